@@ -1,0 +1,179 @@
+/*
+ * sph_amd.h -- C ABI of the MI355X (gfx950) 2D SPH step library (libsph_amd.so).
+ *
+ * One opaque context owns every device buffer of one particle system on one GPU
+ * and one HIP stream. Every call returns SPH_OK (0) or a negative status; the
+ * message of the last failure is returned by sph_last_error(). Calls are
+ * asynchronous on the context stream unless they return host data. Plain
+ * pointers and sizes only: host arrays are float / int32 / uint32 / uint64.
+ *
+ * Each entry point replaces one role of the reference's device "acceleration"
+ * API (jose-pinzon0407/sph-simulations, core/acceleration/); the replaced
+ * declaration is cited beside it. Differences that are deliberate:
+ *   - the context owns the buffers (the reference passes raw device pointers
+ *     held in globals, app/cu_simulation_configuration.h:15-20);
+ *   - errors are reported (the reference checks nothing);
+ *   - the order is an enum, not a std::string (cuda_callbacks.cuh:26);
+ *   - the CSR list capacity grows inside the context instead of being freed and
+ *     reallocated by the callee every step (cuNeighborLists.cu:107-119).
+ */
+#ifndef SPH_AMD_H
+#define SPH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_OK 0
+#define SPH_ERR_INVALID (-1)  /* bad argument / unsupported configuration */
+#define SPH_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define SPH_ERR_CAPACITY (-3) /* more particles than the context was created for */
+#define SPH_ERR_STATE (-4)    /* call order violated (e.g. lists requested but not built) */
+
+enum sph_order { SPH_ORDER_MORTON = 0, SPH_ORDER_ROWMAJOR = 1 };
+enum sph_physics { SPH_PHYS_REFERENCE = 0, SPH_PHYS_WCSPH = 1 };
+enum sph_boundary { SPH_BC_WALLS = 0, SPH_BC_PERIODIC = 1 };
+enum sph_compute { SPH_COMPUTE_SORTED = 0, SPH_COMPUTE_LISTS = 1 };
+
+/* Simulation parameters. Reference physics (SPH_PHYS_REFERENCE) reproduces the
+ * macros of core/ConfigVariables.hpp:9-15 when filled with h=1, eos_k=30,
+ * rho0=25, mu=0.01, gravity_y=-10, mass=1. */
+typedef struct sph_params {
+  double lo[2];      /* Domain::l_corner (core/Domain.h:7) */
+  double hi[2];      /* Domain::h_corner (core/Domain.h:8) */
+  double h;          /* kernel support radius (H) */
+  double cell_size;  /* <= 0 -> h; must be >= h */
+  double dt;
+  double mass;       /* uniform particle mass */
+  double rho0;       /* rest density */
+  double gravity_y;
+  double eos_k;      /* reference physics: p = eos_k (rho - rho0) */
+  double mu;         /* reference physics: viscosity */
+  double c0;         /* WCSPH: numerical sound speed */
+  double alpha;      /* WCSPH: Monaghan artificial viscosity */
+  double eps_visc;   /* WCSPH: mu_ij regulariser, times h^2 */
+  int32_t gamma;     /* WCSPH: integer Tait exponent */
+  int32_t physics;   /* enum sph_physics */
+  int32_t boundary;  /* enum sph_boundary */
+  int32_t order;     /* enum sph_order */
+  int32_t ell_width; /* neighbor slots per particle in the fused step (0 -> 32) */
+  int32_t reserved;
+} sph_params;
+
+/* Uniform cell grid derived from the parameters (cuGridInformation,
+ * core/acceleration/cuParticles.cuh:31-38, plus MortonOrder padding). */
+typedef struct sph_grid_info {
+  float cs_x, cs_y;
+  float lo_x, lo_y;
+  float len_x, len_y;
+  int32_t nx, ny;   /* logical cells per axis */
+  int32_t nxp, nyp; /* padded extents of the key space */
+  int64_t n_keys;   /* cell-table size = largest key + 1 */
+  int32_t key_bits; /* significant key bits (radix passes = ceil(bits / 8)) */
+  int32_t order;
+} sph_grid_info;
+
+/* Device view of the current (sorted) state, for consumers that read it in
+ * place on the same stream (all arrays hold n elements, float unless noted). */
+typedef struct sph_device_view {
+  int64_t n;
+  float *rx, *ry, *vx, *vy, *ax, *ay, *rho, *pr; /* pr = p / rho^2 */
+  uint32_t* pid;
+  int32_t* ell_count; /* neighbors of each particle (self excluded) */
+  int32_t* ell_index; /* n * ell_width slots, stencil order */
+  int32_t ell_width;
+  int32_t pad;
+  const int32_t* cell_range; /* int2 (start, end) per key */
+  void* stream;              /* hipStream_t of the context */
+} sph_device_view;
+
+/* Stage indices of sph_stage_times(), named after the reference timers
+ * (app/simulation_configuration.h:72-91) where one exists. Each launch is
+ * bracketed by hipEvents on the context stream when timing is on. */
+enum sph_stage {
+  SPH_T_INTEGRATION = 0,   /* standalone integration */
+  SPH_T_NEIGHBOR_SEARCH,   /* cell keys; permutation + cell ranges */
+  SPH_T_NEIGHBOR_LIST,     /* CSR build (count, scan, fill) */
+  SPH_T_DENSITY,           /* density + EOS (+ fixed-width neighbor list) */
+  SPH_T_FORCE,             /* forces (+ integration and next keys inside sph_step) */
+  SPH_T_SORT_HIST,         /* radix sort: digit histogram kernel */
+  SPH_T_SORT_PASS,         /* radix sort: one onesweep pass (several per sort) */
+  SPH_T_MEASURE,
+  SPH_T_COUNT
+};
+
+/* ---- lifetime (cuDeviceProperties / cuCreateContainer / cuCreateCellBuffers /
+ *      cuSetupNeighborLists, core/acceleration/cudaLaunchers.cuh:7-15) ---- */
+int sph_create(const sph_params* params, int64_t n_max, int device, void** out_ctx);
+int sph_destroy(void* ctx);
+const char* sph_last_error(const void* ctx); /* ctx may be NULL (creation errors) */
+int sph_get_grid(const void* ctx, sph_grid_info* out);
+/* Derived float constants in the order of the oracle's orc_consts struct. */
+int sph_get_constants(const void* ctx, float* out, int32_t count);
+
+/* ---- data movement (cuReadContainer / cuWriteContainer, cudaLaunchers.cuh:9-10) ---- */
+int sph_upload(void* ctx, int64_t n, const float* rx, const float* ry, const float* vx,
+               const float* vy, const uint32_t* pid /* NULL -> 0..n-1 */);
+int sph_download(void* ctx, float* rx, float* ry, float* vx, float* vy, uint32_t* pid);
+/* Any pointer may be NULL. pres is p (not p / rho^2). */
+int sph_download_fields(void* ctx, float* ax, float* ay, float* rho, float* pres);
+int64_t sph_size(const void* ctx);
+
+/* ---- hot-path stages ---- */
+/* cuCallbackFindNeighbors (cuda_callbacks.cuh:19-26): cell keys, stable radix sort
+ * by key, permutation of r, v, a and particle ids, cell start/end ranges. */
+int sph_find_neighbors(void* ctx);
+/* cuCallbackNeighborLists (cuda_callbacks.cuh:48-57) / MultipassNeighborLists
+ * (core/algorithmic/MultipassNeighborLists.h:14-18): CSR lists in stencil order.
+ * include_self = 1 gives the CPU convention, 0 the GPU convention. */
+int sph_build_neighbor_lists(void* ctx, int32_t include_self, int64_t* total);
+/* cuCallbackApproximateDensitySorted + cuInvertBuffer + cuKernelComputePressureSorted
+ * (cuda_callbacks.cuh:40-46; cudaLaunchers.cuh:17-53), or the list variant
+ * cuKernelApproximateDensityWithNeighborLists for SPH_COMPUTE_LISTS. */
+int sph_density_pressure(void* ctx, int32_t compute);
+/* cuCallbackComputeForcesSorted (cuda_callbacks.cuh:28-38) or
+ * cuKernelComputeForcesWithNeighborLists (cudaLaunchers.cuh:23-33). */
+int sph_forces(void* ctx, int32_t compute);
+/* cuCallbackIntegrate (core/acceleration/integrator/cuIntegratorLauncher.cuh:23-28). */
+int sph_integrate(void* ctx);
+/* cuKernelMeasurementsPerStep (core/acceleration/utils/cuUtilsLauncher.cuh:5-9) in
+ * FP64: out[0] kinetic energy, out[1] max |v|^2, out[2] particles inside the
+ * domain, out[3] potential energy, out[4] total mass. Synchronises. */
+int sph_measure(void* ctx, double* out5);
+
+/* ---- fused pipeline (no reference counterpart; same arithmetic as the stages) ----
+ * sph_prepare: find neighbors + density with the fixed-width neighbor list.
+ * sph_step: n iterations of [forces + integration + keys] -> radix sort ->
+ *   [permutation + cell ranges] -> [density + EOS + neighbor list]; forces are
+ *   left for the state before the last integration (call sph_forces to refresh).
+ * Reference-equivalent run of S iterations (simulation_configuration.h:297-387):
+ *   sph_prepare; sph_step(S-1); sph_forces(SPH_COMPUTE_LISTS). */
+int sph_prepare(void* ctx);
+int sph_step(void* ctx, int32_t n_steps);
+/* Capture one fused iteration in a hipGraph and replay it in sph_step. */
+int sph_set_graph(void* ctx, int32_t enable);
+/* Number of particles whose neighbor count exceeded the fixed list width in the
+ * last density pass (they are handled by a stencil fallback). Synchronises. */
+int sph_ell_overflow(void* ctx, int64_t* count);
+
+/* ---- inspection (host copies; synchronise) ---- */
+int sph_get_sorted_keys(void* ctx, uint64_t* keys);  /* n keys of the last sort */
+int sph_get_permutation(void* ctx, uint32_t* perm);  /* sorted slot -> previous index */
+int sph_get_cell_ranges(void* ctx, int32_t* start, int32_t* end); /* n_keys each */
+int sph_get_neighbor_lists(void* ctx, int64_t* offsets, int32_t* indices); /* CSR */
+int sph_get_ell(void* ctx, int32_t* counts, int32_t* slots); /* n, n*ell_width */
+int sph_get_device_view(void* ctx, sph_device_view* out);
+
+/* ---- timing (hipEvents on the context stream) ---- */
+int sph_set_timing(void* ctx, int32_t enable);
+/* Accumulated milliseconds per enum sph_stage and launch counts; resets them. */
+int sph_stage_times(void* ctx, double* ms, int64_t* launches, int32_t count);
+int sph_synchronize(void* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPH_AMD_H */

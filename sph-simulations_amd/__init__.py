@@ -1,0 +1,13 @@
+"""MI355X-native 2D SPH step (gfx950 HIP kernels behind a C ABI).
+
+Import with importlib, the directory name carries a hyphen:
+    sph = importlib.import_module("sph-simulations_amd")
+"""
+from .capi import (BC_PERIODIC, BC_WALLS, COMPUTE_LISTS, COMPUTE_SORTED, LIB_PATH,
+                   ORDER_MORTON, ORDER_ROWMAJOR, PHYS_REFERENCE, PHYS_WCSPH, STAGES, Context,
+                   GridInfo, Params, SphError, lib)
+from . import scenes
+
+__all__ = ["Context", "Params", "GridInfo", "SphError", "lib", "scenes", "LIB_PATH",
+           "ORDER_MORTON", "ORDER_ROWMAJOR", "PHYS_REFERENCE", "PHYS_WCSPH", "BC_WALLS",
+           "BC_PERIODIC", "COMPUTE_SORTED", "COMPUTE_LISTS", "STAGES"]

@@ -1,0 +1,341 @@
+"""ctypes binding of libsph_amd.so (the C ABI declared in include/sph_amd.h).
+
+The library is the only compute path: importing this module on a machine where
+the library is missing raises immediately, and every call checks the returned
+status and raises SphError with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsph_amd.so")
+
+SPH_OK = 0
+ORDER_MORTON, ORDER_ROWMAJOR = 0, 1
+PHYS_REFERENCE, PHYS_WCSPH = 0, 1
+BC_WALLS, BC_PERIODIC = 0, 1
+COMPUTE_SORTED, COMPUTE_LISTS = 0, 1
+STAGES = ("integration", "neighbor_search", "neighbor_list", "approximate_density",
+          "force_computation", "sort_histogram", "sort_pass", "measure")
+
+
+class SphError(RuntimeError):
+    pass
+
+
+class SphParams(C.Structure):
+    _fields_ = [
+        ("lo", C.c_double * 2), ("hi", C.c_double * 2), ("h", C.c_double),
+        ("cell_size", C.c_double), ("dt", C.c_double), ("mass", C.c_double),
+        ("rho0", C.c_double), ("gravity_y", C.c_double), ("eos_k", C.c_double),
+        ("mu", C.c_double), ("c0", C.c_double), ("alpha", C.c_double),
+        ("eps_visc", C.c_double), ("gamma", C.c_int32), ("physics", C.c_int32),
+        ("boundary", C.c_int32), ("order", C.c_int32), ("ell_width", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class GridInfo(C.Structure):
+    _fields_ = [
+        ("cs_x", C.c_float), ("cs_y", C.c_float), ("lo_x", C.c_float), ("lo_y", C.c_float),
+        ("len_x", C.c_float), ("len_y", C.c_float), ("nx", C.c_int32), ("ny", C.c_int32),
+        ("nxp", C.c_int32), ("nyp", C.c_int32), ("n_keys", C.c_int64),
+        ("key_bits", C.c_int32), ("order", C.c_int32),
+    ]
+
+
+class DeviceView(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64), ("rx", C.c_void_p), ("ry", C.c_void_p), ("vx", C.c_void_p),
+        ("vy", C.c_void_p), ("ax", C.c_void_p), ("ay", C.c_void_p), ("rho", C.c_void_p),
+        ("pr", C.c_void_p), ("pid", C.c_void_p), ("ell_count", C.c_void_p),
+        ("ell_index", C.c_void_p), ("ell_width", C.c_int32), ("pad", C.c_int32),
+        ("cell_range", C.c_void_p), ("stream", C.c_void_p),
+    ]
+
+
+@dataclass
+class Params:
+    """Python mirror of sph_params; fills both the product and oracle structs."""
+    lo: tuple = (0.0, 0.0)
+    hi: tuple = (1.0, 1.0)
+    h: float = 1.0
+    cell_size: float = 0.0
+    dt: float = 0.005
+    mass: float = 1.0
+    rho0: float = 25.0
+    gravity_y: float = -10.0
+    eos_k: float = 30.0
+    mu: float = 0.01
+    c0: float = 0.0
+    alpha: float = 0.0
+    eps_visc: float = 0.01
+    gamma: int = 7
+    physics: int = PHYS_REFERENCE
+    boundary: int = BC_WALLS
+    order: int = ORDER_MORTON
+    ell_width: int = 0
+    extra: dict = field(default_factory=dict)
+
+    def fill(self, s):
+        s.lo[0], s.lo[1] = self.lo
+        s.hi[0], s.hi[1] = self.hi
+        for name in ("h", "cell_size", "dt", "mass", "rho0", "gravity_y", "eos_k", "mu", "c0",
+                     "alpha", "eps_visc", "gamma", "physics", "boundary", "order",
+                     "ell_width"):
+            setattr(s, name, getattr(self, name))
+        s.reserved = 0
+        return s
+
+    def to_c(self) -> SphParams:
+        return self.fill(SphParams())
+
+
+_lib = None
+
+
+def lib():
+    """Load libsph_amd.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SphError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, d = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    P = C.POINTER
+    sig = {
+        "sph_create": (i32, [P(SphParams), i64, C.c_int, P(vp)]),
+        "sph_destroy": (i32, [vp]),
+        "sph_last_error": (C.c_char_p, [vp]),
+        "sph_get_grid": (i32, [vp, P(GridInfo)]),
+        "sph_get_constants": (i32, [vp, P(C.c_float), i32]),
+        "sph_upload": (i32, [vp, i64, vp, vp, vp, vp, vp]),
+        "sph_download": (i32, [vp, vp, vp, vp, vp, vp]),
+        "sph_download_fields": (i32, [vp, vp, vp, vp, vp]),
+        "sph_size": (i64, [vp]),
+        "sph_find_neighbors": (i32, [vp]),
+        "sph_build_neighbor_lists": (i32, [vp, i32, P(i64)]),
+        "sph_density_pressure": (i32, [vp, i32]),
+        "sph_forces": (i32, [vp, i32]),
+        "sph_integrate": (i32, [vp]),
+        "sph_measure": (i32, [vp, P(d)]),
+        "sph_prepare": (i32, [vp]),
+        "sph_step": (i32, [vp, i32]),
+        "sph_set_graph": (i32, [vp, i32]),
+        "sph_ell_overflow": (i32, [vp, P(i64)]),
+        "sph_get_sorted_keys": (i32, [vp, vp]),
+        "sph_get_permutation": (i32, [vp, vp]),
+        "sph_get_cell_ranges": (i32, [vp, vp, vp]),
+        "sph_get_neighbor_lists": (i32, [vp, vp, vp]),
+        "sph_get_ell": (i32, [vp, vp, vp]),
+        "sph_get_device_view": (i32, [vp, P(DeviceView)]),
+        "sph_set_timing": (i32, [vp, i32]),
+        "sph_stage_times": (i32, [vp, P(d), P(i64), i32]),
+        "sph_synchronize": (i32, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_get_constants",
+            "sph_upload", "sph_download", "sph_download_fields", "sph_size",
+            "sph_find_neighbors", "sph_build_neighbor_lists", "sph_density_pressure",
+            "sph_forces", "sph_integrate", "sph_measure", "sph_prepare", "sph_step",
+            "sph_set_graph", "sph_ell_overflow", "sph_get_sorted_keys", "sph_get_permutation",
+            "sph_get_cell_ranges", "sph_get_neighbor_lists", "sph_get_ell",
+            "sph_get_device_view", "sph_set_timing", "sph_stage_times", "sph_synchronize")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Context:
+    """One particle system on one GPU (owns an sph_ctx)."""
+
+    def __init__(self, params: Params, n_max: int, device: int = 0):
+        self._L = lib()
+        self.params = params
+        h = C.c_void_p()
+        rc = self._L.sph_create(C.byref(params.to_c()), int(n_max), int(device), C.byref(h))
+        if rc != SPH_OK:
+            raise SphError(f"sph_create -> {rc}: {self._L.sph_last_error(None).decode()}")
+        self._h = h
+        self.n_max = int(n_max)
+
+    # -- plumbing
+    def _chk(self, rc, what):
+        if rc != SPH_OK:
+            msg = self._L.sph_last_error(self._h).decode()
+            raise SphError(f"{what} -> {rc}: {msg}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.sph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def n(self) -> int:
+        return int(self._L.sph_size(self._h))
+
+    def grid(self) -> GridInfo:
+        g = GridInfo()
+        self._chk(self._L.sph_get_grid(self._h, C.byref(g)), "sph_get_grid")
+        return g
+
+    def constants(self) -> np.ndarray:
+        out = np.zeros(32, np.float32)
+        m = self._L.sph_get_constants(self._h, out.ctypes.data_as(C.POINTER(C.c_float)), 32)
+        if m < 0:
+            self._chk(m, "sph_get_constants")
+        return out[:m]
+
+    # -- data
+    def upload(self, rx, ry, vx, vy, pid=None):
+        rx, ry, vx, vy = _f32(rx), _f32(ry), _f32(vx), _f32(vy)
+        p = None if pid is None else np.ascontiguousarray(pid, dtype=np.uint32)
+        self._chk(self._L.sph_upload(self._h, rx.size, _ptr(rx), _ptr(ry), _ptr(vx), _ptr(vy),
+                                     _ptr(p)), "sph_upload")
+
+    def download(self):
+        n = self.n
+        rx, ry, vx, vy = (np.empty(n, np.float32) for _ in range(4))
+        pid = np.empty(n, np.uint32)
+        self._chk(self._L.sph_download(self._h, _ptr(rx), _ptr(ry), _ptr(vx), _ptr(vy),
+                                       _ptr(pid)), "sph_download")
+        return dict(rx=rx, ry=ry, vx=vx, vy=vy, pid=pid)
+
+    def fields(self):
+        n = self.n
+        ax, ay, rho, p = (np.empty(n, np.float32) for _ in range(4))
+        self._chk(self._L.sph_download_fields(self._h, _ptr(ax), _ptr(ay), _ptr(rho), _ptr(p)),
+                  "sph_download_fields")
+        return dict(ax=ax, ay=ay, rho=rho, p=p)
+
+    # -- stages
+    def find_neighbors(self):
+        self._chk(self._L.sph_find_neighbors(self._h), "sph_find_neighbors")
+
+    def build_neighbor_lists(self, include_self=False) -> int:
+        tot = C.c_int64()
+        self._chk(self._L.sph_build_neighbor_lists(self._h, int(bool(include_self)),
+                                                   C.byref(tot)), "sph_build_neighbor_lists")
+        return int(tot.value)
+
+    def density_pressure(self, compute=COMPUTE_SORTED):
+        self._chk(self._L.sph_density_pressure(self._h, int(compute)), "sph_density_pressure")
+
+    def forces(self, compute=COMPUTE_SORTED):
+        self._chk(self._L.sph_forces(self._h, int(compute)), "sph_forces")
+
+    def integrate(self):
+        self._chk(self._L.sph_integrate(self._h), "sph_integrate")
+
+    def measure(self) -> np.ndarray:
+        out = np.zeros(5, np.float64)
+        self._chk(self._L.sph_measure(self._h, out.ctypes.data_as(C.POINTER(C.c_double))),
+                  "sph_measure")
+        return out
+
+    def prepare(self):
+        self._chk(self._L.sph_prepare(self._h), "sph_prepare")
+
+    def step(self, n_steps=1):
+        self._chk(self._L.sph_step(self._h, int(n_steps)), "sph_step")
+
+    def set_graph(self, enable=True):
+        self._chk(self._L.sph_set_graph(self._h, int(bool(enable))), "sph_set_graph")
+
+    def ell_overflow(self) -> int:
+        c = C.c_int64()
+        self._chk(self._L.sph_ell_overflow(self._h, C.byref(c)), "sph_ell_overflow")
+        return int(c.value)
+
+    def run(self, steps: int):
+        """Reference-equivalent run of `steps` iterations (pre-loop + steps-1 loops)."""
+        self.prepare()
+        if steps > 1:
+            self.step(steps - 1)
+        self.forces(COMPUTE_LISTS)
+
+    # -- inspection
+    def sorted_keys(self):
+        out = np.empty(self.n, np.uint64)
+        self._chk(self._L.sph_get_sorted_keys(self._h, _ptr(out)), "sph_get_sorted_keys")
+        return out
+
+    def permutation(self):
+        out = np.empty(self.n, np.uint32)
+        self._chk(self._L.sph_get_permutation(self._h, _ptr(out)), "sph_get_permutation")
+        return out
+
+    def cell_ranges(self):
+        nk = int(self.grid().n_keys)
+        s = np.empty(nk, np.int32)
+        e = np.empty(nk, np.int32)
+        self._chk(self._L.sph_get_cell_ranges(self._h, _ptr(s), _ptr(e)), "sph_get_cell_ranges")
+        return s, e
+
+    def neighbor_lists(self, total: int):
+        off = np.empty(self.n + 1, np.int64)
+        idx = np.empty(max(total, 1), np.int32)
+        self._chk(self._L.sph_get_neighbor_lists(self._h, _ptr(off), _ptr(idx)),
+                  "sph_get_neighbor_lists")
+        return off, idx[:total]
+
+    def ell(self):
+        w = self.params.ell_width or 32
+        cnt = np.empty(self.n, np.int32)
+        slots = np.empty((self.n, w), np.int32)
+        self._chk(self._L.sph_get_ell(self._h, _ptr(cnt), _ptr(slots)), "sph_get_ell")
+        return cnt, slots
+
+    def device_view(self) -> DeviceView:
+        v = DeviceView()
+        self._chk(self._L.sph_get_device_view(self._h, C.byref(v)), "sph_get_device_view")
+        return v
+
+    # -- timing
+    def set_timing(self, enable=True):
+        self._chk(self._L.sph_set_timing(self._h, int(bool(enable))), "sph_set_timing")
+
+    def stage_times(self):
+        ms = np.zeros(len(STAGES), np.float64)
+        cnt = np.zeros(len(STAGES), np.int64)
+        self._chk(self._L.sph_stage_times(self._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                          cnt.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          len(STAGES)), "sph_stage_times")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(STAGES)}
+
+    def synchronize(self):
+        self._chk(self._L.sph_synchronize(self._h), "sph_synchronize")

@@ -1,0 +1,852 @@
+// C-ABI of libsph_amd.so: context lifetime, buffers, stage orchestration,
+// hipGraph capture of the fused iteration, and hipEvent stage timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sph_amd.h"
+#include "sph_internal.hpp"
+
+using namespace sphamd;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct PendingTiming {
+  int stage;
+  hipEvent_t a, b;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  sph_params prm{};
+  sph_grid_info grid{};
+  Launch L{};
+  int64_t n_max = 0;
+  DevState s{};
+  SortBuffers sb{};
+  std::vector<void*> allocs;
+  int64_t csr_cap = 0, csr_total = -1;
+  int csr_self = -1;
+  int64_t* scan_scratch = nullptr;
+  double* meas_part = nullptr;
+  double* meas_out = nullptr;
+  int64_t* ovf = nullptr;
+  std::string err;
+  // state flags
+  bool sorted = false;         // S is in the order of the last sort, ranges valid
+  bool density_ell = false;    // rho/pr/ELL valid for S
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<PendingTiming> pending;
+  double stage_ms[SPH_T_COUNT] = {};
+  int64_t launches[SPH_T_COUNT] = {};
+  // graph
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_n = -1;
+};
+
+int fail(Ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(Ctx* c, hipError_t e, const char* what) {
+  return fail(c, SPH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, expr)                                  \
+  do {                                                     \
+    hipError_t e_ = (expr);                                \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+int next_pow2(int n) {
+  int p = 1;
+  while (p < n) p *= 2;
+  return p;
+}
+
+uint64_t morton_host(uint32_t x, uint32_t y) {
+  uint64_t z = 0;
+  for (int i = 0; i < 32; ++i) {
+    z |= (uint64_t)((x >> i) & 1u) << (2 * i);
+    z |= (uint64_t)((y >> i) & 1u) << (2 * i + 1);
+  }
+  return z;
+}
+
+// Grid of MortonOrder::Initialize / RowMajorOrder::Initialize
+// (core/alg/orders/MortonOrder.h:15-47, RowMajorOrder.h:13-25): ceil(L / cs) cells
+// per axis in float, Morton extents padded to powers of two; the table is sized
+// by the largest Morton code. Periodic boxes use floor(L / h) cells of L / n.
+int derive_grid(const sph_params& p, sph_grid_info& g, std::string& why) {
+  float lo_x = (float)p.lo[0], lo_y = (float)p.lo[1];
+  float len_x = (float)p.hi[0] - lo_x, len_y = (float)p.hi[1] - lo_y;
+  float cs = (float)(p.cell_size > 0 ? p.cell_size : p.h);
+  if (!(p.h > 0) || !(cs > 0) || !(len_x > 0) || !(len_y > 0)) {
+    why = "h, cell size and box lengths must be positive";
+    return -1;
+  }
+  if (p.cell_size > 0 && p.cell_size < p.h) {
+    why = "cell_size must be >= h (the 3x3 stencil must cover the support)";
+    return -1;
+  }
+  int nx, ny;
+  float csx = cs, csy = cs;
+  if (p.boundary == SPH_BC_PERIODIC) {
+    nx = (int)std::floor(len_x / cs);
+    ny = (int)std::floor(len_y / cs);
+    if (nx < 3 || ny < 3) {
+      why = "periodic boxes need at least 3 cells per axis";
+      return -1;
+    }
+    csx = len_x / (float)nx;
+    csy = len_y / (float)ny;
+  } else {
+    nx = (int)std::ceil(len_x / cs);
+    ny = (int)std::ceil(len_y / cs);
+  }
+  if (nx < 1 || ny < 1 || nx > (1 << 30) || ny > (1 << 30)) {
+    why = "cell grid out of range";
+    return -1;
+  }
+  g.cs_x = csx; g.cs_y = csy;
+  g.lo_x = lo_x; g.lo_y = lo_y;
+  g.len_x = len_x; g.len_y = len_y;
+  g.nx = nx; g.ny = ny;
+  g.order = p.order;
+  if (p.order == SPH_ORDER_MORTON) {
+    g.nxp = next_pow2(nx);
+    g.nyp = next_pow2(ny);
+    g.n_keys = (int64_t)morton_host((uint32_t)(g.nxp - 1), (uint32_t)(g.nyp - 1)) + 1;
+  } else {
+    g.nxp = nx; g.nyp = ny;
+    g.n_keys = (int64_t)nx * ny;
+  }
+  if (g.n_keys > (int64_t)1 << 31) {
+    why = "cell table larger than 2^31 entries";
+    return -1;
+  }
+  int bits = 0;
+  while (bits < 63 && ((uint64_t)1 << bits) < (uint64_t)g.n_keys) ++bits;
+  g.key_bits = std::max(bits, 1);
+  return 0;
+}
+
+// Derived constants; the same double expressions as the oracle, rounded once.
+void derive_phys(const sph_params& p, const sph_grid_info& g, DevPhys& c, DevGrid& dg) {
+  const double PI = 3.14159265358979323846;
+  double h = p.h, h2 = h * h, h4 = h2 * h2, h5 = h4 * h, h6 = h4 * h2, h8 = h4 * h4,
+         h9 = h8 * h;
+  c.h = (float)h;
+  c.h2 = c.h * c.h;
+  c.tol = 1.0e-12f;
+  c.physics = p.physics;
+  c.boundary = p.boundary;
+  c.gamma = p.gamma;
+  c.gx = 0.0f;
+  c.gy = (float)p.gravity_y;
+  c.dt = (float)p.dt;
+  c.rho0 = (float)p.rho0;
+  c.eos_k = (float)p.eos_k;
+  if (p.physics == SPH_PHYS_REFERENCE) {
+    double c_rho = 315.0 / (64.0 * PI * h9);  // poly6, core/Kernel.cpp:8-10
+    double cp = 15.0 / (PI * h6);             // spiky, core/Kernel.cpp:38-40
+    c.mc_rho = (float)(p.mass * c_rho);
+    c.c_pres = (float)(1.5 * cp * p.mass);
+    c.c_visc = (float)(45.0 * p.mu / (PI * h6));  // cuParticles.cuh:51
+    c.inv_rho0 = 0.0f;
+    c.tait_b = 0.0f;
+  } else {
+    double c_rho = 4.0 / (PI * h8);  // poly6 normalised in 2D
+    double cg = 30.0 / (PI * h5);    // spiky gradient normalised in 2D
+    c.mc_rho = (float)(p.mass * c_rho);
+    c.c_pres = (float)(p.mass * cg);
+    c.c_visc = 0.0f;
+    c.inv_rho0 = (float)(1.0 / p.rho0);
+    c.tait_b = (float)(p.rho0 * p.c0 * p.c0 / (double)p.gamma);
+  }
+  c.neg_alpha_c0 = (float)(-p.alpha * p.c0);
+  c.eps_h2 = (float)(p.eps_visc * h2);
+  c.lo_x = (float)p.lo[0];
+  c.lo_y = (float)p.lo[1];
+  c.hi_x = (float)p.hi[0];
+  c.hi_y = (float)p.hi[1];
+  c.len_x = c.hi_x - c.lo_x;
+  c.len_y = c.hi_y - c.lo_y;
+  c.half_x = c.len_x * 0.5f;
+  c.half_y = c.len_y * 0.5f;
+  dg.lo_x = g.lo_x;
+  dg.lo_y = g.lo_y;
+  dg.cs_x = g.cs_x;
+  dg.cs_y = g.cs_y;
+  dg.nx = g.nx;
+  dg.ny = g.ny;
+  dg.order = g.order;
+  dg.periodic = p.boundary == SPH_BC_PERIODIC ? 1 : 0;
+}
+
+template <class T>
+int dev_alloc(Ctx* c, T** p, int64_t count) {
+  void* q = nullptr;
+  size_t bytes = (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
+  c->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return SPH_OK;
+}
+
+Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
+
+// ---- timing helpers
+hipEvent_t take_event(Ctx* c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct StageTimer {
+  Ctx* c;
+  int stage;
+  hipEvent_t a = nullptr;
+  StageTimer(Ctx* ctx, int st) : c(ctx), stage(st) {
+    if (c->timing) {
+      a = take_event(c);
+      if (a) (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~StageTimer() {
+    if (!a) return;
+    hipEvent_t b = take_event(c);
+    if (!b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({stage, a, b});
+  }
+};
+
+int collect_timings(Ctx* c) {
+  if (c->pending.empty()) return SPH_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto& t : c->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      c->stage_ms[t.stage] += ms;
+      c->launches[t.stage] += 1;
+    }
+    c->ev_pool.push_back(t.a);
+    c->ev_pool.push_back(t.b);
+  }
+  c->pending.clear();
+  return SPH_OK;
+}
+
+// ---- pipeline pieces
+int sort_keys(Ctx* c) {
+  SortBuffers b = c->sb;
+  b.clear_n = c->s.n_prev;
+  {
+    StageTimer t(c, SPH_T_SORT_HIST);
+    HIPCHK(c, radix_sort_hist(b, c->s.n, c->grid.key_bits, c->stream));
+  }
+  for (int p = 0; p < sort_passes(c->grid.key_bits); ++p) {
+    StageTimer t(c, SPH_T_SORT_PASS);
+    HIPCHK(c, radix_sort_pass(b, c->s.n, c->grid.key_bits, p, c->stream));
+  }
+  return SPH_OK;
+}
+
+int hist_words(Ctx* c) { return sort_passes(c->grid.key_bits) * 256; }
+
+// Standalone find-neighbors: keys of S, sort, permute S (r, v, a, pid) into T,
+// ranges, then swap S and T.
+int find_neighbors(Ctx* c) {
+  DevState& s = c->s;
+  {
+    StageTimer t(c, SPH_T_NEIGHBOR_SEARCH);
+    HIPCHK(c, launch_keys(c->L, s, c->sb.keys_in));
+  }
+  int rc = sort_keys(c);
+  if (rc) return rc;
+  {
+    StageTimer t(c, SPH_T_NEIGHBOR_SEARCH);
+    HIPCHK(c, launch_gather(c->L, s, s.rx, s.ry, s.vx, s.vy, s.pid, s.ax, s.ay, s.tx, s.ty,
+                            s.tvx, s.tvy, s.pid_alt, s.tax, s.tay, hist_words(c)));
+  }
+  std::swap(s.rx, s.tx);
+  std::swap(s.ry, s.ty);
+  std::swap(s.vx, s.tvx);
+  std::swap(s.vy, s.tvy);
+  std::swap(s.ax, s.tax);
+  std::swap(s.ay, s.tay);
+  std::swap(s.pid, s.pid_alt);
+  s.n_prev = s.n;
+  c->sorted = true;
+  c->density_ell = false;
+  c->csr_total = -1;
+  return SPH_OK;
+}
+
+// One fused iteration: forces(ELL) + integrate + keys -> sort -> gather T->S
+// + ranges -> density + EOS + ELL.
+int fused_iteration(Ctx* c) {
+  DevState& s = c->s;
+  {
+    StageTimer t(c, SPH_T_FORCE);
+    HIPCHK(c, launch_forces(c->L, s, NB_ELL, true, c->sb.keys_in));
+  }
+  int rc = sort_keys(c);
+  if (rc) return rc;
+  {
+    StageTimer t(c, SPH_T_NEIGHBOR_SEARCH);
+    HIPCHK(c, launch_gather(c->L, s, s.tx, s.ty, s.tvx, s.tvy, s.pid, nullptr, nullptr, s.rx,
+                            s.ry, s.vx, s.vy, s.pid_alt, nullptr, nullptr, hist_words(c)));
+  }
+  std::swap(s.pid, s.pid_alt);
+  s.n_prev = s.n;
+  {
+    StageTimer t(c, SPH_T_DENSITY);
+    HIPCHK(c, launch_density(c->L, s, NB_ELL, false));
+  }
+  return SPH_OK;
+}
+
+void destroy_graph(Ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_n = -1;
+}
+
+// The graph holds one fused iteration with fixed buffer pointers. The pid
+// buffers swap once per iteration, so two iterations are captured (an even
+// number of swaps returns every pointer to where it started).
+int capture_graph(Ctx* c) {
+  destroy_graph(c);
+  bool was_timing = c->timing;
+  c->timing = false;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = fused_iteration(c);
+  if (rc == SPH_OK) rc = fused_iteration(c);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->timing = was_timing;
+  if (rc != SPH_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+  c->graph = g;
+  e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    destroy_graph(c);
+    return hip_fail(c, e, "hipGraphInstantiate");
+  }
+  c->graph_n = c->s.n;
+  return SPH_OK;
+}
+
+int check_n(Ctx* c) {
+  if (c->s.n <= 0) return fail(c, SPH_ERR_STATE, "no particles uploaded");
+  return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_create(const sph_params* params, int64_t n_max, int device, void** out_ctx) {
+  g_create_error.clear();
+  if (!params || !out_ctx || n_max <= 0 || n_max >= ((int64_t)1 << 30)) {
+    g_create_error = "sph_create: invalid arguments (n_max must be in [1, 2^30))";
+    return SPH_ERR_INVALID;
+  }
+  const sph_params& p = *params;
+  if (p.physics != SPH_PHYS_REFERENCE && p.physics != SPH_PHYS_WCSPH) {
+    g_create_error = "sph_create: unknown physics";
+    return SPH_ERR_INVALID;
+  }
+  if (p.boundary != SPH_BC_WALLS && p.boundary != SPH_BC_PERIODIC) {
+    g_create_error = "sph_create: unknown boundary";
+    return SPH_ERR_INVALID;
+  }
+  if (p.order != SPH_ORDER_MORTON && p.order != SPH_ORDER_ROWMAJOR) {
+    g_create_error = "sph_create: unknown order";
+    return SPH_ERR_INVALID;
+  }
+  if (p.physics == SPH_PHYS_WCSPH && (p.gamma < 1 || p.gamma > 16 || !(p.rho0 > 0))) {
+    g_create_error = "sph_create: WCSPH needs an integer gamma in [1,16] and rho0 > 0";
+    return SPH_ERR_INVALID;
+  }
+  Ctx* c = new Ctx();
+  c->prm = p;
+  c->n_max = n_max;
+  c->device = device;
+  std::string why;
+  if (derive_grid(p, c->grid, why) != 0) {
+    g_create_error = "sph_create: " + why;
+    delete c;
+    return SPH_ERR_INVALID;
+  }
+  derive_phys(p, c->grid, c->L.phys, c->L.grid);
+  int rc = SPH_OK;
+  auto bail = [&](int code) {
+    g_create_error = c->err.empty() ? std::string("sph_create failed") : c->err;
+    sph_destroy(c);
+    return code;
+  };
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    hip_fail(c, e, "hipSetDevice");
+    return bail(SPH_ERR_HIP);
+  }
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    hip_fail(c, e, "hipStreamCreate");
+    return bail(SPH_ERR_HIP);
+  }
+  c->L.stream = c->stream;
+  DevState& s = c->s;
+  s.ell_width = p.ell_width > 0 ? p.ell_width : 32;
+  s.n_keys = c->grid.n_keys;
+  const int64_t N = n_max;
+  float** farrs[] = {&s.rx, &s.ry, &s.vx, &s.vy, &s.tx, &s.ty, &s.tvx, &s.tvy, &s.ax,
+                     &s.ay, &s.tax, &s.tay, &s.rho, &s.pr, &s.pres};
+  for (float** a : farrs)
+    if ((rc = dev_alloc(c, a, N))) return bail(rc);
+  if ((rc = dev_alloc(c, &s.pid, N)) || (rc = dev_alloc(c, &s.pid_alt, N)) ||
+      (rc = dev_alloc(c, &s.cells, c->grid.n_keys)) ||
+      (rc = dev_alloc(c, &s.ell_count, N)) ||
+      (rc = dev_alloc(c, &s.ell_idx, N * s.ell_width)) ||
+      (rc = dev_alloc(c, &s.csr_count, N)) || (rc = dev_alloc(c, &s.csr_off, N + 1)))
+    return bail(rc);
+  const int tiles = (int)((N + sort_tile_size() - 1) / sort_tile_size());
+  SortBuffers& b = c->sb;
+  if ((rc = dev_alloc(c, &b.keys_in, N)) || (rc = dev_alloc(c, &b.keys_tmp, N)) ||
+      (rc = dev_alloc(c, &b.keys_out, N)) || (rc = dev_alloc(c, &b.vals_a, N)) ||
+      (rc = dev_alloc(c, &b.vals_b, N)) || (rc = dev_alloc(c, &b.vals_out, N)) ||
+      (rc = dev_alloc(c, &b.hist, kMaxPasses * 256)) ||
+      (rc = dev_alloc(c, &b.tickets, kMaxPasses)) ||
+      (rc = dev_alloc(c, &b.lookback, (int64_t)sort_passes(c->grid.key_bits) * tiles * 256)))
+    return bail(rc);
+  b.max_tiles = tiles;
+  int cu = 256;
+  (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device);
+  b.hist_blocks = std::max(cu, 1);
+  b.cells = s.cells;
+  s.keys_sorted = b.keys_out;
+  s.perm = b.vals_out;
+  s.hist = b.hist;
+  if ((rc = dev_alloc(c, &c->scan_scratch, scan_scratch_words((int)N))) ||
+      (rc = dev_alloc(c, &c->meas_part, measure_partial_words((int)N))) ||
+      (rc = dev_alloc(c, &c->meas_out, 8)) || (rc = dev_alloc(c, &c->ovf, 1)))
+    return bail(rc);
+  e = hipMemsetAsync(b.hist, 0, kMaxPasses * 256 * sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, c->grid.n_keys);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    hip_fail(c, e, "initialise buffers");
+    return bail(SPH_ERR_HIP);
+  }
+  *out_ctx = c;
+  return SPH_OK;
+}
+
+int sph_destroy(void* ctx) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  destroy_graph(c);
+  for (auto& t : c->pending) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SPH_OK;
+}
+
+const char* sph_last_error(const void* ctx) {
+  const Ctx* c = static_cast<const Ctx*>(ctx);
+  if (!c) return g_create_error.c_str();
+  return c->err.c_str();
+}
+
+int sph_get_grid(const void* ctx, sph_grid_info* out) {
+  const Ctx* c = static_cast<const Ctx*>(ctx);
+  if (!c || !out) return SPH_ERR_INVALID;
+  *out = c->grid;
+  return SPH_OK;
+}
+
+int sph_get_constants(const void* ctx, float* out, int32_t count) {
+  const Ctx* c = static_cast<const Ctx*>(ctx);
+  if (!c || !out) return SPH_ERR_INVALID;
+  const DevPhys& p = c->L.phys;
+  // order of orc_consts: h, h2, tol, mc_rho, c_pres, c_visc, gx, gy, eos_k, rho0,
+  // inv_rho0, tait_b, neg_alpha_c0, eps_h2, dt, lo_x, lo_y, hi_x, hi_y, len_x, len_y
+  const float v[] = {p.h,      p.h2,     p.tol,    p.mc_rho, p.c_pres, p.c_visc, p.gx,
+                     p.gy,     p.eos_k,  p.rho0,   p.inv_rho0, p.tait_b, p.neg_alpha_c0,
+                     p.eps_h2, p.dt,     p.lo_x,   p.lo_y,   p.hi_x,   p.hi_y,   p.len_x,
+                     p.len_y};
+  int m = std::min<int>(count, (int)(sizeof(v) / sizeof(v[0])));
+  for (int i = 0; i < m; ++i) out[i] = v[i];
+  return m;
+}
+
+int sph_upload(void* ctx, int64_t n, const float* rx, const float* ry, const float* vx,
+               const float* vy, const uint32_t* pid) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  if (n < 0 || n > c->n_max) return fail(c, SPH_ERR_CAPACITY, "n exceeds the context capacity");
+  if (n > 0 && (!rx || !ry || !vx || !vy))
+    return fail(c, SPH_ERR_INVALID, "null position or velocity array");
+  DevState& s = c->s;
+  s.n = (int)n;
+  size_t fb = (size_t)n * sizeof(float);
+  HIPCHK(c, hipMemcpyAsync(s.rx, rx, fb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.ry, ry, fb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.vx, vx, fb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.vy, vy, fb, hipMemcpyHostToDevice, c->stream));
+  std::vector<uint32_t> ids;
+  const uint32_t* src = pid;
+  if (!src) {
+    ids.resize(n);
+    for (int64_t i = 0; i < n; ++i) ids[i] = (uint32_t)i;
+    src = ids.data();
+  }
+  HIPCHK(c, hipMemcpyAsync(s.pid, src, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(s.ax, 0, fb, c->stream));
+  HIPCHK(c, hipMemsetAsync(s.ay, 0, fb, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->sorted = false;
+  c->density_ell = false;
+  c->csr_total = -1;
+  if (c->graph_n != s.n) destroy_graph(c);
+  return SPH_OK;
+}
+
+int sph_download(void* ctx, float* rx, float* ry, float* vx, float* vy, uint32_t* pid) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  DevState& s = c->s;
+  size_t fb = (size_t)s.n * sizeof(float);
+  if (rx) HIPCHK(c, hipMemcpyAsync(rx, s.rx, fb, hipMemcpyDeviceToHost, c->stream));
+  if (ry) HIPCHK(c, hipMemcpyAsync(ry, s.ry, fb, hipMemcpyDeviceToHost, c->stream));
+  if (vx) HIPCHK(c, hipMemcpyAsync(vx, s.vx, fb, hipMemcpyDeviceToHost, c->stream));
+  if (vy) HIPCHK(c, hipMemcpyAsync(vy, s.vy, fb, hipMemcpyDeviceToHost, c->stream));
+  if (pid) HIPCHK(c, hipMemcpyAsync(pid, s.pid, (size_t)s.n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int sph_download_fields(void* ctx, float* ax, float* ay, float* rho, float* pres) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  DevState& s = c->s;
+  size_t fb = (size_t)s.n * sizeof(float);
+  if (ax) HIPCHK(c, hipMemcpyAsync(ax, s.ax, fb, hipMemcpyDeviceToHost, c->stream));
+  if (ay) HIPCHK(c, hipMemcpyAsync(ay, s.ay, fb, hipMemcpyDeviceToHost, c->stream));
+  if (rho) HIPCHK(c, hipMemcpyAsync(rho, s.rho, fb, hipMemcpyDeviceToHost, c->stream));
+  if (pres) HIPCHK(c, hipMemcpyAsync(pres, s.pres, fb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int64_t sph_size(const void* ctx) {
+  const Ctx* c = static_cast<const Ctx*>(ctx);
+  return c ? c->s.n : -1;
+}
+
+int sph_find_neighbors(void* ctx) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  int rc = check_n(c);
+  if (rc) return rc;
+  return find_neighbors(c);
+}
+
+int sph_build_neighbor_lists(void* ctx, int32_t include_self, int64_t* total) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  int rc = check_n(c);
+  if (rc) return rc;
+  if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_find_neighbors first");
+  DevState& s = c->s;
+  StageTimer t(c, SPH_T_NEIGHBOR_LIST);
+  HIPCHK(c, launch_csr_count(c->L, s, include_self != 0));
+  HIPCHK(c, launch_scan(c->L, s.csr_count, s.csr_off, s.n, c->scan_scratch));
+  int64_t tot = 0;
+  HIPCHK(c, hipMemcpyAsync(&tot, s.csr_off + s.n, sizeof(int64_t), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (tot > c->csr_cap) {
+    int64_t cap = std::max<int64_t>(tot + tot / 4, 1024);
+    if (s.csr_idx) {
+      auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)s.csr_idx);
+      if (it != c->allocs.end()) c->allocs.erase(it);
+      HIPCHK(c, hipFree(s.csr_idx));
+      s.csr_idx = nullptr;
+    }
+    if ((rc = dev_alloc(c, &s.csr_idx, cap))) return rc;
+    c->csr_cap = cap;
+  }
+  HIPCHK(c, launch_csr_fill(c->L, s, include_self != 0));
+  c->csr_total = tot;
+  c->csr_self = include_self != 0;
+  if (total) *total = tot;
+  return SPH_OK;
+}
+
+int sph_density_pressure(void* ctx, int32_t compute) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  int rc = check_n(c);
+  if (rc) return rc;
+  if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_find_neighbors first");
+  int nb = NB_STENCIL;
+  if (compute == SPH_COMPUTE_LISTS) {
+    if (c->csr_total < 0 || c->csr_self != 0)
+      return fail(c, SPH_ERR_STATE, "lists compute needs sph_build_neighbor_lists(self=0)");
+    nb = NB_CSR;
+  } else if (compute != SPH_COMPUTE_SORTED) {
+    return fail(c, SPH_ERR_INVALID, "unknown compute type");
+  }
+  StageTimer t(c, SPH_T_DENSITY);
+  HIPCHK(c, launch_density(c->L, c->s, nb, true));
+  c->density_ell = false;
+  return SPH_OK;
+}
+
+int sph_forces(void* ctx, int32_t compute) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  int rc = check_n(c);
+  if (rc) return rc;
+  if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_find_neighbors first");
+  int nb = NB_STENCIL;
+  if (compute == SPH_COMPUTE_LISTS) {
+    if (c->density_ell) {
+      nb = NB_ELL;
+    } else if (c->csr_total >= 0 && c->csr_self == 0) {
+      nb = NB_CSR;
+    } else {
+      return fail(c, SPH_ERR_STATE, "lists compute needs neighbor lists (self excluded)");
+    }
+  } else if (compute != SPH_COMPUTE_SORTED) {
+    return fail(c, SPH_ERR_INVALID, "unknown compute type");
+  }
+  StageTimer t(c, SPH_T_FORCE);
+  HIPCHK(c, launch_forces(c->L, c->s, nb, false, nullptr));
+  return SPH_OK;
+}
+
+int sph_integrate(void* ctx) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  int rc = check_n(c);
+  if (rc) return rc;
+  StageTimer t(c, SPH_T_INTEGRATION);
+  HIPCHK(c, launch_integrate(c->L, c->s));
+  c->density_ell = false;  // positions moved: neighbor data is stale
+  c->csr_total = -1;
+  return SPH_OK;
+}
+
+int sph_measure(void* ctx, double* out5) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !out5) return SPH_ERR_INVALID;
+  {
+    StageTimer t(c, SPH_T_MEASURE);
+    HIPCHK(c, launch_measure(c->L, c->s, c->prm.mass, c->meas_part, c->meas_out));
+  }
+  HIPCHK(c, hipMemcpyAsync(out5, c->meas_out, 5 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int sph_prepare(void* ctx) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  int rc = check_n(c);
+  if (rc) return rc;
+  if ((rc = find_neighbors(c))) return rc;
+  {
+    StageTimer t(c, SPH_T_DENSITY);
+    HIPCHK(c, launch_density(c->L, c->s, NB_ELL, true));
+  }
+  c->density_ell = true;
+  return SPH_OK;
+}
+
+int sph_step(void* ctx, int32_t n_steps) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  int rc = check_n(c);
+  if (rc) return rc;
+  if (!c->density_ell) return fail(c, SPH_ERR_STATE, "call sph_prepare first");
+  int done = 0;
+  if (c->use_graph && !c->timing) {
+    if (c->graph_n != c->s.n && (rc = capture_graph(c))) return rc;
+    for (; done + 2 <= n_steps; done += 2) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+  }
+  for (; done < n_steps; ++done)
+    if ((rc = fused_iteration(c))) return rc;
+  c->csr_total = -1;
+  return SPH_OK;
+}
+
+int sph_set_graph(void* ctx, int32_t enable) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  c->use_graph = enable != 0;
+  if (!c->use_graph) destroy_graph(c);
+  return SPH_OK;
+}
+
+int sph_ell_overflow(void* ctx, int64_t* count) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !count) return SPH_ERR_INVALID;
+  if (!c->density_ell) return fail(c, SPH_ERR_STATE, "no fixed-width list built");
+  HIPCHK(c, hipMemsetAsync(c->ovf, 0, sizeof(int64_t), c->stream));
+  HIPCHK(c, launch_ell_overflow(c->L, c->s, c->ovf));
+  HIPCHK(c, hipMemcpyAsync(count, c->ovf, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int sph_get_sorted_keys(void* ctx, uint64_t* keys) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !keys) return SPH_ERR_INVALID;
+  if (!c->sorted) return fail(c, SPH_ERR_STATE, "no sort has run");
+  HIPCHK(c, hipMemcpyAsync(keys, c->s.keys_sorted, (size_t)c->s.n * 8, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int sph_get_permutation(void* ctx, uint32_t* perm) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !perm) return SPH_ERR_INVALID;
+  if (!c->sorted) return fail(c, SPH_ERR_STATE, "no sort has run");
+  HIPCHK(c, hipMemcpyAsync(perm, c->s.perm, (size_t)c->s.n * 4, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int sph_get_cell_ranges(void* ctx, int32_t* start, int32_t* end) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  if (!c->sorted) return fail(c, SPH_ERR_STATE, "no sort has run");
+  std::vector<int2> cells(c->grid.n_keys);
+  HIPCHK(c, hipMemcpyAsync(cells.data(), c->s.cells, cells.size() * sizeof(int2),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (size_t k = 0; k < cells.size(); ++k) {
+    if (start) start[k] = cells[k].x;
+    if (end) end[k] = cells[k].y;
+  }
+  return SPH_OK;
+}
+
+int sph_get_neighbor_lists(void* ctx, int64_t* offsets, int32_t* indices) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  if (c->csr_total < 0) return fail(c, SPH_ERR_STATE, "lists not built");
+  if (offsets)
+    HIPCHK(c, hipMemcpyAsync(offsets, c->s.csr_off, (size_t)(c->s.n + 1) * 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  if (indices && c->csr_total > 0)
+    HIPCHK(c, hipMemcpyAsync(indices, c->s.csr_idx, (size_t)c->csr_total * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int sph_get_ell(void* ctx, int32_t* counts, int32_t* slots) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  if (!c->density_ell) return fail(c, SPH_ERR_STATE, "no fixed-width list built");
+  const int n = c->s.n, w = c->s.ell_width;
+  std::vector<int32_t> cm((size_t)n * w);
+  if (counts)
+    HIPCHK(c, hipMemcpyAsync(counts, c->s.ell_count, (size_t)n * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+  if (slots)
+    HIPCHK(c, hipMemcpyAsync(cm.data(), c->s.ell_idx, cm.size() * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (slots)
+    for (int k = 0; k < w; ++k)
+      for (int i = 0; i < n; ++i) slots[(size_t)i * w + k] = cm[(size_t)k * n + i];
+  return SPH_OK;
+}
+
+int sph_get_device_view(void* ctx, sph_device_view* out) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !out) return SPH_ERR_INVALID;
+  DevState& s = c->s;
+  out->n = s.n;
+  out->rx = s.rx; out->ry = s.ry; out->vx = s.vx; out->vy = s.vy;
+  out->ax = s.ax; out->ay = s.ay; out->rho = s.rho; out->pr = s.pr;
+  out->pid = s.pid;
+  out->ell_count = s.ell_count;
+  out->ell_index = s.ell_idx;
+  out->ell_width = s.ell_width;
+  out->pad = 0;
+  out->cell_range = reinterpret_cast<const int32_t*>(s.cells);
+  out->stream = c->stream;
+  return SPH_OK;
+}
+
+int sph_set_timing(void* ctx, int32_t enable) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  int rc = collect_timings(c);
+  c->timing = enable != 0;
+  return rc;
+}
+
+int sph_stage_times(void* ctx, double* ms, int64_t* launches, int32_t count) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  int rc = collect_timings(c);
+  if (rc) return rc;
+  for (int i = 0; i < count && i < SPH_T_COUNT; ++i) {
+    if (ms) ms[i] = c->stage_ms[i];
+    if (launches) launches[i] = c->launches[i];
+    c->stage_ms[i] = 0;
+    c->launches[i] = 0;
+  }
+  return SPH_OK;
+}
+
+int sph_synchronize(void* ctx) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+}  // extern "C"

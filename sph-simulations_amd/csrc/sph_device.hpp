@@ -1,0 +1,212 @@
+// Device-side helpers shared by the gfx950 kernels: grid/physics constants,
+// 64-bit Morton keys, cell coordinates and the per-pair SPH arithmetic.
+//
+// Arithmetic contract: every kernel is compiled with -ffp-contract=off and IEEE
+// float division / sqrt, and evaluates each expression in the order written
+// here, so integer outputs (keys, neighbor sets) are bit-exact with the CPU
+// definition and float outputs follow one fixed operation order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sphamd {
+
+constexpr int kWave = 64;
+
+struct DevGrid {
+  float lo_x, lo_y, cs_x, cs_y;
+  int32_t nx, ny;
+  int32_t order;     // 0 Morton, 1 row-major
+  int32_t periodic;  // 1 -> wrapped stencil
+};
+
+struct DevPhys {
+  float h, h2, tol, mc_rho, c_pres, c_visc, gx, gy;
+  float eos_k, rho0, inv_rho0, tait_b, neg_alpha_c0, eps_h2, dt;
+  float lo_x, lo_y, hi_x, hi_y, len_x, len_y, half_x, half_y;
+  int32_t physics, boundary, gamma;
+};
+
+// Spread the low 32 bits of v to the even bit positions of a 64-bit word
+// (Morton dilation; x -> even bits, y -> odd bits as in MortonOrder.h:117-126).
+__device__ __forceinline__ uint64_t dilate(uint32_t v) {
+  uint64_t x = v;
+  x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+  x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+  x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+
+__device__ __forceinline__ uint64_t cell_key(const DevGrid& g, int cx, int cy) {
+  if (g.order == 0) return dilate((uint32_t)cx) | (dilate((uint32_t)cy) << 1);
+  return (uint64_t)cx + (uint64_t)cy * (uint64_t)g.nx;
+}
+
+// floor((x - lo) / cs) clamped to [0, n-1]; NaN -> 0.
+__device__ __forceinline__ int cell_coord(float x, float lo, float cs, int n) {
+  float f = floorf((x - lo) / cs);
+  if (f >= (float)n) return n - 1;
+  if (f >= 0.0f) return (int)f;
+  return 0;
+}
+
+__device__ __forceinline__ float min_image(float d, float len, float half) {
+  if (d > half) return d - len;
+  if (d < -half) return d + len;
+  return d;
+}
+
+template <int BC>
+__device__ __forceinline__ void separation(const DevPhys& c, float xi, float yi, float xj,
+                                           float yj, float& dx, float& dy) {
+  dx = xi - xj;
+  dy = yi - yj;
+  if (BC == 1) {
+    dx = min_image(dx, c.len_x, c.half_x);
+    dy = min_image(dy, c.len_y, c.half_y);
+  }
+}
+
+__device__ __forceinline__ float density_term(float h2, float r2) {
+  float t = h2 - r2;
+  return (t * t) * t;
+}
+
+__device__ __forceinline__ float ipowf(float x, int e) {
+  float r = 1.0f, b = x;
+  while (e > 0) {
+    if (e & 1) r = r * b;
+    e >>= 1;
+    if (e) b = b * b;
+  }
+  return r;
+}
+
+// rho from the kernel sum; p from the EOS; pr = p / rho^2.
+template <int PHYS>
+__device__ __forceinline__ void eos(const DevPhys& c, float acc, float& rho, float& p,
+                                   float& pr) {
+  rho = c.mc_rho * acc;
+  if (PHYS == 0) {
+    p = c.eos_k * (rho - c.rho0);
+  } else {
+    float x = rho * c.inv_rho0;
+    p = c.tait_b * (ipowf(x, c.gamma) - 1.0f);
+  }
+  pr = p / (rho * rho);
+}
+
+struct Acc {
+  float fpx, fpy, fvx, fvy;
+};
+
+// Contribution of neighbor j to the acceleration of i (d = r_i - r_j).
+template <int PHYS>
+__device__ __forceinline__ void force_pair(const DevPhys& c, float dx, float dy, float r2,
+                                           float rho_i, float rho_j, float pr_i, float pr_j,
+                                           float vxi, float vyi, float vxj, float vyj, Acc& a) {
+  float r = sqrtf(r2);
+  float s = c.h - r;
+  if (PHYS == 0) {
+    if (r > c.tol) {
+      float coef = ((c.c_pres * (s * s)) / r) * (pr_i + pr_j);
+      a.fpx += coef * dx;
+      a.fpy += coef * dy;
+    }
+    float cv = (c.c_visc * s) / rho_j;
+    a.fvx += cv * (vxj - vxi);
+    a.fvy += cv * (vyj - vyi);
+  } else {
+    if (r > c.tol) {
+      float dvx = vxi - vxj, dvy = vyi - vyj;
+      float vr = dvx * dx + dvy * dy;
+      float pij = 0.0f;
+      if (vr < 0.0f) {
+        float mu = (c.h * vr) / (r2 + c.eps_h2);
+        pij = (c.neg_alpha_c0 * mu) / ((rho_i + rho_j) * 0.5f);
+      }
+      float coef = ((c.c_pres * (s * s)) / r) * ((pr_i + pr_j) + pij);
+      a.fpx += coef * dx;
+      a.fpy += coef * dy;
+    }
+  }
+}
+
+template <int PHYS>
+__device__ __forceinline__ void force_finish(const DevPhys& c, const Acc& a, float& ax,
+                                             float& ay) {
+  ax = c.gx + a.fpx;
+  ay = c.gy + a.fpy;
+  if (PHYS == 0) {
+    ax = ax + a.fvx;
+    ay = ay + a.fvy;
+  }
+}
+
+// Symplectic Euler and the boundary treatment.
+template <int BC>
+__device__ __forceinline__ void integrate_one(const DevPhys& c, float& x, float& y, float& vx,
+                                              float& vy, float ax, float ay) {
+  vx = vx + c.dt * ax;
+  vy = vy + c.dt * ay;
+  x = x + c.dt * vx;
+  y = y + c.dt * vy;
+  if (BC == 1) {
+    if (x >= c.hi_x) x = x - c.len_x; else if (x < c.lo_x) x = x + c.len_x;
+    if (y >= c.hi_y) y = y - c.len_y; else if (y < c.lo_y) y = y + c.len_y;
+  } else {
+    if (x > c.hi_x) { x = c.hi_x; vx = -vx; }
+    if (x < c.lo_x) { x = c.lo_x; vx = -vx; }
+    if (y > c.hi_y) { y = c.hi_y; vy = -vy; }
+    if (y < c.lo_y) { y = c.lo_y; vy = -vy; }
+  }
+}
+
+// Visit the neighbors of sorted particle i in stencil order (x offset outer,
+// y offset inner, ascending index in a cell), self excluded unless SELF. cells
+// holds one int2 (start, end) per key, -1 for empty cells.
+template <int BC, bool SELF = false, class F>
+__device__ __forceinline__ void for_each_neighbor(const DevGrid& g, const DevPhys& c,
+                                                  const float* __restrict__ rx,
+                                                  const float* __restrict__ ry,
+                                                  const int2* __restrict__ cells, int i,
+                                                  float xi, float yi, F&& fn) {
+  int cx = cell_coord(xi, g.lo_x, g.cs_x, g.nx);
+  int cy = cell_coord(yi, g.lo_y, g.cs_y, g.ny);
+#pragma unroll
+  for (int ox = -1; ox <= 1; ++ox) {
+    int cxj = cx + ox;
+    if (BC == 1) {
+      cxj = cxj < 0 ? cxj + g.nx : (cxj >= g.nx ? cxj - g.nx : cxj);
+    } else if (cxj < 0 || cxj >= g.nx) {
+      continue;
+    }
+#pragma unroll
+    for (int oy = -1; oy <= 1; ++oy) {
+      int cyj = cy + oy;
+      if (BC == 1) {
+        cyj = cyj < 0 ? cyj + g.ny : (cyj >= g.ny ? cyj - g.ny : cyj);
+      } else if (cyj < 0 || cyj >= g.ny) {
+        continue;
+      }
+      int2 se = cells[cell_key(g, cxj, cyj)];
+      if (se.x < 0 || se.y < 0) continue;
+      for (int j = se.x; j < se.y; ++j) {
+        if (j == i) {
+          if (SELF) fn(j, 0.0f, 0.0f, 0.0f);
+          continue;
+        }
+        float dx, dy;
+        separation<BC>(c, xi, yi, rx[j], ry[j], dx, dy);
+        float r2 = dx * dx + dy * dy;
+        if (r2 > c.h2) continue;
+        fn(j, dx, dy, r2);
+      }
+    }
+  }
+}
+
+}  // namespace sphamd

@@ -1,0 +1,96 @@
+// Internal interfaces between the C-ABI layer (sph_capi.hip) and the kernel
+// translation units (sph_kernels.hip, sph_sort.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sph_device.hpp"
+
+namespace sphamd {
+
+constexpr int kMaxPasses = 8;  // 64-bit keys, 8-bit digits
+
+struct SortBuffers {
+  uint64_t* keys_in;   // unsorted keys (clobbered when passes >= 3)
+  uint64_t* keys_tmp;
+  uint64_t* keys_out;  // sorted keys
+  uint32_t* vals_a;
+  uint32_t* vals_b;
+  uint32_t* vals_out;  // permutation: sorted slot -> input index
+  uint32_t* hist;      // [kMaxPasses][256], zero on entry
+  uint32_t* tickets;   // [kMaxPasses]
+  uint32_t* lookback;  // [kMaxPasses][max_tiles][256]
+  int max_tiles;
+  int hist_blocks;
+  // cell ranges written by the previous sort: the histogram kernel clears them
+  // (cells of keys_out[0..clear_n) boundaries) before keys_out is overwritten
+  int2* cells;
+  int clear_n;
+};
+
+int sort_tile_size();
+int sort_passes(int key_bits);
+hipError_t radix_sort_pairs(const SortBuffers& b, int n, int key_bits, hipStream_t stream);
+// the two launch kinds of radix_sort_pairs, for per-kernel timing
+hipError_t radix_sort_hist(const SortBuffers& b, int n, int key_bits, hipStream_t stream);
+hipError_t radix_sort_pass(const SortBuffers& b, int n, int key_bits, int pass,
+                           hipStream_t stream);
+
+// Particle arrays of one context. "S" is the sorted state the stages operate on;
+// "T" holds an unsorted copy (integrated positions in the fused step, or the
+// permutation target of the standalone find-neighbors stage).
+struct DevState {
+  int n;
+  int n_prev;  // particles covered by keys_sorted / the cell table (0: none)
+  float *rx, *ry, *vx, *vy;
+  float *tx, *ty, *tvx, *tvy;
+  float *ax, *ay, *tax, *tay;
+  float *rho, *pr, *pres;
+  uint32_t *pid, *pid_alt;
+  uint64_t* keys_sorted;
+  uint32_t* perm;
+  int2* cells;
+  int64_t n_keys;
+  int32_t *ell_count, *ell_idx;
+  int ell_width;
+  int32_t* csr_count;
+  int64_t* csr_off;
+  int32_t* csr_idx;
+  uint32_t* hist;  // the sort's histogram, cleared by the permutation kernel
+};
+
+enum { NB_STENCIL = 0, NB_CSR = 1, NB_ELL = 2 };
+
+struct Launch {
+  hipStream_t stream;
+  DevGrid grid;
+  DevPhys phys;
+};
+
+hipError_t launch_fill_cells(const Launch& L, int2* cells, int64_t n_keys);
+// keys of S (standalone stage); clears the ranges left by the previous sort
+hipError_t launch_keys(const Launch& L, const DevState& s, uint64_t* keys_out);
+// gather src -> dst through perm, write cell ranges, clear the sort histogram
+hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx,
+                         const float* src_ry, const float* src_vx, const float* src_vy,
+                         const uint32_t* src_pid, const float* src_ax, const float* src_ay,
+                         float* dst_rx, float* dst_ry, float* dst_vx, float* dst_vy,
+                         uint32_t* dst_pid, float* dst_ax, float* dst_ay, int hist_words);
+hipError_t launch_density(const Launch& L, const DevState& s, int nb_mode, bool write_p);
+hipError_t launch_forces(const Launch& L, const DevState& s, int nb_mode, bool fused,
+                         uint64_t* keys_out);
+hipError_t launch_integrate(const Launch& L, const DevState& s);
+hipError_t launch_csr_count(const Launch& L, const DevState& s, bool self);
+hipError_t launch_csr_fill(const Launch& L, const DevState& s, bool self);
+// exclusive scan of csr_count into csr_off[0..n]; scratch >= scan_scratch_words(n)
+int64_t scan_scratch_words(int n);
+hipError_t launch_scan(const Launch& L, const int32_t* in, int64_t* out, int n,
+                       int64_t* scratch);
+// FP64 diagnostics into out5 (device), partials >= measure_partial_words(n)
+int64_t measure_partial_words(int n);
+hipError_t launch_measure(const Launch& L, const DevState& s, double mass, double* partials,
+                          double* out5);
+hipError_t launch_ell_overflow(const Launch& L, const DevState& s, int64_t* out_count);
+
+}  // namespace sphamd

@@ -1,0 +1,515 @@
+// gfx950 kernels of the 2D SPH step other than the radix sort: cell keys,
+// permutation + cell ranges, neighbor lists, density + EOS, forces (optionally
+// fused with integration and the next step's keys), integration, FP64
+// diagnostics. One thread per particle in Morton order, so the neighbor gathers
+// of a wavefront hit a compact patch of cells and coalesce in L2.
+#include <hip/hip_runtime.h>
+
+#include "sph_internal.hpp"
+
+namespace sphamd {
+
+namespace {
+
+constexpr int BLK = 256;
+
+inline int blocks_for(int64_t n, int blk = BLK) { return (int)((n + blk - 1) / blk); }
+
+__global__ __launch_bounds__(BLK) void k_fill_cells(int2* __restrict__ cells, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * BLK)
+    cells[i] = make_int2(-1, -1);
+}
+
+// Cell key of every particle of the sorted state (GenericNeighborSearch.h:200-232).
+__global__ __launch_bounds__(BLK) void k_keys(int n, DevGrid g, const float* __restrict__ rx,
+                                              const float* __restrict__ ry,
+                                              uint64_t* __restrict__ keys) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = cell_key(g, cell_coord(rx[i], g.lo_x, g.cs_x, g.nx),
+                     cell_coord(ry[i], g.lo_y, g.cs_y, g.ny));
+}
+
+// Permutation through the sort result plus cell ranges (cuSetCellRanges,
+// cuFindNeighbors.cu:103-122): run starts are found with one wave64 ballot of
+// "key differs from the previous key"; a run ends where the next one starts, so
+// the end mask is the start mask shifted by one lane (the last lane compares
+// with the next wave's first key). Also clears the sort's digit histogram.
+__global__ __launch_bounds__(BLK) void k_gather(
+    int n, const uint32_t* __restrict__ perm, const uint64_t* __restrict__ keys,
+    const float* __restrict__ srx, const float* __restrict__ sry,
+    const float* __restrict__ svx, const float* __restrict__ svy,
+    const uint32_t* __restrict__ spid, const float* __restrict__ sax,
+    const float* __restrict__ say, float* __restrict__ drx, float* __restrict__ dry,
+    float* __restrict__ dvx, float* __restrict__ dvy, uint32_t* __restrict__ dpid,
+    float* __restrict__ dax, float* __restrict__ day, int2* __restrict__ cells,
+    uint32_t* __restrict__ hist, int hist_words) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (blockIdx.x == 0)
+    for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
+  const bool live = i < n;
+  uint64_t k = live ? keys[i] : ~0ull;
+  const int lane = threadIdx.x & (kWave - 1);
+  uint64_t prev = __shfl_up(k, 1);
+  if (lane == 0) prev = (i > 0 && live) ? keys[i - 1] : ~k;
+  const bool is_start = live && (i == 0 || k != prev);
+  const uint64_t starts = __ballot(is_start);
+  bool is_end;
+  if (lane < kWave - 1) {
+    is_end = live && (i == n - 1 || ((starts >> (lane + 1)) & 1ull));
+  } else {
+    is_end = live && (i == n - 1 || k != keys[i + 1]);
+  }
+  if (!live) return;
+  if (is_start) cells[k].x = i;
+  if (is_end) cells[k].y = i + 1;
+  const uint32_t j = perm[i];
+  drx[i] = srx[j];
+  dry[i] = sry[j];
+  dvx[i] = svx[j];
+  dvy[i] = svy[j];
+  dpid[i] = spid[j];
+  if (sax) {
+    dax[i] = sax[j];
+    day[i] = say[j];
+  }
+}
+
+// Summation density + EOS. NB: stencil traversal, CSR rows, or stencil with the
+// fixed-width (column-major) neighbor list written on the way.
+template <int PHYS, int BC, int NB>
+__global__ __launch_bounds__(BLK) void k_density(
+    int n, DevGrid g, DevPhys c, const float* __restrict__ rx, const float* __restrict__ ry,
+    const int2* __restrict__ cells, const int64_t* __restrict__ csr_off,
+    const int32_t* __restrict__ csr_idx, int32_t* __restrict__ ell_count,
+    int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ rho_out,
+    float* __restrict__ pr_out, float* __restrict__ p_out) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const float xi = rx[i], yi = ry[i];
+  float acc = density_term(c.h2, 0.0f);
+  if (NB == NB_CSR) {
+    for (int64_t k = csr_off[i]; k < csr_off[i + 1]; ++k) {
+      int j = csr_idx[k];
+      float dx, dy;
+      separation<BC>(c, xi, yi, rx[j], ry[j], dx, dy);
+      acc += density_term(c.h2, dx * dx + dy * dy);
+    }
+  } else {
+    int cnt = 0;
+    for_each_neighbor<BC>(g, c, rx, ry, cells, i, xi, yi, [&](int j, float, float, float r2) {
+      acc += density_term(c.h2, r2);
+      if (NB == NB_ELL) {
+        if (cnt < ell_w) ell_idx[(int64_t)cnt * n + i] = j;
+        ++cnt;
+      }
+    });
+    if (NB == NB_ELL) ell_count[i] = cnt;
+  }
+  float rho, p, pr;
+  eos<PHYS>(c, acc, rho, p, pr);
+  rho_out[i] = rho;
+  pr_out[i] = pr;
+  if (p_out) p_out[i] = p;
+}
+
+// Accelerations. FUSED additionally integrates (symplectic Euler) into the T
+// arrays and writes the cell key of the new position for the next sort.
+template <int PHYS, int BC, int NB, bool FUSED>
+__global__ __launch_bounds__(BLK) void k_forces(
+    int n, DevGrid g, DevPhys c, const float* __restrict__ rx, const float* __restrict__ ry,
+    const float* __restrict__ vx, const float* __restrict__ vy,
+    const float* __restrict__ rho, const float* __restrict__ pr,
+    const int2* __restrict__ cells, const int64_t* __restrict__ csr_off,
+    const int32_t* __restrict__ csr_idx, const int32_t* __restrict__ ell_count,
+    const int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ ax_out,
+    float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
+    float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const float xi = rx[i], yi = ry[i], vxi = vx[i], vyi = vy[i], rhoi = rho[i], pri = pr[i];
+  Acc a{0.f, 0.f, 0.f, 0.f};
+  auto pair = [&](int j, float dx, float dy, float r2) {
+    force_pair<PHYS>(c, dx, dy, r2, rhoi, rho[j], pri, pr[j], vxi, vyi, vx[j], vy[j], a);
+  };
+  auto from_list = [&](int j) {
+    float dx, dy;
+    separation<BC>(c, xi, yi, rx[j], ry[j], dx, dy);
+    pair(j, dx, dy, dx * dx + dy * dy);
+  };
+  if (NB == NB_CSR) {
+    for (int64_t k = csr_off[i]; k < csr_off[i + 1]; ++k) from_list(csr_idx[k]);
+  } else if (NB == NB_ELL) {
+    const int cnt = ell_count[i];
+    if (cnt <= ell_w) {
+      for (int k = 0; k < cnt; ++k) from_list(ell_idx[(int64_t)k * n + i]);
+    } else {
+      for_each_neighbor<BC>(g, c, rx, ry, cells, i, xi, yi, pair);
+    }
+  } else {
+    for_each_neighbor<BC>(g, c, rx, ry, cells, i, xi, yi, pair);
+  }
+  float axi, ayi;
+  force_finish<PHYS>(c, a, axi, ayi);
+  ax_out[i] = axi;
+  ay_out[i] = ayi;
+  if (FUSED) {
+    float x = xi, y = yi, u = vxi, v = vyi;
+    integrate_one<BC>(c, x, y, u, v, axi, ayi);
+    tx[i] = x;
+    ty[i] = y;
+    tvx[i] = u;
+    tvy[i] = v;
+    keys_out[i] = cell_key(g, cell_coord(x, g.lo_x, g.cs_x, g.nx),
+                           cell_coord(y, g.lo_y, g.cs_y, g.ny));
+  }
+}
+
+template <int BC>
+__global__ __launch_bounds__(BLK) void k_integrate(int n, DevPhys c, float* __restrict__ rx,
+                                                   float* __restrict__ ry,
+                                                   float* __restrict__ vx,
+                                                   float* __restrict__ vy,
+                                                   const float* __restrict__ ax,
+                                                   const float* __restrict__ ay) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  float x = rx[i], y = ry[i], u = vx[i], v = vy[i];
+  integrate_one<BC>(c, x, y, u, v, ax[i], ay[i]);
+  rx[i] = x;
+  ry[i] = y;
+  vx[i] = u;
+  vy[i] = v;
+}
+
+// Multipass CSR lists (cuCountNeighborsPerParticle / cuAssignIndices,
+// cuNeighborLists.cuh:6-136).
+template <int BC, bool SELF>
+__global__ __launch_bounds__(BLK) void k_csr_count(int n, DevGrid g, DevPhys c,
+                                                   const float* __restrict__ rx,
+                                                   const float* __restrict__ ry,
+                                                   const int2* __restrict__ cells,
+                                                   int32_t* __restrict__ count) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  int k = 0;
+  for_each_neighbor<BC, SELF>(g, c, rx, ry, cells, i, rx[i], ry[i],
+                              [&](int, float, float, float) { ++k; });
+  count[i] = k;
+}
+
+template <int BC, bool SELF>
+__global__ __launch_bounds__(BLK) void k_csr_fill(int n, DevGrid g, DevPhys c,
+                                                  const float* __restrict__ rx,
+                                                  const float* __restrict__ ry,
+                                                  const int2* __restrict__ cells,
+                                                  const int64_t* __restrict__ off,
+                                                  int32_t* __restrict__ idx) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  int64_t o = off[i];
+  for_each_neighbor<BC, SELF>(g, c, rx, ry, cells, i, rx[i], ry[i],
+                              [&](int j, float, float, float) { idx[o++] = j; });
+}
+
+// ---------------------------------------------------------------- scan
+constexpr int SCAN_BLK = 1024;
+
+__device__ __forceinline__ int64_t block_incl_scan64(int64_t v, int64_t* s_w) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  int64_t x = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    int64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) s_w[w] = x;
+  __syncthreads();
+  int64_t off = 0;
+  for (int k = 0; k < w; ++k) off += s_w[k];
+  __syncthreads();
+  return x + off;
+}
+
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_reduce(const int32_t* __restrict__ in, int n,
+                                                          int64_t* __restrict__ part) {
+  __shared__ int64_t s_w[SCAN_BLK / kWave];
+  const int i = blockIdx.x * SCAN_BLK + threadIdx.x;
+  int64_t v = i < n ? in[i] : 0;
+  int64_t incl = block_incl_scan64(v, s_w);
+  if (threadIdx.x == SCAN_BLK - 1) part[blockIdx.x] = incl;
+}
+
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_top(int64_t* __restrict__ part, int nparts) {
+  __shared__ int64_t s_w[SCAN_BLK / kWave];
+  __shared__ int64_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nparts; base += SCAN_BLK) {
+    int i = base + threadIdx.x;
+    int64_t v = i < nparts ? part[i] : 0;
+    int64_t incl = block_incl_scan64(v, s_w);
+    int64_t carry = s_carry;
+    if (i < nparts) part[i] = carry + incl - v;  // exclusive
+    __syncthreads();
+    if (threadIdx.x == SCAN_BLK - 1) s_carry = carry + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[nparts] = s_carry;
+}
+
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_down(const int32_t* __restrict__ in, int n,
+                                                        const int64_t* __restrict__ part,
+                                                        int64_t* __restrict__ out) {
+  __shared__ int64_t s_w[SCAN_BLK / kWave];
+  const int i = blockIdx.x * SCAN_BLK + threadIdx.x;
+  int64_t v = i < n ? in[i] : 0;
+  int64_t incl = block_incl_scan64(v, s_w);
+  if (i < n) out[i] = part[blockIdx.x] + incl - v;
+  if (i == n - 1) out[n] = part[blockIdx.x] + incl;
+}
+
+// ----------------------------------------------------------- diagnostics
+// Per-block FP64 partials of sum 1/2 m |v|^2, sum m (-g) (y - lo), max |v|^2 and
+// the count inside the domain (MeasuringFunctions.h:5-62), then one block that
+// adds the partials in a fixed order.
+__global__ __launch_bounds__(BLK) void k_measure_partial(int n, DevPhys c, double mass,
+                                                         const float* __restrict__ rx,
+                                                         const float* __restrict__ ry,
+                                                         const float* __restrict__ vx,
+                                                         const float* __restrict__ vy,
+                                                         double* __restrict__ part) {
+  __shared__ double s_ke[BLK], s_pe[BLK], s_vm[BLK], s_in[BLK];
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  double ke = 0, pe = 0, vm = 0, in = 0;
+  if (i < n) {
+    double u = vx[i], v = vy[i];
+    double v2 = u * u + v * v;
+    ke = 0.5 * mass * v2;
+    pe = mass * (-(double)c.gy) * ((double)ry[i] - (double)c.lo_y);
+    vm = v2;
+    float x = rx[i], y = ry[i];
+    bool inside = isfinite(x) && isfinite(y) && !(x < c.lo_x) && !(x > c.hi_x) &&
+                  !(y < c.lo_y) && !(y > c.hi_y);
+    in = inside ? 1.0 : 0.0;
+  }
+  s_ke[threadIdx.x] = ke;
+  s_pe[threadIdx.x] = pe;
+  s_vm[threadIdx.x] = vm;
+  s_in[threadIdx.x] = in;
+  __syncthreads();
+  for (int s = BLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      s_ke[threadIdx.x] += s_ke[threadIdx.x + s];
+      s_pe[threadIdx.x] += s_pe[threadIdx.x + s];
+      s_vm[threadIdx.x] = fmax(s_vm[threadIdx.x], s_vm[threadIdx.x + s]);
+      s_in[threadIdx.x] += s_in[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x + 0] = s_ke[0];
+    part[4 * blockIdx.x + 1] = s_pe[0];
+    part[4 * blockIdx.x + 2] = s_vm[0];
+    part[4 * blockIdx.x + 3] = s_in[0];
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_measure_final(int nparts, int n, double mass,
+                                                       const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+  __shared__ double s_ke[BLK], s_pe[BLK], s_vm[BLK], s_in[BLK];
+  double ke = 0, pe = 0, vm = 0, in = 0;
+  for (int b = threadIdx.x; b < nparts; b += BLK) {
+    ke += part[4 * b + 0];
+    pe += part[4 * b + 1];
+    vm = fmax(vm, part[4 * b + 2]);
+    in += part[4 * b + 3];
+  }
+  s_ke[threadIdx.x] = ke;
+  s_pe[threadIdx.x] = pe;
+  s_vm[threadIdx.x] = vm;
+  s_in[threadIdx.x] = in;
+  __syncthreads();
+  for (int s = BLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      s_ke[threadIdx.x] += s_ke[threadIdx.x + s];
+      s_pe[threadIdx.x] += s_pe[threadIdx.x + s];
+      s_vm[threadIdx.x] = fmax(s_vm[threadIdx.x], s_vm[threadIdx.x + s]);
+      s_in[threadIdx.x] += s_in[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s_ke[0];
+    out[1] = s_vm[0];
+    out[2] = s_in[0];
+    out[3] = s_pe[0];
+    out[4] = mass * (double)n;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_ell_overflow(int n, int w,
+                                                      const int32_t* __restrict__ cnt,
+                                                      unsigned long long* __restrict__ out) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  bool over = i < n && cnt[i] > w;
+  uint64_t m = __ballot(over);
+  if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(out, (unsigned long long)__popcll(m));
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_fill_cells(const Launch& L, int2* cells, int64_t n_keys) {
+  int b = (int)std::min<int64_t>(blocks_for(n_keys), 8192);
+  k_fill_cells<<<std::max(b, 1), BLK, 0, L.stream>>>(cells, n_keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_keys(const Launch& L, const DevState& s, uint64_t* keys_out) {
+  if (s.n > 0) k_keys<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, L.grid, s.rx, s.ry, keys_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx,
+                         const float* src_ry, const float* src_vx, const float* src_vy,
+                         const uint32_t* src_pid, const float* src_ax, const float* src_ay,
+                         float* dst_rx, float* dst_ry, float* dst_vx, float* dst_vy,
+                         uint32_t* dst_pid, float* dst_ax, float* dst_ay, int hist_words) {
+  if (s.n > 0)
+    k_gather<<<blocks_for(s.n), BLK, 0, L.stream>>>(
+        s.n, s.perm, s.keys_sorted, src_rx, src_ry, src_vx, src_vy, src_pid, src_ax, src_ay,
+        dst_rx, dst_ry, dst_vx, dst_vy, dst_pid, dst_ax, dst_ay, s.cells, s.hist, hist_words);
+  return hipGetLastError();
+}
+
+template <int PHYS, int BC>
+static hipError_t density_dispatch(const Launch& L, const DevState& s, int nb, bool write_p) {
+  float* p = write_p ? s.pres : nullptr;
+  const int nb_ = s.n > 0 ? blocks_for(s.n) : 0;
+  if (nb_ == 0) return hipSuccess;
+  if (nb == NB_CSR)
+    k_density<PHYS, BC, NB_CSR><<<nb_, BLK, 0, L.stream>>>(
+        s.n, L.grid, L.phys, s.rx, s.ry, s.cells, s.csr_off, s.csr_idx, s.ell_count, s.ell_idx,
+        s.ell_width, s.rho, s.pr, p);
+  else if (nb == NB_ELL)
+    k_density<PHYS, BC, NB_ELL><<<nb_, BLK, 0, L.stream>>>(
+        s.n, L.grid, L.phys, s.rx, s.ry, s.cells, s.csr_off, s.csr_idx, s.ell_count, s.ell_idx,
+        s.ell_width, s.rho, s.pr, p);
+  else
+    k_density<PHYS, BC, NB_STENCIL><<<nb_, BLK, 0, L.stream>>>(
+        s.n, L.grid, L.phys, s.rx, s.ry, s.cells, s.csr_off, s.csr_idx, s.ell_count, s.ell_idx,
+        s.ell_width, s.rho, s.pr, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_density(const Launch& L, const DevState& s, int nb, bool write_p) {
+  const int ph = L.phys.physics, bc = L.phys.boundary;
+  if (ph == 0 && bc == 0) return density_dispatch<0, 0>(L, s, nb, write_p);
+  if (ph == 0 && bc == 1) return density_dispatch<0, 1>(L, s, nb, write_p);
+  if (ph == 1 && bc == 0) return density_dispatch<1, 0>(L, s, nb, write_p);
+  return density_dispatch<1, 1>(L, s, nb, write_p);
+}
+
+template <int PHYS, int BC, int NB, bool FUSED>
+static hipError_t forces_go(const Launch& L, const DevState& s, uint64_t* keys_out) {
+  k_forces<PHYS, BC, NB, FUSED><<<blocks_for(s.n), BLK, 0, L.stream>>>(
+      s.n, L.grid, L.phys, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.csr_off, s.csr_idx,
+      s.ell_count, s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out);
+  return hipGetLastError();
+}
+
+template <int PHYS, int BC>
+static hipError_t forces_dispatch(const Launch& L, const DevState& s, int nb, bool fused,
+                                  uint64_t* keys_out) {
+  if (s.n <= 0) return hipSuccess;
+  if (fused) {
+    if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, true>(L, s, keys_out);
+    if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, true>(L, s, keys_out);
+    return forces_go<PHYS, BC, NB_STENCIL, true>(L, s, keys_out);
+  }
+  if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, false>(L, s, keys_out);
+  if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, false>(L, s, keys_out);
+  return forces_go<PHYS, BC, NB_STENCIL, false>(L, s, keys_out);
+}
+
+hipError_t launch_forces(const Launch& L, const DevState& s, int nb, bool fused,
+                         uint64_t* keys_out) {
+  const int ph = L.phys.physics, bc = L.phys.boundary;
+  if (ph == 0 && bc == 0) return forces_dispatch<0, 0>(L, s, nb, fused, keys_out);
+  if (ph == 0 && bc == 1) return forces_dispatch<0, 1>(L, s, nb, fused, keys_out);
+  if (ph == 1 && bc == 0) return forces_dispatch<1, 0>(L, s, nb, fused, keys_out);
+  return forces_dispatch<1, 1>(L, s, nb, fused, keys_out);
+}
+
+hipError_t launch_integrate(const Launch& L, const DevState& s) {
+  if (s.n <= 0) return hipSuccess;
+  if (L.phys.boundary == 1)
+    k_integrate<1><<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, L.phys, s.rx, s.ry, s.vx, s.vy,
+                                                          s.ax, s.ay);
+  else
+    k_integrate<0><<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, L.phys, s.rx, s.ry, s.vx, s.vy,
+                                                          s.ax, s.ay);
+  return hipGetLastError();
+}
+
+template <int BC, bool SELF>
+static void csr_count_go(const Launch& L, const DevState& s) {
+  k_csr_count<BC, SELF><<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, L.grid, L.phys, s.rx, s.ry,
+                                                               s.cells, s.csr_count);
+}
+template <int BC, bool SELF>
+static void csr_fill_go(const Launch& L, const DevState& s) {
+  k_csr_fill<BC, SELF><<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, L.grid, L.phys, s.rx, s.ry,
+                                                              s.cells, s.csr_off, s.csr_idx);
+}
+
+hipError_t launch_csr_count(const Launch& L, const DevState& s, bool self) {
+  if (s.n <= 0) return hipSuccess;
+  const int bc = L.phys.boundary;
+  if (bc == 0) self ? csr_count_go<0, true>(L, s) : csr_count_go<0, false>(L, s);
+  else self ? csr_count_go<1, true>(L, s) : csr_count_go<1, false>(L, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_fill(const Launch& L, const DevState& s, bool self) {
+  if (s.n <= 0) return hipSuccess;
+  const int bc = L.phys.boundary;
+  if (bc == 0) self ? csr_fill_go<0, true>(L, s) : csr_fill_go<0, false>(L, s);
+  else self ? csr_fill_go<1, true>(L, s) : csr_fill_go<1, false>(L, s);
+  return hipGetLastError();
+}
+
+int64_t scan_scratch_words(int n) { return (int64_t)blocks_for(n, SCAN_BLK) + 2; }
+
+hipError_t launch_scan(const Launch& L, const int32_t* in, int64_t* out, int n,
+                       int64_t* scratch) {
+  if (n <= 0) return hipSuccess;
+  const int nb = blocks_for(n, SCAN_BLK);
+  k_scan_reduce<<<nb, SCAN_BLK, 0, L.stream>>>(in, n, scratch);
+  k_scan_top<<<1, SCAN_BLK, 0, L.stream>>>(scratch, nb);
+  k_scan_down<<<nb, SCAN_BLK, 0, L.stream>>>(in, n, scratch, out);
+  return hipGetLastError();
+}
+
+int64_t measure_partial_words(int n) { return 4 * (int64_t)std::max(blocks_for(n), 1); }
+
+hipError_t launch_measure(const Launch& L, const DevState& s, double mass, double* partials,
+                          double* out5) {
+  const int nb = std::max(blocks_for(s.n), 1);
+  k_measure_partial<<<nb, BLK, 0, L.stream>>>(s.n, L.phys, mass, s.rx, s.ry, s.vx, s.vy,
+                                              partials);
+  k_measure_final<<<1, BLK, 0, L.stream>>>(nb, s.n, mass, partials, out5);
+  return hipGetLastError();
+}
+
+hipError_t launch_ell_overflow(const Launch& L, const DevState& s, int64_t* out_count) {
+  if (s.n <= 0) return hipSuccess;
+  k_ell_overflow<<<blocks_for(s.n), BLK, 0, L.stream>>>(
+      s.n, s.ell_width, s.ell_count, reinterpret_cast<unsigned long long*>(out_count));
+  return hipGetLastError();
+}
+
+}  // namespace sphamd

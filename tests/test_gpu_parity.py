@@ -1,0 +1,312 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle, through the C ABI.
+
+Integer outputs (keys, permutation, cell ranges, neighbor lists) must be
+bit-exact. Float outputs are compared with the FP32 oracle (same operation
+order; stated tolerance 4 ulp-scale: rtol 1e-6) and with the FP64 oracle
+(stated tolerances per quantity below).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# FP64-oracle tolerances (FP32 on the GPU)
+RHO_RTOL = 2e-6          # density: relative, per particle
+ACC_RTOL = 2e-4          # acceleration: |a - a64| <= ACC_RTOL * max|a64|
+POS_ATOL_CELLS = 1e-5    # positions after a few steps, in units of h
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def scenes(sph):
+    rng = np.random.default_rng(42)
+    out = {}
+    out["reference_example"] = sph.scenes.reference_example()
+    out["reference_rowmajor"] = sph.scenes.reference_example(order=sph.ORDER_ROWMAJOR)
+    out["dam_break_16k"] = sph.scenes.dam_break()
+    s = sph.scenes.periodic_shear(n_side=96, seed=9)
+    out["periodic_9k"] = s
+    # random gas with clusters / duplicates
+    d = sph.scenes.dam_break(80, 80, seed=5, name="random")
+    d.rx = rng.random(d.n).astype(np.float32) * 0.3
+    d.ry = rng.random(d.n).astype(np.float32) * 0.3
+    d.rx[:100] = d.rx[100:200]
+    d.ry[:100] = d.ry[100:200]
+    d.vx = rng.normal(0, 0.3, d.n).astype(np.float32)
+    d.vy = rng.normal(0, 0.3, d.n).astype(np.float32)
+    out["random_clusters"] = d
+    return out
+
+
+@pytest.fixture(scope="module")
+def all_scenes(sph):
+    return scenes(sph)
+
+
+def oracle_sorted(orc, p, rx, ry, vx, vy):
+    k = orc.keys(p, rx, ry)
+    perm, sk = orc.stable_sort(k)
+    g = orc.grid(p)
+    st, en = orc.cell_ranges(sk, g.n_keys)
+    return dict(keys=k, perm=perm, sk=sk, start=st, end=en, rx=rx[perm], ry=ry[perm],
+                vx=vx[perm], vy=vy[perm])
+
+
+NAMES = ["reference_example", "reference_rowmajor", "dam_break_16k", "periodic_9k",
+         "random_clusters"]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_find_neighbors_bit_exact(sph, orc, all_scenes, name):
+    s = all_scenes[name]
+    note(f"[gpu] find_neighbors {name} n={s.n}")
+    o = oracle_sorted(orc, s.params, s.rx, s.ry, s.vx, s.vy)
+    with sph.Context(s.params, s.n) as ctx:
+        g = ctx.grid()
+        og = orc.grid(s.params)
+        assert (g.nx, g.ny, g.nxp, g.nyp, g.n_keys, g.key_bits) == (
+            og.nx, og.ny, og.nxp, og.nyp, og.n_keys, og.key_bits)
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.find_neighbors()
+        assert np.array_equal(ctx.sorted_keys(), o["sk"])
+        assert np.array_equal(ctx.permutation(), o["perm"])
+        st, en = ctx.cell_ranges()
+        assert np.array_equal(st, o["start"]) and np.array_equal(en, o["end"])
+        d = ctx.download()
+        assert np.array_equal(d["rx"], o["rx"]) and np.array_equal(d["vy"], o["vy"])
+        assert np.array_equal(d["pid"], o["perm"])
+        # a second sort of the already sorted state is the identity and leaves
+        # no stale cell ranges behind
+        ctx.find_neighbors()
+        assert np.array_equal(ctx.permutation(), np.arange(s.n, dtype=np.uint32))
+        st2, en2 = ctx.cell_ranges()
+        assert np.array_equal(st2, o["start"]) and np.array_equal(en2, o["end"])
+
+
+@pytest.mark.parametrize("name", NAMES)
+@pytest.mark.parametrize("include_self", [False, True])
+def test_csr_lists_bit_exact(sph, orc, all_scenes, name, include_self):
+    s = all_scenes[name]
+    note(f"[gpu] csr {name} self={include_self}")
+    o = oracle_sorted(orc, s.params, s.rx, s.ry, s.vx, s.vy)
+    cnt, off, idx = orc.neighbor_csr(s.params, o["rx"], o["ry"], o["start"], o["end"],
+                                     include_self=include_self)
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.find_neighbors()
+        tot = ctx.build_neighbor_lists(include_self)
+        assert tot == idx.size
+        goff, gidx = ctx.neighbor_lists(tot)
+        assert np.array_equal(goff, off)
+        assert np.array_equal(gidx, idx)   # same stencil order, element by element
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_density_forces_stages(sph, orc, all_scenes, name):
+    s = all_scenes[name]
+    note(f"[gpu] density/forces {name}")
+    o = oracle_sorted(orc, s.params, s.rx, s.ry, s.vx, s.vy)
+    cnt, off, idx = orc.neighbor_csr(s.params, o["rx"], o["ry"], o["start"], o["end"])
+    rho, pres, pr = orc.density(s.params, o["rx"], o["ry"], off, idx)
+    ax, ay = orc.forces(s.params, o["rx"], o["ry"], o["vx"], o["vy"], rho, pr, off, idx)
+    X = [o[k].astype(np.float64) for k in ("rx", "ry", "vx", "vy")]
+    rho64, _, pr64 = orc.density(s.params, X[0], X[1], off, idx, dtype=np.float64)
+    ax64, ay64 = orc.forces(s.params, *X, rho64, pr64, off, idx, dtype=np.float64)
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.find_neighbors()
+        results = {}
+        for mode in ("sorted", "lists", "fused"):
+            if mode == "sorted":
+                ctx.density_pressure(sph.COMPUTE_SORTED)
+                ctx.forces(sph.COMPUTE_SORTED)
+            elif mode == "lists":
+                ctx.build_neighbor_lists(False)
+                ctx.density_pressure(sph.COMPUTE_LISTS)
+                ctx.forces(sph.COMPUTE_LISTS)
+            else:
+                ctx.prepare()   # re-sorts an already sorted state: identity
+                ctx.forces(sph.COMPUTE_LISTS)
+            results[mode] = ctx.fields()
+        for mode, f in results.items():
+            # FP32 oracle: same operation order
+            assert np.allclose(f["rho"], rho, rtol=1e-6, atol=0), mode
+            assert np.allclose(f["p"], pres, rtol=1e-5, atol=1e-6 * np.abs(pres).max()), mode
+            amax = max(np.abs(ax).max(), np.abs(ay).max())
+            assert np.abs(f["ax"] - ax).max() <= 1e-5 * amax, mode
+            assert np.abs(f["ay"] - ay).max() <= 1e-5 * amax, mode
+            # FP64 oracle
+            assert np.max(np.abs(f["rho"] - rho64) / np.abs(rho64)) <= RHO_RTOL, mode
+            a64 = max(np.abs(ax64).max(), np.abs(ay64).max())
+            assert np.abs(f["ax"] - ax64).max() <= ACC_RTOL * a64, mode
+            assert np.abs(f["ay"] - ay64).max() <= ACC_RTOL * a64, mode
+        exact = {m: (np.array_equal(f["rho"], rho), np.array_equal(f["ax"], ax)) for m, f in
+                 results.items()}
+        note(f"[gpu] bit-identical to FP32 oracle (rho, ax): {exact}")
+
+
+@pytest.mark.parametrize("name", ["dam_break_16k", "periodic_9k", "reference_example"])
+def test_ell_lists_bit_exact(sph, orc, all_scenes, name):
+    s = all_scenes[name]
+    o = oracle_sorted(orc, s.params, s.rx, s.ry, s.vx, s.vy)
+    cnt, off, idx = orc.neighbor_csr(s.params, o["rx"], o["ry"], o["start"], o["end"])
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.prepare()
+        gcnt, slots = ctx.ell()
+        assert np.array_equal(gcnt, cnt)
+        w = slots.shape[1]
+        for i in range(s.n):
+            k = min(cnt[i], w)
+            assert np.array_equal(slots[i, :k], idx[off[i]:off[i] + k])
+        assert ctx.ell_overflow() == int(np.sum(cnt > w))
+
+
+def test_ell_overflow_fallback(sph, orc):
+    # a narrow list forces the stencil fallback; forces must not change
+    s = sph.scenes.reference_example()
+    s.params.ell_width = 4
+    o = oracle_sorted(orc, s.params, s.rx, s.ry, s.vx, s.vy)
+    cnt, off, idx = orc.neighbor_csr(s.params, o["rx"], o["ry"], o["start"], o["end"])
+    rho, pres, pr = orc.density(s.params, o["rx"], o["ry"], off, idx)
+    ax, ay = orc.forces(s.params, o["rx"], o["ry"], o["vx"], o["vy"], rho, pr, off, idx)
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.prepare()
+        assert ctx.ell_overflow() == int(np.sum(cnt > 4)) > 0
+        ctx.forces(sph.COMPUTE_LISTS)
+        f = ctx.fields()
+        amax = np.abs(ay).max()
+        assert np.abs(f["ax"] - ax).max() <= 1e-5 * amax
+        assert np.abs(f["ay"] - ay).max() <= 1e-5 * amax
+
+
+@pytest.mark.parametrize("name,steps", [("reference_example", 50), ("dam_break_16k", 40),
+                                        ("periodic_9k", 40), ("random_clusters", 20)])
+def test_multistep_run_matches_oracle(sph, orc, all_scenes, name, steps):
+    s = all_scenes[name]
+    note(f"[gpu] run {name} steps={steps}")
+    o32 = orc.run(s.params, s.rx, s.ry, s.vx, s.vy, steps)
+    o64 = orc.run(s.params, s.rx, s.ry, s.vx, s.vy, steps, dtype=np.float64)
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.run(steps)
+        d = ctx.download()
+        f = ctx.fields()
+    # identical particle order and ids: the sort of every step was bit-exact
+    assert np.array_equal(d["pid"], o32["pid"])
+    same = np.array_equal(d["rx"], o32["rx"]) and np.array_equal(d["ry"], o32["ry"])
+    note(f"[gpu] {name}: positions bit-identical to FP32 oracle after {steps} steps: {same}")
+    h = s.params.h
+    assert np.abs(d["rx"] - o32["rx"]).max() <= 1e-6 * h * steps
+    # FP64 oracle, matched by particle id
+    i32 = np.argsort(d["pid"])
+    i64 = np.argsort(o64["pid"])
+    err = np.hypot(d["rx"][i32] - o64["rx"][i64], d["ry"][i32] - o64["ry"][i64])
+    note(f"[gpu] {name}: max position error vs FP64 = {err.max() / h:.3e} h")
+    assert err.max() <= POS_ATOL_CELLS * h * steps
+    rerr = np.abs(f["rho"][i32] - o64["rho"][i64]) / o64["rho"][i64]
+    note(f"[gpu] {name}: max density rel error vs FP64 = {rerr.max():.3e}")
+    assert rerr.max() <= 1e-4
+
+
+def test_fused_step_equals_stage_path(sph, all_scenes):
+    s = all_scenes["dam_break_16k"]
+    with sph.Context(s.params, s.n) as a, sph.Context(s.params, s.n) as b:
+        a.upload(s.rx, s.ry, s.vx, s.vy)
+        b.upload(s.rx, s.ry, s.vx, s.vy)
+        a.run(6)
+        # stage by stage, the reference's loop order
+        b.find_neighbors(); b.build_neighbor_lists(False)
+        b.density_pressure(sph.COMPUTE_LISTS); b.forces(sph.COMPUTE_LISTS)
+        for _ in range(5):
+            b.integrate(); b.find_neighbors(); b.build_neighbor_lists(False)
+            b.density_pressure(sph.COMPUTE_LISTS); b.forces(sph.COMPUTE_LISTS)
+        da, db = a.download(), b.download()
+        fa, fb = a.fields(), b.fields()
+        for k in ("rx", "ry", "vx", "vy", "pid"):
+            assert np.array_equal(da[k], db[k]), k
+        for k in ("ax", "ay", "rho"):
+            assert np.array_equal(fa[k], fb[k]), k
+
+
+def test_graph_replay_equals_direct(sph, all_scenes):
+    s = all_scenes["periodic_9k"]
+    with sph.Context(s.params, s.n) as a, sph.Context(s.params, s.n) as b:
+        for c in (a, b):
+            c.upload(s.rx, s.ry, s.vx, s.vy)
+            c.prepare()
+        b.set_graph(True)
+        for _ in range(3):   # several replays with eager work in between
+            a.step(4)
+            b.step(4)
+            a.measure(); b.measure()
+            da, db = a.download(), b.download()
+            for k in ("rx", "ry", "vx", "vy", "pid"):
+                assert np.array_equal(da[k], db[k]), k
+        b.step(3)   # odd count: one direct iteration after the graph
+        a.step(3)
+        assert np.array_equal(a.download()["rx"], b.download()["rx"])
+
+
+@pytest.mark.parametrize("name", ["dam_break_16k", "random_clusters"])
+def test_measure_fp64(sph, orc, all_scenes, name):
+    s = all_scenes[name]
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        m = ctx.measure()
+    o = orc.measure(s.params, s.rx, s.ry, s.vx, s.vy)
+    assert m[0] == pytest.approx(o[0], rel=1e-12, abs=1e-300)
+    assert m[1] == o[1]
+    assert m[2] == o[2]
+    assert m[3] == pytest.approx(o[3], rel=1e-12)
+    assert m[4] == pytest.approx(o[4], rel=1e-15)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 2047, 2048, 2049, 4097])
+def test_sort_edge_sizes(sph, orc, n):
+    # ragged tails of the 2048-key tile and the 64-lane wave; one particle
+    rng = np.random.default_rng(n)
+    p = sph.scenes.wcsph_params(0.02, (0, 0), (1, 1))
+    rx = rng.random(n).astype(np.float32)
+    ry = rng.random(n).astype(np.float32)
+    rx[: n // 3] = rx[0]
+    ry[: n // 3] = ry[0]
+    o = oracle_sorted(orc, p, rx, ry, rx, ry)
+    with sph.Context(p, 5000) as ctx:
+        ctx.upload(rx, ry, rx, ry)
+        ctx.find_neighbors()
+        assert np.array_equal(ctx.permutation(), o["perm"])
+        st, en = ctx.cell_ranges()
+        assert np.array_equal(st, o["start"]) and np.array_equal(en, o["end"])
+
+
+def test_reupload_smaller_clears_old_ranges(sph, orc):
+    p = sph.scenes.wcsph_params(0.02, (0, 0), (1, 1))
+    rng = np.random.default_rng(1)
+    with sph.Context(p, 4000) as ctx:
+        for n in (4000, 37, 1500):
+            rx = rng.random(n).astype(np.float32)
+            ry = rng.random(n).astype(np.float32)
+            o = oracle_sorted(orc, p, rx, ry, rx, ry)
+            ctx.upload(rx, ry, rx, ry)
+            ctx.find_neighbors()
+            st, en = ctx.cell_ranges()
+            assert np.array_equal(st, o["start"]) and np.array_equal(en, o["end"])
+
+
+def test_capacity_and_state_errors(sph):
+    p = sph.scenes.wcsph_params(0.02, (0, 0), (1, 1))
+    with sph.Context(p, 10) as ctx:
+        with pytest.raises(sph.SphError, match="capacity"):
+            ctx.upload(*(np.zeros(11, np.float32),) * 4)
+        ctx.upload(*(np.full(10, 0.5, np.float32),) * 4)
+        with pytest.raises(sph.SphError, match="find_neighbors"):
+            ctx.density_pressure()
+        with pytest.raises(sph.SphError, match="prepare"):
+            ctx.step(1)
