@@ -1,0 +1,186 @@
+"""Benchmark: particle-steps/s of the plain 2D SPH step on one MI355X.
+
+Workload: the 1,048,576-particle dam break (1024 x 1024 block in a 4 x 2 box,
+dx = 0.25/128, h = 1.3 dx, Tait EOS, Monaghan viscosity, reflective walls),
+FP32, synthetic seeded initial state. One step = forces + integration + cell
+keys -> stable radix sort -> permutation + cell ranges -> density + EOS +
+neighbor list (the fused pipeline of sph_step).
+
+Prints one JSON line. `roofline` is for the kernel with the largest share of
+the timed region, from hipEvents recorded on the context stream around every
+launch; `stages` gives every kernel's share. `cpu_baseline` times the
+reference-style OpenMP CPU step (the oracle's baseline build) on a bounded
+sample of the same workload.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def alg_bytes_per_particle(stage, k, occ, passes):
+    """Algorithmic HBM bytes per particle of one launch of each kernel (FP32
+    state, u64 keys, u32 indices; neighbor re-reads counted as cache hits)."""
+    return {
+        # R r,v (16) rho,pr (8) count (4) list (4k); W a (8) T r,v (16) key (8)
+        "force_computation": 60.0 + 4.0 * k,
+        # R new keys (8) + previous sorted keys (8); clear previous ranges (8/occupied cell)
+        "sort_histogram": 16.0 + 8.0 * occ,
+        # first pass R key 8, W key+idx 12; later passes R 12, W 12 (mean per launch)
+        "sort_pass": (20.0 + 24.0 * (passes - 1)) / passes,
+        # R perm 4, sorted key 8, T r,v 16, pid 4; W r,v 16, pid 4; ranges 8/occupied cell
+        "neighbor_search": 52.0 + 8.0 * occ,
+        # R r 8; W rho,pr 8, count 4, list 4k
+        "approximate_density": 20.0 + 4.0 * k,
+    }[stage]
+
+
+def cpu_baseline(sph, scene, budget_s):
+    from oracle import oracle as orc
+    threads, t, _ = orc.baseline_run(scene.params, scene.rx, scene.ry, scene.vx, scene.vy, 2)
+    per_step = t["main_loop"] / 2.0
+    steps = int(max(2, min(60, budget_s / max(per_step, 1e-6))))
+    log(f"cpu baseline: {per_step:.3f} s/step on {threads} threads -> sampling {steps} steps")
+    threads, t, _ = orc.baseline_run(scene.params, scene.rx, scene.ry, scene.vx, scene.vy, steps)
+    value = scene.n * steps / t["main_loop"]
+    return {
+        "value": value,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{scene.name}: {scene.n} particles x {steps} steps of the reference-style "
+                  f"OpenMP mp-lists/Morton CPU step (serial sort/gather/ranges/scan/EOS/"
+                  f"integrator), -O3 -ffast-math",
+        "stage_seconds": {k: round(v, 4) for k, v in t.items()},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="dam_break_1m")
+    ap.add_argument("--cpu-budget", type=float, default=15.0,
+                    help="seconds of CPU work for the cpu_baseline sample (0 disables)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE "
+              f"{world}); multi-GPU sharding of this path is described in DESIGN.md, not run.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    sph = importlib.import_module("sph-simulations_amd")
+    scene = sph.scenes.by_name(args.workload)
+    n = scene.n
+    log(f"workload {scene.name}: n={n}")
+    ctx = sph.Context(scene.params, n)
+    ctx.upload(scene.rx, scene.ry, scene.vx, scene.vy)
+    ctx.prepare()
+    ctx.set_graph(True)
+    ctx.step(args.warmup)
+    ctx.synchronize()
+    log(f"warmup {args.warmup} steps done")
+
+    # timed region: every launch bracketed by hipEvents on the context stream
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    stage = ctx.stage_times()
+    ctx.set_timing(False)
+    wall = t1 - t0
+    value = n * args.steps / wall
+    log(f"timed {args.steps} steps: {wall * 1e3 / args.steps:.4f} ms/step")
+
+    # same steps replayed from the captured hipGraph (no per-launch events)
+    ctx.synchronize()
+    g0 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.synchronize()
+    g1 = time.perf_counter()
+    graph_ms = (g1 - g0) * 1e3 / args.steps
+    log(f"graph replay: {graph_ms:.4f} ms/step")
+
+    cnt, _ = ctx.ell()
+    k = float(cnt.mean())
+    st, _ = ctx.cell_ranges()
+    occ = float((st >= 0).sum()) / n
+    passes = (int(ctx.grid().key_bits) + 7) // 8
+    m = ctx.measure()
+
+    stages = {}
+    kern_total = 0.0
+    for name, (ms, launches) in stage.items():
+        if launches == 0:
+            continue
+        per_launch_ms = ms / launches
+        entry = {"launches": launches, "ms_total": round(ms, 4),
+                 "us_per_launch": round(per_launch_ms * 1e3, 3)}
+        try:
+            b = alg_bytes_per_particle(name, k, occ, passes) * n
+            entry["alg_bytes_per_launch"] = int(b)
+            entry["achieved_GBs"] = round(b / (per_launch_ms * 1e-3) / 1e9, 1)
+        except KeyError:
+            pass
+        stages[name] = entry
+        kern_total += ms
+    dom_name = max((s for s in stages if "alg_bytes_per_launch" in stages[s]),
+                   key=lambda s: stages[s]["ms_total"])
+    dom = stages[dom_name]
+    achieved = dom["achieved_GBs"]
+    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes) * (passes if s == "sort_pass" else 1)
+                     for s in ("force_computation", "sort_histogram", "sort_pass",
+                               "neighbor_search", "approximate_density"))
+    out = {
+        "metric": "particle-steps/s at 1M particles: plain GPU SPH step (surrogate path not built)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded dam-break lattice with jitter)",
+        "config": {"workload": scene.name, "particles": n, "box": list(scene.params.hi),
+                   "h": scene.params.h, "dt": scene.params.dt, "physics": "wcsph",
+                   "mean_neighbors": round(k, 3), "occupied_cells_per_particle": round(occ, 4),
+                   "radix_passes": passes},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None},
+        "stages": stages,
+        "step": {"alg_bytes_per_particle": round(step_bytes, 1),
+                 "step_frac_of_hbm": step_bytes * value / (HBM_PEAK_GBS * 1e9),
+                 "kernel_ms_per_step": kern_total / args.steps,
+                 "graph_ms_per_step": graph_ms,
+                 "graph_particle_steps_per_s": n / (graph_ms * 1e-3)},
+        "diagnostics": {"kinetic_energy": m[0], "max_v2": m[1], "inside": int(m[2])},
+    }
+    ctx.close()
+    if args.cpu_budget > 0:
+        out["cpu_baseline"] = cpu_baseline(sph, scene, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

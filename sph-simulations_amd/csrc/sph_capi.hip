@@ -54,7 +54,26 @@ struct Ctx {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int graph_n = -1;
+  const void* graph_sig[9] = {};  // buffer assignment the graph was captured with
 };
+
+// Buffer assignment the fused iteration reads and writes; the stages swap
+// these pointers, so a captured graph is only valid for the assignment it saw.
+void graph_signature(const Ctx* c, const void** sig) {
+  const DevState& s = c->s;
+  const void* v[9] = {s.rx, s.tx, s.vx, s.tvx, s.ax, s.tax, s.pid, s.pid_alt,
+                      reinterpret_cast<const void*>((intptr_t)s.n)};
+  for (int i = 0; i < 9; ++i) sig[i] = v[i];
+}
+
+bool graph_matches(const Ctx* c) {
+  if (!c->gexec) return false;
+  const void* sig[9];
+  graph_signature(c, sig);
+  for (int i = 0; i < 9; ++i)
+    if (sig[i] != c->graph_sig[i]) return false;
+  return true;
+}
 
 int fail(Ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -360,6 +379,16 @@ int capture_graph(Ctx* c) {
     return hip_fail(c, e, "hipGraphInstantiate");
   }
   c->graph_n = c->s.n;
+  graph_signature(c, c->graph_sig);
+  return SPH_OK;
+}
+
+// Synchronise and report a radix-sort look-back that gave up (see sph_sort.hip).
+int sync_and_check(Ctx* c) {
+  uint32_t st = 0;
+  HIPCHK(c, hipMemcpyAsync(&st, c->sb.status, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (st != 0) return fail(c, SPH_ERR_HIP, "radix sort look-back gave up waiting (device status)");
   return SPH_OK;
 }
 
@@ -443,7 +472,7 @@ int sph_create(const sph_params* params, int64_t n_max, int device, void** out_c
       (rc = dev_alloc(c, &b.keys_out, N)) || (rc = dev_alloc(c, &b.vals_a, N)) ||
       (rc = dev_alloc(c, &b.vals_b, N)) || (rc = dev_alloc(c, &b.vals_out, N)) ||
       (rc = dev_alloc(c, &b.hist, kMaxPasses * 256)) ||
-      (rc = dev_alloc(c, &b.tickets, kMaxPasses)) ||
+      (rc = dev_alloc(c, &b.tickets, kMaxPasses)) || (rc = dev_alloc(c, &b.status, 1)) ||
       (rc = dev_alloc(c, &b.lookback, (int64_t)sort_passes(c->grid.key_bits) * tiles * 256)))
     return bail(rc);
   b.max_tiles = tiles;
@@ -451,6 +480,7 @@ int sph_create(const sph_params* params, int64_t n_max, int device, void** out_c
   (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device);
   b.hist_blocks = std::max(cu, 1);
   b.cells = s.cells;
+  b.n_keys = (uint64_t)c->grid.n_keys;
   s.keys_sorted = b.keys_out;
   s.perm = b.vals_out;
   s.hist = b.hist;
@@ -459,6 +489,7 @@ int sph_create(const sph_params* params, int64_t n_max, int device, void** out_c
       (rc = dev_alloc(c, &c->meas_out, 8)) || (rc = dev_alloc(c, &c->ovf, 1)))
     return bail(rc);
   e = hipMemsetAsync(b.hist, 0, kMaxPasses * 256 * sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(b.status, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, c->grid.n_keys);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
@@ -555,8 +586,7 @@ int sph_download(void* ctx, float* rx, float* ry, float* vx, float* vy, uint32_t
   if (vx) HIPCHK(c, hipMemcpyAsync(vx, s.vx, fb, hipMemcpyDeviceToHost, c->stream));
   if (vy) HIPCHK(c, hipMemcpyAsync(vy, s.vy, fb, hipMemcpyDeviceToHost, c->stream));
   if (pid) HIPCHK(c, hipMemcpyAsync(pid, s.pid, (size_t)s.n * 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return SPH_OK;
+  return sync_and_check(c);
 }
 
 int sph_download_fields(void* ctx, float* ax, float* ay, float* rho, float* pres) {
@@ -707,7 +737,7 @@ int sph_step(void* ctx, int32_t n_steps) {
   if (!c->density_ell) return fail(c, SPH_ERR_STATE, "call sph_prepare first");
   int done = 0;
   if (c->use_graph && !c->timing) {
-    if (c->graph_n != c->s.n && (rc = capture_graph(c))) return rc;
+    if (n_steps >= 2 && !graph_matches(c) && (rc = capture_graph(c))) return rc;
     for (; done + 2 <= n_steps; done += 2) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
   }
   for (; done < n_steps; ++done)
@@ -845,8 +875,7 @@ int sph_stage_times(void* ctx, double* ms, int64_t* launches, int32_t count) {
 int sph_synchronize(void* ctx) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return SPH_OK;
+  return sync_and_check(c);
 }
 
 }  // extern "C"

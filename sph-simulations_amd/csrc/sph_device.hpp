@@ -172,7 +172,7 @@ template <int BC, bool SELF = false, class F>
 __device__ __forceinline__ void for_each_neighbor(const DevGrid& g, const DevPhys& c,
                                                   const float* __restrict__ rx,
                                                   const float* __restrict__ ry,
-                                                  const int2* __restrict__ cells, int i,
+                                                  const int2* __restrict__ cells, int n, int i,
                                                   float xi, float yi, F&& fn) {
   int cx = cell_coord(xi, g.lo_x, g.cs_x, g.nx);
   int cy = cell_coord(yi, g.lo_y, g.cs_y, g.ny);
@@ -194,7 +194,8 @@ __device__ __forceinline__ void for_each_neighbor(const DevGrid& g, const DevPhy
       }
       int2 se = cells[cell_key(g, cxj, cyj)];
       if (se.x < 0 || se.y < 0) continue;
-      for (int j = se.x; j < se.y; ++j) {
+      const int je = min(se.y, n);
+      for (int j = se.x; j < je; ++j) {
         if (j == i) {
           if (SELF) fn(j, 0.0f, 0.0f, 0.0f);
           continue;

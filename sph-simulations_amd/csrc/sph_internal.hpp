@@ -27,6 +27,8 @@ struct SortBuffers {
   // (cells of keys_out[0..clear_n) boundaries) before keys_out is overwritten
   int2* cells;
   int clear_n;
+  uint64_t n_keys;
+  uint32_t* status;  // nonzero after a look-back gave up (never expected)
 };
 
 int sort_tile_size();
@@ -52,7 +54,7 @@ struct DevState {
   uint32_t* perm;
   int2* cells;
   int64_t n_keys;
-  int32_t *ell_count, *ell_idx;
+  int32_t *ell_count, *ell_idx;  // ELL: slot k of particle i at ell_idx[k * n + i]
   int ell_width;
   int32_t* csr_count;
   int64_t* csr_off;

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(BLK) void k_gather(
     const float* __restrict__ say, float* __restrict__ drx, float* __restrict__ dry,
     float* __restrict__ dvx, float* __restrict__ dvy, uint32_t* __restrict__ dpid,
     float* __restrict__ dax, float* __restrict__ day, int2* __restrict__ cells,
-    uint32_t* __restrict__ hist, int hist_words) {
+    int64_t n_keys, uint32_t* __restrict__ hist, int hist_words) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
@@ -62,9 +62,11 @@ __global__ __launch_bounds__(BLK) void k_gather(
     is_end = live && (i == n - 1 || k != keys[i + 1]);
   }
   if (!live) return;
-  if (is_start) cells[k].x = i;
-  if (is_end) cells[k].y = i + 1;
-  const uint32_t j = perm[i];
+  if (k < (uint64_t)n_keys) {
+    if (is_start) cells[k].x = i;
+    if (is_end) cells[k].y = i + 1;
+  }
+  const uint32_t j = min(perm[i], (uint32_t)(n - 1));  // in bounds even if perm were bad
   drx[i] = srx[j];
   dry[i] = sry[j];
   dvx[i] = svx[j];
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(BLK) void k_density(
     }
   } else {
     int cnt = 0;
-    for_each_neighbor<BC>(g, c, rx, ry, cells, i, xi, yi, [&](int j, float, float, float r2) {
+    for_each_neighbor<BC>(g, c, rx, ry, cells, n, i, xi, yi, [&](int j, float, float, float r2) {
       acc += density_term(c.h2, r2);
       if (NB == NB_ELL) {
         if (cnt < ell_w) ell_idx[(int64_t)cnt * n + i] = j;
@@ -145,10 +147,10 @@ __global__ __launch_bounds__(BLK) void k_forces(
     if (cnt <= ell_w) {
       for (int k = 0; k < cnt; ++k) from_list(ell_idx[(int64_t)k * n + i]);
     } else {
-      for_each_neighbor<BC>(g, c, rx, ry, cells, i, xi, yi, pair);
+      for_each_neighbor<BC>(g, c, rx, ry, cells, n, i, xi, yi, pair);
     }
   } else {
-    for_each_neighbor<BC>(g, c, rx, ry, cells, i, xi, yi, pair);
+    for_each_neighbor<BC>(g, c, rx, ry, cells, n, i, xi, yi, pair);
   }
   float axi, ayi;
   force_finish<PHYS>(c, a, axi, ayi);
@@ -194,7 +196,7 @@ __global__ __launch_bounds__(BLK) void k_csr_count(int n, DevGrid g, DevPhys c,
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   int k = 0;
-  for_each_neighbor<BC, SELF>(g, c, rx, ry, cells, i, rx[i], ry[i],
+  for_each_neighbor<BC, SELF>(g, c, rx, ry, cells, n, i, rx[i], ry[i],
                               [&](int, float, float, float) { ++k; });
   count[i] = k;
 }
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(BLK) void k_csr_fill(int n, DevGrid g, DevPhys c,
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   int64_t o = off[i];
-  for_each_neighbor<BC, SELF>(g, c, rx, ry, cells, i, rx[i], ry[i],
+  for_each_neighbor<BC, SELF>(g, c, rx, ry, cells, n, i, rx[i], ry[i],
                               [&](int j, float, float, float) { idx[o++] = j; });
 }
 
@@ -381,7 +383,8 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
   if (s.n > 0)
     k_gather<<<blocks_for(s.n), BLK, 0, L.stream>>>(
         s.n, s.perm, s.keys_sorted, src_rx, src_ry, src_vx, src_vy, src_pid, src_ax, src_ay,
-        dst_rx, dst_ry, dst_vx, dst_vy, dst_pid, dst_ax, dst_ay, s.cells, s.hist, hist_words);
+        dst_rx, dst_ry, dst_vx, dst_vy, dst_pid, dst_ax, dst_ay, s.cells, s.n_keys, s.hist,
+        hist_words);
   return hipGetLastError();
 }
 

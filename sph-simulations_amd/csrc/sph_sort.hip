@@ -27,6 +27,10 @@ constexpr int HIST_IPT = 16;
 constexpr uint32_t FLAG_AGG = 1u << 30;
 constexpr uint32_t FLAG_PREFIX = 2u << 30;
 constexpr uint32_t COUNT_MASK = (1u << 30) - 1;
+// Bound on look-back polls of one predecessor word (each poll sleeps ~64 clk).
+// Reaching it means a predecessor never published: the pass records an error in
+// the status word instead of spinning forever.
+constexpr uint32_t MAX_POLLS = 1u << 18;
 
 __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -69,7 +73,8 @@ __global__ __launch_bounds__(WG) void k_hist(const uint64_t* __restrict__ keys, 
                                              uint32_t lookback_words,
                                              uint32_t* __restrict__ tickets,
                                              const uint64_t* __restrict__ old_sorted,
-                                             int old_n, int2* __restrict__ cells) {
+                                             int old_n, int2* __restrict__ cells,
+                                             uint64_t n_keys) {
   __shared__ uint32_t s_h[kMaxPasses * RADIX];
   for (int i = threadIdx.x; i < passes * RADIX; i += WG) s_h[i] = 0;
   __syncthreads();
@@ -100,7 +105,7 @@ __global__ __launch_bounds__(WG) void k_hist(const uint64_t* __restrict__ keys, 
   }
   for (int i = gtid; i < old_n; i += gsize) {
     uint64_t k = old_sorted[i];
-    if (i == 0 || k != old_sorted[i - 1]) cells[k] = make_int2(-1, -1);
+    if ((i == 0 || k != old_sorted[i - 1]) && k < n_keys) cells[k] = make_int2(-1, -1);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < passes * RADIX; i += WG)
@@ -117,7 +122,8 @@ __global__ __launch_bounds__(WG) void k_onesweep(int n, int shift, int pass,
                                                  uint32_t* __restrict__ vout,
                                                  const uint32_t* __restrict__ hist,
                                                  uint32_t* __restrict__ lookback,
-                                                 uint32_t* __restrict__ tickets) {
+                                                 uint32_t* __restrict__ tickets,
+                                                 uint32_t* __restrict__ status) {
   __shared__ uint64_t s_keys[TILE];
   __shared__ uint32_t s_vals[TILE];
   __shared__ uint32_t s_whist[WAVES * RADIX];
@@ -189,13 +195,21 @@ __global__ __launch_bounds__(WG) void k_onesweep(int n, int shift, int pass,
     st_agent(&lb[d], FLAG_AGG | cnt);
     uint32_t excl = 0;
     int t = (int)tile - 1;
-    while (true) {
+    uint32_t polls = 0;
+    while (t >= 0) {
       uint32_t w = ld_agent(&lookback[(size_t)t * RADIX + d]);
       uint32_t flag = w & ~COUNT_MASK;
       if (flag == 0) {
-        __builtin_amdgcn_s_sleep(1);
+        if (++polls < MAX_POLLS) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        atomicOr(status, 1u);  // give up on this predecessor: result is wrong, flagged
+        polls = 0;
+        --t;
         continue;
       }
+      polls = 0;
       excl += w & COUNT_MASK;
       if (flag == FLAG_PREFIX) break;
       --t;
@@ -222,8 +236,10 @@ __global__ __launch_bounds__(WG) void k_onesweep(int n, int shift, int pass,
     uint64_t kk = s_keys[k];
     uint32_t dd = (uint32_t)(kk >> shift) & (RADIX - 1);
     uint32_t g = s_gbase[dd] + ((uint32_t)k - s_dstart[dd]);
-    kout[g] = kk;
-    vout[g] = s_vals[k];
+    if (g < (uint32_t)n) {
+      kout[g] = kk;
+      vout[g] = s_vals[k];
+    }
   }
 }
 
@@ -247,7 +263,7 @@ hipError_t radix_sort_hist(const SortBuffers& b, int n, int key_bits, hipStream_
   // clears `passes` look-back arrays laid out back to back
   k_hist<<<hist_blocks, WG, 0, stream>>>(b.keys_in, n, passes, b.hist, b.lookback,
                                          lookback_words_per_pass(n) * passes, b.tickets,
-                                         b.keys_out, b.clear_n, b.cells);
+                                         b.keys_out, b.clear_n, b.cells, b.n_keys);
   return hipGetLastError();
 }
 
@@ -262,7 +278,8 @@ hipError_t radix_sort_pass(const SortBuffers& b, int n, int key_bits, int p,
   const uint32_t* vin = p == 0 ? nullptr : ((p & 1) ? b.vals_a : b.vals_b);
   uint32_t* vout = p == passes - 1 ? b.vals_out : ((p & 1) ? b.vals_b : b.vals_a);
   k_onesweep<<<tiles, WG, 0, stream>>>(n, p * RB, p, kin, vin, kout, vout, b.hist,
-                                       b.lookback + (size_t)p * lb_words, b.tickets);
+                                       b.lookback + (size_t)p * lb_words, b.tickets,
+                                       b.status);
   return hipGetLastError();
 }
 
