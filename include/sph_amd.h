@@ -139,6 +139,12 @@ int sph_density_pressure(void* ctx, int32_t compute);
 int sph_forces(void* ctx, int32_t compute);
 /* cuCallbackIntegrate (core/acceleration/integrator/cuIntegratorLauncher.cuh:23-28). */
 int sph_integrate(void* ctx);
+/* cuKernelUpdateVelocities / cuKernelUpdatePositions (cuIntegratorLauncher.cuh:5-21):
+ * the two halves of sph_integrate, v += dt a, then r += dt v with the boundary. */
+int sph_update_velocities(void* ctx);
+int sph_update_positions(void* ctx);
+/* Time step of the integration stages (the dt argument of cuCallbackIntegrate). */
+int sph_set_dt(void* ctx, double dt);
 /* cuKernelMeasurementsPerStep (core/acceleration/utils/cuUtilsLauncher.cuh:5-9) in
  * FP64: out[0] kinetic energy, out[1] max |v|^2, out[2] particles inside the
  * domain, out[3] potential energy, out[4] total mass. Synchronises. */

@@ -124,6 +124,9 @@ def lib():
         "sph_density_pressure": (i32, [vp, i32]),
         "sph_forces": (i32, [vp, i32]),
         "sph_integrate": (i32, [vp]),
+        "sph_update_velocities": (i32, [vp]),
+        "sph_update_positions": (i32, [vp]),
+        "sph_set_dt": (i32, [vp, d]),
         "sph_measure": (i32, [vp, P(d)]),
         "sph_prepare": (i32, [vp]),
         "sph_step": (i32, [vp, i32]),
@@ -150,7 +153,8 @@ def lib():
 EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_get_constants",
             "sph_upload", "sph_download", "sph_download_fields", "sph_size",
             "sph_find_neighbors", "sph_build_neighbor_lists", "sph_density_pressure",
-            "sph_forces", "sph_integrate", "sph_measure", "sph_prepare", "sph_step",
+            "sph_forces", "sph_integrate", "sph_update_velocities", "sph_update_positions",
+            "sph_set_dt", "sph_measure", "sph_prepare", "sph_step",
             "sph_set_graph", "sph_ell_overflow", "sph_get_sorted_keys", "sph_get_permutation",
             "sph_get_cell_ranges", "sph_get_neighbor_lists", "sph_get_ell",
             "sph_get_device_view", "sph_set_timing", "sph_stage_times", "sph_synchronize")
@@ -260,6 +264,15 @@ class Context:
 
     def integrate(self):
         self._chk(self._L.sph_integrate(self._h), "sph_integrate")
+
+    def set_dt(self, dt: float):
+        self._chk(self._L.sph_set_dt(self._h, float(dt)), "sph_set_dt")
+
+    def update_velocities(self):
+        self._chk(self._L.sph_update_velocities(self._h), "sph_update_velocities")
+
+    def update_positions(self):
+        self._chk(self._L.sph_update_positions(self._h), "sph_update_positions")
 
     def measure(self) -> np.ndarray:
         out = np.zeros(5, np.float64)

@@ -702,6 +702,39 @@ int sph_integrate(void* ctx) {
   return SPH_OK;
 }
 
+int sph_set_dt(void* ctx, double dt) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SPH_ERR_INVALID, "dt must be positive");
+  c->prm.dt = dt;
+  c->L.phys.dt = (float)dt;
+  destroy_graph(c);  // dt is baked into the captured launches
+  return SPH_OK;
+}
+
+int sph_update_velocities(void* ctx) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  int rc = check_n(c);
+  if (rc) return rc;
+  StageTimer t(c, SPH_T_INTEGRATION);
+  HIPCHK(c, launch_update_velocities(c->L, c->s));
+  c->density_ell = false;
+  return SPH_OK;
+}
+
+int sph_update_positions(void* ctx) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  int rc = check_n(c);
+  if (rc) return rc;
+  StageTimer t(c, SPH_T_INTEGRATION);
+  HIPCHK(c, launch_update_positions(c->L, c->s));
+  c->density_ell = false;
+  c->csr_total = -1;
+  return SPH_OK;
+}
+
 int sph_measure(void* ctx, double* out5) {
   Ctx* c = as_ctx(ctx);
   if (!c || !out5) return SPH_ERR_INVALID;

@@ -83,6 +83,8 @@ hipError_t launch_density(const Launch& L, const DevState& s, int nb_mode, bool 
 hipError_t launch_forces(const Launch& L, const DevState& s, int nb_mode, bool fused,
                          uint64_t* keys_out);
 hipError_t launch_integrate(const Launch& L, const DevState& s);
+hipError_t launch_update_velocities(const Launch& L, const DevState& s);
+hipError_t launch_update_positions(const Launch& L, const DevState& s);
 hipError_t launch_csr_count(const Launch& L, const DevState& s, bool self);
 hipError_t launch_csr_fill(const Launch& L, const DevState& s, bool self);
 // exclusive scan of csr_count into csr_off[0..n]; scratch >= scan_scratch_words(n)

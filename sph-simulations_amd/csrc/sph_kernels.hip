@@ -185,6 +185,45 @@ __global__ __launch_bounds__(BLK) void k_integrate(int n, DevPhys c, float* __re
   vy[i] = v;
 }
 
+// The two halves of integrate_one, for callers that drive them separately
+// (cuUpdateVelocities / cuUpdatePositions, cuIntegrator.cu:60-121).
+__global__ __launch_bounds__(BLK) void k_update_velocities(int n, DevPhys c,
+                                                           float* __restrict__ vx,
+                                                           float* __restrict__ vy,
+                                                           const float* __restrict__ ax,
+                                                           const float* __restrict__ ay) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  vx[i] = vx[i] + c.dt * ax[i];
+  vy[i] = vy[i] + c.dt * ay[i];
+}
+
+template <int BC>
+__global__ __launch_bounds__(BLK) void k_update_positions(int n, DevPhys c,
+                                                          float* __restrict__ rx,
+                                                          float* __restrict__ ry,
+                                                          float* __restrict__ vx,
+                                                          float* __restrict__ vy) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  float x = rx[i], y = ry[i], u = vx[i], v = vy[i];
+  x = x + c.dt * u;
+  y = y + c.dt * v;
+  if (BC == 1) {
+    if (x >= c.hi_x) x = x - c.len_x; else if (x < c.lo_x) x = x + c.len_x;
+    if (y >= c.hi_y) y = y - c.len_y; else if (y < c.lo_y) y = y + c.len_y;
+  } else {
+    if (x > c.hi_x) { x = c.hi_x; u = -u; }
+    if (x < c.lo_x) { x = c.lo_x; u = -u; }
+    if (y > c.hi_y) { y = c.hi_y; v = -v; }
+    if (y < c.lo_y) { y = c.lo_y; v = -v; }
+  }
+  rx[i] = x;
+  ry[i] = y;
+  vx[i] = u;
+  vy[i] = v;
+}
+
 // Multipass CSR lists (cuCountNeighborsPerParticle / cuAssignIndices,
 // cuNeighborLists.cuh:6-136).
 template <int BC, bool SELF>
@@ -455,6 +494,24 @@ hipError_t launch_integrate(const Launch& L, const DevState& s) {
   else
     k_integrate<0><<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, L.phys, s.rx, s.ry, s.vx, s.vy,
                                                           s.ax, s.ay);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_velocities(const Launch& L, const DevState& s) {
+  if (s.n <= 0) return hipSuccess;
+  k_update_velocities<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, L.phys, s.vx, s.vy, s.ax,
+                                                              s.ay);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_positions(const Launch& L, const DevState& s) {
+  if (s.n <= 0) return hipSuccess;
+  if (L.phys.boundary == 1)
+    k_update_positions<1><<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, L.phys, s.rx, s.ry, s.vx,
+                                                                 s.vy);
+  else
+    k_update_positions<0><<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, L.phys, s.rx, s.ry, s.vx,
+                                                                 s.vy);
   return hipGetLastError();
 }
 
