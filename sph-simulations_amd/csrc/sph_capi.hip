@@ -740,7 +740,7 @@ int sph_measure(void* ctx, double* out5) {
   if (!c || !out5) return SPH_ERR_INVALID;
   {
     StageTimer t(c, SPH_T_MEASURE);
-    HIPCHK(c, launch_measure(c->L, c->s, c->prm.mass, c->meas_part, c->meas_out));
+    HIPCHK(c, launch_measure(c->L, c->s, c->prm.mass, c->prm.gravity_y, c->meas_part, c->meas_out));
   }
   HIPCHK(c, hipMemcpyAsync(out5, c->meas_out, 5 * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));

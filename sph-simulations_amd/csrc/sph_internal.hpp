@@ -93,8 +93,8 @@ hipError_t launch_scan(const Launch& L, const int32_t* in, int64_t* out, int n,
                        int64_t* scratch);
 // FP64 diagnostics into out5 (device), partials >= measure_partial_words(n)
 int64_t measure_partial_words(int n);
-hipError_t launch_measure(const Launch& L, const DevState& s, double mass, double* partials,
-                          double* out5);
+hipError_t launch_measure(const Launch& L, const DevState& s, double mass, double gravity_y,
+                          double* partials, double* out5);
 hipError_t launch_ell_overflow(const Launch& L, const DevState& s, int64_t* out_count);
 
 }  // namespace sphamd

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(SCAN_BLK) void k_scan_down(const int32_t* __restric
 // Per-block FP64 partials of sum 1/2 m |v|^2, sum m (-g) (y - lo), max |v|^2 and
 // the count inside the domain (MeasuringFunctions.h:5-62), then one block that
 // adds the partials in a fixed order.
-__global__ __launch_bounds__(BLK) void k_measure_partial(int n, DevPhys c, double mass,
+__global__ __launch_bounds__(BLK) void k_measure_partial(int n, DevPhys c, double mass, double g,
                                                          const float* __restrict__ rx,
                                                          const float* __restrict__ ry,
                                                          const float* __restrict__ vx,
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(BLK) void k_measure_partial(int n, DevPhys c, doubl
     double u = vx[i], v = vy[i];
     double v2 = u * u + v * v;
     ke = 0.5 * mass * v2;
-    pe = mass * (-(double)c.gy) * ((double)ry[i] - (double)c.lo_y);
+    pe = mass * g * ((double)ry[i] - (double)c.lo_y);
     vm = v2;
     float x = rx[i], y = ry[i];
     bool inside = isfinite(x) && isfinite(y) && !(x < c.lo_x) && !(x > c.hi_x) &&
@@ -556,11 +556,11 @@ hipError_t launch_scan(const Launch& L, const int32_t* in, int64_t* out, int n,
 
 int64_t measure_partial_words(int n) { return 4 * (int64_t)std::max(blocks_for(n), 1); }
 
-hipError_t launch_measure(const Launch& L, const DevState& s, double mass, double* partials,
-                          double* out5) {
+hipError_t launch_measure(const Launch& L, const DevState& s, double mass, double gravity_y,
+                          double* partials, double* out5) {
   const int nb = std::max(blocks_for(s.n), 1);
-  k_measure_partial<<<nb, BLK, 0, L.stream>>>(s.n, L.phys, mass, s.rx, s.ry, s.vx, s.vy,
-                                              partials);
+  k_measure_partial<<<nb, BLK, 0, L.stream>>>(s.n, L.phys, mass, -gravity_y, s.rx, s.ry, s.vx,
+                                              s.vy, partials);
   k_measure_final<<<1, BLK, 0, L.stream>>>(nb, s.n, mass, partials, out5);
   return hipGetLastError();
 }

@@ -12,10 +12,9 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-# FP64-oracle tolerances (FP32 on the GPU)
+# FP64-oracle tolerances of one evaluation (FP32 on the GPU)
 RHO_RTOL = 2e-6          # density: relative, per particle
 ACC_RTOL = 2e-4          # acceleration: |a - a64| <= ACC_RTOL * max|a64|
-POS_ATOL_CELLS = 1e-5    # positions after a few steps, in units of h
 
 
 def note(msg):
@@ -188,31 +187,54 @@ def test_ell_overflow_fallback(sph, orc):
 
 @pytest.mark.parametrize("name,steps", [("reference_example", 50), ("dam_break_16k", 40),
                                         ("periodic_9k", 40), ("random_clusters", 20)])
-def test_multistep_run_matches_oracle(sph, orc, all_scenes, name, steps):
+def test_multistep_run_bit_exact_fp32(sph, orc, all_scenes, name, steps):
+    """The whole pipeline, `steps` iterations: identical sort permutations every
+    step (same particle ids in the same order) and bit-identical FP32 state."""
     s = all_scenes[name]
     note(f"[gpu] run {name} steps={steps}")
     o32 = orc.run(s.params, s.rx, s.ry, s.vx, s.vy, steps)
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.run(steps)
+        d = ctx.download()
+        f = ctx.fields()
+    assert np.array_equal(d["pid"], o32["pid"])
+    for k in ("rx", "ry", "vx", "vy"):
+        assert np.array_equal(d[k], o32[k]), k
+    assert np.array_equal(f["rho"], o32["rho"])
+    assert np.array_equal(f["ax"], o32["ax"]) and np.array_equal(f["ay"], o32["ay"])
+
+
+# FP64 comparison horizon per scene (the random-cluster gas is chaotic: nearly
+# coincident particles), and stated tolerances after that many steps
+FP64_CASES = [("reference_example", 10), ("dam_break_16k", 40), ("periodic_9k", 40),
+              ("random_clusters", 2)]
+FP64_POS_TOL_H = 1e-3   # max |r - r64| / h
+FP64_RHO_RTOL = 1e-3    # max |rho - rho64| / rho64
+FP64_ACC_TOL = 1e-2     # max |a - a64| / max |a64|
+
+
+@pytest.mark.parametrize("name,steps", FP64_CASES)
+def test_multistep_run_vs_fp64(sph, orc, all_scenes, name, steps):
+    s = all_scenes[name]
     o64 = orc.run(s.params, s.rx, s.ry, s.vx, s.vy, steps, dtype=np.float64)
     with sph.Context(s.params, s.n) as ctx:
         ctx.upload(s.rx, s.ry, s.vx, s.vy)
         ctx.run(steps)
         d = ctx.download()
         f = ctx.fields()
-    # identical particle order and ids: the sort of every step was bit-exact
-    assert np.array_equal(d["pid"], o32["pid"])
-    same = np.array_equal(d["rx"], o32["rx"]) and np.array_equal(d["ry"], o32["ry"])
-    note(f"[gpu] {name}: positions bit-identical to FP32 oracle after {steps} steps: {same}")
     h = s.params.h
-    assert np.abs(d["rx"] - o32["rx"]).max() <= 1e-6 * h * steps
-    # FP64 oracle, matched by particle id
     i32 = np.argsort(d["pid"])
     i64 = np.argsort(o64["pid"])
-    err = np.hypot(d["rx"][i32] - o64["rx"][i64], d["ry"][i32] - o64["ry"][i64])
-    note(f"[gpu] {name}: max position error vs FP64 = {err.max() / h:.3e} h")
-    assert err.max() <= POS_ATOL_CELLS * h * steps
-    rerr = np.abs(f["rho"][i32] - o64["rho"][i64]) / o64["rho"][i64]
-    note(f"[gpu] {name}: max density rel error vs FP64 = {rerr.max():.3e}")
-    assert rerr.max() <= 1e-4
+    perr = np.hypot(d["rx"][i32] - o64["rx"][i64], d["ry"][i32] - o64["ry"][i64]).max() / h
+    rerr = (np.abs(f["rho"][i32] - o64["rho"][i64]) / o64["rho"][i64]).max()
+    a64 = np.hypot(o64["ax"], o64["ay"]).max()
+    aerr = np.hypot(f["ax"][i32] - o64["ax"][i64], f["ay"][i32] - o64["ay"][i64]).max() / a64
+    note(f"[gpu] {name} after {steps} steps vs FP64: position {perr:.3e} h, density "
+         f"{rerr:.3e} rel, acceleration {aerr:.3e} of max")
+    assert perr <= FP64_POS_TOL_H
+    assert rerr <= FP64_RHO_RTOL
+    assert aerr <= FP64_ACC_TOL
 
 
 def test_fused_step_equals_stage_path(sph, all_scenes):
