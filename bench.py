@@ -33,6 +33,8 @@ def alg_bytes_per_particle(stage, k, occ, passes):
     """Algorithmic HBM bytes per particle of one launch of each kernel (FP32
     state, u64 keys, u32 indices; neighbor re-reads counted as cache hits)."""
     return {
+        # R r,v (16); W rho,pr (8) a (8) T r,v (16) key (8); halo re-reads are cache hits
+        "block_step": 56.0,
         # R r,v (16) rho,pr (8) count (4) list (4k); W a (8) T r,v (16) key (8)
         "force_computation": 60.0 + 4.0 * k,
         # R new keys (8) + previous sorted keys (8); clear previous ranges (8/occupied cell)
@@ -116,8 +118,7 @@ def main():
     graph_ms = (g1 - g0) * 1e3 / args.steps
     log(f"graph replay: {graph_ms:.4f} ms/step")
 
-    cnt, _ = ctx.ell()
-    k = float(cnt.mean())
+    k = ctx.build_neighbor_lists(False) / n
     st, _ = ctx.cell_ranges()
     occ = float((st >= 0).sum()) / n
     passes = (int(ctx.grid().key_bits) + 7) // 8
@@ -143,9 +144,8 @@ def main():
                    key=lambda s: stages[s]["ms_total"])
     dom = stages[dom_name]
     achieved = dom["achieved_GBs"]
-    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes) * (passes if s == "sort_pass" else 1)
-                     for s in ("force_computation", "sort_histogram", "sort_pass",
-                               "neighbor_search", "approximate_density"))
+    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes) * stages[s]["launches"] / args.steps
+                     for s in stages if "alg_bytes_per_launch" in stages[s])
     out = {
         "metric": "particle-steps/s at 1M particles: plain GPU SPH step (surrogate path not built)",
         "value": value,

@@ -101,6 +101,7 @@ enum sph_stage {
   SPH_T_FORCE,             /* forces (+ integration and next keys inside sph_step) */
   SPH_T_SORT_HIST,         /* radix sort: digit histogram kernel */
   SPH_T_SORT_PASS,         /* radix sort: one onesweep pass (several per sort) */
+  SPH_T_BLOCK_STEP,        /* fused density + forces + integration + keys (sph_step) */
   SPH_T_MEASURE,
   SPH_T_COUNT
 };
@@ -152,11 +153,14 @@ int sph_measure(void* ctx, double* out5);
 
 /* ---- fused pipeline (no reference counterpart; same arithmetic as the stages) ----
  * sph_prepare: find neighbors + density with the fixed-width neighbor list.
- * sph_step: n iterations of [forces + integration + keys] -> radix sort ->
- *   [permutation + cell ranges] -> [density + EOS + neighbor list]; forces are
- *   left for the state before the last integration (call sph_forces to refresh).
+ * sph_step: n iterations of [density + forces + integration + keys] -> radix
+ *   sort -> [permutation + cell ranges]; density and forces are left for the
+ *   state before the last integration (sph_density_pressure + sph_forces refresh).
  * Reference-equivalent run of S iterations (simulation_configuration.h:297-387):
- *   sph_prepare; sph_step(S-1); sph_forces(SPH_COMPUTE_LISTS). */
+ *   sph_prepare; sph_step(S-1); sph_density_pressure(SORTED); sph_forces(SORTED).
+ * With Morton order one iteration is three kernels kinds: a fused block kernel
+ * (density + EOS + forces + integration + next keys over occupied 8x8-cell
+ * blocks staged in LDS), the radix sort, and the permutation + ranges kernel. */
 int sph_prepare(void* ctx);
 int sph_step(void* ctx, int32_t n_steps);
 /* Capture one fused iteration in a hipGraph and replay it in sph_step. */

@@ -21,7 +21,7 @@ PHYS_REFERENCE, PHYS_WCSPH = 0, 1
 BC_WALLS, BC_PERIODIC = 0, 1
 COMPUTE_SORTED, COMPUTE_LISTS = 0, 1
 STAGES = ("integration", "neighbor_search", "neighbor_list", "approximate_density",
-          "force_computation", "sort_histogram", "sort_pass", "measure")
+          "force_computation", "sort_histogram", "sort_pass", "block_step", "measure")
 
 
 class SphError(RuntimeError):
@@ -295,11 +295,14 @@ class Context:
         return int(c.value)
 
     def run(self, steps: int):
-        """Reference-equivalent run of `steps` iterations (pre-loop + steps-1 loops)."""
+        """Reference-equivalent run of `steps` iterations (pre-loop + steps-1 loops):
+        the last iteration's density, pressure and forces are evaluated on the
+        final state."""
         self.prepare()
         if steps > 1:
             self.step(steps - 1)
-        self.forces(COMPUTE_LISTS)
+        self.density_pressure(COMPUTE_SORTED)
+        self.forces(COMPUTE_SORTED)
 
     # -- inspection
     def sorted_keys(self):

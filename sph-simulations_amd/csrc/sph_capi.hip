@@ -55,6 +55,7 @@ struct Ctx {
   hipGraph_t graph = nullptr;
   int graph_n = -1;
   const void* graph_sig[9] = {};  // buffer assignment the graph was captured with
+  int block_grid = 0;             // workgroups of the fused block kernel (multiple of 8)
 };
 
 // Buffer assignment the fused iteration reads and writes; the stages swap
@@ -326,7 +327,16 @@ int find_neighbors(Ctx* c) {
 // + ranges -> density + EOS + ELL.
 int fused_iteration(Ctx* c) {
   DevState& s = c->s;
-  {
+  if (s.block_bits) {
+    // Morton order: one block kernel does density, forces, integration and keys
+    StageTimer t(c, SPH_T_BLOCK_STEP);
+    HIPCHK(c, launch_block_step(c->L, s, c->sb.keys_in, c->block_grid));
+  } else {
+    // row-major order: density with the fixed-width list, then forces from it
+    {
+      StageTimer t(c, SPH_T_DENSITY);
+      HIPCHK(c, launch_density(c->L, s, NB_ELL, false));
+    }
     StageTimer t(c, SPH_T_FORCE);
     HIPCHK(c, launch_forces(c->L, s, NB_ELL, true, c->sb.keys_in));
   }
@@ -339,10 +349,6 @@ int fused_iteration(Ctx* c) {
   }
   std::swap(s.pid, s.pid_alt);
   s.n_prev = s.n;
-  {
-    StageTimer t(c, SPH_T_DENSITY);
-    HIPCHK(c, launch_density(c->L, s, NB_ELL, false));
-  }
   return SPH_OK;
 }
 
@@ -476,6 +482,18 @@ int sph_create(const sph_params* params, int64_t n_max, int device, void** out_c
       (rc = dev_alloc(c, &b.lookback, (int64_t)sort_passes(c->grid.key_bits) * tiles * 256)))
     return bail(rc);
   b.max_tiles = tiles;
+  s.block_bits = p.order == SPH_ORDER_MORTON ? 3 : 0;
+  s.n_blocks = s.block_bits ? ((c->grid.n_keys + (1 << (2 * s.block_bits)) - 1) >> (2 * s.block_bits))
+                            : 0;
+  if (s.block_bits) {
+    if ((rc = dev_alloc(c, &s.blocks, s.n_blocks)) || (rc = dev_alloc(c, &s.active, s.n_blocks)) ||
+        (rc = dev_alloc(c, &s.n_active, 1)))
+      return bail(rc);
+  }
+  b.blocks = s.blocks;
+  b.block_bits = s.block_bits;
+  b.n_active = s.n_active;
+  c->block_grid = (int)std::min<int64_t>(2048, ((std::max<int64_t>(s.n_blocks, 1) + 7) / 8) * 8);
   int cu = 256;
   (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device);
   b.hist_blocks = std::max(cu, 1);
@@ -491,6 +509,8 @@ int sph_create(const sph_params* params, int64_t n_max, int device, void** out_c
   e = hipMemsetAsync(b.hist, 0, kMaxPasses * 256 * sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(b.status, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, c->grid.n_keys);
+  if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);
+  if (e == hipSuccess && s.block_bits) e = hipMemsetAsync(s.n_active, 0, sizeof(int), c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
     hip_fail(c, e, "initialise buffers");
@@ -767,7 +787,7 @@ int sph_step(void* ctx, int32_t n_steps) {
   if (!c) return SPH_ERR_INVALID;
   int rc = check_n(c);
   if (rc) return rc;
-  if (!c->density_ell) return fail(c, SPH_ERR_STATE, "call sph_prepare first");
+  if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_prepare first");
   int done = 0;
   if (c->use_graph && !c->timing) {
     if (n_steps >= 2 && !graph_matches(c) && (rc = capture_graph(c))) return rc;
@@ -776,6 +796,7 @@ int sph_step(void* ctx, int32_t n_steps) {
   for (; done < n_steps; ++done)
     if ((rc = fused_iteration(c))) return rc;
   c->csr_total = -1;
+  if (n_steps > 0) c->density_ell = false;
   return SPH_OK;
 }
 

@@ -44,7 +44,8 @@ __global__ __launch_bounds__(BLK) void k_gather(
     const float* __restrict__ say, float* __restrict__ drx, float* __restrict__ dry,
     float* __restrict__ dvx, float* __restrict__ dvy, uint32_t* __restrict__ dpid,
     float* __restrict__ dax, float* __restrict__ day, int2* __restrict__ cells,
-    int64_t n_keys, uint32_t* __restrict__ hist, int hist_words) {
+    int64_t n_keys, uint32_t* __restrict__ hist, int hist_words, int2* __restrict__ blocks,
+    int block_bits, int* __restrict__ active, int* __restrict__ n_active) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
@@ -60,6 +61,31 @@ __global__ __launch_bounds__(BLK) void k_gather(
     is_end = live && (i == n - 1 || ((starts >> (lane + 1)) & 1ull));
   } else {
     is_end = live && (i == n - 1 || k != keys[i + 1]);
+  }
+  if (block_bits > 0) {
+    // the same boundary detection one level up: Morton blocks of 4^block_bits cells
+    const int sh = 2 * block_bits;
+    const bool b_start = live && (i == 0 || (k >> sh) != (prev >> sh));
+    const uint64_t bstarts = __ballot(b_start);
+    bool b_end;
+    if (lane < kWave - 1)
+      b_end = live && (i == n - 1 || ((bstarts >> (lane + 1)) & 1ull));
+    else
+      b_end = live && (i == n - 1 || (k >> sh) != (keys[i + 1] >> sh));
+    // one atomic per wave appends this wave's new blocks to the occupied list
+    int base = 0;
+    if (bstarts) {
+      if (lane == 0) base = atomicAdd(n_active, __popcll(bstarts));
+      base = __shfl(base, 0);
+    }
+    if (live && k < (uint64_t)n_keys) {
+      const int bk = (int)(k >> sh);
+      if (b_start) {
+        blocks[bk].x = i;
+        active[base + __popcll(bstarts & ((1ull << lane) - 1ull))] = bk;
+      }
+      if (b_end) blocks[bk].y = i + 1;
+    }
   }
   if (!live) return;
   if (k < (uint64_t)n_keys) {
@@ -423,7 +449,7 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
     k_gather<<<blocks_for(s.n), BLK, 0, L.stream>>>(
         s.n, s.perm, s.keys_sorted, src_rx, src_ry, src_vx, src_vy, src_pid, src_ax, src_ay,
         dst_rx, dst_ry, dst_vx, dst_vy, dst_pid, dst_ax, dst_ay, s.cells, s.n_keys, s.hist,
-        hist_words);
+        hist_words, s.blocks, s.block_bits, s.active, s.n_active);
   return hipGetLastError();
 }
 

@@ -74,7 +74,8 @@ __global__ __launch_bounds__(WG) void k_hist(const uint64_t* __restrict__ keys, 
                                              uint32_t* __restrict__ tickets,
                                              const uint64_t* __restrict__ old_sorted,
                                              int old_n, int2* __restrict__ cells,
-                                             uint64_t n_keys) {
+                                             uint64_t n_keys, int2* __restrict__ blocks,
+                                             int block_bits, int* __restrict__ n_active) {
   __shared__ uint32_t s_h[kMaxPasses * RADIX];
   for (int i = threadIdx.x; i < passes * RADIX; i += WG) s_h[i] = 0;
   __syncthreads();
@@ -105,8 +106,12 @@ __global__ __launch_bounds__(WG) void k_hist(const uint64_t* __restrict__ keys, 
   }
   for (int i = gtid; i < old_n; i += gsize) {
     uint64_t k = old_sorted[i];
-    if ((i == 0 || k != old_sorted[i - 1]) && k < n_keys) cells[k] = make_int2(-1, -1);
+    uint64_t kp = i > 0 ? old_sorted[i - 1] : ~k;
+    if (k != kp && k < n_keys) cells[k] = make_int2(-1, -1);
+    if (block_bits > 0 && (k >> (2 * block_bits)) != (kp >> (2 * block_bits)) && k < n_keys)
+      blocks[k >> (2 * block_bits)] = make_int2(-1, -1);
   }
+  if (gtid == 0 && n_active) *n_active = 0;
   __syncthreads();
   for (int i = threadIdx.x; i < passes * RADIX; i += WG)
     if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
@@ -263,7 +268,8 @@ hipError_t radix_sort_hist(const SortBuffers& b, int n, int key_bits, hipStream_
   // clears `passes` look-back arrays laid out back to back
   k_hist<<<hist_blocks, WG, 0, stream>>>(b.keys_in, n, passes, b.hist, b.lookback,
                                          lookback_words_per_pass(n) * passes, b.tickets,
-                                         b.keys_out, b.clear_n, b.cells, b.n_keys);
+                                         b.keys_out, b.clear_n, b.cells, b.n_keys, b.blocks,
+                                         b.block_bits, b.n_active);
   return hipGetLastError();
 }
 

@@ -39,6 +39,14 @@ def scenes(sph):
     d.vx = rng.normal(0, 0.3, d.n).astype(np.float32)
     d.vy = rng.normal(0, 0.3, d.n).astype(np.float32)
     out["random_clusters"] = d
+    # more particles in one Morton block's halo box than the LDS stage holds:
+    # exercises the global-memory path of the block kernel
+    e = sph.scenes.dam_break(60, 60, seed=6, name="dense")
+    h = e.params.h
+    e.rx = (0.3 + rng.random(e.n) * 4 * h).astype(np.float32)
+    e.ry = (0.3 + rng.random(e.n) * 4 * h).astype(np.float32)
+    e.rx[:1000] = (0.5 + rng.random(1000) * 30 * h).astype(np.float32)
+    out["dense_cluster"] = e
     return out
 
 
@@ -185,8 +193,9 @@ def test_ell_overflow_fallback(sph, orc):
         assert np.abs(f["ay"] - ay).max() <= 1e-5 * amax
 
 
-@pytest.mark.parametrize("name,steps", [("reference_example", 50), ("dam_break_16k", 40),
-                                        ("periodic_9k", 40), ("random_clusters", 20)])
+@pytest.mark.parametrize("name,steps", [("reference_example", 50), ("reference_rowmajor", 20),
+                                        ("dam_break_16k", 40), ("periodic_9k", 40),
+                                        ("random_clusters", 20), ("dense_cluster", 3)])
 def test_multistep_run_bit_exact_fp32(sph, orc, all_scenes, name, steps):
     """The whole pipeline, `steps` iterations: identical sort permutations every
     step (same particle ids in the same order) and bit-identical FP32 state."""
