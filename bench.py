@@ -29,12 +29,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_particle(stage, k, occ, passes):
+def alg_bytes_per_particle(stage, k, occ, passes, split=True):
     """Algorithmic HBM bytes per particle of one launch of each kernel (FP32
     state, u64 keys, u32 indices; neighbor re-reads counted as cache hits)."""
     return {
-        # R r,v (16); W rho,pr (8) a (8) T r,v (16) key (8); halo re-reads are cache hits
-        "block_step": 56.0,
+        # split: density R r 8, W rho,pr 8; force R r,v,rho,pr 24, W a 8, T 16, key 8
+        # (fused: R r,v 16, W rho,pr 8, a 8, T 16, key 8); halo re-reads are cache hits
+        "block_step": 72.0 if split else 56.0,
         # R r,v (16) rho,pr (8) count (4) list (4k); W a (8) T r,v (16) key (8)
         "force_computation": 60.0 + 4.0 * k,
         # R new keys (8) + previous sorted keys (8); clear previous ranges (8/occupied cell)
@@ -96,6 +97,25 @@ def main():
     ctx.synchronize()
     log(f"warmup {args.warmup} steps done")
 
+    # step-kernel choice: time each variant on graph replay (all give identical results)
+    tune = {}
+    for variant, vname in ((sph.STEP_SPLIT_BLOCK, "split_block"),
+                           (sph.STEP_FUSED_BLOCK, "fused_block"),
+                           (sph.STEP_PARTICLE, "particle")):
+        ctx.set_step_kernel(variant)
+        ctx.step(4)
+        ctx.synchronize()
+        a0 = time.perf_counter()
+        ctx.step(20)
+        ctx.synchronize()
+        tune[vname] = (time.perf_counter() - a0) * 1e3 / 20
+        log(f"variant {vname}: {tune[vname]:.4f} ms/step")
+    best = min(tune, key=tune.get)
+    ctx.set_step_kernel({"split_block": sph.STEP_SPLIT_BLOCK, "fused_block": sph.STEP_FUSED_BLOCK,
+                         "particle": sph.STEP_PARTICLE}[best])
+    bstats = ctx.block_stats()
+    log(f"block stats {bstats}")
+
     # timed region: every launch bracketed by hipEvents on the context stream
     ctx.set_timing(True)
     ctx.synchronize()
@@ -133,7 +153,7 @@ def main():
         entry = {"launches": launches, "ms_total": round(ms, 4),
                  "us_per_launch": round(per_launch_ms * 1e3, 3)}
         try:
-            b = alg_bytes_per_particle(name, k, occ, passes) * n
+            b = alg_bytes_per_particle(name, k, occ, passes, best == "split_block") * n
             entry["alg_bytes_per_launch"] = int(b)
             entry["achieved_GBs"] = round(b / (per_launch_ms * 1e-3) / 1e9, 1)
         except KeyError:
@@ -144,7 +164,8 @@ def main():
                    key=lambda s: stages[s]["ms_total"])
     dom = stages[dom_name]
     achieved = dom["achieved_GBs"]
-    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes) * stages[s]["launches"] / args.steps
+    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes, best == "split_block")
+                     * stages[s]["launches"] / args.steps
                      for s in stages if "alg_bytes_per_launch" in stages[s])
     out = {
         "metric": "particle-steps/s at 1M particles: plain GPU SPH step (surrogate path not built)",
@@ -162,7 +183,9 @@ def main():
         "config": {"workload": scene.name, "particles": n, "box": list(scene.params.hi),
                    "h": scene.params.h, "dt": scene.params.dt, "physics": "wcsph",
                    "mean_neighbors": round(k, 3), "occupied_cells_per_particle": round(occ, 4),
-                   "radix_passes": passes},
+                   "radix_passes": passes, "step_kernel": best,
+                   "step_kernel_ms": {k2: round(v, 4) for k2, v in tune.items()},
+                   "block_stats": bstats},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None},

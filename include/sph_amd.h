@@ -37,6 +37,13 @@ enum sph_order { SPH_ORDER_MORTON = 0, SPH_ORDER_ROWMAJOR = 1 };
 enum sph_physics { SPH_PHYS_REFERENCE = 0, SPH_PHYS_WCSPH = 1 };
 enum sph_boundary { SPH_BC_WALLS = 0, SPH_BC_PERIODIC = 1 };
 enum sph_compute { SPH_COMPUTE_SORTED = 0, SPH_COMPUTE_LISTS = 1 };
+/* Physics kernels of the fused step (sph_step); all give bit-identical results.
+ * Block variants need Morton order (row-major always uses PARTICLE). */
+enum sph_step_kernel {
+  SPH_STEP_SPLIT_BLOCK = 0, /* density kernel + force kernel over LDS-staged Morton blocks */
+  SPH_STEP_FUSED_BLOCK = 1, /* one kernel, two-cell halo, densities never leave LDS */
+  SPH_STEP_PARTICLE = 2     /* one thread per particle, fixed-width neighbor list */
+};
 
 /* Simulation parameters. Reference physics (SPH_PHYS_REFERENCE) reproduces the
  * macros of core/ConfigVariables.hpp:9-15 when filled with h=1, eos_k=30,
@@ -168,6 +175,12 @@ int sph_set_graph(void* ctx, int32_t enable);
 /* Number of particles whose neighbor count exceeded the fixed list width in the
  * last density pass (they are handled by a stencil fallback). Synchronises. */
 int sph_ell_overflow(void* ctx, int64_t* count);
+/* Choose the physics kernels of sph_step (enum sph_step_kernel). */
+int sph_set_step_kernel(void* ctx, int32_t variant);
+/* Re-evaluate the block kernels on the current state with counters on:
+ * out[0..2] density (or fused) kernel: occupied blocks, blocks over the LDS
+ * capacity, staged particles; out[3..5] the same for the force kernel. */
+int sph_block_stats(void* ctx, int64_t* out6);
 
 /* ---- inspection (host copies; synchronise) ---- */
 int sph_get_sorted_keys(void* ctx, uint64_t* keys);  /* n keys of the last sort */

@@ -20,6 +20,7 @@ ORDER_MORTON, ORDER_ROWMAJOR = 0, 1
 PHYS_REFERENCE, PHYS_WCSPH = 0, 1
 BC_WALLS, BC_PERIODIC = 0, 1
 COMPUTE_SORTED, COMPUTE_LISTS = 0, 1
+STEP_SPLIT_BLOCK, STEP_FUSED_BLOCK, STEP_PARTICLE = 0, 1, 2
 STAGES = ("integration", "neighbor_search", "neighbor_list", "approximate_density",
           "force_computation", "sort_histogram", "sort_pass", "block_step", "measure")
 
@@ -132,6 +133,8 @@ def lib():
         "sph_step": (i32, [vp, i32]),
         "sph_set_graph": (i32, [vp, i32]),
         "sph_ell_overflow": (i32, [vp, P(i64)]),
+        "sph_set_step_kernel": (i32, [vp, i32]),
+        "sph_block_stats": (i32, [vp, P(i64)]),
         "sph_get_sorted_keys": (i32, [vp, vp]),
         "sph_get_permutation": (i32, [vp, vp]),
         "sph_get_cell_ranges": (i32, [vp, vp, vp]),
@@ -155,7 +158,7 @@ EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_
             "sph_find_neighbors", "sph_build_neighbor_lists", "sph_density_pressure",
             "sph_forces", "sph_integrate", "sph_update_velocities", "sph_update_positions",
             "sph_set_dt", "sph_measure", "sph_prepare", "sph_step",
-            "sph_set_graph", "sph_ell_overflow", "sph_get_sorted_keys", "sph_get_permutation",
+            "sph_set_graph", "sph_ell_overflow", "sph_set_step_kernel", "sph_block_stats", "sph_get_sorted_keys", "sph_get_permutation",
             "sph_get_cell_ranges", "sph_get_neighbor_lists", "sph_get_ell",
             "sph_get_device_view", "sph_set_timing", "sph_stage_times", "sph_synchronize")
 
@@ -288,6 +291,17 @@ class Context:
 
     def set_graph(self, enable=True):
         self._chk(self._L.sph_set_graph(self._h, int(bool(enable))), "sph_set_graph")
+
+    def set_step_kernel(self, variant: int):
+        self._chk(self._L.sph_set_step_kernel(self._h, int(variant)), "sph_set_step_kernel")
+
+    def block_stats(self):
+        out = np.zeros(6, np.int64)
+        self._chk(self._L.sph_block_stats(self._h, out.ctypes.data_as(C.POINTER(C.c_int64))),
+                  "sph_block_stats")
+        keys = ("blocks", "over_capacity", "staged")
+        return {"density": dict(zip(keys, out[:3].tolist())),
+                "force": dict(zip(keys, out[3:].tolist()))}
 
     def ell_overflow(self) -> int:
         c = C.c_int64()

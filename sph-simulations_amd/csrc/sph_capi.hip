@@ -55,7 +55,9 @@ struct Ctx {
   hipGraph_t graph = nullptr;
   int graph_n = -1;
   const void* graph_sig[9] = {};  // buffer assignment the graph was captured with
-  int block_grid = 0;             // workgroups of the fused block kernel (multiple of 8)
+  int block_grid = 0;             // persistent workgroups of the block kernels
+  int step_kernel = SPH_STEP_SPLIT_BLOCK;
+  unsigned long long* block_stats = nullptr;
 };
 
 // Buffer assignment the fused iteration reads and writes; the stages swap
@@ -327,10 +329,10 @@ int find_neighbors(Ctx* c) {
 // + ranges -> density + EOS + ELL.
 int fused_iteration(Ctx* c) {
   DevState& s = c->s;
-  if (s.block_bits) {
-    // Morton order: one block kernel does density, forces, integration and keys
+  if (s.block_bits && c->step_kernel != SPH_STEP_PARTICLE) {
+    // Morton order: density, forces, integration and keys over LDS-staged blocks
     StageTimer t(c, SPH_T_BLOCK_STEP);
-    HIPCHK(c, launch_block_step(c->L, s, c->sb.keys_in, c->block_grid));
+    HIPCHK(c, launch_block_step(c->L, s, c->sb.keys_in, c->block_grid, c->step_kernel, nullptr));
   } else {
     // row-major order: density with the fixed-width list, then forces from it
     {
@@ -485,18 +487,14 @@ int sph_create(const sph_params* params, int64_t n_max, int device, void** out_c
   s.block_bits = p.order == SPH_ORDER_MORTON ? 3 : 0;
   s.n_blocks = s.block_bits ? ((c->grid.n_keys + (1 << (2 * s.block_bits)) - 1) >> (2 * s.block_bits))
                             : 0;
-  if (s.block_bits) {
-    if ((rc = dev_alloc(c, &s.blocks, s.n_blocks)) || (rc = dev_alloc(c, &s.active, s.n_blocks)) ||
-        (rc = dev_alloc(c, &s.n_active, 1)))
-      return bail(rc);
-  }
+  if (s.block_bits && (rc = dev_alloc(c, &s.blocks, s.n_blocks))) return bail(rc);
+  if ((rc = dev_alloc(c, &c->block_stats, 6))) return bail(rc);
   b.blocks = s.blocks;
   b.block_bits = s.block_bits;
-  b.n_active = s.n_active;
-  c->block_grid = (int)std::min<int64_t>(2048, ((std::max<int64_t>(s.n_blocks, 1) + 7) / 8) * 8);
   int cu = 256;
   (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device);
   b.hist_blocks = std::max(cu, 1);
+  c->block_grid = block_step_grid(s.n_blocks, cu);
   b.cells = s.cells;
   b.n_keys = (uint64_t)c->grid.n_keys;
   s.keys_sorted = b.keys_out;
@@ -510,7 +508,6 @@ int sph_create(const sph_params* params, int64_t n_max, int device, void** out_c
   if (e == hipSuccess) e = hipMemsetAsync(b.status, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, c->grid.n_keys);
   if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);
-  if (e == hipSuccess && s.block_bits) e = hipMemsetAsync(s.n_active, 0, sizeof(int), c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
     hip_fail(c, e, "initialise buffers");
@@ -816,6 +813,33 @@ int sph_ell_overflow(void* ctx, int64_t* count) {
   HIPCHK(c, launch_ell_overflow(c->L, c->s, c->ovf));
   HIPCHK(c, hipMemcpyAsync(count, c->ovf, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int sph_set_step_kernel(void* ctx, int32_t variant) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  if (variant < SPH_STEP_SPLIT_BLOCK || variant > SPH_STEP_PARTICLE)
+    return fail(c, SPH_ERR_INVALID, "unknown step kernel");
+  c->step_kernel = variant;
+  destroy_graph(c);
+  return SPH_OK;
+}
+
+int sph_block_stats(void* ctx, int64_t* out6) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !out6) return SPH_ERR_INVALID;
+  if (!c->sorted || !c->s.block_bits)
+    return fail(c, SPH_ERR_STATE, "needs a sorted Morton-order state");
+  HIPCHK(c, hipMemsetAsync(c->block_stats, 0, 6 * sizeof(unsigned long long), c->stream));
+  int variant = c->step_kernel == SPH_STEP_PARTICLE ? SPH_STEP_SPLIT_BLOCK : c->step_kernel;
+  // writes only the unsorted twin, keys_in and the per-particle fields
+  HIPCHK(c, launch_block_step(c->L, c->s, c->sb.keys_in, c->block_grid, variant,
+                              c->block_stats));
+  unsigned long long v[6];
+  HIPCHK(c, hipMemcpyAsync(v, c->block_stats, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 6; ++i) out6[i] = (int64_t)v[i];
   return SPH_OK;
 }
 

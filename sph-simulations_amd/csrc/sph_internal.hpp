@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/sph_amd.h"
 #include "sph_device.hpp"
 
 namespace sphamd {
@@ -32,7 +33,6 @@ struct SortBuffers {
   // Morton blocks of 2^block_bits x 2^block_bits cells (0: no block table)
   int2* blocks;
   int block_bits;
-  int* n_active;     // zeroed by the histogram kernel, counted by the permutation kernel
 };
 
 int sort_tile_size();
@@ -65,12 +65,10 @@ struct DevState {
   int32_t* csr_idx;
   uint32_t* hist;  // the sort's histogram, cleared by the permutation kernel
   // Morton blocks (square groups of 2^block_bits cells per side, a contiguous key
-  // range): particle range per block and the list of occupied blocks
+  // range): particle range per block, -1 when empty
   int block_bits;   // 0 when the order has no block structure (row-major)
   int2* blocks;
   int64_t n_blocks;
-  int* active;
-  int* n_active;
 };
 
 enum { NB_STENCIL = 0, NB_CSR = 1, NB_ELL = 2 };
@@ -94,10 +92,11 @@ hipError_t launch_density(const Launch& L, const DevState& s, int nb_mode, bool 
 hipError_t launch_forces(const Launch& L, const DevState& s, int nb_mode, bool fused,
                          uint64_t* keys_out);
 hipError_t launch_integrate(const Launch& L, const DevState& s);
-// fused density + forces + integration + next keys over occupied Morton blocks,
-// each staged with a two-cell halo in LDS
-hipError_t launch_block_step(const Launch& L, const DevState& s, uint64_t* keys_out, int grid);
-int block_step_max_particles(int block_bits);
+// density + forces + integration + next keys over LDS-staged Morton blocks
+// (variant: enum sph_step_kernel); stats (may be null): 3 counters per kernel
+hipError_t launch_block_step(const Launch& L, const DevState& s, uint64_t* keys_out, int grid,
+                             int variant, unsigned long long* stats);
+int block_step_grid(int64_t n_blocks, int cus);
 hipError_t launch_update_velocities(const Launch& L, const DevState& s);
 hipError_t launch_update_positions(const Launch& L, const DevState& s);
 hipError_t launch_csr_count(const Launch& L, const DevState& s, bool self);

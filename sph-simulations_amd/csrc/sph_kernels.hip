@@ -45,7 +45,7 @@ __global__ __launch_bounds__(BLK) void k_gather(
     float* __restrict__ dvx, float* __restrict__ dvy, uint32_t* __restrict__ dpid,
     float* __restrict__ dax, float* __restrict__ day, int2* __restrict__ cells,
     int64_t n_keys, uint32_t* __restrict__ hist, int hist_words, int2* __restrict__ blocks,
-    int block_bits, int* __restrict__ active, int* __restrict__ n_active) {
+    int block_bits) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
@@ -72,18 +72,9 @@ __global__ __launch_bounds__(BLK) void k_gather(
       b_end = live && (i == n - 1 || ((bstarts >> (lane + 1)) & 1ull));
     else
       b_end = live && (i == n - 1 || (k >> sh) != (keys[i + 1] >> sh));
-    // one atomic per wave appends this wave's new blocks to the occupied list
-    int base = 0;
-    if (bstarts) {
-      if (lane == 0) base = atomicAdd(n_active, __popcll(bstarts));
-      base = __shfl(base, 0);
-    }
     if (live && k < (uint64_t)n_keys) {
       const int bk = (int)(k >> sh);
-      if (b_start) {
-        blocks[bk].x = i;
-        active[base + __popcll(bstarts & ((1ull << lane) - 1ull))] = bk;
-      }
+      if (b_start) blocks[bk].x = i;
       if (b_end) blocks[bk].y = i + 1;
     }
   }
@@ -449,7 +440,7 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
     k_gather<<<blocks_for(s.n), BLK, 0, L.stream>>>(
         s.n, s.perm, s.keys_sorted, src_rx, src_ry, src_vx, src_vy, src_pid, src_ax, src_ay,
         dst_rx, dst_ry, dst_vx, dst_vy, dst_pid, dst_ax, dst_ay, s.cells, s.n_keys, s.hist,
-        hist_words, s.blocks, s.block_bits, s.active, s.n_active);
+        hist_words, s.blocks, s.block_bits);
   return hipGetLastError();
 }
 

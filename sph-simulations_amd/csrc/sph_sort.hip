@@ -75,7 +75,7 @@ __global__ __launch_bounds__(WG) void k_hist(const uint64_t* __restrict__ keys, 
                                              const uint64_t* __restrict__ old_sorted,
                                              int old_n, int2* __restrict__ cells,
                                              uint64_t n_keys, int2* __restrict__ blocks,
-                                             int block_bits, int* __restrict__ n_active) {
+                                             int block_bits) {
   __shared__ uint32_t s_h[kMaxPasses * RADIX];
   for (int i = threadIdx.x; i < passes * RADIX; i += WG) s_h[i] = 0;
   __syncthreads();
@@ -111,7 +111,6 @@ __global__ __launch_bounds__(WG) void k_hist(const uint64_t* __restrict__ keys, 
     if (block_bits > 0 && (k >> (2 * block_bits)) != (kp >> (2 * block_bits)) && k < n_keys)
       blocks[k >> (2 * block_bits)] = make_int2(-1, -1);
   }
-  if (gtid == 0 && n_active) *n_active = 0;
   __syncthreads();
   for (int i = threadIdx.x; i < passes * RADIX; i += WG)
     if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
@@ -269,7 +268,7 @@ hipError_t radix_sort_hist(const SortBuffers& b, int n, int key_bits, hipStream_
   k_hist<<<hist_blocks, WG, 0, stream>>>(b.keys_in, n, passes, b.hist, b.lookback,
                                          lookback_words_per_pass(n) * passes, b.tickets,
                                          b.keys_out, b.clear_n, b.cells, b.n_keys, b.blocks,
-                                         b.block_bits, b.n_active);
+                                         b.block_bits);
   return hipGetLastError();
 }
 

@@ -193,17 +193,20 @@ def test_ell_overflow_fallback(sph, orc):
         assert np.abs(f["ay"] - ay).max() <= 1e-5 * amax
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("name,steps", [("reference_example", 50), ("reference_rowmajor", 20),
                                         ("dam_break_16k", 40), ("periodic_9k", 40),
                                         ("random_clusters", 20), ("dense_cluster", 3)])
-def test_multistep_run_bit_exact_fp32(sph, orc, all_scenes, name, steps):
-    """The whole pipeline, `steps` iterations: identical sort permutations every
+def test_multistep_run_bit_exact_fp32(sph, orc, all_scenes, name, steps, variant):
+    """The whole pipeline, `steps` iterations, with each step-kernel variant
+    (split blocks, fused blocks, per particle): identical sort permutations every
     step (same particle ids in the same order) and bit-identical FP32 state."""
     s = all_scenes[name]
-    note(f"[gpu] run {name} steps={steps}")
+    note(f"[gpu] run {name} steps={steps} step_kernel={variant}")
     o32 = orc.run(s.params, s.rx, s.ry, s.vx, s.vy, steps)
     with sph.Context(s.params, s.n) as ctx:
         ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.set_step_kernel(variant)
         ctx.run(steps)
         d = ctx.download()
         f = ctx.fields()
@@ -244,6 +247,23 @@ def test_multistep_run_vs_fp64(sph, orc, all_scenes, name, steps):
     assert perr <= FP64_POS_TOL_H
     assert rerr <= FP64_RHO_RTOL
     assert aerr <= FP64_ACC_TOL
+
+
+def test_block_stats_counts_capacity_fallbacks(sph, all_scenes):
+    s = all_scenes["dense_cluster"]
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.prepare()
+        st = ctx.block_stats()
+        note(f"[gpu] dense_cluster block stats {st}")
+        assert st["density"]["blocks"] > 0 and st["density"]["over_capacity"] > 0
+        d = sph.scenes.dam_break()
+    with sph.Context(d.params, d.n) as ctx:
+        ctx.upload(d.rx, d.ry, d.vx, d.vy)
+        ctx.prepare()
+        st = ctx.block_stats()
+        assert st["density"]["over_capacity"] == 0 and st["force"]["over_capacity"] == 0
+        assert st["density"]["staged"] >= d.n
 
 
 def test_fused_step_equals_stage_path(sph, all_scenes):
