@@ -29,24 +29,29 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_particle(stage, k, occ, passes, split=True):
+def alg_bytes_per_particle(stage, k, occ, passes, variant):
     """Algorithmic HBM bytes per particle of one launch of each kernel (FP32
-    state, u64 keys, u32 indices; neighbor re-reads counted as cache hits)."""
-    return {
-        # split: density R r 8, W rho,pr 8; force R r,v,rho,pr 24, W a 8, T 16, key 8
-        # (fused: R r,v 16, W rho,pr 8, a 8, T 16, key 8); halo re-reads are cache hits
-        "block_step": 72.0 if split else 56.0,
-        # R r,v (16) rho,pr (8) count (4) list (4k); W a (8) T r,v (16) key (8)
-        "force_computation": 60.0 + 4.0 * k,
+    state, u64 keys, u32 indices; neighbor and halo re-reads counted as cache
+    hits)."""
+    if variant == "split_block":
+        # density: R r 8, W rho,pr 8; force: R r,v,rho,pr 24, W a 8, T r,v 16, key 8
+        phys = {"approximate_density": 16.0, "force_computation": 56.0}
+    elif variant == "fused_block":
+        # R r,v 16; W rho,pr 8, a 8, T r,v 16, key 8
+        phys = {"block_step": 56.0}
+    else:
+        # density: R r 8, W rho,pr 8, count 4, list 4k;
+        # force: R r,v 16, rho,pr 8, count 4, list 4k, W a 8, T 16, key 8
+        phys = {"approximate_density": 20.0 + 4.0 * k, "force_computation": 60.0 + 4.0 * k}
+    sort = {
         # R new keys (8) + previous sorted keys (8); clear previous ranges (8/occupied cell)
         "sort_histogram": 16.0 + 8.0 * occ,
         # first pass R key 8, W key+idx 12; later passes R 12, W 12 (mean per launch)
         "sort_pass": (20.0 + 24.0 * (passes - 1)) / passes,
         # R perm 4, sorted key 8, T r,v 16, pid 4; W r,v 16, pid 4; ranges 8/occupied cell
         "neighbor_search": 52.0 + 8.0 * occ,
-        # R r 8; W rho,pr 8, count 4, list 4k
-        "approximate_density": 20.0 + 4.0 * k,
-    }[stage]
+    }
+    return {**phys, **sort}[stage]
 
 
 def cpu_baseline(sph, scene, budget_s):
@@ -153,7 +158,7 @@ def main():
         entry = {"launches": launches, "ms_total": round(ms, 4),
                  "us_per_launch": round(per_launch_ms * 1e3, 3)}
         try:
-            b = alg_bytes_per_particle(name, k, occ, passes, best == "split_block") * n
+            b = alg_bytes_per_particle(name, k, occ, passes, best) * n
             entry["alg_bytes_per_launch"] = int(b)
             entry["achieved_GBs"] = round(b / (per_launch_ms * 1e-3) / 1e9, 1)
         except KeyError:
@@ -164,7 +169,7 @@ def main():
                    key=lambda s: stages[s]["ms_total"])
     dom = stages[dom_name]
     achieved = dom["achieved_GBs"]
-    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes, best == "split_block")
+    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes, best)
                      * stages[s]["launches"] / args.steps
                      for s in stages if "alg_bytes_per_launch" in stages[s])
     out = {

@@ -362,7 +362,7 @@ constexpr int kMaxPSplit = 1024;
 
 template <int PHYS, int BC>
 hipError_t blocks_go(const Launch& L, const DevState& s, uint64_t* keys_out, int grid,
-                     int variant, unsigned long long* stats) {
+                     int variant, unsigned long long* stats, int part) {
   if (s.block_bits != 3 || grid <= 0) return hipErrorInvalidValue;
   const int nbk = (int)s.n_blocks;
   if (variant == SPH_STEP_FUSED_BLOCK) {
@@ -370,14 +370,17 @@ hipError_t blocks_go(const Launch& L, const DevState& s, uint64_t* keys_out, int
         s.n, nbk, L.grid, L.phys, s.blocks, s.cells, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.ax,
         s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out, stats);
   } else {
-    k_blocks<PHYS, BC, 3, 1, SPLIT_DENSITY, kMaxPSplit><<<grid, BWG, 0, L.stream>>>(
-        s.n, nbk, L.grid, L.phys, s.blocks, s.cells, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.ax,
-        s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out, stats);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    k_blocks<PHYS, BC, 3, 1, SPLIT_FORCE, kMaxPSplit><<<grid, BWG, 0, L.stream>>>(
-        s.n, nbk, L.grid, L.phys, s.blocks, s.cells, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.ax,
-        s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out, stats ? stats + 3 : nullptr);
+    if (part != 2) {
+      k_blocks<PHYS, BC, 3, 1, SPLIT_DENSITY, kMaxPSplit><<<grid, BWG, 0, L.stream>>>(
+          s.n, nbk, L.grid, L.phys, s.blocks, s.cells, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr,
+          s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out, stats);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    if (part != 1)
+      k_blocks<PHYS, BC, 3, 1, SPLIT_FORCE, kMaxPSplit><<<grid, BWG, 0, L.stream>>>(
+          s.n, nbk, L.grid, L.phys, s.blocks, s.cells, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr,
+          s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out, stats ? stats + 3 : nullptr);
   }
   return hipGetLastError();
 }
@@ -390,13 +393,13 @@ int block_step_grid(int64_t n_blocks, int cus) {
 }
 
 hipError_t launch_block_step(const Launch& L, const DevState& s, uint64_t* keys_out, int grid,
-                             int variant, unsigned long long* stats) {
+                             int variant, unsigned long long* stats, int part) {
   if (s.n <= 0) return hipSuccess;
   const int ph = L.phys.physics, bc = L.phys.boundary;
-  if (ph == 0 && bc == 0) return blocks_go<0, 0>(L, s, keys_out, grid, variant, stats);
-  if (ph == 0 && bc == 1) return blocks_go<0, 1>(L, s, keys_out, grid, variant, stats);
-  if (ph == 1 && bc == 0) return blocks_go<1, 0>(L, s, keys_out, grid, variant, stats);
-  return blocks_go<1, 1>(L, s, keys_out, grid, variant, stats);
+  if (ph == 0 && bc == 0) return blocks_go<0, 0>(L, s, keys_out, grid, variant, stats, part);
+  if (ph == 0 && bc == 1) return blocks_go<0, 1>(L, s, keys_out, grid, variant, stats, part);
+  if (ph == 1 && bc == 0) return blocks_go<1, 0>(L, s, keys_out, grid, variant, stats, part);
+  return blocks_go<1, 1>(L, s, keys_out, grid, variant, stats, part);
 }
 
 }  // namespace sphamd

@@ -329,10 +329,20 @@ int find_neighbors(Ctx* c) {
 // + ranges -> density + EOS + ELL.
 int fused_iteration(Ctx* c) {
   DevState& s = c->s;
-  if (s.block_bits && c->step_kernel != SPH_STEP_PARTICLE) {
-    // Morton order: density, forces, integration and keys over LDS-staged blocks
+  if (s.block_bits && c->step_kernel == SPH_STEP_FUSED_BLOCK) {
+    // Morton order, one kernel: density, forces, integration and keys
     StageTimer t(c, SPH_T_BLOCK_STEP);
     HIPCHK(c, launch_block_step(c->L, s, c->sb.keys_in, c->block_grid, c->step_kernel, nullptr));
+  } else if (s.block_bits && c->step_kernel == SPH_STEP_SPLIT_BLOCK) {
+    // Morton order: block density kernel, then block force + integration + keys
+    {
+      StageTimer t(c, SPH_T_DENSITY);
+      HIPCHK(c, launch_block_step(c->L, s, c->sb.keys_in, c->block_grid, c->step_kernel,
+                                  nullptr, 1));
+    }
+    StageTimer t(c, SPH_T_FORCE);
+    HIPCHK(c, launch_block_step(c->L, s, c->sb.keys_in, c->block_grid, c->step_kernel, nullptr,
+                                2));
   } else {
     // row-major order: density with the fixed-width list, then forces from it
     {

@@ -93,9 +93,10 @@ hipError_t launch_forces(const Launch& L, const DevState& s, int nb_mode, bool f
                          uint64_t* keys_out);
 hipError_t launch_integrate(const Launch& L, const DevState& s);
 // density + forces + integration + next keys over LDS-staged Morton blocks
-// (variant: enum sph_step_kernel); stats (may be null): 3 counters per kernel
+// (variant: enum sph_step_kernel); stats (may be null): 3 counters per kernel;
+// part selects the kernels of the split variant: 0 both, 1 density, 2 force
 hipError_t launch_block_step(const Launch& L, const DevState& s, uint64_t* keys_out, int grid,
-                             int variant, unsigned long long* stats);
+                             int variant, unsigned long long* stats, int part = 0);
 int block_step_grid(int64_t n_blocks, int cus);
 hipError_t launch_update_velocities(const Launch& L, const DevState& s);
 hipError_t launch_update_positions(const Launch& L, const DevState& s);
