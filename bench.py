@@ -29,7 +29,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_particle(stage, k, occ, passes, variant):
+def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers_frac=0.0):
     """Algorithmic HBM bytes per particle of one launch of each kernel (FP32
     state, u64 keys, u32 indices; neighbor and halo re-reads counted as cache
     hits)."""
@@ -43,14 +43,26 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant):
         # density: R r 8, W rho,pr 8, count 4, list 4k;
         # force: R r,v 16, rho,pr 8, count 4, list 4k, W a 8, T 16, key 8
         phys = {"approximate_density": 20.0 + 4.0 * k, "force_computation": 60.0 + 4.0 * k}
-    sort = {
-        # R new keys (8) + previous sorted keys (8); clear previous ranges (8/occupied cell)
-        "sort_histogram": 16.0 + 8.0 * occ,
-        # first pass R key 8, W key+idx 12; later passes R 12, W 12 (mean per launch)
-        "sort_pass": (20.0 + 24.0 * (passes - 1)) / passes,
-        # R perm 4, sorted key 8, T r,v 16, pid 4; W r,v 16, pid 4; ranges 8/occupied cell
-        "neighbor_search": 52.0 + 8.0 * occ,
-    }
+    if resort == "full":
+        sort = {
+            # R new keys (8) + previous sorted keys (8); clear previous ranges (8/occ. cell)
+            "sort_histogram": 16.0 + 8.0 * occ,
+            # first pass R key 8, W key+idx 12; later passes R 12, W 12 (mean per launch)
+            "sort_pass": (20.0 + 24.0 * (passes - 1)) / passes,
+            # R perm 4, sorted key 8, T r,v 16, pid 4; W r,v 16, pid 4; ranges 8/occ. cell
+            "neighbor_search": 52.0 + 8.0 * occ,
+        }
+    else:
+        f = movers_frac
+        sort = {
+            # movers: R new + old keys 16, clear old ranges 8 occ, W flags ~0.1, movers 12 f;
+            # mover histogram: R 8 f  (mean of the two launches)
+            "sort_histogram": (16.0 + 8.0 * occ + 12.0 * f + 8.0 * f) / 2,
+            "sort_pass": (20.0 + 24.0 * (passes - 1)) / passes * f,
+            # merge: R old key 8, T r,v 16, pid 4; W r,v 16, pid 4, key 8, perm 4;
+            # ranges: R key 8, W 8 occ  (mean of the two launches)
+            "neighbor_search": (60.0 + 8.0 + 8.0 * occ) / 2,
+        }
     return {**phys, **sort}[stage]
 
 
@@ -102,22 +114,27 @@ def main():
     ctx.synchronize()
     log(f"warmup {args.warmup} steps done")
 
-    # step-kernel choice: time each variant on graph replay (all give identical results)
+    # step-kernel and re-sort choice: time each combination on graph replay (all
+    # give identical results)
+    kernels = {"split_block": sph.STEP_SPLIT_BLOCK, "fused_block": sph.STEP_FUSED_BLOCK,
+               "particle": sph.STEP_PARTICLE}
+    resorts = {"full": sph.RESORT_FULL, "incremental": sph.RESORT_INCREMENTAL}
     tune = {}
-    for variant, vname in ((sph.STEP_SPLIT_BLOCK, "split_block"),
-                           (sph.STEP_FUSED_BLOCK, "fused_block"),
-                           (sph.STEP_PARTICLE, "particle")):
-        ctx.set_step_kernel(variant)
-        ctx.step(4)
-        ctx.synchronize()
-        a0 = time.perf_counter()
-        ctx.step(20)
-        ctx.synchronize()
-        tune[vname] = (time.perf_counter() - a0) * 1e3 / 20
-        log(f"variant {vname}: {tune[vname]:.4f} ms/step")
-    best = min(tune, key=tune.get)
-    ctx.set_step_kernel({"split_block": sph.STEP_SPLIT_BLOCK, "fused_block": sph.STEP_FUSED_BLOCK,
-                         "particle": sph.STEP_PARTICLE}[best])
+    for kname, kv in kernels.items():
+        for rname, rv in resorts.items():
+            ctx.set_step_kernel(kv)
+            ctx.set_resort(rv)
+            ctx.step(4)
+            ctx.synchronize()
+            a0 = time.perf_counter()
+            ctx.step(20)
+            ctx.synchronize()
+            tune[f"{kname}/{rname}"] = (time.perf_counter() - a0) * 1e3 / 20
+            log(f"variant {kname}/{rname}: {tune[f'{kname}/{rname}']:.4f} ms/step")
+    best_combo = min(tune, key=tune.get)
+    best, best_resort = best_combo.split("/")
+    ctx.set_step_kernel(kernels[best])
+    ctx.set_resort(resorts[best_resort])
     bstats = ctx.block_stats()
     log(f"block stats {bstats}")
 
@@ -143,6 +160,7 @@ def main():
     graph_ms = (g1 - g0) * 1e3 / args.steps
     log(f"graph replay: {graph_ms:.4f} ms/step")
 
+    mfrac = ctx.movers() / n
     k = ctx.build_neighbor_lists(False) / n
     st, _ = ctx.cell_ranges()
     occ = float((st >= 0).sum()) / n
@@ -158,7 +176,7 @@ def main():
         entry = {"launches": launches, "ms_total": round(ms, 4),
                  "us_per_launch": round(per_launch_ms * 1e3, 3)}
         try:
-            b = alg_bytes_per_particle(name, k, occ, passes, best) * n
+            b = alg_bytes_per_particle(name, k, occ, passes, best, best_resort, mfrac) * n
             entry["alg_bytes_per_launch"] = int(b)
             entry["achieved_GBs"] = round(b / (per_launch_ms * 1e-3) / 1e9, 1)
         except KeyError:
@@ -169,7 +187,7 @@ def main():
                    key=lambda s: stages[s]["ms_total"])
     dom = stages[dom_name]
     achieved = dom["achieved_GBs"]
-    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes, best)
+    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes, best, best_resort, mfrac)
                      * stages[s]["launches"] / args.steps
                      for s in stages if "alg_bytes_per_launch" in stages[s])
     out = {
@@ -188,7 +206,8 @@ def main():
         "config": {"workload": scene.name, "particles": n, "box": list(scene.params.hi),
                    "h": scene.params.h, "dt": scene.params.dt, "physics": "wcsph",
                    "mean_neighbors": round(k, 3), "occupied_cells_per_particle": round(occ, 4),
-                   "radix_passes": passes, "step_kernel": best,
+                   "radix_passes": passes, "step_kernel": best, "resort": best_resort,
+                   "movers_last_step": ctx.movers(),
                    "step_kernel_ms": {k2: round(v, 4) for k2, v in tune.items()},
                    "block_stats": bstats},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,

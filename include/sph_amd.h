@@ -44,6 +44,11 @@ enum sph_step_kernel {
   SPH_STEP_FUSED_BLOCK = 1, /* one kernel, two-cell halo, densities never leave LDS */
   SPH_STEP_PARTICLE = 2     /* one thread per particle, fixed-width neighbor list */
 };
+/* Re-sort inside sph_step. FULL: radix sort of every key, then the permutation.
+ * INCREMENTAL: only particles whose cell key changed are radix-sorted, then merged
+ * with the unchanged ones (already in order); the result is the same stable
+ * order by (key, previous index). */
+enum sph_resort { SPH_RESORT_FULL = 0, SPH_RESORT_INCREMENTAL = 1 };
 
 /* Simulation parameters. Reference physics (SPH_PHYS_REFERENCE) reproduces the
  * macros of core/ConfigVariables.hpp:9-15 when filled with h=1, eos_k=30,
@@ -177,6 +182,10 @@ int sph_set_graph(void* ctx, int32_t enable);
 int sph_ell_overflow(void* ctx, int64_t* count);
 /* Choose the physics kernels of sph_step (enum sph_step_kernel). */
 int sph_set_step_kernel(void* ctx, int32_t variant);
+/* How sph_step re-sorts (enum sph_resort); both give the same permutation. */
+int sph_set_resort(void* ctx, int32_t mode);
+/* Particles whose cell key changed in the last incremental re-sort. Synchronises. */
+int sph_movers(void* ctx, int64_t* count);
 /* Re-evaluate the block kernels on the current state with counters on:
  * out[0..2] density (or fused) kernel: occupied blocks, blocks over the LDS
  * capacity, staged particles; out[3..5] the same for the force kernel. */

@@ -21,6 +21,7 @@ PHYS_REFERENCE, PHYS_WCSPH = 0, 1
 BC_WALLS, BC_PERIODIC = 0, 1
 COMPUTE_SORTED, COMPUTE_LISTS = 0, 1
 STEP_SPLIT_BLOCK, STEP_FUSED_BLOCK, STEP_PARTICLE = 0, 1, 2
+RESORT_FULL, RESORT_INCREMENTAL = 0, 1
 STAGES = ("integration", "neighbor_search", "neighbor_list", "approximate_density",
           "force_computation", "sort_histogram", "sort_pass", "block_step", "measure")
 
@@ -134,6 +135,8 @@ def lib():
         "sph_set_graph": (i32, [vp, i32]),
         "sph_ell_overflow": (i32, [vp, P(i64)]),
         "sph_set_step_kernel": (i32, [vp, i32]),
+        "sph_set_resort": (i32, [vp, i32]),
+        "sph_movers": (i32, [vp, P(i64)]),
         "sph_block_stats": (i32, [vp, P(i64)]),
         "sph_get_sorted_keys": (i32, [vp, vp]),
         "sph_get_permutation": (i32, [vp, vp]),
@@ -158,7 +161,7 @@ EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_
             "sph_find_neighbors", "sph_build_neighbor_lists", "sph_density_pressure",
             "sph_forces", "sph_integrate", "sph_update_velocities", "sph_update_positions",
             "sph_set_dt", "sph_measure", "sph_prepare", "sph_step",
-            "sph_set_graph", "sph_ell_overflow", "sph_set_step_kernel", "sph_block_stats", "sph_get_sorted_keys", "sph_get_permutation",
+            "sph_set_graph", "sph_ell_overflow", "sph_set_step_kernel", "sph_set_resort", "sph_movers", "sph_block_stats", "sph_get_sorted_keys", "sph_get_permutation",
             "sph_get_cell_ranges", "sph_get_neighbor_lists", "sph_get_ell",
             "sph_get_device_view", "sph_set_timing", "sph_stage_times", "sph_synchronize")
 
@@ -294,6 +297,14 @@ class Context:
 
     def set_step_kernel(self, variant: int):
         self._chk(self._L.sph_set_step_kernel(self._h, int(variant)), "sph_set_step_kernel")
+
+    def set_resort(self, mode: int):
+        self._chk(self._L.sph_set_resort(self._h, int(mode)), "sph_set_resort")
+
+    def movers(self) -> int:
+        c = C.c_int64()
+        self._chk(self._L.sph_movers(self._h, C.byref(c)), "sph_movers")
+        return int(c.value)
 
     def block_stats(self):
         out = np.zeros(6, np.int64)

@@ -54,9 +54,11 @@ struct Ctx {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int graph_n = -1;
-  const void* graph_sig[9] = {};  // buffer assignment the graph was captured with
+  const void* graph_sig[10] = {};  // buffer assignment the graph was captured with
   int block_grid = 0;             // persistent workgroups of the block kernels
-  int step_kernel = SPH_STEP_SPLIT_BLOCK;
+  int step_kernel = SPH_STEP_PARTICLE;
+  int resort = SPH_RESORT_INCREMENTAL;  // how sph_step re-sorts
+  ResortBuffers rb{};
   unsigned long long* block_stats = nullptr;
 };
 
@@ -64,16 +66,16 @@ struct Ctx {
 // these pointers, so a captured graph is only valid for the assignment it saw.
 void graph_signature(const Ctx* c, const void** sig) {
   const DevState& s = c->s;
-  const void* v[9] = {s.rx, s.tx, s.vx, s.tvx, s.ax, s.tax, s.pid, s.pid_alt,
-                      reinterpret_cast<const void*>((intptr_t)s.n)};
-  for (int i = 0; i < 9; ++i) sig[i] = v[i];
+  const void* v[10] = {s.rx, s.tx, s.vx, s.tvx, s.ax, s.tax, s.pid, s.pid_alt,
+                       s.keys_sorted, reinterpret_cast<const void*>((intptr_t)s.n)};
+  for (int i = 0; i < 10; ++i) sig[i] = v[i];
 }
 
 bool graph_matches(const Ctx* c) {
   if (!c->gexec) return false;
-  const void* sig[9];
+  const void* sig[10];
   graph_signature(c, sig);
-  for (int i = 0; i < 9; ++i)
+  for (int i = 0; i < 10; ++i)
     if (sig[i] != c->graph_sig[i]) return false;
   return true;
 }
@@ -325,6 +327,48 @@ int find_neighbors(Ctx* c) {
   return SPH_OK;
 }
 
+// Re-sort of the integrated state T (keys in sb.keys_in) against the previous
+// sorted order: movers -> radix sort of the movers -> merge/scatter T -> S ->
+// ranges. Same permutation as sort_keys + gather.
+int incremental_resort(Ctx* c) {
+  DevState& s = c->s;
+  ResortBuffers& r = c->rb;
+  {
+    StageTimer t(c, SPH_T_SORT_HIST);
+    HIPCHK(c, launch_movers(c->L, s, r, c->sb.keys_in));
+  }
+  SortBuffers mb = c->sb;
+  mb.n_dev = r.n_movers;
+  mb.vals_in = r.midx;
+  mb.keys_in = r.mkey;
+  mb.keys_tmp = r.ktmp;
+  mb.keys_out = r.skey;
+  mb.vals_out = r.sidx;
+  mb.clear_n = 0;
+  mb.block_bits = 0;
+  {
+    StageTimer t(c, SPH_T_SORT_HIST);
+    HIPCHK(c, radix_sort_hist(mb, s.n, c->grid.key_bits, c->stream));
+  }
+  for (int p = 0; p < sort_passes(c->grid.key_bits); ++p) {
+    StageTimer t(c, SPH_T_SORT_PASS);
+    HIPCHK(c, radix_sort_pass(mb, s.n, c->grid.key_bits, p, c->stream));
+  }
+  {
+    StageTimer t(c, SPH_T_NEIGHBOR_SEARCH);
+    HIPCHK(c, launch_merge(c->L, s, r, r.keys_alt));
+  }
+  std::swap(s.keys_sorted, r.keys_alt);
+  c->sb.keys_out = s.keys_sorted;
+  std::swap(s.pid, s.pid_alt);
+  {
+    StageTimer t(c, SPH_T_NEIGHBOR_SEARCH);
+    HIPCHK(c, launch_ranges(c->L, s, hist_words(c)));
+  }
+  s.n_prev = s.n;
+  return SPH_OK;
+}
+
 // One fused iteration: forces(ELL) + integrate + keys -> sort -> gather T->S
 // + ranges -> density + EOS + ELL.
 int fused_iteration(Ctx* c) {
@@ -352,6 +396,7 @@ int fused_iteration(Ctx* c) {
     StageTimer t(c, SPH_T_FORCE);
     HIPCHK(c, launch_forces(c->L, s, NB_ELL, true, c->sb.keys_in));
   }
+  if (c->resort == SPH_RESORT_INCREMENTAL) return incremental_resort(c);
   int rc = sort_keys(c);
   if (rc) return rc;
   {
@@ -499,6 +544,17 @@ int sph_create(const sph_params* params, int64_t n_max, int device, void** out_c
                             : 0;
   if (s.block_bits && (rc = dev_alloc(c, &s.blocks, s.n_blocks))) return bail(rc);
   if ((rc = dev_alloc(c, &c->block_stats, 6))) return bail(rc);
+  {
+    ResortBuffers& r = c->rb;
+    if ((rc = dev_alloc(c, &r.mkey, N)) || (rc = dev_alloc(c, &r.midx, N)) ||
+        (rc = dev_alloc(c, &r.skey, N)) || (rc = dev_alloc(c, &r.sidx, N)) ||
+        (rc = dev_alloc(c, &r.ktmp, N)) || (rc = dev_alloc(c, &r.keys_alt, N)) ||
+        (rc = dev_alloc(c, &r.flags, resort_flag_words((int)N))) ||
+        (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
+        (rc = dev_alloc(c, &r.lookback, resort_tiles((int)N))) ||
+        (rc = dev_alloc(c, &r.ticket, 1)) || (rc = dev_alloc(c, &r.n_movers, 1)))
+      return bail(rc);
+  }
   b.blocks = s.blocks;
   b.block_bits = s.block_bits;
   int cu = 256;
@@ -516,6 +572,10 @@ int sph_create(const sph_params* params, int64_t n_max, int device, void** out_c
     return bail(rc);
   e = hipMemsetAsync(b.hist, 0, kMaxPasses * 256 * sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(b.status, 0, sizeof(uint32_t), c->stream);
+  c->rb.status = b.status;
+  if (e == hipSuccess)
+    e = hipMemsetAsync(c->rb.lookback, 0, resort_tiles((int)N) * sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->rb.ticket, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, c->grid.n_keys);
   if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -833,6 +893,26 @@ int sph_set_step_kernel(void* ctx, int32_t variant) {
     return fail(c, SPH_ERR_INVALID, "unknown step kernel");
   c->step_kernel = variant;
   destroy_graph(c);
+  return SPH_OK;
+}
+
+int sph_set_resort(void* ctx, int32_t mode) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  if (mode != SPH_RESORT_FULL && mode != SPH_RESORT_INCREMENTAL)
+    return fail(c, SPH_ERR_INVALID, "unknown re-sort mode");
+  c->resort = mode;
+  destroy_graph(c);
+  return SPH_OK;
+}
+
+int sph_movers(void* ctx, int64_t* count) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !count) return SPH_ERR_INVALID;
+  int m = 0;
+  HIPCHK(c, hipMemcpyAsync(&m, c->rb.n_movers, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *count = m;
   return SPH_OK;
 }
 

@@ -13,6 +13,10 @@ namespace sphamd {
 constexpr int kMaxPasses = 8;  // 64-bit keys, 8-bit digits
 
 struct SortBuffers {
+  // n passed to the launchers sizes the grids; when n_dev is set the kernels
+  // sort the first *n_dev elements (a count produced on the device)
+  const int* n_dev = nullptr;
+  const uint32_t* vals_in = nullptr;  // first-pass values; null -> 0..n-1
   uint64_t* keys_in;   // unsorted keys (clobbered when passes >= 3)
   uint64_t* keys_tmp;
   uint64_t* keys_out;  // sorted keys
@@ -73,6 +77,22 @@ struct DevState {
 
 enum { NB_STENCIL = 0, NB_CSR = 1, NB_ELL = 2 };
 
+// Buffers of the incremental re-sort (sph_resort.hip).
+struct ResortBuffers {
+  uint64_t* mkey;                  // movers' new keys, previous-index order
+  uint32_t* midx;                  // movers' previous indices
+  uint64_t* skey;                  // movers sorted by (key, previous index)
+  uint32_t* sidx;
+  uint64_t* ktmp;                  // sort scratch
+  unsigned long long* flags;       // mover bit per particle, one word per wave
+  uint32_t* tile_base;             // movers before each 256-particle tile
+  uint32_t* lookback;              // per tile, cleared by k_merge
+  uint32_t* ticket;
+  int* n_movers;
+  uint32_t* status;
+  uint64_t* keys_alt;              // second sorted-key buffer (ping-pong)
+};
+
 struct Launch {
   hipStream_t stream;
   DevGrid grid;
@@ -98,6 +118,15 @@ hipError_t launch_integrate(const Launch& L, const DevState& s);
 hipError_t launch_block_step(const Launch& L, const DevState& s, uint64_t* keys_out, int grid,
                              int variant, unsigned long long* stats, int part = 0);
 int block_step_grid(int64_t n_blocks, int cus);
+// incremental re-sort: movers -> (sort movers) -> merge -> ranges
+int64_t resort_flag_words(int n);
+int64_t resort_tiles(int n);
+hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers& r,
+                         const uint64_t* knew);
+hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
+                        uint64_t* keys_dst);
+// cell and block ranges of the sorted keys s.keys_sorted; clears the histogram
+hipError_t launch_ranges(const Launch& L, const DevState& s, int hist_words);
 hipError_t launch_update_velocities(const Launch& L, const DevState& s);
 hipError_t launch_update_positions(const Launch& L, const DevState& s);
 hipError_t launch_csr_count(const Launch& L, const DevState& s, bool self);

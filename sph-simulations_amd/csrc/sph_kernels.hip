@@ -31,24 +31,14 @@ __global__ __launch_bounds__(BLK) void k_keys(int n, DevGrid g, const float* __r
                      cell_coord(ry[i], g.lo_y, g.cs_y, g.ny));
 }
 
-// Permutation through the sort result plus cell ranges (cuSetCellRanges,
-// cuFindNeighbors.cu:103-122): run starts are found with one wave64 ballot of
-// "key differs from the previous key"; a run ends where the next one starts, so
-// the end mask is the start mask shifted by one lane (the last lane compares
-// with the next wave's first key). Also clears the sort's digit histogram.
-__global__ __launch_bounds__(BLK) void k_gather(
-    int n, const uint32_t* __restrict__ perm, const uint64_t* __restrict__ keys,
-    const float* __restrict__ srx, const float* __restrict__ sry,
-    const float* __restrict__ svx, const float* __restrict__ svy,
-    const uint32_t* __restrict__ spid, const float* __restrict__ sax,
-    const float* __restrict__ say, float* __restrict__ drx, float* __restrict__ dry,
-    float* __restrict__ dvx, float* __restrict__ dvy, uint32_t* __restrict__ dpid,
-    float* __restrict__ dax, float* __restrict__ day, int2* __restrict__ cells,
-    int64_t n_keys, uint32_t* __restrict__ hist, int hist_words, int2* __restrict__ blocks,
-    int block_bits) {
-  const int i = blockIdx.x * BLK + threadIdx.x;
-  if (blockIdx.x == 0)
-    for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
+// Cell start/end of sorted index i (cuSetCellRanges, cuFindNeighbors.cu:103-122):
+// run starts are found with one wave64 ballot of "key differs from the previous
+// key"; a run ends where the next one starts, so the end mask is the start mask
+// shifted by one lane (the last lane compares with the next wave's first key).
+// The same is done one level up for Morton blocks. Returns i < n.
+__device__ __forceinline__ bool write_ranges(int i, int n, const uint64_t* __restrict__ keys,
+                                             int2* __restrict__ cells, int64_t n_keys,
+                                             int2* __restrict__ blocks, int block_bits) {
   const bool live = i < n;
   uint64_t k = live ? keys[i] : ~0ull;
   const int lane = threadIdx.x & (kWave - 1);
@@ -78,11 +68,30 @@ __global__ __launch_bounds__(BLK) void k_gather(
       if (b_end) blocks[bk].y = i + 1;
     }
   }
-  if (!live) return;
   if (k < (uint64_t)n_keys) {
     if (is_start) cells[k].x = i;
     if (is_end) cells[k].y = i + 1;
   }
+  return live;
+}
+
+// Permutation through the sort result plus cell ranges. Also clears the sort's
+// digit histogram.
+__global__ __launch_bounds__(BLK) void k_gather(
+    int n, const uint32_t* __restrict__ perm, const uint64_t* __restrict__ keys,
+    const float* __restrict__ srx, const float* __restrict__ sry,
+    const float* __restrict__ svx, const float* __restrict__ svy,
+    const uint32_t* __restrict__ spid, const float* __restrict__ sax,
+    const float* __restrict__ say, float* __restrict__ drx, float* __restrict__ dry,
+    float* __restrict__ dvx, float* __restrict__ dvy, uint32_t* __restrict__ dpid,
+    float* __restrict__ dax, float* __restrict__ day, int2* __restrict__ cells,
+    int64_t n_keys, uint32_t* __restrict__ hist, int hist_words, int2* __restrict__ blocks,
+    int block_bits) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (blockIdx.x == 0)
+    for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
+  const bool live = write_ranges(i, n, keys, cells, n_keys, blocks, block_bits);
+  if (!live) return;
   const uint32_t j = min(perm[i], (uint32_t)(n - 1));  // in bounds even if perm were bad
   drx[i] = srx[j];
   dry[i] = sry[j];
@@ -93,6 +102,18 @@ __global__ __launch_bounds__(BLK) void k_gather(
     dax[i] = sax[j];
     day[i] = say[j];
   }
+}
+
+// Cell and block ranges of an order produced by the incremental re-sort; clears
+// the digit histogram its mover sort used.
+__global__ __launch_bounds__(BLK) void k_ranges(int n, const uint64_t* __restrict__ keys,
+                                                int2* __restrict__ cells, int64_t n_keys,
+                                                int2* __restrict__ blocks, int block_bits,
+                                                uint32_t* __restrict__ hist, int hist_words) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (blockIdx.x == 0)
+    for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
+  write_ranges(i, n, keys, cells, n_keys, blocks, block_bits);
 }
 
 // Summation density + EOS. NB: stencil traversal, CSR rows, or stencil with the
@@ -441,6 +462,13 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
         s.n, s.perm, s.keys_sorted, src_rx, src_ry, src_vx, src_vy, src_pid, src_ax, src_ay,
         dst_rx, dst_ry, dst_vx, dst_vy, dst_pid, dst_ax, dst_ay, s.cells, s.n_keys, s.hist,
         hist_words, s.blocks, s.block_bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_ranges(const Launch& L, const DevState& s, int hist_words) {
+  if (s.n > 0)
+    k_ranges<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, s.keys_sorted, s.cells, s.n_keys,
+                                                    s.blocks, s.block_bits, s.hist, hist_words);
   return hipGetLastError();
 }
 

@@ -67,16 +67,18 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wsum
 // passes about to run, the tile tickets, and the cell ranges of the previous
 // sort (read from its sorted keys before the last pass overwrites them). The
 // global histogram must be zero on entry (the permutation kernel clears it).
-__global__ __launch_bounds__(WG) void k_hist(const uint64_t* __restrict__ keys, int n, int passes,
+__global__ __launch_bounds__(WG) void k_hist(const uint64_t* __restrict__ keys, int n,
+                                             const int* __restrict__ n_dev, int passes,
                                              uint32_t* __restrict__ hist,
                                              uint32_t* __restrict__ lookback,
-                                             uint32_t lookback_words,
+                                             uint32_t lookback_stride,
                                              uint32_t* __restrict__ tickets,
                                              const uint64_t* __restrict__ old_sorted,
                                              int old_n, int2* __restrict__ cells,
                                              uint64_t n_keys, int2* __restrict__ blocks,
                                              int block_bits) {
   __shared__ uint32_t s_h[kMaxPasses * RADIX];
+  if (n_dev) n = *n_dev;  // count known on the device only (incremental re-sort)
   for (int i = threadIdx.x; i < passes * RADIX; i += WG) s_h[i] = 0;
   __syncthreads();
   const int gtid = blockIdx.x * WG + threadIdx.x, gsize = gridDim.x * WG;
@@ -114,12 +116,16 @@ __global__ __launch_bounds__(WG) void k_hist(const uint64_t* __restrict__ keys, 
   __syncthreads();
   for (int i = threadIdx.x; i < passes * RADIX; i += WG)
     if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
-  uint4* lb = reinterpret_cast<uint4*>(lookback);
-  for (uint32_t i = gtid; i < lookback_words / 4; i += gsize) lb[i] = make_uint4(0, 0, 0, 0);
+  const uint32_t used = (uint32_t)((n + TILE - 1) / TILE) * RADIX;  // words per pass in use
+  for (int p = 0; p < passes; ++p) {
+    uint4* lb = reinterpret_cast<uint4*>(lookback + (size_t)p * lookback_stride);
+    for (uint32_t i = gtid; i < used / 4; i += gsize) lb[i] = make_uint4(0, 0, 0, 0);
+  }
   if (blockIdx.x == 0 && threadIdx.x < kMaxPasses) tickets[threadIdx.x] = 0;
 }
 
-__global__ __launch_bounds__(WG) void k_onesweep(int n, int shift, int pass,
+__global__ __launch_bounds__(WG) void k_onesweep(int n, const int* __restrict__ n_dev,
+                                                 int shift, int pass,
                                                  const uint64_t* __restrict__ kin,
                                                  const uint32_t* __restrict__ vin,
                                                  uint64_t* __restrict__ kout,
@@ -141,6 +147,8 @@ __global__ __launch_bounds__(WG) void k_onesweep(int n, int shift, int pass,
   for (int i = tid; i < WAVES * RADIX; i += WG) s_whist[i] = 0;
   __syncthreads();
   const uint32_t tile = s_tile;
+  if (n_dev) n = *n_dev;
+  if ((int64_t)tile * TILE >= n) return;  // more workgroups than tiles (device-side count)
   const int base = (int)tile * TILE + wave * (kWave * IPT);
 
   uint64_t key[IPT];
@@ -264,11 +272,10 @@ hipError_t radix_sort_hist(const SortBuffers& b, int n, int key_bits, hipStream_
   if (passes > kMaxPasses || tiles > b.max_tiles) return hipErrorInvalidValue;
   const int hist_chunks = (n + HIST_IPT - 1) / HIST_IPT;
   const int hist_blocks = max(1, min(b.hist_blocks, (hist_chunks + WG - 1) / WG));
-  // clears `passes` look-back arrays laid out back to back
-  k_hist<<<hist_blocks, WG, 0, stream>>>(b.keys_in, n, passes, b.hist, b.lookback,
-                                         lookback_words_per_pass(n) * passes, b.tickets,
-                                         b.keys_out, b.clear_n, b.cells, b.n_keys, b.blocks,
-                                         b.block_bits);
+  // `passes` look-back arrays laid out back to back with stride lookback_words_per_pass(n)
+  k_hist<<<hist_blocks, WG, 0, stream>>>(b.keys_in, n, b.n_dev, passes, b.hist, b.lookback,
+                                         lookback_words_per_pass(n), b.tickets, b.keys_out,
+                                         b.clear_n, b.cells, b.n_keys, b.blocks, b.block_bits);
   return hipGetLastError();
 }
 
@@ -280,9 +287,9 @@ hipError_t radix_sort_pass(const SortBuffers& b, int n, int key_bits, int p,
   const uint32_t lb_words = lookback_words_per_pass(n);
   const uint64_t* kin = p == 0 ? b.keys_in : ((p & 1) ? b.keys_tmp : b.keys_in);
   uint64_t* kout = p == passes - 1 ? b.keys_out : ((p & 1) ? b.keys_in : b.keys_tmp);
-  const uint32_t* vin = p == 0 ? nullptr : ((p & 1) ? b.vals_a : b.vals_b);
+  const uint32_t* vin = p == 0 ? b.vals_in : ((p & 1) ? b.vals_a : b.vals_b);
   uint32_t* vout = p == passes - 1 ? b.vals_out : ((p & 1) ? b.vals_b : b.vals_a);
-  k_onesweep<<<tiles, WG, 0, stream>>>(n, p * RB, p, kin, vin, kout, vout, b.hist,
+  k_onesweep<<<tiles, WG, 0, stream>>>(n, b.n_dev, p * RB, p, kin, vin, kout, vout, b.hist,
                                        b.lookback + (size_t)p * lb_words, b.tickets,
                                        b.status);
   return hipGetLastError();

@@ -193,20 +193,23 @@ def test_ell_overflow_fallback(sph, orc):
         assert np.abs(f["ay"] - ay).max() <= 1e-5 * amax
 
 
+@pytest.mark.parametrize("resort", [0, 1])
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("name,steps", [("reference_example", 50), ("reference_rowmajor", 20),
                                         ("dam_break_16k", 40), ("periodic_9k", 40),
                                         ("random_clusters", 20), ("dense_cluster", 3)])
-def test_multistep_run_bit_exact_fp32(sph, orc, all_scenes, name, steps, variant):
+def test_multistep_run_bit_exact_fp32(sph, orc, all_scenes, name, steps, variant, resort):
     """The whole pipeline, `steps` iterations, with each step-kernel variant
-    (split blocks, fused blocks, per particle): identical sort permutations every
-    step (same particle ids in the same order) and bit-identical FP32 state."""
+    (split blocks, fused blocks, per particle) and each re-sort (full radix,
+    incremental): identical sort permutations every step (same particle ids in
+    the same order) and bit-identical FP32 state."""
     s = all_scenes[name]
-    note(f"[gpu] run {name} steps={steps} step_kernel={variant}")
+    note(f"[gpu] run {name} steps={steps} step_kernel={variant} resort={resort}")
     o32 = orc.run(s.params, s.rx, s.ry, s.vx, s.vy, steps)
     with sph.Context(s.params, s.n) as ctx:
         ctx.upload(s.rx, s.ry, s.vx, s.vy)
         ctx.set_step_kernel(variant)
+        ctx.set_resort(resort)
         ctx.run(steps)
         d = ctx.download()
         f = ctx.fields()
@@ -247,6 +250,32 @@ def test_multistep_run_vs_fp64(sph, orc, all_scenes, name, steps):
     assert perr <= FP64_POS_TOL_H
     assert rerr <= FP64_RHO_RTOL
     assert aerr <= FP64_ACC_TOL
+
+
+@pytest.mark.parametrize("name", ["dam_break_16k", "random_clusters", "periodic_9k"])
+def test_incremental_resort_movers_and_permutation(sph, orc, all_scenes, name):
+    """One step with the incremental re-sort: the mover count equals the number of
+    particles whose cell key changed, and the permutation equals the oracle's
+    stable sort of the integrated keys."""
+    s = all_scenes[name]
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.prepare()
+        before = ctx.download()
+        k_before = ctx.sorted_keys()
+        ctx.step(1)
+        after = ctx.download()
+        perm = ctx.permutation()
+        m = ctx.movers()
+    # integrated positions in the previous order: after[...] through perm
+    x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
+    x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
+    k_new = orc.keys(s.params, x_prev, y_prev)
+    assert m == int(np.sum(k_new != k_before))
+    operm, _ = orc.stable_sort(k_new)
+    assert np.array_equal(perm, operm)
+    assert np.array_equal(after["pid"], before["pid"][perm])
+    note(f"[gpu] {name}: {m} movers of {s.n}")
 
 
 def test_block_stats_counts_capacity_fallbacks(sph, all_scenes):
