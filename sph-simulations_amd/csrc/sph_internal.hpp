@@ -17,6 +17,7 @@ struct SortBuffers {
   // sort the first *n_dev elements (a count produced on the device)
   const int* n_dev = nullptr;
   const uint32_t* vals_in = nullptr;  // first-pass values; null -> 0..n-1
+  int pool = 64;                      // persistent workgroups when n_dev is set
   uint64_t* keys_in;   // unsorted keys (clobbered when passes >= 3)
   uint64_t* keys_tmp;
   uint64_t* keys_out;  // sorted keys

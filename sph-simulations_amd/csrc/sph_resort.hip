@@ -72,39 +72,49 @@ __global__ __launch_bounds__(TW) void k_movers(
     flags[(size_t)tile * (TW / kWave) + wave] = bal;
   }
   __syncthreads();
-  if (tid == 0) {
+  // Tile prefix by a windowed look-back: wave 0 reads the 64 closest predecessor
+  // words at once, sums back to the nearest inclusive prefix, and only re-polls
+  // while a predecessor inside that span has not published yet.
+  if (wave == 0) {
     uint32_t cnt = 0;
     for (int w = 0; w < TW / kWave; ++w) cnt += s_wcnt[w];
     uint32_t excl = 0;
     if (tile == 0) {
-      st_agent(&lookback[0], FLAG_PREFIX | cnt);
+      if (lane == 0) st_agent(&lookback[0], FLAG_PREFIX | cnt);
     } else {
-      st_agent(&lookback[tile], FLAG_AGG | cnt);
-      int t = (int)tile - 1;
+      if (lane == 0) st_agent(&lookback[tile], FLAG_AGG | cnt);
+      int top = (int)tile - 1;
       uint32_t polls = 0;
-      while (t >= 0) {
-        const uint32_t w = ld_agent(&lookback[t]);
+      while (true) {
+        const int t = top - lane;
+        const uint32_t w = t >= 0 ? ld_agent(&lookback[t]) : FLAG_PREFIX;
         const uint32_t f = w & ~COUNT_MASK;
-        if (f == 0) {
+        const unsigned long long pm = __ballot(f == FLAG_PREFIX);
+        const unsigned long long zm = __ballot(f == 0);
+        const int first_p = pm ? __ffsll((long long)pm) - 1 : kWave;
+        const int first_z = zm ? __ffsll((long long)zm) - 1 : kWave;
+        if (first_z < first_p) {
           if (++polls < MAX_POLLS) {
             __builtin_amdgcn_s_sleep(1);
             continue;
           }
-          atomicOr(status, 2u);
-          polls = 0;
-          --t;
-          continue;
+          if (lane == 0) atomicOr(status, 2u);  // a predecessor never published
+          break;
         }
-        polls = 0;
-        excl += w & COUNT_MASK;
-        if (f == FLAG_PREFIX) break;
-        --t;
+        uint32_t v = lane <= first_p ? (w & COUNT_MASK) : 0u;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        excl += v;
+        if (first_p < kWave) break;
+        top -= kWave;
       }
-      st_agent(&lookback[tile], FLAG_PREFIX | (excl + cnt));
+      if (lane == 0) st_agent(&lookback[tile], FLAG_PREFIX | (excl + cnt));
     }
-    s_base = excl;
-    tile_base[tile] = excl;
-    if (tile == gridDim.x - 1) *n_movers = (int)(excl + cnt);
+    if (lane == 0) {
+      s_base = excl;
+      tile_base[tile] = excl;
+      if (tile == gridDim.x - 1) *n_movers = (int)(excl + cnt);
+    }
   }
   __syncthreads();
   if (mv) {

@@ -143,12 +143,15 @@ __global__ __launch_bounds__(WG) void k_onesweep(int n, const int* __restrict__ 
   __shared__ uint32_t s_tile;
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  if (n_dev) n = *n_dev;
+  // With a device-side count the grid is a small persistent pool that keeps taking
+  // tickets until every tile is done; otherwise one tile per workgroup.
+  for (;;) {
   if (tid == 0) s_tile = atomicAdd(&tickets[pass], 1u);
   for (int i = tid; i < WAVES * RADIX; i += WG) s_whist[i] = 0;
   __syncthreads();
   const uint32_t tile = s_tile;
-  if (n_dev) n = *n_dev;
-  if ((int64_t)tile * TILE >= n) return;  // more workgroups than tiles (device-side count)
+  if ((int64_t)tile * TILE >= n) return;
   const int base = (int)tile * TILE + wave * (kWave * IPT);
 
   uint64_t key[IPT];
@@ -253,6 +256,9 @@ __global__ __launch_bounds__(WG) void k_onesweep(int n, const int* __restrict__ 
       vout[g] = s_vals[k];
     }
   }
+  if (!n_dev) return;
+  __syncthreads();  // LDS reused by the next tile
+  }
 }
 
 }  // namespace
@@ -271,7 +277,8 @@ hipError_t radix_sort_hist(const SortBuffers& b, int n, int key_bits, hipStream_
   const int tiles = (n + TILE - 1) / TILE;
   if (passes > kMaxPasses || tiles > b.max_tiles) return hipErrorInvalidValue;
   const int hist_chunks = (n + HIST_IPT - 1) / HIST_IPT;
-  const int hist_blocks = max(1, min(b.hist_blocks, (hist_chunks + WG - 1) / WG));
+  const int hist_blocks =
+      max(1, min(b.n_dev ? b.pool : b.hist_blocks, (hist_chunks + WG - 1) / WG));
   // `passes` look-back arrays laid out back to back with stride lookback_words_per_pass(n)
   k_hist<<<hist_blocks, WG, 0, stream>>>(b.keys_in, n, b.n_dev, passes, b.hist, b.lookback,
                                          lookback_words_per_pass(n), b.tickets, b.keys_out,
@@ -289,7 +296,8 @@ hipError_t radix_sort_pass(const SortBuffers& b, int n, int key_bits, int p,
   uint64_t* kout = p == passes - 1 ? b.keys_out : ((p & 1) ? b.keys_in : b.keys_tmp);
   const uint32_t* vin = p == 0 ? b.vals_in : ((p & 1) ? b.vals_a : b.vals_b);
   uint32_t* vout = p == passes - 1 ? b.vals_out : ((p & 1) ? b.vals_b : b.vals_a);
-  k_onesweep<<<tiles, WG, 0, stream>>>(n, b.n_dev, p * RB, p, kin, vin, kout, vout, b.hist,
+  const int grid = b.n_dev ? min(tiles, b.pool) : tiles;
+  k_onesweep<<<grid, WG, 0, stream>>>(n, b.n_dev, p * RB, p, kin, vin, kout, vout, b.hist,
                                        b.lookback + (size_t)p * lb_words, b.tickets,
                                        b.status);
   return hipGetLastError();
