@@ -7,9 +7,9 @@
 // unchanged keys. The new order is the stable merge of the two sorted sequences
 // by (key, previous index) -- exactly the order a stable sort of all keys gives.
 //
-//   k_movers  flag movers, compact them in index order (decoupled look-back over
-//             256-particle tiles), record per-tile mover counts, and clear the
-//             cell / block ranges of the previous order
+//   k_movers  flag movers, compact them in index order (windowed decoupled
+//             look-back over 256-particle tiles), record per-tile mover counts,
+//             and clear the cell / block ranges of the previous order
 //   (radix sort of the movers, element count read on the device)
 //   k_merge   final slot of every particle: stayer j -> (stayers before j) +
 //             (movers ordered before it); mover r -> r + (stayers ordered before
@@ -49,12 +49,14 @@ __global__ __launch_bounds__(TW) void k_movers(
     unsigned long long* __restrict__ flags, uint32_t* __restrict__ tile_base,
     uint32_t* __restrict__ lookback, uint32_t* __restrict__ ticket, int* __restrict__ n_movers,
     uint32_t* __restrict__ status) {
-  __shared__ uint32_t s_tile, s_base;
+  __shared__ uint32_t s_base;
   __shared__ uint32_t s_wcnt[TW / kWave];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
+  // Tiles in workgroup-id order: a 1-D grid is dispatched in id order on each
+  // XCD, so the lowest unfinished tile is always resident and every look-back
+  // wait ends (a same-address ticket counter would serialise 4096 atomics).
+  const uint32_t tile = blockIdx.x;
+  (void)ticket;
   const int i = (int)tile * TW + tid;
   const bool live = i < n;
   const uint64_t kn = live ? knew[i] : 0, ko = live ? kold[i] : 0;

@@ -1,9 +1,10 @@
 // Stable LSD radix sort of (u64 key, u32 value) pairs for gfx950: one digit
 // histogram pass over all digits, then one "onesweep" kernel per 8-bit digit.
 //
-// Onesweep pass (decoupled look-back): each workgroup takes a tile ticket from
-// an atomic counter (tickets are handed out in start order, so every tile it
-// waits on is already resident), ranks its 2048 keys stably in LDS with wave64
+// Onesweep pass (decoupled look-back): each workgroup takes the tile of its id
+// (workgroups are dispatched in id order on each XCD, so every tile it waits on
+// is resident or done; a pool sorting a device-side count takes tiles from an
+// atomic ticket instead), ranks its 2048 keys stably in LDS with wave64
 // ballots (rank = items of the same digit in earlier waves / slots / lanes),
 // publishes its per-digit count, walks back over predecessor tiles until it
 // meets an inclusive prefix, publishes its own inclusive prefix, reorders the
@@ -147,7 +148,9 @@ __global__ __launch_bounds__(WG) void k_onesweep(int n, const int* __restrict__ 
   // With a device-side count the grid is a small persistent pool that keeps taking
   // tickets until every tile is done; otherwise one tile per workgroup.
   for (;;) {
-  if (tid == 0) s_tile = atomicAdd(&tickets[pass], 1u);
+  // one tile per workgroup in id order (dispatch order per XCD keeps every
+  // predecessor resident); the persistent pool takes tickets instead
+  if (tid == 0) s_tile = n_dev ? atomicAdd(&tickets[pass], 1u) : blockIdx.x;
   for (int i = tid; i < WAVES * RADIX; i += WG) s_whist[i] = 0;
   __syncthreads();
   const uint32_t tile = s_tile;
