@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -86,12 +88,85 @@ def cpu_baseline(sph, scene, budget_s):
     }
 
 
+def bench_batched(sph, steps, trial_steps=10):
+    """The 256-scene batched workload: the library's autotuner picks cell size,
+    tile and re-sort from its built-in candidates, then `steps` graph-replayed
+    steps are timed (host clock) and once more with per-launch hipEvents."""
+    b = sph.scenes.batched_dam_breaks()
+    n = b.n
+    log(f"batched: {len(b.counts)} scenes, n={n}")
+    ctx = sph.Batch(b.params, b.counts)
+    ctx.upload(b.rx, b.ry, b.vx, b.vy)
+    a0 = time.perf_counter()
+    best, times = ctx.autotune(trial_steps=trial_steps)
+    tune_s = time.perf_counter() - a0
+    log(f"batched autotune {tune_s:.2f} s -> {best}")
+    ctx.prepare()
+    ctx.set_graph(True)
+    ctx.step(10)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    value = n * steps / wall
+    ctx.set_timing(True)
+    ctx.step(min(steps, 50))
+    ctx.synchronize()
+    stage = ctx.stage_times()
+    ctx.set_timing(False)
+    ev_steps = min(steps, 50)
+    mfrac = ctx.movers() / n
+    keys = ctx.sorted_keys()
+    occ = float(np.unique(keys).size) / n
+    ctx.density_pressure()
+    k = float(ctx.ell_counts().mean())
+    passes = (int(ctx.grid().key_bits) + 7) // 8
+    resort = "incremental" if best[2] == sph.RESORT_INCREMENTAL else "full"
+    stages = {}
+    step_bytes = 0.0
+    kern_ms = 0.0
+    for name, (ms, launches) in stage.items():
+        if launches == 0:
+            continue
+        per = ms / launches
+        bpp = alg_bytes_per_particle(name, k, occ, passes, "particle", resort, mfrac)
+        stages[name] = {"launches": launches, "us_per_launch": round(per * 1e3, 3),
+                        "alg_bytes_per_launch": int(bpp * n),
+                        "achieved_GBs": round(bpp * n / (per * 1e-3) / 1e9, 1)}
+        step_bytes += bpp * launches / ev_steps
+        kern_ms += ms
+    dom = max(stages, key=lambda s: stages[s]["us_per_launch"] * stages[s]["launches"])
+    ctx.close()
+    return {
+        "workload": b.name, "scenes": int(len(b.counts)), "particles": n,
+        "counts_min_max": [int(b.counts.min()), int(b.counts.max())],
+        "value": value, "unit": "particle-steps/s", "steps": steps,
+        "ms_per_step": wall * 1e3 / steps,
+        "tuning": {"cell_factor": best[0], "tile": best[1], "resort": resort},
+        "autotune_s": round(tune_s, 2),
+        "autotune_ms_per_step": {f"{c[0]}/{c[1]}/{'inc' if c[2] else 'full'}": round(v, 4)
+                                 for c, v in times.items()},
+        "mean_neighbors": round(k, 3), "occupied_cells_per_particle": round(occ, 4),
+        "radix_passes": passes,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": stages[dom]["achieved_GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": stages[dom]["achieved_GBs"] / HBM_PEAK_GBS},
+        "step_alg_bytes_per_particle": round(step_bytes, 1),
+        "step_frac_of_hbm": step_bytes * value / (HBM_PEAK_GBS * 1e9),
+        "kernel_ms_per_step": kern_ms / ev_steps,
+        "stages": stages,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="dam_break_1m")
+    ap.add_argument("--batched-steps", type=int, default=200,
+                    help="steps of the 256-scene batched workload (0 disables)")
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU work for the cpu_baseline sample (0 disables)")
     args = ap.parse_args()
@@ -222,6 +297,8 @@ def main():
         "diagnostics": {"kinetic_energy": m[0], "max_v2": m[1], "inside": int(m[2])},
     }
     ctx.close()
+    if args.batched_steps > 0:
+        out["batched"] = bench_batched(sph, args.batched_steps)
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(sph, scene, args.cpu_budget)
     else:

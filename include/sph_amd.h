@@ -191,6 +191,58 @@ int sph_movers(void* ctx, int64_t* count);
  * capacity, staged particles; out[3..5] the same for the force kernel. */
 int sph_block_stats(void* ctx, int64_t* out6);
 
+/* ---- batched scenes (no reference counterpart: the reference runs one scene
+ * per process) ----
+ * A batch context holds n_scenes independent particle systems that share
+ * physics, boundary, order and ell_width but each have their own box, h, dt,
+ * mass and derived constants. The scenes' keys live in one key space (scene s
+ * owns the key range [base_s, base_s + its cell-table size)), so one radix sort
+ * per step serves all scenes and each scene keeps a fixed, contiguous segment
+ * [first_s, first_s + count_s) of the sorted order; every workgroup of the
+ * physics kernels covers one tile of one segment. Upload and download take the
+ * scenes' particles concatenated in scene order (sph_upload with n = the total
+ * count); particle ids are global. On a batch context these calls work:
+ * sph_upload, sph_download(_fields), sph_size, sph_get_grid (the shared key
+ * space), sph_find_neighbors, sph_prepare, sph_step, sph_set_graph,
+ * sph_set_resort, sph_density_pressure and sph_forces (both through the
+ * fixed-width list, same arithmetic as SORTED), sph_get_sorted_keys,
+ * sph_get_permutation, sph_get_cell_ranges, sph_get_ell, sph_movers, timing and
+ * sph_synchronize; the others return SPH_ERR_STATE. Every scene's trajectory is
+ * bit-identical to a single-scene context with the same parameters and
+ * cell_size = cell_factor * h. */
+typedef struct sph_batch_tuning {
+  double cell_factor; /* cell size = cell_factor * h of every scene; >= 1 */
+  int32_t tile;       /* particles per workgroup: 64, 128, 256 or 512 */
+  int32_t resort;     /* enum sph_resort used by sph_step */
+} sph_batch_tuning;
+
+/* scenes[0..n_scenes) with counts[s] >= 1 particles each; the scenes' cell_size
+ * fields are ignored (the tuning sets the cell size). Starts with tuning
+ * {1.0, 256, SPH_RESORT_INCREMENTAL}. */
+int sph_batch_create(const sph_params* scenes, int32_t n_scenes, const int64_t* counts,
+                     int device, void** out_ctx);
+int32_t sph_batch_scenes(const void* ctx); /* 0 for a single-scene context */
+/* Segment of scene s in the concatenated / sorted order and its grid. */
+int sph_batch_scene_info(const void* ctx, int32_t scene, int64_t* first, int64_t* count,
+                         sph_grid_info* grid);
+/* Apply a tuning: rebuilds the key space and tiles; the state is kept but must be
+ * re-sorted, so call sph_prepare before sph_step. */
+int sph_batch_set_tuning(void* ctx, const sph_batch_tuning* tuning);
+int sph_batch_get_tuning(const void* ctx, sph_batch_tuning* tuning);
+/* The built-in candidate set of the autotuner (cell_factor x tile x resort);
+ * returns the number of candidates (writes at most cap). */
+int32_t sph_batch_default_candidates(sph_batch_tuning* out, int32_t cap);
+/* Autotuner: for each candidate, from a device copy of the current state,
+ * applies it, prepares, runs 2 warm-up steps and times trial_steps steps of
+ * graph-replayed sph_step with hipEvents; ms_per_step[i] receives the time (-1
+ * for a candidate that cannot be applied). Applies the fastest, restores the
+ * state it started from, and leaves the context as after sph_upload (call
+ * sph_prepare next). *best = index of the chosen candidate. */
+int sph_batch_autotune(void* ctx, const sph_batch_tuning* candidates, int32_t n_candidates,
+                       int32_t trial_steps, double* ms_per_step, int32_t* best);
+/* sph_measure per scene: out[5 * s + k] as out5 of sph_measure. Synchronises. */
+int sph_batch_measure(void* ctx, double* out);
+
 /* ---- inspection (host copies; synchronise) ---- */
 int sph_get_sorted_keys(void* ctx, uint64_t* keys);  /* n keys of the last sort */
 int sph_get_permutation(void* ctx, uint32_t* perm);  /* sorted slot -> previous index */

@@ -51,6 +51,13 @@ class GridInfo(C.Structure):
     ]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("cell_factor", C.c_double), ("tile", C.c_int32), ("resort", C.c_int32)]
+
+    def as_tuple(self):
+        return (float(self.cell_factor), int(self.tile), int(self.resort))
+
+
 class DeviceView(C.Structure):
     _fields_ = [
         ("n", C.c_int64), ("rx", C.c_void_p), ("ry", C.c_void_p), ("vx", C.c_void_p),
@@ -147,6 +154,14 @@ def lib():
         "sph_set_timing": (i32, [vp, i32]),
         "sph_stage_times": (i32, [vp, P(d), P(i64), i32]),
         "sph_synchronize": (i32, [vp]),
+        "sph_batch_create": (i32, [P(SphParams), i32, P(i64), C.c_int, P(vp)]),
+        "sph_batch_scenes": (i32, [vp]),
+        "sph_batch_scene_info": (i32, [vp, i32, P(i64), P(i64), P(GridInfo)]),
+        "sph_batch_set_tuning": (i32, [vp, P(BatchTuning)]),
+        "sph_batch_get_tuning": (i32, [vp, P(BatchTuning)]),
+        "sph_batch_default_candidates": (i32, [P(BatchTuning), i32]),
+        "sph_batch_autotune": (i32, [vp, P(BatchTuning), i32, i32, P(d), P(i32)]),
+        "sph_batch_measure": (i32, [vp, P(d)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -163,7 +178,10 @@ EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_
             "sph_set_dt", "sph_measure", "sph_prepare", "sph_step",
             "sph_set_graph", "sph_ell_overflow", "sph_set_step_kernel", "sph_set_resort", "sph_movers", "sph_block_stats", "sph_get_sorted_keys", "sph_get_permutation",
             "sph_get_cell_ranges", "sph_get_neighbor_lists", "sph_get_ell",
-            "sph_get_device_view", "sph_set_timing", "sph_stage_times", "sph_synchronize")
+            "sph_get_device_view", "sph_set_timing", "sph_stage_times", "sph_synchronize",
+            "sph_batch_create", "sph_batch_scenes", "sph_batch_scene_info",
+            "sph_batch_set_tuning", "sph_batch_get_tuning", "sph_batch_default_candidates",
+            "sph_batch_autotune", "sph_batch_measure")
 
 
 def _ptr(a):
@@ -361,6 +379,11 @@ class Context:
         self._chk(self._L.sph_get_ell(self._h, _ptr(cnt), _ptr(slots)), "sph_get_ell")
         return cnt, slots
 
+    def ell_counts(self):
+        cnt = np.empty(self.n, np.int32)
+        self._chk(self._L.sph_get_ell(self._h, _ptr(cnt), None), "sph_get_ell")
+        return cnt
+
     def device_view(self) -> DeviceView:
         v = DeviceView()
         self._chk(self._L.sph_get_device_view(self._h, C.byref(v)), "sph_get_device_view")
@@ -380,3 +403,75 @@ class Context:
 
     def synchronize(self):
         self._chk(self._L.sph_synchronize(self._h), "sph_synchronize")
+
+
+class Batch(Context):
+    """Many independent scenes in one context (sph_batch_*): one radix sort and
+    one launch per stage serve every scene. Arrays passed to upload() and
+    returned by download()/fields() hold the scenes concatenated in scene order
+    (download returns the sorted order plus global ids, as Context does)."""
+
+    def __init__(self, params_list, counts, device: int = 0):
+        self._L = lib()
+        self.params = params_list[0]
+        self.scene_params = list(params_list)
+        arr = (SphParams * len(params_list))()
+        for k, p in enumerate(params_list):
+            p.fill(arr[k])
+        cnt = np.ascontiguousarray(counts, dtype=np.int64)
+        h = C.c_void_p()
+        rc = self._L.sph_batch_create(arr, len(params_list),
+                                      cnt.ctypes.data_as(C.POINTER(C.c_int64)), int(device),
+                                      C.byref(h))
+        if rc != SPH_OK:
+            raise SphError(f"sph_batch_create -> {rc}: {self._L.sph_last_error(None).decode()}")
+        self._h = h
+        self.n_max = int(cnt.sum())
+        self.counts = cnt
+        self.first = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int64)
+
+    @property
+    def n_scenes(self) -> int:
+        return int(self._L.sph_batch_scenes(self._h))
+
+    def scene_info(self, scene: int):
+        f, n, g = C.c_int64(), C.c_int64(), GridInfo()
+        self._chk(self._L.sph_batch_scene_info(self._h, int(scene), C.byref(f), C.byref(n),
+                                               C.byref(g)), "sph_batch_scene_info")
+        return int(f.value), int(n.value), g
+
+    def set_tuning(self, cell_factor=1.0, tile=256, resort=RESORT_INCREMENTAL):
+        t = BatchTuning(float(cell_factor), int(tile), int(resort))
+        self._chk(self._L.sph_batch_set_tuning(self._h, C.byref(t)), "sph_batch_set_tuning")
+
+    def tuning(self):
+        t = BatchTuning()
+        self._chk(self._L.sph_batch_get_tuning(self._h, C.byref(t)), "sph_batch_get_tuning")
+        return t.as_tuple()
+
+    @staticmethod
+    def default_candidates():
+        L = lib()
+        m = L.sph_batch_default_candidates(None, 0)
+        arr = (BatchTuning * m)()
+        L.sph_batch_default_candidates(arr, m)
+        return [arr[k].as_tuple() for k in range(m)]
+
+    def autotune(self, candidates=None, trial_steps: int = 10):
+        """Run the library's autotuner; returns (best tuning, {tuning: ms/step})."""
+        cands = candidates if candidates is not None else self.default_candidates()
+        arr = (BatchTuning * len(cands))()
+        for k, (f, t, r) in enumerate(cands):
+            arr[k] = BatchTuning(float(f), int(t), int(r))
+        ms = np.zeros(len(cands), np.float64)
+        best = C.c_int32(-1)
+        self._chk(self._L.sph_batch_autotune(self._h, arr, len(cands), int(trial_steps),
+                                             ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                             C.byref(best)), "sph_batch_autotune")
+        return tuple(cands[best.value]), {tuple(c): float(m) for c, m in zip(cands, ms)}
+
+    def measure_scenes(self) -> np.ndarray:
+        out = np.zeros((self.n_scenes, 5), np.float64)
+        self._chk(self._L.sph_batch_measure(self._h, out.ctypes.data_as(C.POINTER(C.c_double))),
+                  "sph_batch_measure")
+        return out

@@ -1,0 +1,142 @@
+// gfx950 kernels of the batched step: many independent scenes advance in one
+// launch per stage. Every workgroup covers one tile of one scene's contiguous
+// segment of the sorted order (the tile table is static because scene segments
+// never move), so the scene's constants are workgroup-uniform and come in
+// through scalar loads; the per-particle arithmetic is the single-scene code.
+#include <hip/hip_runtime.h>
+
+#include "sph_internal.hpp"
+
+namespace sphamd {
+
+namespace {
+
+template <int BC>
+__global__ __launch_bounds__(kMaxTile) void k_bkeys(const TileDev* __restrict__ tiles,
+                                                    const SceneDev* __restrict__ scenes,
+                                                    const float* __restrict__ rx,
+                                                    const float* __restrict__ ry,
+                                                    uint64_t* __restrict__ keys) {
+  const TileDev t = tiles[blockIdx.x];
+  const int i = t.begin + (int)threadIdx.x;
+  if (i >= t.end) return;
+  const DevGrid g = scenes[t.scene].g;
+  const uint64_t base = scenes[t.scene].base;
+  keys[i] = base + cell_key(g, cell_coord(rx[i], g.lo_x, g.cs_x, g.nx),
+                            cell_coord(ry[i], g.lo_y, g.cs_y, g.ny));
+}
+
+template <int PHYS, int BC>
+__global__ __launch_bounds__(kMaxTile) void k_bdensity(
+    int n, const TileDev* __restrict__ tiles, const SceneDev* __restrict__ scenes,
+    const float* __restrict__ rx, const float* __restrict__ ry, const int2* __restrict__ cells,
+    int32_t* __restrict__ ell_count, int32_t* __restrict__ ell_idx, int ell_w,
+    float* __restrict__ rho_out, float* __restrict__ pr_out, float* __restrict__ p_out) {
+  extern __shared__ int s_list[];  // kEllStage x blockDim.x
+  const TileDev t = tiles[blockIdx.x];
+  const int i = t.begin + (int)threadIdx.x;
+  if (i >= t.end) return;
+  const SceneDev& S = scenes[t.scene];
+  const DevGrid g = S.g;
+  const DevPhys c = S.c;
+  const float xi = rx[i], yi = ry[i];
+  const float acc = density_ell<BC>(g, c, rx, ry, cells + S.base, n, i, xi, yi,
+                                    s_list + threadIdx.x, (int)blockDim.x, ell_count, ell_idx,
+                                    ell_w);
+  float rho, p, pr;
+  eos<PHYS>(c, acc, rho, p, pr);
+  rho_out[i] = rho;
+  pr_out[i] = pr;
+  if (p_out) p_out[i] = p;
+}
+
+template <int PHYS, int BC>
+__global__ __launch_bounds__(kMaxTile) void k_bforces(
+    int n, const TileDev* __restrict__ tiles, const SceneDev* __restrict__ scenes,
+    const float* __restrict__ rx, const float* __restrict__ ry, const float* __restrict__ vx,
+    const float* __restrict__ vy, const float* __restrict__ rho, const float* __restrict__ pr,
+    const int2* __restrict__ cells, const int32_t* __restrict__ ell_count,
+    const int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ ax_out,
+    float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
+    float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out) {
+  const TileDev t = tiles[blockIdx.x];
+  const int i = t.begin + (int)threadIdx.x;
+  if (i >= t.end) return;
+  const SceneDev& S = scenes[t.scene];
+  const DevGrid g = S.g;
+  const DevPhys c = S.c;
+  const uint64_t base = S.base;
+  const float xi = rx[i], yi = ry[i], vxi = vx[i], vyi = vy[i], rhoi = rho[i], pri = pr[i];
+  const Acc a = forces_ell<PHYS, BC>(g, c, rx, ry, vx, vy, rho, pr, cells + base, ell_count,
+                                     ell_idx, ell_w, n, i, xi, yi, vxi, vyi, rhoi, pri);
+  float axi, ayi;
+  force_finish<PHYS>(c, a, axi, ayi);
+  ax_out[i] = axi;
+  ay_out[i] = ayi;
+  float x = xi, y = yi, u = vxi, v = vyi;
+  integrate_one<BC>(c, x, y, u, v, axi, ayi);
+  tx[i] = x;
+  ty[i] = y;
+  tvx[i] = u;
+  tvy[i] = v;
+  keys_out[i] = base + cell_key(g, cell_coord(x, g.lo_x, g.cs_x, g.nx),
+                                cell_coord(y, g.lo_y, g.cs_y, g.ny));
+}
+
+bool tile_ok(const BatchLaunch& B) {
+  return B.n_tiles > 0 && B.tile >= kWave && B.tile <= kMaxTile && B.tile % kWave == 0;
+}
+
+}  // namespace
+
+hipError_t launch_batch_keys(const Launch& L, const BatchLaunch& B, const DevState& s,
+                             uint64_t* keys_out) {
+  if (s.n <= 0) return hipSuccess;
+  if (!tile_ok(B)) return hipErrorInvalidValue;
+  if (L.phys.boundary == 1)
+    k_bkeys<1><<<B.n_tiles, B.tile, 0, L.stream>>>(B.tiles, B.scenes, s.rx, s.ry, keys_out);
+  else
+    k_bkeys<0><<<B.n_tiles, B.tile, 0, L.stream>>>(B.tiles, B.scenes, s.rx, s.ry, keys_out);
+  return hipGetLastError();
+}
+
+template <int PHYS, int BC>
+static void bdensity_go(const Launch& L, const BatchLaunch& B, const DevState& s) {
+  const size_t lds = (size_t)kEllStage * B.tile * sizeof(int);
+  k_bdensity<PHYS, BC><<<B.n_tiles, B.tile, lds, L.stream>>>(
+      s.n, B.tiles, B.scenes, s.rx, s.ry, s.cells, s.ell_count, s.ell_idx, s.ell_width, s.rho,
+      s.pr, s.pres);
+}
+
+hipError_t launch_batch_density(const Launch& L, const BatchLaunch& B, const DevState& s) {
+  if (s.n <= 0) return hipSuccess;
+  if (!tile_ok(B)) return hipErrorInvalidValue;
+  const int ph = L.phys.physics, bc = L.phys.boundary;
+  if (ph == 0 && bc == 0) bdensity_go<0, 0>(L, B, s);
+  else if (ph == 0 && bc == 1) bdensity_go<0, 1>(L, B, s);
+  else if (ph == 1 && bc == 0) bdensity_go<1, 0>(L, B, s);
+  else bdensity_go<1, 1>(L, B, s);
+  return hipGetLastError();
+}
+
+template <int PHYS, int BC>
+static void bforces_go(const Launch& L, const BatchLaunch& B, const DevState& s,
+                       uint64_t* keys_out) {
+  k_bforces<PHYS, BC><<<B.n_tiles, B.tile, 0, L.stream>>>(
+      s.n, B.tiles, B.scenes, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.ell_count,
+      s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out);
+}
+
+hipError_t launch_batch_forces(const Launch& L, const BatchLaunch& B, const DevState& s,
+                               uint64_t* keys_out) {
+  if (s.n <= 0) return hipSuccess;
+  if (!tile_ok(B)) return hipErrorInvalidValue;
+  const int ph = L.phys.physics, bc = L.phys.boundary;
+  if (ph == 0 && bc == 0) bforces_go<0, 0>(L, B, s, keys_out);
+  else if (ph == 0 && bc == 1) bforces_go<0, 1>(L, B, s, keys_out);
+  else if (ph == 1 && bc == 0) bforces_go<1, 0>(L, B, s, keys_out);
+  else bforces_go<1, 1>(L, B, s, keys_out);
+  return hipGetLastError();
+}
+
+}  // namespace sphamd

@@ -60,6 +60,21 @@ struct Ctx {
   int resort = SPH_RESORT_INCREMENTAL;  // how sph_step re-sorts
   ResortBuffers rb{};
   unsigned long long* block_stats = nullptr;
+  // batched scenes (n_scenes == 0: single-scene context)
+  int n_scenes = 0;
+  std::vector<sph_params> scene_prm;
+  std::vector<int64_t> scene_first, scene_count;
+  std::vector<sph_grid_info> scene_grid;
+  std::vector<SceneDev> scene_dev;
+  sph_batch_tuning tuning{1.0, 256, SPH_RESORT_INCREMENTAL};
+  SceneDev* d_scenes = nullptr;
+  TileDev* d_tiles = nullptr;
+  int64_t tiles_cap = 0;
+  int64_t cells_cap = 0;
+  int passes_cap = 0;
+  BatchLaunch B{};
+  float* backup = nullptr;  // autotuner's copy of rx, ry, vx, vy, ax, ay (6 n_max)
+  uint32_t* backup_pid = nullptr;
 };
 
 // Buffer assignment the fused iteration reads and writes; the stages swap
@@ -304,7 +319,10 @@ int find_neighbors(Ctx* c) {
   DevState& s = c->s;
   {
     StageTimer t(c, SPH_T_NEIGHBOR_SEARCH);
-    HIPCHK(c, launch_keys(c->L, s, c->sb.keys_in));
+    if (c->n_scenes)
+      HIPCHK(c, launch_batch_keys(c->L, c->B, s, c->sb.keys_in));
+    else
+      HIPCHK(c, launch_keys(c->L, s, c->sb.keys_in));
   }
   int rc = sort_keys(c);
   if (rc) return rc;
@@ -373,7 +391,15 @@ int incremental_resort(Ctx* c) {
 // + ranges -> density + EOS + ELL.
 int fused_iteration(Ctx* c) {
   DevState& s = c->s;
-  if (s.block_bits && c->step_kernel == SPH_STEP_FUSED_BLOCK) {
+  if (c->n_scenes) {
+    // batched scenes: per-tile scene constants, fixed-width list
+    {
+      StageTimer t(c, SPH_T_DENSITY);
+      HIPCHK(c, launch_batch_density(c->L, c->B, s));
+    }
+    StageTimer t(c, SPH_T_FORCE);
+    HIPCHK(c, launch_batch_forces(c->L, c->B, s, c->sb.keys_in));
+  } else if (s.block_bits && c->step_kernel == SPH_STEP_FUSED_BLOCK) {
     // Morton order, one kernel: density, forces, integration and keys
     StageTimer t(c, SPH_T_BLOCK_STEP);
     HIPCHK(c, launch_block_step(c->L, s, c->sb.keys_in, c->block_grid, c->step_kernel, nullptr));
@@ -455,8 +481,199 @@ int sync_and_check(Ctx* c) {
   return SPH_OK;
 }
 
+int single_only(const Ctx* c) {
+  return fail(const_cast<Ctx*>(c), SPH_ERR_STATE,
+              "not available on a batch context (single-scene call)");
+}
+
 int check_n(Ctx* c) {
   if (c->s.n <= 0) return fail(c, SPH_ERR_STATE, "no particles uploaded");
+  return SPH_OK;
+}
+
+// Stream and every device buffer of a context for n_max particles and a cell
+// table of n_keys entries (keys of at most key_bits bits); block_bits 0 -> no
+// Morton block table.
+int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_bits,
+                int64_t meas_words) {
+  int rc = SPH_OK;
+  auto bail = [&](int code) { return code; };
+  const sph_params& p = c->prm;
+  const int device = c->device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    hip_fail(c, e, "hipSetDevice");
+    return bail(SPH_ERR_HIP);
+  }
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    hip_fail(c, e, "hipStreamCreate");
+    return bail(SPH_ERR_HIP);
+  }
+  c->L.stream = c->stream;
+  DevState& s = c->s;
+  s.ell_width = p.ell_width > 0 ? p.ell_width : 32;
+  s.n_keys = n_keys;
+  const int64_t N = n_max;
+  float** farrs[] = {&s.rx, &s.ry, &s.vx, &s.vy, &s.tx, &s.ty, &s.tvx, &s.tvy, &s.ax,
+                     &s.ay, &s.tax, &s.tay, &s.rho, &s.pr, &s.pres};
+  for (float** a : farrs)
+    if ((rc = dev_alloc(c, a, N))) return bail(rc);
+  if ((rc = dev_alloc(c, &s.pid, N)) || (rc = dev_alloc(c, &s.pid_alt, N)) ||
+      (rc = dev_alloc(c, &s.cells, n_keys)) ||
+      (rc = dev_alloc(c, &s.ell_count, N)) ||
+      (rc = dev_alloc(c, &s.ell_idx, N * s.ell_width)) ||
+      (rc = dev_alloc(c, &s.csr_count, N)) || (rc = dev_alloc(c, &s.csr_off, N + 1)))
+    return bail(rc);
+  const int tiles = (int)((N + sort_tile_size() - 1) / sort_tile_size());
+  SortBuffers& b = c->sb;
+  if ((rc = dev_alloc(c, &b.keys_in, N)) || (rc = dev_alloc(c, &b.keys_tmp, N)) ||
+      (rc = dev_alloc(c, &b.keys_out, N)) || (rc = dev_alloc(c, &b.vals_a, N)) ||
+      (rc = dev_alloc(c, &b.vals_b, N)) || (rc = dev_alloc(c, &b.vals_out, N)) ||
+      (rc = dev_alloc(c, &b.hist, kMaxPasses * 256)) ||
+      (rc = dev_alloc(c, &b.tickets, kMaxPasses)) || (rc = dev_alloc(c, &b.status, 1)) ||
+      (rc = dev_alloc(c, &b.lookback, (int64_t)sort_passes(key_bits) * tiles * 256)))
+    return bail(rc);
+  b.max_tiles = tiles;
+  s.block_bits = block_bits;
+  s.n_blocks = s.block_bits ? ((n_keys + (1 << (2 * s.block_bits)) - 1) >> (2 * s.block_bits))
+                            : 0;
+  if (s.block_bits && (rc = dev_alloc(c, &s.blocks, s.n_blocks))) return bail(rc);
+  if ((rc = dev_alloc(c, &c->block_stats, 6))) return bail(rc);
+  {
+    ResortBuffers& r = c->rb;
+    if ((rc = dev_alloc(c, &r.mkey, N)) || (rc = dev_alloc(c, &r.midx, N)) ||
+        (rc = dev_alloc(c, &r.skey, N)) || (rc = dev_alloc(c, &r.sidx, N)) ||
+        (rc = dev_alloc(c, &r.ktmp, N)) || (rc = dev_alloc(c, &r.keys_alt, N)) ||
+        (rc = dev_alloc(c, &r.flags, resort_flag_words((int)N))) ||
+        (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
+        (rc = dev_alloc(c, &r.lookback, resort_tiles((int)N))) ||
+        (rc = dev_alloc(c, &r.ticket, 1)) || (rc = dev_alloc(c, &r.n_movers, 1)))
+      return bail(rc);
+  }
+  b.blocks = s.blocks;
+  b.block_bits = s.block_bits;
+  int cu = 256;
+  (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device);
+  b.hist_blocks = std::max(cu, 1);
+  c->block_grid = block_step_grid(s.n_blocks, cu);
+  b.cells = s.cells;
+  b.n_keys = (uint64_t)n_keys;
+  s.keys_sorted = b.keys_out;
+  s.perm = b.vals_out;
+  s.hist = b.hist;
+  if ((rc = dev_alloc(c, &c->scan_scratch, scan_scratch_words((int)N))) ||
+      (rc = dev_alloc(c, &c->meas_part, measure_partial_words((int)N))) ||
+      (rc = dev_alloc(c, &c->meas_out, meas_words)) || (rc = dev_alloc(c, &c->ovf, 1)))
+    return bail(rc);
+  e = hipMemsetAsync(b.hist, 0, kMaxPasses * 256 * sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(b.status, 0, sizeof(uint32_t), c->stream);
+  c->rb.status = b.status;
+  if (e == hipSuccess)
+    e = hipMemsetAsync(c->rb.lookback, 0, resort_tiles((int)N) * sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->rb.ticket, 0, sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, n_keys);
+  if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    hip_fail(c, e, "initialise buffers");
+    return bail(SPH_ERR_HIP);
+  }
+  c->cells_cap = n_keys;
+  c->passes_cap = sort_passes(key_bits);
+  return SPH_OK;
+}
+
+
+// Grids, constants and key bases of every scene of a batch for one tuning, and
+// the per-scene totals of the shared key space; no device work.
+int batch_layout(const Ctx* c, const sph_batch_tuning& t, std::vector<sph_grid_info>& grids,
+                 std::vector<SceneDev>& dev, int64_t& total, std::string& why) {
+  if (!(t.cell_factor >= 1.0) || !std::isfinite(t.cell_factor)) {
+    why = "cell_factor must be >= 1";
+    return -1;
+  }
+  if (t.tile < kWave || t.tile > kMaxTile || t.tile % kWave != 0 || (t.tile & (t.tile - 1))) {
+    why = "tile must be 64, 128, 256 or 512";
+    return -1;
+  }
+  if (t.resort != SPH_RESORT_FULL && t.resort != SPH_RESORT_INCREMENTAL) {
+    why = "unknown re-sort mode";
+    return -1;
+  }
+  const int S = c->n_scenes;
+  grids.assign(S, sph_grid_info{});
+  dev.assign(S, SceneDev{});
+  total = 0;
+  for (int i = 0; i < S; ++i) {
+    sph_params p = c->scene_prm[i];
+    p.cell_size = t.cell_factor * p.h;
+    if (derive_grid(p, grids[i], why) != 0) {
+      why = "scene " + std::to_string(i) + ": " + why;
+      return -1;
+    }
+    derive_phys(p, grids[i], dev[i].c, dev[i].g);
+    dev[i].base = (uint64_t)total;
+    total += grids[i].n_keys;
+    if (total > ((int64_t)1 << 31)) {
+      why = "batch cell table larger than 2^31 entries";
+      return -1;
+    }
+  }
+  return 0;
+}
+
+int key_bits_of(int64_t n_keys) {
+  int bits = 0;
+  while (bits < 63 && ((uint64_t)1 << bits) < (uint64_t)n_keys) ++bits;
+  return std::max(bits, 1);
+}
+
+// Apply a tuning to a batch context: scene constants and key bases, tile table,
+// a cleared cell table; the state must be re-sorted afterwards.
+int batch_configure(Ctx* c, const sph_batch_tuning& t) {
+  std::vector<sph_grid_info> grids;
+  std::vector<SceneDev> dev;
+  int64_t total = 0;
+  std::string why;
+  if (batch_layout(c, t, grids, dev, total, why) != 0) return fail(c, SPH_ERR_INVALID, why);
+  const int bits = key_bits_of(total);
+  if (total > c->cells_cap || sort_passes(bits) > c->passes_cap)
+    return fail(c, SPH_ERR_INVALID, "tuning needs a larger cell table than the context holds");
+  std::vector<TileDev> tiles;
+  for (int i = 0; i < c->n_scenes; ++i) {
+    const int64_t b = c->scene_first[i], e = b + c->scene_count[i];
+    for (int64_t q = b; q < e; q += t.tile)
+      tiles.push_back(TileDev{i, (int32_t)q, (int32_t)std::min<int64_t>(q + t.tile, e), 0});
+  }
+  if ((int64_t)tiles.size() > c->tiles_cap) return fail(c, SPH_ERR_INVALID, "tile table overflow");
+  destroy_graph(c);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_scenes, dev.data(), dev.size() * sizeof(SceneDev),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tiles, tiles.data(), tiles.size() * sizeof(TileDev),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_fill_cells(c->L, c->s.cells, total));
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors go out of scope
+  c->scene_grid = grids;
+  c->scene_dev = dev;
+  c->tuning = t;
+  c->resort = t.resort;
+  c->B.scenes = c->d_scenes;
+  c->B.tiles = c->d_tiles;
+  c->B.n_tiles = (int)tiles.size();
+  c->B.tile = t.tile;
+  sph_grid_info& g = c->grid;
+  g = sph_grid_info{};
+  g.n_keys = total;
+  g.key_bits = bits;
+  g.order = c->prm.order;
+  c->s.n_keys = total;
+  c->sb.n_keys = (uint64_t)total;
+  c->s.n_prev = 0;  // the whole table was just cleared
+  c->sorted = false;
+  c->density_ell = false;
+  c->csr_total = -1;
   return SPH_OK;
 }
 
@@ -498,90 +715,12 @@ int sph_create(const sph_params* params, int64_t n_max, int device, void** out_c
     return SPH_ERR_INVALID;
   }
   derive_phys(p, c->grid, c->L.phys, c->L.grid);
-  int rc = SPH_OK;
-  auto bail = [&](int code) {
+  int rc = alloc_state(c, n_max, c->grid.n_keys, c->grid.key_bits,
+                       p.order == SPH_ORDER_MORTON ? 3 : 0, 8);
+  if (rc != SPH_OK) {
     g_create_error = c->err.empty() ? std::string("sph_create failed") : c->err;
     sph_destroy(c);
-    return code;
-  };
-  hipError_t e = hipSetDevice(device);
-  if (e != hipSuccess) {
-    hip_fail(c, e, "hipSetDevice");
-    return bail(SPH_ERR_HIP);
-  }
-  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    hip_fail(c, e, "hipStreamCreate");
-    return bail(SPH_ERR_HIP);
-  }
-  c->L.stream = c->stream;
-  DevState& s = c->s;
-  s.ell_width = p.ell_width > 0 ? p.ell_width : 32;
-  s.n_keys = c->grid.n_keys;
-  const int64_t N = n_max;
-  float** farrs[] = {&s.rx, &s.ry, &s.vx, &s.vy, &s.tx, &s.ty, &s.tvx, &s.tvy, &s.ax,
-                     &s.ay, &s.tax, &s.tay, &s.rho, &s.pr, &s.pres};
-  for (float** a : farrs)
-    if ((rc = dev_alloc(c, a, N))) return bail(rc);
-  if ((rc = dev_alloc(c, &s.pid, N)) || (rc = dev_alloc(c, &s.pid_alt, N)) ||
-      (rc = dev_alloc(c, &s.cells, c->grid.n_keys)) ||
-      (rc = dev_alloc(c, &s.ell_count, N)) ||
-      (rc = dev_alloc(c, &s.ell_idx, N * s.ell_width)) ||
-      (rc = dev_alloc(c, &s.csr_count, N)) || (rc = dev_alloc(c, &s.csr_off, N + 1)))
-    return bail(rc);
-  const int tiles = (int)((N + sort_tile_size() - 1) / sort_tile_size());
-  SortBuffers& b = c->sb;
-  if ((rc = dev_alloc(c, &b.keys_in, N)) || (rc = dev_alloc(c, &b.keys_tmp, N)) ||
-      (rc = dev_alloc(c, &b.keys_out, N)) || (rc = dev_alloc(c, &b.vals_a, N)) ||
-      (rc = dev_alloc(c, &b.vals_b, N)) || (rc = dev_alloc(c, &b.vals_out, N)) ||
-      (rc = dev_alloc(c, &b.hist, kMaxPasses * 256)) ||
-      (rc = dev_alloc(c, &b.tickets, kMaxPasses)) || (rc = dev_alloc(c, &b.status, 1)) ||
-      (rc = dev_alloc(c, &b.lookback, (int64_t)sort_passes(c->grid.key_bits) * tiles * 256)))
-    return bail(rc);
-  b.max_tiles = tiles;
-  s.block_bits = p.order == SPH_ORDER_MORTON ? 3 : 0;
-  s.n_blocks = s.block_bits ? ((c->grid.n_keys + (1 << (2 * s.block_bits)) - 1) >> (2 * s.block_bits))
-                            : 0;
-  if (s.block_bits && (rc = dev_alloc(c, &s.blocks, s.n_blocks))) return bail(rc);
-  if ((rc = dev_alloc(c, &c->block_stats, 6))) return bail(rc);
-  {
-    ResortBuffers& r = c->rb;
-    if ((rc = dev_alloc(c, &r.mkey, N)) || (rc = dev_alloc(c, &r.midx, N)) ||
-        (rc = dev_alloc(c, &r.skey, N)) || (rc = dev_alloc(c, &r.sidx, N)) ||
-        (rc = dev_alloc(c, &r.ktmp, N)) || (rc = dev_alloc(c, &r.keys_alt, N)) ||
-        (rc = dev_alloc(c, &r.flags, resort_flag_words((int)N))) ||
-        (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
-        (rc = dev_alloc(c, &r.lookback, resort_tiles((int)N))) ||
-        (rc = dev_alloc(c, &r.ticket, 1)) || (rc = dev_alloc(c, &r.n_movers, 1)))
-      return bail(rc);
-  }
-  b.blocks = s.blocks;
-  b.block_bits = s.block_bits;
-  int cu = 256;
-  (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device);
-  b.hist_blocks = std::max(cu, 1);
-  c->block_grid = block_step_grid(s.n_blocks, cu);
-  b.cells = s.cells;
-  b.n_keys = (uint64_t)c->grid.n_keys;
-  s.keys_sorted = b.keys_out;
-  s.perm = b.vals_out;
-  s.hist = b.hist;
-  if ((rc = dev_alloc(c, &c->scan_scratch, scan_scratch_words((int)N))) ||
-      (rc = dev_alloc(c, &c->meas_part, measure_partial_words((int)N))) ||
-      (rc = dev_alloc(c, &c->meas_out, 8)) || (rc = dev_alloc(c, &c->ovf, 1)))
-    return bail(rc);
-  e = hipMemsetAsync(b.hist, 0, kMaxPasses * 256 * sizeof(uint32_t), c->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(b.status, 0, sizeof(uint32_t), c->stream);
-  c->rb.status = b.status;
-  if (e == hipSuccess)
-    e = hipMemsetAsync(c->rb.lookback, 0, resort_tiles((int)N) * sizeof(uint32_t), c->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(c->rb.ticket, 0, sizeof(uint32_t), c->stream);
-  if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, c->grid.n_keys);
-  if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) {
-    hip_fail(c, e, "initialise buffers");
-    return bail(SPH_ERR_HIP);
+    return rc;
   }
   *out_ctx = c;
   return SPH_OK;
@@ -619,6 +758,7 @@ int sph_get_grid(const void* ctx, sph_grid_info* out) {
 int sph_get_constants(const void* ctx, float* out, int32_t count) {
   const Ctx* c = static_cast<const Ctx*>(ctx);
   if (!c || !out) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
   const DevPhys& p = c->L.phys;
   // order of orc_consts: h, h2, tol, mc_rho, c_pres, c_visc, gx, gy, eos_k, rho0,
   // inv_rho0, tait_b, neg_alpha_c0, eps_h2, dt, lo_x, lo_y, hi_x, hi_y, len_x, len_y
@@ -636,6 +776,8 @@ int sph_upload(void* ctx, int64_t n, const float* rx, const float* ry, const flo
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
   if (n < 0 || n > c->n_max) return fail(c, SPH_ERR_CAPACITY, "n exceeds the context capacity");
+  if (c->n_scenes && n != c->n_max)
+    return fail(c, SPH_ERR_INVALID, "a batch upload holds every scene's particles");
   if (n > 0 && (!rx || !ry || !vx || !vy))
     return fail(c, SPH_ERR_INVALID, "null position or velocity array");
   DevState& s = c->s;
@@ -705,6 +847,7 @@ int sph_find_neighbors(void* ctx) {
 int sph_build_neighbor_lists(void* ctx, int32_t include_self, int64_t* total) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
   int rc = check_n(c);
   if (rc) return rc;
   if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_find_neighbors first");
@@ -740,6 +883,14 @@ int sph_density_pressure(void* ctx, int32_t compute) {
   int rc = check_n(c);
   if (rc) return rc;
   if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_find_neighbors first");
+  if (c->n_scenes) {
+    if (compute != SPH_COMPUTE_SORTED && compute != SPH_COMPUTE_LISTS)
+      return fail(c, SPH_ERR_INVALID, "unknown compute type");
+    StageTimer t(c, SPH_T_DENSITY);
+    HIPCHK(c, launch_batch_density(c->L, c->B, c->s));
+    c->density_ell = true;
+    return SPH_OK;
+  }
   int nb = NB_STENCIL;
   if (compute == SPH_COMPUTE_LISTS) {
     if (c->csr_total < 0 || c->csr_self != 0)
@@ -760,6 +911,15 @@ int sph_forces(void* ctx, int32_t compute) {
   int rc = check_n(c);
   if (rc) return rc;
   if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_find_neighbors first");
+  if (c->n_scenes) {
+    if (compute != SPH_COMPUTE_SORTED && compute != SPH_COMPUTE_LISTS)
+      return fail(c, SPH_ERR_INVALID, "unknown compute type");
+    if (!c->density_ell) return fail(c, SPH_ERR_STATE, "call sph_density_pressure first");
+    // the fused kernel also writes the integrated twin and keys into scratch
+    StageTimer t(c, SPH_T_FORCE);
+    HIPCHK(c, launch_batch_forces(c->L, c->B, c->s, c->sb.keys_in));
+    return SPH_OK;
+  }
   int nb = NB_STENCIL;
   if (compute == SPH_COMPUTE_LISTS) {
     if (c->density_ell) {
@@ -780,6 +940,7 @@ int sph_forces(void* ctx, int32_t compute) {
 int sph_integrate(void* ctx) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
   int rc = check_n(c);
   if (rc) return rc;
   StageTimer t(c, SPH_T_INTEGRATION);
@@ -792,6 +953,7 @@ int sph_integrate(void* ctx) {
 int sph_set_dt(void* ctx, double dt) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
   if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SPH_ERR_INVALID, "dt must be positive");
   c->prm.dt = dt;
   c->L.phys.dt = (float)dt;
@@ -802,6 +964,7 @@ int sph_set_dt(void* ctx, double dt) {
 int sph_update_velocities(void* ctx) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
   int rc = check_n(c);
   if (rc) return rc;
   StageTimer t(c, SPH_T_INTEGRATION);
@@ -813,6 +976,7 @@ int sph_update_velocities(void* ctx) {
 int sph_update_positions(void* ctx) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
   int rc = check_n(c);
   if (rc) return rc;
   StageTimer t(c, SPH_T_INTEGRATION);
@@ -825,6 +989,7 @@ int sph_update_positions(void* ctx) {
 int sph_measure(void* ctx, double* out5) {
   Ctx* c = as_ctx(ctx);
   if (!c || !out5) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
   {
     StageTimer t(c, SPH_T_MEASURE);
     HIPCHK(c, launch_measure(c->L, c->s, c->prm.mass, c->prm.gravity_y, c->meas_part, c->meas_out));
@@ -843,7 +1008,10 @@ int sph_prepare(void* ctx) {
   if ((rc = find_neighbors(c))) return rc;
   {
     StageTimer t(c, SPH_T_DENSITY);
-    HIPCHK(c, launch_density(c->L, c->s, NB_ELL, true));
+    if (c->n_scenes)
+      HIPCHK(c, launch_batch_density(c->L, c->B, c->s));
+    else
+      HIPCHK(c, launch_density(c->L, c->s, NB_ELL, true));
   }
   c->density_ell = true;
   return SPH_OK;
@@ -889,6 +1057,7 @@ int sph_ell_overflow(void* ctx, int64_t* count) {
 int sph_set_step_kernel(void* ctx, int32_t variant) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
   if (variant < SPH_STEP_SPLIT_BLOCK || variant > SPH_STEP_PARTICLE)
     return fail(c, SPH_ERR_INVALID, "unknown step kernel");
   c->step_kernel = variant;
@@ -902,6 +1071,7 @@ int sph_set_resort(void* ctx, int32_t mode) {
   if (mode != SPH_RESORT_FULL && mode != SPH_RESORT_INCREMENTAL)
     return fail(c, SPH_ERR_INVALID, "unknown re-sort mode");
   c->resort = mode;
+  c->tuning.resort = mode;
   destroy_graph(c);
   return SPH_OK;
 }
@@ -919,6 +1089,7 @@ int sph_movers(void* ctx, int64_t* count) {
 int sph_block_stats(void* ctx, int64_t* out6) {
   Ctx* c = as_ctx(ctx);
   if (!c || !out6) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
   if (!c->sorted || !c->s.block_bits)
     return fail(c, SPH_ERR_STATE, "needs a sorted Morton-order state");
   HIPCHK(c, hipMemsetAsync(c->block_stats, 0, 6 * sizeof(unsigned long long), c->stream));
@@ -971,6 +1142,7 @@ int sph_get_cell_ranges(void* ctx, int32_t* start, int32_t* end) {
 int sph_get_neighbor_lists(void* ctx, int64_t* offsets, int32_t* indices) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
   if (c->csr_total < 0) return fail(c, SPH_ERR_STATE, "lists not built");
   if (offsets)
     HIPCHK(c, hipMemcpyAsync(offsets, c->s.csr_off, (size_t)(c->s.n + 1) * 8,
@@ -987,7 +1159,7 @@ int sph_get_ell(void* ctx, int32_t* counts, int32_t* slots) {
   if (!c) return SPH_ERR_INVALID;
   if (!c->density_ell) return fail(c, SPH_ERR_STATE, "no fixed-width list built");
   const int n = c->s.n, w = c->s.ell_width;
-  std::vector<int32_t> cm((size_t)n * w);
+  std::vector<int32_t> cm(slots ? (size_t)n * w : 0);
   if (counts)
     HIPCHK(c, hipMemcpyAsync(counts, c->s.ell_count, (size_t)n * 4, hipMemcpyDeviceToHost,
                              c->stream));
@@ -1044,6 +1216,236 @@ int sph_synchronize(void* ctx) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
   return sync_and_check(c);
+}
+
+int sph_batch_create(const sph_params* scenes, int32_t n_scenes, const int64_t* counts,
+                     int device, void** out_ctx) {
+  g_create_error.clear();
+  if (!scenes || !counts || !out_ctx || n_scenes < 1 || n_scenes > (1 << 20)) {
+    g_create_error = "sph_batch_create: invalid arguments";
+    return SPH_ERR_INVALID;
+  }
+  int64_t total = 0;
+  for (int i = 0; i < n_scenes; ++i) {
+    if (counts[i] < 1) {
+      g_create_error = "sph_batch_create: every scene needs at least one particle";
+      return SPH_ERR_INVALID;
+    }
+    total += counts[i];
+  }
+  if (total >= ((int64_t)1 << 30)) {
+    g_create_error = "sph_batch_create: more than 2^30 particles";
+    return SPH_ERR_INVALID;
+  }
+  const sph_params& p0 = scenes[0];
+  for (int i = 0; i < n_scenes; ++i) {
+    const sph_params& p = scenes[i];
+    if (p.physics != p0.physics || p.boundary != p0.boundary || p.order != p0.order ||
+        p.ell_width != p0.ell_width) {
+      g_create_error = "sph_batch_create: scenes must share physics, boundary, order, ell_width";
+      return SPH_ERR_INVALID;
+    }
+    if (p.physics == SPH_PHYS_WCSPH && (p.gamma < 1 || p.gamma > 16 || !(p.rho0 > 0))) {
+      g_create_error = "sph_batch_create: WCSPH needs an integer gamma in [1,16] and rho0 > 0";
+      return SPH_ERR_INVALID;
+    }
+  }
+  if ((p0.physics != SPH_PHYS_REFERENCE && p0.physics != SPH_PHYS_WCSPH) ||
+      (p0.boundary != SPH_BC_WALLS && p0.boundary != SPH_BC_PERIODIC) ||
+      (p0.order != SPH_ORDER_MORTON && p0.order != SPH_ORDER_ROWMAJOR)) {
+    g_create_error = "sph_batch_create: unknown physics, boundary or order";
+    return SPH_ERR_INVALID;
+  }
+  Ctx* c = new Ctx();
+  c->prm = p0;
+  c->n_max = total;
+  c->device = device;
+  c->n_scenes = n_scenes;
+  c->scene_prm.assign(scenes, scenes + n_scenes);
+  c->scene_count.assign(counts, counts + n_scenes);
+  c->scene_first.resize(n_scenes);
+  int64_t off = 0;
+  for (int i = 0; i < n_scenes; ++i) {
+    c->scene_first[i] = off;
+    off += counts[i];
+  }
+  c->step_kernel = SPH_STEP_PARTICLE;
+  // the cell table is sized for cell_factor 1, the largest any tuning needs
+  std::vector<sph_grid_info> grids;
+  std::vector<SceneDev> dev;
+  int64_t n_keys = 0;
+  std::string why;
+  sph_batch_tuning t0{1.0, 256, SPH_RESORT_INCREMENTAL};
+  if (batch_layout(c, t0, grids, dev, n_keys, why) != 0) {
+    g_create_error = "sph_batch_create: " + why;
+    delete c;
+    return SPH_ERR_INVALID;
+  }
+  derive_phys(p0, grids[0], c->L.phys, c->L.grid);  // physics / boundary selectors
+  int rc = alloc_state(c, total, n_keys, key_bits_of(n_keys), 0,
+                       std::max<int64_t>(8, 5 * (int64_t)n_scenes));
+  if (rc == SPH_OK) {
+    c->tiles_cap = total / kWave + n_scenes + 1;
+    rc = dev_alloc(c, &c->d_scenes, n_scenes);
+    if (rc == SPH_OK) rc = dev_alloc(c, &c->d_tiles, c->tiles_cap);
+  }
+  if (rc == SPH_OK) rc = batch_configure(c, t0);
+  if (rc != SPH_OK) {
+    g_create_error = c->err.empty() ? std::string("sph_batch_create failed") : c->err;
+    sph_destroy(c);
+    return rc;
+  }
+  *out_ctx = c;
+  return SPH_OK;
+}
+
+int32_t sph_batch_scenes(const void* ctx) {
+  const Ctx* c = static_cast<const Ctx*>(ctx);
+  return c ? c->n_scenes : SPH_ERR_INVALID;
+}
+
+int sph_batch_scene_info(const void* ctx, int32_t scene, int64_t* first, int64_t* count,
+                         sph_grid_info* grid) {
+  const Ctx* c = static_cast<const Ctx*>(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  if (scene < 0 || scene >= c->n_scenes)
+    return fail(const_cast<Ctx*>(c), SPH_ERR_INVALID, "scene index out of range");
+  if (first) *first = c->scene_first[scene];
+  if (count) *count = c->scene_count[scene];
+  if (grid) *grid = c->scene_grid[scene];
+  return SPH_OK;
+}
+
+int sph_batch_set_tuning(void* ctx, const sph_batch_tuning* tuning) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !tuning) return SPH_ERR_INVALID;
+  if (!c->n_scenes) return fail(c, SPH_ERR_STATE, "not a batch context");
+  return batch_configure(c, *tuning);
+}
+
+int sph_batch_get_tuning(const void* ctx, sph_batch_tuning* tuning) {
+  const Ctx* c = static_cast<const Ctx*>(ctx);
+  if (!c || !tuning) return SPH_ERR_INVALID;
+  if (!c->n_scenes) return fail(const_cast<Ctx*>(c), SPH_ERR_STATE, "not a batch context");
+  *tuning = c->tuning;
+  return SPH_OK;
+}
+
+int32_t sph_batch_default_candidates(sph_batch_tuning* out, int32_t cap) {
+  const double factors[] = {1.0, 1.25, 1.5};
+  const int32_t tiles[] = {64, 128, 256, 512};
+  const int32_t resorts[] = {SPH_RESORT_INCREMENTAL, SPH_RESORT_FULL};
+  int32_t n = 0;
+  for (double f : factors)
+    for (int32_t t : tiles)
+      for (int32_t r : resorts) {
+        if (out && n < cap) out[n] = sph_batch_tuning{f, t, r};
+        ++n;
+      }
+  return n;
+}
+
+int sph_batch_autotune(void* ctx, const sph_batch_tuning* candidates, int32_t n_candidates,
+                       int32_t trial_steps, double* ms_per_step, int32_t* best) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !candidates || n_candidates < 1 || trial_steps < 1) return SPH_ERR_INVALID;
+  if (!c->n_scenes) return fail(c, SPH_ERR_STATE, "not a batch context");
+  int rc = check_n(c);
+  if (rc) return rc;
+  DevState& s = c->s;
+  const int64_t N = c->n_max;
+  if (!c->backup) {
+    if ((rc = dev_alloc(c, &c->backup, 6 * N)) || (rc = dev_alloc(c, &c->backup_pid, N)))
+      return rc;
+  }
+  const size_t fb = (size_t)s.n * sizeof(float);
+  auto copy_state = [&](bool save) -> hipError_t {
+    float* arrs[] = {s.rx, s.ry, s.vx, s.vy, s.ax, s.ay};
+    for (int k = 0; k < 6; ++k) {
+      float* bk = c->backup + (size_t)k * N;
+      hipError_t e = hipMemcpyAsync(save ? bk : arrs[k], save ? arrs[k] : bk, fb,
+                                    hipMemcpyDeviceToDevice, c->stream);
+      if (e != hipSuccess) return e;
+    }
+    return hipMemcpyAsync(save ? c->backup_pid : s.pid, save ? s.pid : c->backup_pid,
+                          (size_t)s.n * 4, hipMemcpyDeviceToDevice, c->stream);
+  };
+  HIPCHK(c, copy_state(true));
+  const sph_batch_tuning start = c->tuning;
+  const bool was_graph = c->use_graph, was_timing = c->timing;
+  if ((rc = collect_timings(c))) return rc;
+  c->use_graph = true;
+  c->timing = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  int best_i = -1;
+  double best_ms = 0;
+  for (int i = 0; i < n_candidates && rc == SPH_OK; ++i) {
+    if (ms_per_step) ms_per_step[i] = -1.0;
+    if (batch_configure(c, candidates[i]) != SPH_OK) continue;  // not applicable
+    hipError_t e = copy_state(false);
+    if (e != hipSuccess) { rc = hip_fail(c, e, "restore state"); break; }
+    if ((rc = sph_prepare(c)) || (rc = sph_step(c, 2))) break;
+    if ((e = hipEventRecord(e0, c->stream)) != hipSuccess) { rc = hip_fail(c, e, "event"); break; }
+    if ((rc = sph_step(c, trial_steps))) break;
+    if ((e = hipEventRecord(e1, c->stream)) != hipSuccess) { rc = hip_fail(c, e, "event"); break; }
+    if ((rc = sync_and_check(c))) break;
+    float ms = 0;
+    if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) { rc = hip_fail(c, e, "timing"); break; }
+    const double per = (double)ms / trial_steps;
+    if (ms_per_step) ms_per_step[i] = per;
+    if (best_i < 0 || per < best_ms) {
+      best_i = i;
+      best_ms = per;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->use_graph = was_graph;
+  c->timing = was_timing;
+  if (rc == SPH_OK && best_i < 0) rc = fail(c, SPH_ERR_INVALID, "no candidate could be applied");
+  const std::string err = c->err;
+  // leave the chosen (or, after a failure, the starting) tuning and the saved state
+  int rc2 = batch_configure(c, rc == SPH_OK ? candidates[best_i] : start);
+  if (rc2 == SPH_OK) {
+    hipError_t e = copy_state(false);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc2 = hip_fail(c, e, "restore state");
+  }
+  if (rc != SPH_OK) {
+    c->err = err;
+    return rc;
+  }
+  if (rc2 != SPH_OK) return rc2;
+  if (best) *best = best_i;
+  return SPH_OK;
+}
+
+int sph_batch_measure(void* ctx, double* out) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !out) return SPH_ERR_INVALID;
+  if (!c->n_scenes) return fail(c, SPH_ERR_STATE, "not a batch context");
+  int rc = check_n(c);
+  if (rc) return rc;
+  {
+    StageTimer t(c, SPH_T_MEASURE);
+    for (int i = 0; i < c->n_scenes; ++i) {
+      // scene i occupies [first, first + count) of every particle array
+      DevState v = c->s;
+      const int64_t f = c->scene_first[i];
+      v.rx += f; v.ry += f; v.vx += f; v.vy += f;
+      v.n = (int)c->scene_count[i];
+      Launch L = c->L;
+      L.phys = c->scene_dev[i].c;
+      HIPCHK(c, launch_measure(L, v, c->scene_prm[i].mass, c->scene_prm[i].gravity_y,
+                               c->meas_part, c->meas_out + 5 * (int64_t)i));
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(out, c->meas_out, 5 * sizeof(double) * c->n_scenes,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
 }
 
 }  // extern "C"

@@ -210,4 +210,113 @@ __device__ __forceinline__ void for_each_neighbor(const DevGrid& g, const DevPhy
   }
 }
 
+constexpr int kEllStage = 16;  // list slots per thread staged in LDS before the store
+
+// Cell ranges of the 3x3 stencil of position (x, y) in visiting order (x offset
+// outer, y offset inner); cells outside a walled box and empty cells give the
+// empty range (0, 0). Issuing the nine loads together keeps them in flight at once.
+template <int BC>
+__device__ __forceinline__ void stencil_ranges(const DevGrid& g, const int2* __restrict__ cells,
+                                               float x, float y, int2* rng) {
+  const int cx = cell_coord(x, g.lo_x, g.cs_x, g.nx);
+  const int cy = cell_coord(y, g.lo_y, g.cs_y, g.ny);
+#pragma unroll
+  for (int ox = -1; ox <= 1; ++ox) {
+#pragma unroll
+    for (int oy = -1; oy <= 1; ++oy) {
+      int cxj = cx + ox, cyj = cy + oy;
+      bool ok = true;
+      if (BC == 1) {
+        cxj = cxj < 0 ? cxj + g.nx : (cxj >= g.nx ? cxj - g.nx : cxj);
+        cyj = cyj < 0 ? cyj + g.ny : (cyj >= g.ny ? cyj - g.ny : cyj);
+      } else {
+        ok = cxj >= 0 && cxj < g.nx && cyj >= 0 && cyj < g.ny;
+      }
+      int2 se = ok ? cells[cell_key(g, cxj, cyj)] : make_int2(-1, -1);
+      if (se.x < 0 || se.y < 0) se = make_int2(0, 0);
+      rng[(ox + 1) * 3 + (oy + 1)] = se;
+    }
+  }
+}
+
+// Kernel sum of the summation density of sorted particle i (self term first,
+// then the neighbors in stencil order), writing the neighbor indices (self
+// excluded) to the column-major fixed-width list: slot k of i at ell_idx[k*n+i].
+// The first kEllStage slots go through stage[k * stride] (this thread's LDS
+// column) and are stored slot by slot at the end, so a wavefront's stores of one
+// slot are consecutive addresses. Returns the sum; ell_count[i] gets the count
+// (which may exceed ell_w: the force pass then walks the stencil instead).
+template <int BC>
+__device__ __forceinline__ float density_ell(const DevGrid& g, const DevPhys& c,
+                                             const float* __restrict__ rx,
+                                             const float* __restrict__ ry,
+                                             const int2* __restrict__ cells, int n, int i,
+                                             float xi, float yi, int* stage, int stride,
+                                             int32_t* __restrict__ ell_count,
+                                             int32_t* __restrict__ ell_idx, int ell_w) {
+  float acc = density_term(c.h2, 0.0f);
+  int2 rng[9];
+  stencil_ranges<BC>(g, cells, xi, yi, rng);
+  int cnt = 0;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const int je = min(rng[q].y, n);
+    for (int j = rng[q].x; j < je; ++j) {
+      if (j == i) continue;
+      float dx, dy;
+      separation<BC>(c, xi, yi, rx[j], ry[j], dx, dy);
+      const float r2 = dx * dx + dy * dy;
+      if (r2 > c.h2) continue;
+      acc += density_term(c.h2, r2);
+      if (cnt < kEllStage) stage[cnt * stride] = j;
+      else if (cnt < ell_w) ell_idx[(int64_t)cnt * n + i] = j;
+      ++cnt;
+    }
+  }
+  ell_count[i] = cnt;
+  const int staged = min(cnt, min(kEllStage, ell_w));
+  for (int k = 0; k < staged; ++k) ell_idx[(int64_t)k * n + i] = stage[k * stride];
+  return acc;
+}
+
+// Pressure (+ viscous) sum of sorted particle i over its fixed-width list, or
+// over the stencil when the list overflowed.
+template <int PHYS, int BC>
+__device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
+                                          const float* __restrict__ rx,
+                                          const float* __restrict__ ry,
+                                          const float* __restrict__ vx,
+                                          const float* __restrict__ vy,
+                                          const float* __restrict__ rho,
+                                          const float* __restrict__ pr,
+                                          const int2* __restrict__ cells,
+                                          const int32_t* __restrict__ ell_count,
+                                          const int32_t* __restrict__ ell_idx, int ell_w, int n,
+                                          int i, float xi, float yi, float vxi, float vyi,
+                                          float rhoi, float pri) {
+  Acc a{0.f, 0.f, 0.f, 0.f};
+  auto pair = [&](int j, float dx, float dy, float r2) {
+    force_pair<PHYS>(c, dx, dy, r2, rhoi, rho[j], pri, pr[j], vxi, vyi, vx[j], vy[j], a);
+  };
+  auto from_list = [&](int j) {
+    float dx, dy;
+    separation<BC>(c, xi, yi, rx[j], ry[j], dx, dy);
+    pair(j, dx, dy, dx * dx + dy * dy);
+  };
+  const int cnt = ell_count[i];
+  if (cnt <= ell_w) {
+    // first 8 slots fetched up front so their neighbor loads can all be in flight
+    int jl[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) jl[k] = k < cnt ? ell_idx[(int64_t)k * n + i] : i;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < cnt) from_list(jl[k]);
+    for (int k = 8; k < cnt; ++k) from_list(ell_idx[(int64_t)k * n + i]);
+  } else {
+    for_each_neighbor<BC>(g, c, rx, ry, cells, n, i, xi, yi, pair);
+  }
+  return a;
+}
+
 }  // namespace sphamd

@@ -142,4 +142,38 @@ hipError_t launch_measure(const Launch& L, const DevState& s, double mass, doubl
                           double* partials, double* out5);
 hipError_t launch_ell_overflow(const Launch& L, const DevState& s, int64_t* out_count);
 
+// ------------------------------------------------------------ batched scenes
+// Per-scene constants of a batch: the scene's grid and physics, and the offset
+// of its cell table in the shared key space (keys of scene s lie in
+// [base, base + its table size), so one global sort keeps every scene's
+// particles in one contiguous, fixed segment of the sorted order).
+struct SceneDev {
+  DevGrid g;
+  DevPhys c;
+  uint64_t base;
+};
+
+// One workgroup's particles: [begin, end) of one scene's segment.
+struct TileDev {
+  int32_t scene, begin, end, pad;
+};
+
+struct BatchLaunch {
+  const SceneDev* scenes;
+  const TileDev* tiles;
+  int n_tiles;
+  int tile;  // workgroup size: 64, 128, 256 or 512
+};
+
+constexpr int kMaxTile = 512;
+
+// keys of S (scene base + cell key in the scene's grid)
+hipError_t launch_batch_keys(const Launch& L, const BatchLaunch& B, const DevState& s,
+                             uint64_t* keys_out);
+// density + EOS + fixed-width list of every scene
+hipError_t launch_batch_density(const Launch& L, const BatchLaunch& B, const DevState& s);
+// forces + integration into T + next keys of every scene
+hipError_t launch_batch_forces(const Launch& L, const BatchLaunch& B, const DevState& s,
+                               uint64_t* keys_out);
+
 }  // namespace sphamd

@@ -116,34 +116,6 @@ __global__ __launch_bounds__(BLK) void k_ranges(int n, const uint64_t* __restric
   write_ranges(i, n, keys, cells, n_keys, blocks, block_bits);
 }
 
-constexpr int ELL_STAGE = 16;  // list slots staged in LDS per thread
-
-// Cell ranges of the 3x3 stencil of position (x, y) in visiting order (x offset
-// outer, y offset inner); cells outside a walled box get an empty range.
-template <int BC>
-__device__ __forceinline__ void stencil_ranges(const DevGrid& g, const int2* __restrict__ cells,
-                                               float x, float y, int2* rng) {
-  const int cx = cell_coord(x, g.lo_x, g.cs_x, g.nx);
-  const int cy = cell_coord(y, g.lo_y, g.cs_y, g.ny);
-#pragma unroll
-  for (int ox = -1; ox <= 1; ++ox) {
-#pragma unroll
-    for (int oy = -1; oy <= 1; ++oy) {
-      int cxj = cx + ox, cyj = cy + oy;
-      bool ok = true;
-      if (BC == 1) {
-        cxj = cxj < 0 ? cxj + g.nx : (cxj >= g.nx ? cxj - g.nx : cxj);
-        cyj = cyj < 0 ? cyj + g.ny : (cyj >= g.ny ? cyj - g.ny : cyj);
-      } else {
-        ok = cxj >= 0 && cxj < g.nx && cyj >= 0 && cyj < g.ny;
-      }
-      int2 se = ok ? cells[cell_key(g, cxj, cyj)] : make_int2(-1, -1);
-      if (se.x < 0 || se.y < 0) se = make_int2(0, 0);
-      rng[(ox + 1) * 3 + (oy + 1)] = se;
-    }
-  }
-}
-
 // Summation density + EOS. NB: stencil traversal, CSR rows, or stencil with the
 // fixed-width (column-major) neighbor list written on the way.
 template <int PHYS, int BC, int NB>
@@ -168,31 +140,9 @@ __global__ __launch_bounds__(BLK) void k_density(
     for_each_neighbor<BC>(g, c, rx, ry, cells, n, i, xi, yi,
                           [&](int, float, float, float r2) { acc += density_term(c.h2, r2); });
   } else {
-    // the 9 stencil ranges are loaded up front (independent loads), the list is
-    // staged per thread in LDS and written slot by slot so a wavefront's stores
-    // of one slot are consecutive addresses
-    __shared__ int s_list[ELL_STAGE * BLK];
-    int2 rng[9];
-    stencil_ranges<BC>(g, cells, xi, yi, rng);
-    int cnt = 0;
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      const int je = min(rng[q].y, n);
-      for (int j = rng[q].x; j < je; ++j) {
-        if (j == i) continue;
-        float dx, dy;
-        separation<BC>(c, xi, yi, rx[j], ry[j], dx, dy);
-        const float r2 = dx * dx + dy * dy;
-        if (r2 > c.h2) continue;
-        acc += density_term(c.h2, r2);
-        if (cnt < ELL_STAGE) s_list[cnt * BLK + threadIdx.x] = j;
-        else if (cnt < ell_w) ell_idx[(int64_t)cnt * n + i] = j;
-        ++cnt;
-      }
-    }
-    ell_count[i] = cnt;
-    const int staged = min(cnt, min(ELL_STAGE, ell_w));
-    for (int k = 0; k < staged; ++k) ell_idx[(int64_t)k * n + i] = s_list[k * BLK + threadIdx.x];
+    __shared__ int s_list[kEllStage * BLK];
+    acc = density_ell<BC>(g, c, rx, ry, cells, n, i, xi, yi, s_list + threadIdx.x, BLK,
+                          ell_count, ell_idx, ell_w);
   }
   float rho, p, pr;
   eos<PHYS>(c, acc, rho, p, pr);
@@ -228,19 +178,8 @@ __global__ __launch_bounds__(BLK) void k_forces(
   if (NB == NB_CSR) {
     for (int64_t k = csr_off[i]; k < csr_off[i + 1]; ++k) from_list(csr_idx[k]);
   } else if (NB == NB_ELL) {
-    const int cnt = ell_count[i];
-    if (cnt <= ell_w) {
-      // first 8 slots fetched up front so their neighbor loads can all be in flight
-      int jl[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) jl[k] = k < cnt ? ell_idx[(int64_t)k * n + i] : i;
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (k < cnt) from_list(jl[k]);
-      for (int k = 8; k < cnt; ++k) from_list(ell_idx[(int64_t)k * n + i]);
-    } else {
-      for_each_neighbor<BC>(g, c, rx, ry, cells, n, i, xi, yi, pair);
-    }
+    a = forces_ell<PHYS, BC>(g, c, rx, ry, vx, vy, rho, pr, cells, ell_count, ell_idx, ell_w, n,
+                             i, xi, yi, vxi, vyi, rhoi, pri);
   } else {
     for_each_neighbor<BC>(g, c, rx, ry, cells, n, i, xi, yi, pair);
   }

@@ -143,6 +143,69 @@ def dam_break_1m(seed: int = 0x5EED0004, steps: int = 5000) -> Scene:
     return dam_break(1024, 1024, box=(4.0, 2.0), seed=seed, steps=steps, name="dam_break_1m")
 
 
+@dataclass
+class BatchScene:
+    """Independent scenes advanced together; arrays concatenated in scene order."""
+    name: str
+    params: list
+    counts: np.ndarray
+    rx: np.ndarray
+    ry: np.ndarray
+    vx: np.ndarray
+    vy: np.ndarray
+    steps: int
+
+    @property
+    def n(self) -> int:
+        return int(self.rx.size)
+
+    @property
+    def first(self) -> np.ndarray:
+        return np.concatenate([[0], np.cumsum(self.counts)[:-1]]).astype(np.int64)
+
+    def scene(self, s: int) -> Scene:
+        a, b = int(self.first[s]), int(self.first[s] + self.counts[s])
+        return Scene(f"{self.name}[{s}]", self.params[s], self.rx[a:b], self.ry[a:b],
+                     self.vx[a:b], self.vy[a:b], self.steps)
+
+
+def batched_dam_breaks(n_scenes: int = 256, seed: int = 0x5EED0002, n_lo: int = 2048,
+                       n_hi: int = 65536, steps: int = 500, name: str = "batched_dam_breaks",
+                       order=ORDER_MORTON) -> BatchScene:
+    """Independent dam breaks in their own unit boxes. Scene s draws (from
+    uniform(seed), three numbers per scene) a particle count log-uniform in
+    [n_lo, n_hi], a block aspect ratio A = height/width uniform in [0.5, 2] and a
+    block width W uniform in [0.2, 0.5]. Lattice nx = round(sqrt(n / A)),
+    ny = round(n / nx) (at most 1/dx), dx = W / nx from the origin with +-0.01 dx
+    jitter (seeded seed + 1 + s); h = 1.3 dx, mass 1000 dx^2, Tait gamma 7 with
+    c0 = 10 sqrt(2 g H) for the block height H = ny dx, dt = 0.25 h / c0,
+    reflective walls, gravity -9.81."""
+    u = uniform(seed, 3 * n_scenes)
+    params, xs, ys = [], [], []
+    counts = np.zeros(n_scenes, np.int64)
+    for s in range(n_scenes):
+        n = round(math.exp(math.log(n_lo) + u[3 * s] * (math.log(n_hi) - math.log(n_lo))))
+        A = 0.5 + 1.5 * u[3 * s + 1]
+        W = 0.2 + 0.3 * u[3 * s + 2]
+        nx = max(1, round(math.sqrt(n / A)))
+        dx = W / nx
+        ny = max(1, min(round(n / nx), int(math.floor(1.0 / dx))))
+        X, Y = lattice(nx, ny, dx, dx)
+        jx, jy = _jitter(seed + 1 + s, X.size, 0.01 * dx)
+        p = wcsph_params(dx, (0.0, 0.0), (1.0, 1.0), order=order)
+        H = ny * dx
+        p.c0 = 10.0 * math.sqrt(2.0 * 9.81 * H)
+        p.dt = 0.25 * p.h / p.c0
+        params.append(p)
+        xs.append((X + jx).astype(np.float32))
+        ys.append((Y + jy).astype(np.float32))
+        counts[s] = X.size
+    rx = np.concatenate(xs)
+    ry = np.concatenate(ys)
+    z = np.zeros(rx.size, np.float32)
+    return BatchScene(name, params, counts, rx, ry, z, z.copy(), steps)
+
+
 def by_name(name: str) -> Scene:
     return {
         "reference_example": reference_example,
@@ -152,6 +215,6 @@ def by_name(name: str) -> Scene:
     }[name]()
 
 
-__all__ = ["Scene", "lattice", "reference_example", "dam_break", "periodic_shear",
+__all__ = ["Scene", "BatchScene", "batched_dam_breaks", "lattice", "reference_example", "dam_break", "periodic_shear",
            "dam_break_1m", "wcsph_params", "splitmix64", "uniform", "gaussian", "by_name",
            "ORDER_MORTON", "ORDER_ROWMAJOR"]

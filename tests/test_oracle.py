@@ -376,3 +376,27 @@ def test_fp32_restatement_tracks_fp64(sph, orc):
     amag = np.sqrt(a64["ax"] ** 2 + a64["ay"] ** 2)
     err = np.hypot(a32["ax"][o32] - a64["ax"][o64], a32["ay"][o32] - a64["ay"][o64])
     assert err.max() < 1e-3 * amag.max()
+
+
+def test_batched_scene_generator(sph):
+    """Seeded, deterministic, counts within the drawn range, blocks inside the
+    unit box, and per-scene physics from each scene's own lattice spacing."""
+    b1 = sph.scenes.batched_dam_breaks(n_scenes=16)
+    b2 = sph.scenes.batched_dam_breaks(n_scenes=16)
+    assert np.array_equal(b1.rx, b2.rx) and np.array_equal(b1.counts, b2.counts)
+    assert b1.n == int(b1.counts.sum()) and b1.rx.dtype == np.float32
+    assert np.all(b1.counts >= 1900) and np.all(b1.counts <= 66000)
+    for s in range(16):
+        sc = b1.scene(s)
+        p = sc.params
+        dx = p.h / 1.3
+        assert abs(p.mass - 1000.0 * dx * dx) < 1e-12
+        assert abs(p.dt - 0.25 * p.h / p.c0) < 1e-15
+        assert sc.ry.max() < 1.0 and sc.rx.max() < 0.51
+    assert len({p.h for p in b1.params}) == 16
+
+
+def test_batch_default_candidates(sph):
+    c = sph.Batch.default_candidates()
+    assert len(c) == 24 and len(set(c)) == 24
+    assert {t for _, t, _ in c} == {64, 128, 256, 512}
