@@ -243,6 +243,73 @@ int sph_batch_autotune(void* ctx, const sph_batch_tuning* candidates, int32_t n_
 /* sph_measure per scene: out[5 * s + k] as out5 of sph_measure. Synchronises. */
 int sph_batch_measure(void* ctx, double* out);
 
+/* ---- device-resident state exchange (no reference counterpart) ----
+ * For data pipelines that keep snapshots in device memory: export writes the
+ * current state into caller device memory out[4 n] = rx | ry | vx | vy in
+ * particle-id order; upload_device replaces the state from caller device arrays
+ * (ids 0..n-1), like sph_upload. Both synchronise the context stream. */
+int sph_export_state(void* ctx, float* d_out);
+int sph_upload_device(void* ctx, int64_t n, const float* d_rx, const float* d_ry,
+                      const float* d_vx, const float* d_vy);
+
+/* ---- learned surrogate (no reference counterpart) ----
+ * A message-passing network over the fixed-width neighbor lists predicts each
+ * particle's acceleration from invariant features (distances, relative
+ * velocities, densities and p / rho^2 of up to ell_width neighbors) with an
+ * equivariant readout (separation vectors times learned edge scalars). Node
+ * linear layers run as BF16 MFMA GEMMs with FP32 accumulation, edge layers in
+ * FP32, training with Adam on FP32 master weights. A network handle is tied to
+ * no context; every compute call takes the context whose sorted state (after
+ * sph_prepare / sph_density_pressure) it reads and whose stream it uses. */
+typedef struct sph_nn_config {
+  int32_t width;     /* hidden width; must be 128 */
+  int32_t layers;    /* message-passing layers, 1..8 */
+  int32_t train;     /* 1: keep per-layer activations for backpropagation */
+  int32_t periodic;  /* minimum-image separations in a periodic box */
+  int64_t max_nodes; /* largest particle count of a call */
+  double h;          /* smoothing length of the scenes it is used on */
+  double rho0, c0;   /* density and pressure feature scales */
+  double v_scale;    /* relative-velocity feature scale */
+  double a_scale;    /* acceleration scale of the readout and the loss */
+  double gravity_x, gravity_y; /* added to the readout */
+  double len_x, len_y;         /* periodic box lengths (periodic = 1) */
+} sph_nn_config;
+
+int sph_nn_create(const sph_nn_config* cfg, int device, void** out_nn);
+int sph_nn_destroy(void* nn);
+const char* sph_nn_last_error(const void* nn); /* nn may be NULL (creation errors) */
+int64_t sph_nn_num_params(const void* nn);
+/* FP32 parameters in the order of the network's parameter layout. */
+int sph_nn_set_params(void* nn, const float* host, int64_t count);
+int sph_nn_get_params(void* nn, float* host, int64_t count);
+int sph_nn_get_grad(void* nn, float* host, int64_t count); /* last sph_nn_loss_grad */
+int sph_nn_reset_optimizer(void* nn);
+/* Predicted accelerations of the context's sorted particles; ax, ay (host, n
+ * each, sorted order) may be NULL. Needs the fixed-width list (sph_prepare or
+ * sph_density_pressure on a batch / after sph_prepare). */
+int sph_nn_forward(void* nn, void* ctx, float* ax, float* ay);
+/* Forward, loss against the context's accelerations (call sph_forces first),
+ * backward into the gradient. out3 = {loss, RMS relative error, RMS target}.
+ * Needs symmetric lists: fails if a list overflowed ell_width. Synchronises. */
+int sph_nn_loss_grad(void* nn, void* ctx, double* out3);
+int sph_nn_adam(void* nn, void* ctx, double lr, double beta1, double beta2, double eps);
+/* loss_grad + adam(lr, 0.9, 0.999, 1e-8). */
+int sph_nn_train_step(void* nn, void* ctx, double lr, double* out3);
+
+/* Surrogate in the loop: n_steps steps of the context where the network supplies
+ * the accelerations; every check_every steps the plain SPH forces are computed
+ * as well and the RMS relative error sqrt(sum |a_nn - a|^2 / sum |a|^2) (FP64)
+ * is compared with threshold. mode 0 (step): a failing check step integrates
+ * with the plain forces. mode 1 (window): a failing check restores the state at
+ * the start of the window and advances the window with plain SPH. Needs
+ * sph_prepare first; leaves the context prepared for sph_step. */
+typedef struct sph_surrogate_stats {
+  int64_t steps, checks, failed_checks, nn_steps, plain_steps;
+  double last_error, max_error, mean_error;
+} sph_surrogate_stats;
+int sph_surrogate_run(void* ctx, void* nn, int32_t n_steps, int32_t check_every,
+                      double threshold, int32_t mode, sph_surrogate_stats* stats);
+
 /* ---- inspection (host copies; synchronise) ---- */
 int sph_get_sorted_keys(void* ctx, uint64_t* keys);  /* n keys of the last sort */
 int sph_get_permutation(void* ctx, uint32_t* perm);  /* sorted slot -> previous index */

@@ -58,6 +58,24 @@ class BatchTuning(C.Structure):
         return (float(self.cell_factor), int(self.tile), int(self.resort))
 
 
+class NnConfig(C.Structure):
+    _fields_ = [("width", C.c_int32), ("layers", C.c_int32), ("train", C.c_int32),
+                ("periodic", C.c_int32), ("max_nodes", C.c_int64), ("h", C.c_double),
+                ("rho0", C.c_double), ("c0", C.c_double), ("v_scale", C.c_double),
+                ("a_scale", C.c_double), ("gravity_x", C.c_double), ("gravity_y", C.c_double),
+                ("len_x", C.c_double), ("len_y", C.c_double)]
+
+
+class SurrogateStats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("checks", C.c_int64), ("failed_checks", C.c_int64),
+                ("nn_steps", C.c_int64), ("plain_steps", C.c_int64),
+                ("last_error", C.c_double), ("max_error", C.c_double),
+                ("mean_error", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class DeviceView(C.Structure):
     _fields_ = [
         ("n", C.c_int64), ("rx", C.c_void_p), ("ry", C.c_void_p), ("vx", C.c_void_p),
@@ -162,6 +180,21 @@ def lib():
         "sph_batch_default_candidates": (i32, [P(BatchTuning), i32]),
         "sph_batch_autotune": (i32, [vp, P(BatchTuning), i32, i32, P(d), P(i32)]),
         "sph_batch_measure": (i32, [vp, P(d)]),
+        "sph_export_state": (i32, [vp, vp]),
+        "sph_upload_device": (i32, [vp, i64, vp, vp, vp, vp]),
+        "sph_nn_create": (i32, [P(NnConfig), C.c_int, P(vp)]),
+        "sph_nn_destroy": (i32, [vp]),
+        "sph_nn_last_error": (C.c_char_p, [vp]),
+        "sph_nn_num_params": (i64, [vp]),
+        "sph_nn_set_params": (i32, [vp, vp, i64]),
+        "sph_nn_get_params": (i32, [vp, vp, i64]),
+        "sph_nn_get_grad": (i32, [vp, vp, i64]),
+        "sph_nn_reset_optimizer": (i32, [vp]),
+        "sph_nn_forward": (i32, [vp, vp, vp, vp]),
+        "sph_nn_loss_grad": (i32, [vp, vp, P(d)]),
+        "sph_nn_adam": (i32, [vp, vp, d, d, d, d]),
+        "sph_nn_train_step": (i32, [vp, vp, d, P(d)]),
+        "sph_surrogate_run": (i32, [vp, vp, i32, i32, d, i32, P(SurrogateStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -181,7 +214,11 @@ EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_
             "sph_get_device_view", "sph_set_timing", "sph_stage_times", "sph_synchronize",
             "sph_batch_create", "sph_batch_scenes", "sph_batch_scene_info",
             "sph_batch_set_tuning", "sph_batch_get_tuning", "sph_batch_default_candidates",
-            "sph_batch_autotune", "sph_batch_measure")
+            "sph_batch_autotune", "sph_batch_measure", "sph_export_state",
+            "sph_upload_device", "sph_nn_create", "sph_nn_destroy", "sph_nn_last_error",
+            "sph_nn_num_params", "sph_nn_set_params", "sph_nn_get_params", "sph_nn_get_grad",
+            "sph_nn_reset_optimizer", "sph_nn_forward", "sph_nn_loss_grad", "sph_nn_adam",
+            "sph_nn_train_step", "sph_surrogate_run")
 
 
 def _ptr(a):
@@ -383,6 +420,15 @@ class Context:
         cnt = np.empty(self.n, np.int32)
         self._chk(self._L.sph_get_ell(self._h, _ptr(cnt), None), "sph_get_ell")
         return cnt
+
+    def export_state(self, dev_ptr: int):
+        """State (rx | ry | vx | vy, particle-id order) into device memory at dev_ptr."""
+        self._chk(self._L.sph_export_state(self._h, C.c_void_p(dev_ptr)), "sph_export_state")
+
+    def upload_device(self, n: int, rx_ptr: int, ry_ptr: int, vx_ptr: int, vy_ptr: int):
+        self._chk(self._L.sph_upload_device(self._h, int(n), C.c_void_p(rx_ptr),
+                                            C.c_void_p(ry_ptr), C.c_void_p(vx_ptr),
+                                            C.c_void_p(vy_ptr)), "sph_upload_device")
 
     def device_view(self) -> DeviceView:
         v = DeviceView()

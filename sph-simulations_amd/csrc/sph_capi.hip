@@ -11,6 +11,7 @@
 
 #include "../../include/sph_amd.h"
 #include "sph_internal.hpp"
+#include "sph_nn.hpp"
 
 using namespace sphamd;
 
@@ -387,6 +388,8 @@ int incremental_resort(Ctx* c) {
   return SPH_OK;
 }
 
+int resort_integrated(Ctx* c);
+
 // One fused iteration: forces(ELL) + integrate + keys -> sort -> gather T->S
 // + ranges -> density + EOS + ELL.
 int fused_iteration(Ctx* c) {
@@ -422,6 +425,12 @@ int fused_iteration(Ctx* c) {
     StageTimer t(c, SPH_T_FORCE);
     HIPCHK(c, launch_forces(c->L, s, NB_ELL, true, c->sb.keys_in));
   }
+  return resort_integrated(c);
+}
+
+// Re-sort the integrated twin T (keys in sb.keys_in) into S with new ranges.
+int resort_integrated(Ctx* c) {
+  DevState& s = c->s;
   if (c->resort == SPH_RESORT_INCREMENTAL) return incremental_resort(c);
   int rc = sort_keys(c);
   if (rc) return rc;
@@ -674,6 +683,39 @@ int batch_configure(Ctx* c, const sph_batch_tuning& t) {
   c->sorted = false;
   c->density_ell = false;
   c->csr_total = -1;
+  return SPH_OK;
+}
+
+
+// ---- learned surrogate handles
+thread_local std::string g_nn_create_error;
+
+struct NnH {
+  Net net;
+};
+
+NnH* as_nn(void* p) { return static_cast<NnH*>(p); }
+
+int nn_fail(NnH* h, int code, const std::string& msg) {
+  if (h) h->net.err = msg;
+  return code;
+}
+
+#define NNCHK(h, expr)                                                               \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return nn_fail(h, SPH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Shared checks of the calls that run the network on a context's sorted state.
+int nn_ctx_check(NnH* h, Ctx* c, bool need_train) {
+  if (!h || !c) return SPH_ERR_INVALID;
+  if (c->s.n <= 0) return nn_fail(h, SPH_ERR_STATE, "context holds no particles");
+  if (!c->sorted || !c->density_ell)
+    return nn_fail(h, SPH_ERR_STATE, "context needs a fixed-width neighbor list (sph_prepare)");
+  if (c->s.n > h->net.n_cap) return nn_fail(h, SPH_ERR_CAPACITY, "more particles than max_nodes");
+  if (need_train && !h->net.train) return nn_fail(h, SPH_ERR_STATE, "network created with train = 0");
   return SPH_OK;
 }
 
@@ -1445,6 +1487,307 @@ int sph_batch_measure(void* ctx, double* out) {
   HIPCHK(c, hipMemcpyAsync(out, c->meas_out, 5 * sizeof(double) * c->n_scenes,
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int sph_export_state(void* ctx, float* d_out) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !d_out) return SPH_ERR_INVALID;
+  HIPCHK(c, launch_export_state(c->L, c->s, d_out));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int sph_upload_device(void* ctx, int64_t n, const float* d_rx, const float* d_ry,
+                      const float* d_vx, const float* d_vy) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  if (n < 0 || n > c->n_max) return fail(c, SPH_ERR_CAPACITY, "n exceeds the context capacity");
+  if (c->n_scenes && n != c->n_max)
+    return fail(c, SPH_ERR_INVALID, "a batch upload holds every scene's particles");
+  if (n > 0 && (!d_rx || !d_ry || !d_vx || !d_vy))
+    return fail(c, SPH_ERR_INVALID, "null position or velocity array");
+  DevState& s = c->s;
+  s.n = (int)n;
+  const size_t fb = (size_t)n * sizeof(float);
+  HIPCHK(c, hipMemcpyAsync(s.rx, d_rx, fb, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.ry, d_ry, fb, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.vx, d_vx, fb, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.vy, d_vy, fb, hipMemcpyDeviceToDevice, c->stream));
+  std::vector<uint32_t> ids(n);
+  for (int64_t i = 0; i < n; ++i) ids[i] = (uint32_t)i;
+  HIPCHK(c, hipMemcpyAsync(s.pid, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(s.ax, 0, fb, c->stream));
+  HIPCHK(c, hipMemsetAsync(s.ay, 0, fb, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->sorted = false;
+  c->density_ell = false;
+  c->csr_total = -1;
+  if (c->graph_n != s.n) destroy_graph(c);
+  return SPH_OK;
+}
+
+int sph_nn_create(const sph_nn_config* cfg, int device, void** out_nn) {
+  g_nn_create_error.clear();
+  if (!cfg || !out_nn) {
+    g_nn_create_error = "sph_nn_create: invalid arguments";
+    return SPH_ERR_INVALID;
+  }
+  const sph_nn_config& q = *cfg;
+  if (q.width != kNnWidth || q.layers < 1 || q.layers > 8 || q.max_nodes < 1 ||
+      q.max_nodes >= ((int64_t)1 << 30) || !(q.h > 0) || !(q.rho0 > 0) || !(q.c0 > 0) ||
+      !(q.v_scale > 0) || !(q.a_scale > 0) ||
+      (q.periodic && (!(q.len_x > 0) || !(q.len_y > 0)))) {
+    g_nn_create_error = "sph_nn_create: width must be 128, layers 1..8, positive scales";
+    return SPH_ERR_INVALID;
+  }
+  NnConsts k{};
+  k.h = (float)q.h;
+  k.inv_h = (float)(1.0 / q.h);
+  k.rho0 = (float)q.rho0;
+  k.inv_rho0 = (float)(1.0 / q.rho0);
+  k.rs = 50.0f;
+  k.ps = (float)(50.0 * q.rho0 / (q.c0 * q.c0));
+  k.inv_vs = (float)(1.0 / q.v_scale);
+  k.a_s = (float)q.a_scale;
+  k.gx = (float)q.gravity_x;
+  k.gy = (float)q.gravity_y;
+  k.len_x = (float)q.len_x;
+  k.len_y = (float)q.len_y;
+  k.half_x = k.len_x * 0.5f;
+  k.half_y = k.len_y * 0.5f;
+  k.periodic = q.periodic ? 1 : 0;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    g_nn_create_error = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    return SPH_ERR_HIP;
+  }
+  NnH* h = new NnH();
+  h->net.device = device;
+  if (nn_create(&h->net, q.layers, q.train != 0, q.max_nodes, k) != 0) {
+    g_nn_create_error = h->net.err;
+    nn_free(&h->net);
+    delete h;
+    return SPH_ERR_HIP;
+  }
+  *out_nn = h;
+  return SPH_OK;
+}
+
+int sph_nn_destroy(void* nn) {
+  NnH* h = as_nn(nn);
+  if (!h) return SPH_OK;
+  (void)hipDeviceSynchronize();
+  nn_free(&h->net);
+  delete h;
+  return SPH_OK;
+}
+
+const char* sph_nn_last_error(const void* nn) {
+  const NnH* h = static_cast<const NnH*>(nn);
+  return h ? h->net.err.c_str() : g_nn_create_error.c_str();
+}
+
+int64_t sph_nn_num_params(const void* nn) {
+  const NnH* h = static_cast<const NnH*>(nn);
+  return h ? h->net.n_params : SPH_ERR_INVALID;
+}
+
+int sph_nn_set_params(void* nn, const float* host, int64_t count) {
+  NnH* h = as_nn(nn);
+  if (!h || !host) return SPH_ERR_INVALID;
+  if (count != h->net.n_params) return nn_fail(h, SPH_ERR_INVALID, "parameter count mismatch");
+  NNCHK(h, hipDeviceSynchronize());
+  NNCHK(h, hipMemcpy(h->net.theta, host, count * sizeof(float), hipMemcpyHostToDevice));
+  NNCHK(h, nn_sync_weights(&h->net, nullptr));
+  NNCHK(h, hipDeviceSynchronize());
+  return SPH_OK;
+}
+
+int sph_nn_get_params(void* nn, float* host, int64_t count) {
+  NnH* h = as_nn(nn);
+  if (!h || !host) return SPH_ERR_INVALID;
+  if (count != h->net.n_params) return nn_fail(h, SPH_ERR_INVALID, "parameter count mismatch");
+  NNCHK(h, hipDeviceSynchronize());
+  NNCHK(h, hipMemcpy(host, h->net.theta, count * sizeof(float), hipMemcpyDeviceToHost));
+  return SPH_OK;
+}
+
+int sph_nn_get_grad(void* nn, float* host, int64_t count) {
+  NnH* h = as_nn(nn);
+  if (!h || !host) return SPH_ERR_INVALID;
+  if (count != h->net.n_params) return nn_fail(h, SPH_ERR_INVALID, "parameter count mismatch");
+  NNCHK(h, hipDeviceSynchronize());
+  NNCHK(h, hipMemcpy(host, h->net.grad, count * sizeof(float), hipMemcpyDeviceToHost));
+  return SPH_OK;
+}
+
+int sph_nn_reset_optimizer(void* nn) {
+  NnH* h = as_nn(nn);
+  if (!h) return SPH_ERR_INVALID;
+  NNCHK(h, hipDeviceSynchronize());
+  NNCHK(h, hipMemset(h->net.m, 0, h->net.n_params * sizeof(float)));
+  NNCHK(h, hipMemset(h->net.v, 0, h->net.n_params * sizeof(float)));
+  h->net.adam_t = 0;
+  return SPH_OK;
+}
+
+int sph_nn_forward(void* nn, void* ctx, float* ax, float* ay) {
+  NnH* h = as_nn(nn);
+  Ctx* c = as_ctx(ctx);
+  int rc = nn_ctx_check(h, c, false);
+  if (rc) return rc;
+  Net& net = h->net;
+  float* dax = net.a_out;
+  float* day = net.a_out + net.n_cap;
+  NNCHK(h, nn_forward(&net, c->s, c->stream, dax, day));
+  const size_t fb = (size_t)c->s.n * sizeof(float);
+  if (ax) NNCHK(h, hipMemcpyAsync(ax, dax, fb, hipMemcpyDeviceToHost, c->stream));
+  if (ay) NNCHK(h, hipMemcpyAsync(ay, day, fb, hipMemcpyDeviceToHost, c->stream));
+  NNCHK(h, hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int sph_nn_loss_grad(void* nn, void* ctx, double* out3) {
+  NnH* h = as_nn(nn);
+  Ctx* c = as_ctx(ctx);
+  int rc = nn_ctx_check(h, c, true);
+  if (rc) return rc;
+  Net& net = h->net;
+  // the sender half of the edge backward walks each list backwards: the lists
+  // must be complete (no particle over ell_width neighbors)
+  int64_t over = 0;
+  NNCHK(h, hipMemsetAsync(c->ovf, 0, sizeof(int64_t), c->stream));
+  NNCHK(h, launch_ell_overflow(c->L, c->s, c->ovf));
+  NNCHK(h, hipMemcpyAsync(&over, c->ovf, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  NNCHK(h, hipStreamSynchronize(c->stream));
+  if (over != 0) return nn_fail(h, SPH_ERR_STATE, "a neighbor list overflowed ell_width");
+  NNCHK(h, nn_loss_backward(&net, c->s, c->stream, c->s.ax, c->s.ay));
+  double sums[2] = {0, 0};
+  NNCHK(h, hipMemcpyAsync(sums, net.red_out, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+  NNCHK(h, hipStreamSynchronize(c->stream));
+  if (out3) {
+    const double n = c->s.n, as = net.k.a_s;
+    out3[0] = sums[0] / (n * as * as);
+    out3[1] = sums[1] > 0 ? std::sqrt(sums[0] / sums[1]) : 0.0;
+    out3[2] = std::sqrt(sums[1] / n);
+  }
+  return SPH_OK;
+}
+
+int sph_nn_adam(void* nn, void* ctx, double lr, double beta1, double beta2, double eps) {
+  NnH* h = as_nn(nn);
+  Ctx* c = as_ctx(ctx);
+  if (!h || !c) return SPH_ERR_INVALID;
+  if (!h->net.train) return nn_fail(h, SPH_ERR_STATE, "network created with train = 0");
+  if (!(lr > 0) || !(beta1 >= 0 && beta1 < 1) || !(beta2 >= 0 && beta2 < 1) || !(eps > 0))
+    return nn_fail(h, SPH_ERR_INVALID, "invalid Adam hyper-parameters");
+  NNCHK(h, nn_adam(&h->net, c->stream, (float)lr, (float)beta1, (float)beta2, (float)eps));
+  return SPH_OK;
+}
+
+int sph_nn_train_step(void* nn, void* ctx, double lr, double* out3) {
+  int rc = sph_nn_loss_grad(nn, ctx, out3);
+  if (rc) return rc;
+  return sph_nn_adam(nn, ctx, lr, 0.9, 0.999, 1e-8);
+}
+
+int sph_surrogate_run(void* ctx, void* nn, int32_t n_steps, int32_t check_every,
+                      double threshold, int32_t mode, sph_surrogate_stats* stats) {
+  Ctx* c = as_ctx(ctx);
+  NnH* h = as_nn(nn);
+  if (!c || !h || n_steps < 0 || check_every < 1 || !(threshold > 0) || (mode != 0 && mode != 1))
+    return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
+  if (c->s.n <= 0) return fail(c, SPH_ERR_STATE, "no particles uploaded");
+  if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_prepare first");
+  if (c->s.n > h->net.n_cap) return fail(c, SPH_ERR_CAPACITY, "more particles than max_nodes");
+  DevState& s = c->s;
+  const int64_t N = c->n_max;
+  int rc = SPH_OK;
+  if (mode == 1 && !c->backup) {
+    if ((rc = dev_alloc(c, &c->backup, 6 * N)) || (rc = dev_alloc(c, &c->backup_pid, N)))
+      return rc;
+  }
+  sph_surrogate_stats st{};
+  double err_sum = 0;
+  const size_t fb = (size_t)s.n * sizeof(float);
+  auto snapshot = [&](bool save) -> hipError_t {
+    float* arrs[] = {s.rx, s.ry, s.vx, s.vy};
+    for (int k = 0; k < 4; ++k) {
+      float* bk = c->backup + (size_t)k * N;
+      hipError_t e = hipMemcpyAsync(save ? bk : arrs[k], save ? arrs[k] : bk, fb,
+                                    hipMemcpyDeviceToDevice, c->stream);
+      if (e != hipSuccess) return e;
+    }
+    return hipMemcpyAsync(save ? c->backup_pid : s.pid, save ? s.pid : c->backup_pid,
+                          (size_t)s.n * 4, hipMemcpyDeviceToDevice, c->stream);
+  };
+  double* d_sums = h->net.red_out + 2;
+  for (int t = 0; t < n_steps;) {
+    const int k = std::min(check_every, n_steps - t);
+    if (mode == 1) HIPCHK(c, snapshot(true));
+    bool failed = false;
+    int nn_done = 0, plain_done = 0;
+    for (int q = 0; q < k; ++q) {
+      {
+        StageTimer tm(c, SPH_T_DENSITY);
+        HIPCHK(c, launch_density(c->L, s, NB_ELL, false));
+      }
+      {
+        StageTimer tm(c, SPH_T_FORCE);
+        HIPCHK(c, nn_forward(&h->net, s, c->stream, s.tax, s.tay));
+      }
+      const float* ax = s.tax;
+      const float* ay = s.tay;
+      if (q == k - 1) {
+        {
+          StageTimer tm(c, SPH_T_FORCE);
+          HIPCHK(c, launch_forces(c->L, s, NB_ELL, false, nullptr));
+        }
+        double sums[2] = {0, 0};
+        HIPCHK(c, nn_error_sums(&h->net, s.n, s.tax, s.tay, s.ax, s.ay, c->stream, d_sums));
+        HIPCHK(c, hipMemcpyAsync(sums, d_sums, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const double err = sums[1] > 0 ? std::sqrt(sums[0] / sums[1]) : 0.0;
+        st.checks += 1;
+        st.last_error = err;
+        st.max_error = std::max(st.max_error, err);
+        err_sum += err;
+        if (err > threshold) {
+          st.failed_checks += 1;
+          failed = true;
+          if (mode == 1) break;
+          ax = s.ax;  // this step integrates with the plain forces
+          ay = s.ay;
+        }
+      }
+      {
+        StageTimer tm(c, SPH_T_INTEGRATION);
+        HIPCHK(c, launch_integrate_keys(c->L, s, ax, ay, c->sb.keys_in));
+      }
+      if ((rc = resort_integrated(c))) return rc;
+      if (ax == s.ax) ++plain_done; else ++nn_done;
+    }
+    if (failed && mode == 1) {
+      // discard the window: back to its start, then plain SPH for its k steps
+      HIPCHK(c, snapshot(false));
+      if ((rc = find_neighbors(c))) return rc;
+      for (int q = 0; q < k; ++q)
+        if ((rc = fused_iteration(c))) return rc;
+      nn_done = 0;
+      plain_done = k;
+    }
+    st.nn_steps += nn_done;
+    st.plain_steps += plain_done;
+    st.steps += k;
+    t += k;
+  }
+  if ((rc = sync_and_check(c))) return rc;
+  st.mean_error = st.checks ? err_sum / (double)st.checks : 0.0;
+  c->density_ell = false;
+  c->csr_total = -1;
+  if (stats) *stats = st;
   return SPH_OK;
 }
 
