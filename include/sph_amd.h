@@ -249,6 +249,11 @@ int sph_batch_measure(void* ctx, double* out);
  * particle-id order; upload_device replaces the state from caller device arrays
  * (ids 0..n-1), like sph_upload. Both synchronise the context stream. */
 int sph_export_state(void* ctx, float* d_out);
+/* Device memory for such pipelines (hipMalloc / hipFree / hipMemcpy on a device;
+ * kind: 0 host->device, 1 device->host, 2 device->device). */
+int sph_device_malloc(int device, int64_t bytes, void** out_ptr);
+int sph_device_free(void* ptr);
+int sph_device_copy(void* dst, const void* src, int64_t bytes, int32_t kind);
 int sph_upload_device(void* ctx, int64_t n, const float* d_rx, const float* d_ry,
                       const float* d_vx, const float* d_vy);
 
@@ -270,7 +275,7 @@ typedef struct sph_nn_config {
   double h;          /* smoothing length of the scenes it is used on */
   double rho0, c0;   /* density and pressure feature scales */
   double v_scale;    /* relative-velocity feature scale */
-  double a_scale;    /* acceleration scale of the readout and the loss */
+  double a_scale;    /* acceleration scale of the readout */
   double gravity_x, gravity_y; /* added to the readout */
   double len_x, len_y;         /* periodic box lengths (periodic = 1) */
 } sph_nn_config;
@@ -288,8 +293,9 @@ int sph_nn_reset_optimizer(void* nn);
  * each, sorted order) may be NULL. Needs the fixed-width list (sph_prepare or
  * sph_density_pressure on a batch / after sph_prepare). */
 int sph_nn_forward(void* nn, void* ctx, float* ax, float* ay);
-/* Forward, loss against the context's accelerations (call sph_forces first),
- * backward into the gradient. out3 = {loss, RMS relative error, RMS target}.
+/* Forward, loss against the context's accelerations t (call sph_forces first),
+ * backward into the gradient. Loss = sum |a - t|^2 / sum |t|^2 (FP64 sums).
+ * out3 = {loss, RMS relative error = sqrt(loss), RMS |t|}.
  * Needs symmetric lists: fails if a list overflowed ell_width. Synchronises. */
 int sph_nn_loss_grad(void* nn, void* ctx, double* out3);
 int sph_nn_adam(void* nn, void* ctx, double lr, double beta1, double beta2, double eps);

@@ -20,9 +20,13 @@ Network (width W, L message-passing layers, 8 node and 8 edge features):
   M_i  = D S_i + deg_i d                     (edge MLP layer 2, after the sum:
                                                linear layers commute with it)
   h'   = h + F silu(E [h, M] + e) + f                              (node MLP)
-  a_i  = g + a_s sum_j (r_i - r_j)/h (w . silu(Ao h_i + Bo h_j + Co f_ij + co) + bo)
+  a_i  = g + a_s sum_j (r_i - r_j)/h (w . silu(Ao h_i + Bo h_j + Co f_ij + co) + bo
+                                     + u . f_ij)
 The readout is an equivariant edge readout: invariant edge scalars times the
-separation vector, so the predicted acceleration rotates with the particles.
+separation vector, so the predicted acceleration rotates with the particles;
+u . f_ij is a linear path from the edge features (one of them has the form of
+the SPH pressure term). Loss: sum |a - t|^2 / sum |t|^2 over the batch (the
+squared RMS relative error).
 """
 from __future__ import annotations
 
@@ -43,7 +47,7 @@ def param_layout(width=128, layers=3):
                 (f"mp{l}_E", (W, 2 * W)), (f"mp{l}_e", (W,)), (f"mp{l}_F", (W, W)),
                 (f"mp{l}_f", (W,))]
     out += [("ro_A", (W, W)), ("ro_B", (W, W)), ("ro_C", (W, EDGE_F)), ("ro_c", (W,)),
-            ("ro_w", (1, W)), ("ro_b", (1,))]
+            ("ro_w", (1, W)), ("ro_b", (1,)), ("ro_u", (EDGE_F,))]
     return out
 
 
@@ -135,7 +139,7 @@ def edge_features(g, cfg):
     f[:, 4] = np.sqrt(dvx * dvx + dvy * dvy) / vs
     f[:, 5] = rs * (g.rho[j] / cfg["rho0"] - 1.0)
     f[:, 6] = ps * g.pr[j]
-    f[:, 7] = rs * (g.rho[i] / cfg["rho0"] - 1.0)
+    f[:, 7] = f[:, 2] * (ps * g.pr[i] + ps * g.pr[j])
     return f
 
 
@@ -181,7 +185,7 @@ class Net:
         Qo = self.G(h, p["ro_B"])
         zo = Po[g.recv] + Qo[g.send] + g.f @ r(p["ro_C"]).T + p["ro_c"]
         so = silu(zo)
-        phi = so @ r(p["ro_w"])[0] + p["ro_b"][0]
+        phi = so @ r(p["ro_w"])[0] + p["ro_b"][0] + g.f @ p["ro_u"]
         s = cfg["a_scale"] / cfg["h"]
         ax = cfg.get("gx", 0.0) + s * _segsum(g.dx * phi, g.recv, g.n)
         ay = cfg["gy"] + s * _segsum(g.dy * phi, g.recv, g.n)
@@ -190,9 +194,9 @@ class Net:
 
     def loss(self, g, target):
         a, c = self.forward(g)
-        a_s = g.cfg["a_scale"]
-        e = a - np.asarray(target, np.float64)
-        return float((e * e).sum() / (g.n * a_s * a_s)), a, c
+        t = np.asarray(target, np.float64)
+        e = a - t
+        return float((e * e).sum() / (t * t).sum()), a, c
 
     def backward(self, g, da, c):
         """Gradient of sum(da * a) with respect to the flat parameters."""
@@ -204,6 +208,7 @@ class Net:
         dphi = s * (da[g.recv, 0] * g.dx + da[g.recv, 1] * g.dy)
         grads["ro_w"] = (dphi[:, None] * c["so"]).sum(0)[None, :]
         grads["ro_b"] = np.array([dphi.sum()])
+        grads["ro_u"] = dphi @ g.f
         dzo = dphi[:, None] * r(p["ro_w"])[0][None, :] * dsilu(c["zo"])
         dPo = _segsum(dzo, g.recv, g.n)
         dQo = _segsum(dzo, g.send, g.n)
@@ -246,7 +251,7 @@ class Net:
 
     def loss_grad(self, g, target):
         L, a, c = self.loss(g, target)
-        a_s = g.cfg["a_scale"]
-        da = 2.0 * (a - np.asarray(target, np.float64)) / (g.n * a_s * a_s)
+        t = np.asarray(target, np.float64)
+        da = 2.0 * (a - t) / (t * t).sum()
         return L, self.backward(g, da, c), a
 

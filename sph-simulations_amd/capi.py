@@ -181,6 +181,9 @@ def lib():
         "sph_batch_autotune": (i32, [vp, P(BatchTuning), i32, i32, P(d), P(i32)]),
         "sph_batch_measure": (i32, [vp, P(d)]),
         "sph_export_state": (i32, [vp, vp]),
+        "sph_device_malloc": (i32, [C.c_int, i64, P(vp)]),
+        "sph_device_free": (i32, [vp]),
+        "sph_device_copy": (i32, [vp, vp, i64, i32]),
         "sph_upload_device": (i32, [vp, i64, vp, vp, vp, vp]),
         "sph_nn_create": (i32, [P(NnConfig), C.c_int, P(vp)]),
         "sph_nn_destroy": (i32, [vp]),
@@ -215,10 +218,52 @@ EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_
             "sph_batch_create", "sph_batch_scenes", "sph_batch_scene_info",
             "sph_batch_set_tuning", "sph_batch_get_tuning", "sph_batch_default_candidates",
             "sph_batch_autotune", "sph_batch_measure", "sph_export_state",
+            "sph_device_malloc", "sph_device_free", "sph_device_copy",
             "sph_upload_device", "sph_nn_create", "sph_nn_destroy", "sph_nn_last_error",
             "sph_nn_num_params", "sph_nn_set_params", "sph_nn_get_params", "sph_nn_get_grad",
             "sph_nn_reset_optimizer", "sph_nn_forward", "sph_nn_loss_grad", "sph_nn_adam",
             "sph_nn_train_step", "sph_surrogate_run")
+
+
+class DeviceBuffer:
+    """Raw device memory owned by Python (sph_device_malloc); float32 views are
+    addressed by element offset."""
+
+    def __init__(self, n_floats: int, device: int = 0):
+        self._L = lib()
+        p = C.c_void_p()
+        rc = self._L.sph_device_malloc(int(device), int(n_floats) * 4, C.byref(p))
+        if rc != SPH_OK:
+            raise SphError(f"sph_device_malloc({n_floats * 4} bytes) -> {rc}")
+        self._p = p.value
+        self.n = int(n_floats)
+
+    def ptr(self, offset: int = 0) -> int:
+        if not 0 <= offset <= self.n:
+            raise IndexError("offset outside the buffer")
+        return self._p + 4 * int(offset)
+
+    def to_host(self, offset=0, count=None) -> np.ndarray:
+        count = self.n - offset if count is None else int(count)
+        if offset + count > self.n:
+            raise IndexError("range outside the buffer")
+        out = np.empty(count, np.float32)
+        rc = self._L.sph_device_copy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr(offset)),
+                                     count * 4, 1)
+        if rc != SPH_OK:
+            raise SphError(f"sph_device_copy -> {rc}")
+        return out
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._L.sph_device_free(C.c_void_p(self._p))
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _ptr(a):

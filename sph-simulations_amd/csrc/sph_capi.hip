@@ -1498,6 +1498,24 @@ int sph_export_state(void* ctx, float* d_out) {
   return SPH_OK;
 }
 
+int sph_device_malloc(int device, int64_t bytes, void** out_ptr) {
+  if (!out_ptr || bytes <= 0) return SPH_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return SPH_ERR_HIP;
+  return hipMalloc(out_ptr, (size_t)bytes) == hipSuccess ? SPH_OK : SPH_ERR_HIP;
+}
+
+int sph_device_free(void* ptr) {
+  if (!ptr) return SPH_OK;
+  return hipFree(ptr) == hipSuccess ? SPH_OK : SPH_ERR_HIP;
+}
+
+int sph_device_copy(void* dst, const void* src, int64_t bytes, int32_t kind) {
+  if (!dst || !src || bytes < 0 || kind < 0 || kind > 2) return SPH_ERR_INVALID;
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                    : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  return hipMemcpy(dst, src, (size_t)bytes, k) == hipSuccess ? SPH_OK : SPH_ERR_HIP;
+}
+
 int sph_upload_device(void* ctx, int64_t n, const float* d_rx, const float* d_ry,
                       const float* d_vx, const float* d_vy) {
   Ctx* c = as_ctx(ctx);
@@ -1667,9 +1685,9 @@ int sph_nn_loss_grad(void* nn, void* ctx, double* out3) {
   NNCHK(h, hipMemcpyAsync(sums, net.red_out, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
   NNCHK(h, hipStreamSynchronize(c->stream));
   if (out3) {
-    const double n = c->s.n, as = net.k.a_s;
-    out3[0] = sums[0] / (n * as * as);
-    out3[1] = sums[1] > 0 ? std::sqrt(sums[0] / sums[1]) : 0.0;
+    const double n = c->s.n;
+    out3[0] = sums[1] > 0 ? sums[0] / sums[1] : 0.0;
+    out3[1] = std::sqrt(out3[0]);
     out3[2] = std::sqrt(sums[1] / n);
   }
   return SPH_OK;

@@ -26,7 +26,7 @@ typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 constexpr int W = kNnWidth;
 constexpr int kNodesPerWave = 16;         // edge kernels: nodes per wavefront
 constexpr int kEdgeNodesPerBlock = 4 * kNodesPerWave;
-constexpr int kEdgePartial = W * kEdgeF + W + W + 1;  // dC, dc, dw, db per workgroup
+constexpr int kEdgePartial = W * kEdgeF + W + W + 1 + kEdgeF;  // dC, dc, dw, db, du
 constexpr int kRowsPerChunk = 1024;       // weight-gradient / bias-gradient row chunks
 
 __device__ __forceinline__ uint16_t f2bf(float x) {
@@ -267,10 +267,11 @@ __global__ __launch_bounds__(256) void k_node_features(NodeIn in, NnConsts k,
   for (int c = 0; c < kNodeK; ++c) xr[c] = c < kNodeF ? f2bf(f[c]) : (uint16_t)0;
 }
 
-// Invariant features of edge (receiver i, sender j); (dx, dy) = r_i - r_j.
+// Invariant features of edge (receiver i, sender j); (dx, dy) = r_i - r_j;
+// pri is p / rho^2 of the receiver.
 template <int BC>
 __device__ __forceinline__ void edge_feat(const NnConsts& k, float xi, float yi, float vxi,
-                                          float vyi, float rhoi, float xj, float yj, float vxj,
+                                          float vyi, float pri, float xj, float yj, float vxj,
                                           float vyj, float rhoj, float prj, float& dx, float& dy,
                                           float* f) {
   dx = xi - xj;
@@ -290,7 +291,7 @@ __device__ __forceinline__ void edge_feat(const NnConsts& k, float xi, float yi,
   f[4] = sqrtf(dvx * dvx + dvy * dvy) * k.inv_vs;
   f[5] = k.rs * (rhoj * k.inv_rho0 - 1.0f);
   f[6] = k.ps * prj;
-  f[7] = k.rs * (rhoi * k.inv_rho0 - 1.0f);
+  f[7] = f[2] * (k.ps * pri + k.ps * prj);  // the form of the SPH pressure term
 }
 
 // --------------------------------------------------------- edge forward
@@ -304,6 +305,7 @@ __global__ __launch_bounds__(256) void k_edge_fwd(NodeIn in, NnConsts k, const f
                                                   const float* __restrict__ cb,
                                                   const uint16_t* __restrict__ wvec,
                                                   const float* __restrict__ bo,
+                                                  const float* __restrict__ uvec,
                                                   float* __restrict__ S,
                                                   uint16_t* __restrict__ Sb,
                                                   float* __restrict__ ax, float* __restrict__ ay) {
@@ -316,7 +318,9 @@ __global__ __launch_bounds__(256) void k_edge_fwd(NodeIn in, NnConsts k, const f
     C1[e] = bf2f(Cb[(c0 + 1) * kEdgeF + e]);
   }
   const float cb0 = cb[c0], cb1 = cb[c0 + 1];
-  float w0 = 0.0f, w1 = 0.0f, b = 0.0f;
+  float w0 = 0.0f, w1 = 0.0f, b = 0.0f, u[kEdgeF];
+#pragma unroll
+  for (int q = 0; q < kEdgeF; ++q) u[q] = MODE == 1 ? uvec[q] : 0.0f;
   if (MODE == 1) {
     w0 = bf2f(wvec[c0]);
     w1 = bf2f(wvec[c0 + 1]);
@@ -327,15 +331,15 @@ __global__ __launch_bounds__(256) void k_edge_fwd(NodeIn in, NnConsts k, const f
   for (int t = 0; t < kNodesPerWave; ++t) {
     const int i = base + t;
     if (i >= in.n) break;
-    const float xi = in.rx[i], yi = in.ry[i], vxi = in.vx[i], vyi = in.vy[i], rhoi = in.rho[i];
+    const float xi = in.rx[i], yi = in.ry[i], vxi = in.vx[i], vyi = in.vy[i], pri = in.pr[i];
     const int deg = min(in.cnt[i], in.w);
     const float2 p = *reinterpret_cast<const float2*>(P + (int64_t)i * W + c0);
     float s0 = 0.0f, s1 = 0.0f, tx0 = 0.0f, ty0 = 0.0f, tx1 = 0.0f, ty1 = 0.0f, sdx = 0.0f,
-          sdy = 0.0f;
+          sdy = 0.0f, lx = 0.0f, ly = 0.0f;
     for (int e = 0; e < deg; ++e) {
       const int j = in.idx[(int64_t)e * in.n + i];
       float dx, dy, f[kEdgeF];
-      edge_feat<BC>(k, xi, yi, vxi, vyi, rhoi, in.rx[j], in.ry[j], in.vx[j], in.vy[j],
+      edge_feat<BC>(k, xi, yi, vxi, vyi, pri, in.rx[j], in.ry[j], in.vx[j], in.vy[j],
                     in.rho[j], in.pr[j], dx, dy, f);
       const float2 qv = *reinterpret_cast<const float2*>(Q + (int64_t)j * W + c0);
       float z0 = p.x + qv.x + cb0, z1 = p.y + qv.y + cb1;
@@ -355,6 +359,11 @@ __global__ __launch_bounds__(256) void k_edge_fwd(NodeIn in, NnConsts k, const f
         ty1 += dy * a1;
         sdx += dx;
         sdy += dy;
+        float uf = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kEdgeF; ++q) uf += u[q] * f[q];
+        lx += dx * uf;
+        ly += dy * uf;
       }
     }
     if (MODE == 0) {
@@ -370,8 +379,8 @@ __global__ __launch_bounds__(256) void k_edge_fwd(NodeIn in, NnConsts k, const f
         py += __shfl_xor(py, o);
       }
       if (lane == 0) {
-        ax[i] = k.gx + sc * (px + b * sdx);
-        ay[i] = k.gy + sc * (py + b * sdy);
+        ax[i] = k.gx + sc * (px + b * sdx + lx);
+        ay[i] = k.gy + sc * (py + b * sdy + ly);
       }
     }
   }
@@ -380,7 +389,7 @@ __global__ __launch_bounds__(256) void k_edge_fwd(NodeIn in, NnConsts k, const f
 // --------------------------------------------------------- edge backward
 // One wavefront per node k, lane l owns columns 2l, 2l + 1. Receiver part:
 // dz_kj = g_kj * silu'(z_kj) over k's list gives dP_k and the C, c (and readout
-// w, b) gradient partials; sender part: dQ_k = sum over receivers i with k in
+// w, b, u) gradient partials; sender part: dQ_k = sum over receivers i with k in
 // i's list of dz_ik, visited through k's own list (the lists are symmetric).
 // MODE 0: g_kj = dS_k. MODE 1: g_kj = (a_s / h) (da_k . (r_k - r_j)) w.
 template <int MODE, int BC>
@@ -407,7 +416,9 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
     w0 = bf2f(wvec[c0]);
     w1 = bf2f(wvec[c0 + 1]);
   }
-  float gc0 = 0.0f, gc1 = 0.0f, gw0 = 0.0f, gw1 = 0.0f, gb = 0.0f;
+  float gc0 = 0.0f, gc1 = 0.0f, gw0 = 0.0f, gw1 = 0.0f, gb = 0.0f, gu[kEdgeF];
+#pragma unroll
+  for (int q = 0; q < kEdgeF; ++q) gu[q] = 0.0f;
   const float sc = k.a_s * k.inv_h;
   const int base = (blockIdx.x * 4 + wave) * kNodesPerWave;
   for (int t = 0; t < kNodesPerWave; ++t) {
@@ -434,7 +445,7 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
       // k receives from j
       {
         float dx, dy, f[kEdgeF];
-        edge_feat<BC>(k, xk, yk, vxk, vyk, rhok, xj, yj, vxj, vyj, rhoj, prj, dx, dy, f);
+        edge_feat<BC>(k, xk, yk, vxk, vyk, prk, xj, yj, vxj, vyj, rhoj, prj, dx, dy, f);
         const float2 qj = *reinterpret_cast<const float2*>(Q + (int64_t)j * W + c0);
         float z0 = pk.x + qj.x + cb0, z1 = pk.y + qj.y + cb1;
 #pragma unroll
@@ -451,6 +462,8 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
           gw0 += dphi * silu_f(z0);
           gw1 += dphi * silu_f(z1);
           gb += dphi;
+#pragma unroll
+          for (int q = 0; q < kEdgeF; ++q) gu[q] += dphi * f[q];
           g0 = dphi * w0;
           g1 = dphi * w1;
         }
@@ -468,7 +481,7 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
       // j receives from k (the same pair seen from j's list)
       {
         float dx, dy, f[kEdgeF];
-        edge_feat<BC>(k, xj, yj, vxj, vyj, rhoj, xk, yk, vxk, vyk, rhok, prk, dx, dy, f);
+        edge_feat<BC>(k, xj, yj, vxj, vyj, prj, xk, yk, vxk, vyk, rhok, prk, dx, dy, f);
         const float2 pj = *reinterpret_cast<const float2*>(P + (int64_t)j * W + c0);
         float z0 = pj.x + qk.x + cb0, z1 = pj.y + qk.y + cb1;
 #pragma unroll
@@ -509,7 +522,11 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
   r[W * kEdgeF + c0 + 1] = gc1;
   r[W * kEdgeF + W + c0] = gw0;
   r[W * kEdgeF + W + c0 + 1] = gw1;
-  if (lane == 0) r[W * kEdgeF + 2 * W] = gb;
+  if (lane == 0) {
+    r[W * kEdgeF + 2 * W] = gb;
+#pragma unroll
+    for (int q = 0; q < kEdgeF; ++q) r[W * kEdgeF + 2 * W + 1 + q] = gu[q];
+  }
   __syncthreads();
   for (int idx = threadIdx.x; idx < kEdgePartial; idx += 256)
     partial[(int64_t)blockIdx.x * kEdgePartial + idx] =
@@ -517,22 +534,17 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
 }
 
 // ------------------------------------------------------------ loss / error
-// da = 2 (a - t) / (n a_s^2) (if da is set) and per-block FP64 sums of |a - t|^2
-// and |t|^2.
+// Per-block FP64 sums of |a - t|^2 and |t|^2.
 __global__ __launch_bounds__(256) void k_loss(int n, const float* __restrict__ ax,
                                               const float* __restrict__ ay,
                                               const float* __restrict__ tx,
-                                              const float* __restrict__ ty, float inv_scale,
-                                              float* __restrict__ da, double* __restrict__ part) {
+                                              const float* __restrict__ ty,
+                                              double* __restrict__ part) {
   __shared__ double s0[256], s1[256];
   const int i = blockIdx.x * 256 + threadIdx.x;
   double e2 = 0.0, t2 = 0.0;
   if (i < n) {
     const float ex = ax[i] - tx[i], ey = ay[i] - ty[i];
-    if (da) {
-      da[2 * (int64_t)i] = 2.0f * ex * inv_scale;
-      da[2 * (int64_t)i + 1] = 2.0f * ey * inv_scale;
-    }
     e2 = (double)ex * ex + (double)ey * ey;
     t2 = (double)tx[i] * tx[i] + (double)ty[i] * ty[i];
   }
@@ -550,6 +562,20 @@ __global__ __launch_bounds__(256) void k_loss(int n, const float* __restrict__ a
     part[2 * blockIdx.x] = s0[0];
     part[2 * blockIdx.x + 1] = s1[0];
   }
+}
+
+// da = 2 (a - t) / sum |t|^2 (the gradient of the normalised loss), interleaved.
+__global__ __launch_bounds__(256) void k_loss_grad(int n, const float* __restrict__ ax,
+                                                   const float* __restrict__ ay,
+                                                   const float* __restrict__ tx,
+                                                   const float* __restrict__ ty,
+                                                   const double* __restrict__ sums,
+                                                   float* __restrict__ da) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float inv = sums[1] > 0.0 ? (float)(2.0 / sums[1]) : 0.0f;
+  da[2 * (int64_t)i] = (ax[i] - tx[i]) * inv;
+  da[2 * (int64_t)i + 1] = (ay[i] - ty[i]) * inv;
 }
 
 __global__ __launch_bounds__(256) void k_sum2(int parts, const double* __restrict__ part,
@@ -624,7 +650,7 @@ hipError_t nn_alloc(Net* net, T** p, int64_t count) {
 
 enum { T_ENC_W1 = 0, T_ENC_B1, T_ENC_W2, T_ENC_B2, T_LAYER0 };
 enum { L_A = 0, L_B, L_C, L_c, L_D, L_d, L_E, L_e, L_F, L_f, L_COUNT };
-enum { R_A = 0, R_B, R_C, R_c, R_w, R_b };
+enum { R_A = 0, R_B, R_C, R_c, R_w, R_b, R_u };
 
 const NnTensor& ten(const Net* n, int idx) { return n->t[idx]; }
 const NnTensor& lt(const Net* n, int l, int which) { return n->t[T_LAYER0 + l * L_COUNT + which]; }
@@ -668,24 +694,25 @@ NodeIn node_in(const DevState& s) {
 template <int MODE>
 void edge_fwd(const Net* net, const DevState& s, hipStream_t st, const float* P, const float* Q,
               const NnTensor& C, const NnTensor& c, const NnTensor* w, const NnTensor* b,
-              float* S, uint16_t* Sb, float* ax, float* ay) {
+              const NnTensor* u, float* S, uint16_t* Sb, float* ax, float* ay) {
   const NodeIn in = node_in(s);
   const int blocks = cdiv(s.n, kEdgeNodesPerBlock);
   const uint16_t* wv = w ? WB(net, *w) : nullptr;
   const float* bv = b ? P_(net, *b) : nullptr;
+  const float* uv = u ? P_(net, *u) : nullptr;
   if (net->k.periodic)
     k_edge_fwd<MODE, 1><<<blocks, 256, 0, st>>>(in, net->k, P, Q, WB(net, C), P_(net, c), wv, bv,
-                                                S, Sb, ax, ay);
+                                                uv, S, Sb, ax, ay);
   else
     k_edge_fwd<MODE, 0><<<blocks, 256, 0, st>>>(in, net->k, P, Q, WB(net, C), P_(net, c), wv, bv,
-                                                S, Sb, ax, ay);
+                                                uv, S, Sb, ax, ay);
 }
 
 // edge backward + reduction of its partials into the C, c (, w, b) gradients
 template <int MODE>
 void edge_bwd(Net* net, const DevState& s, hipStream_t st, const float* P, const float* Q,
               const NnTensor& C, const NnTensor& c, const NnTensor* w, const NnTensor* b,
-              const float* G) {
+              const NnTensor* u, const float* G) {
   const NodeIn in = node_in(s);
   const int blocks = cdiv(s.n, kEdgeNodesPerBlock);
   const uint16_t* wv = w ? WB(net, *w) : nullptr;
@@ -697,7 +724,7 @@ void edge_bwd(Net* net, const DevState& s, hipStream_t st, const float* P, const
     k_edge_bwd<MODE, 0><<<blocks, 256, 0, st>>>(in, net->k, P, Q, WB(net, C), P_(net, c), wv, G,
                                                 net->dP, net->dQ, net->dPb, net->dQb,
                                                 net->partial);
-  // partial rows: [dC (W x 8) | dc (W) | dw (W) | db]
+  // partial rows: [dC (W x 8) | dc (W) | dw (W) | db | du (8)]
   const int stride = kEdgePartial;
   k_reduce<<<cdiv(W * kEdgeF, 256), 256, 0, st>>>(blocks, 1, stride, W * kEdgeF, net->partial,
                                                   G_(net, C));
@@ -706,6 +733,8 @@ void edge_bwd(Net* net, const DevState& s, hipStream_t st, const float* P, const
     k_reduce<<<1, 256, 0, st>>>(blocks, 1, stride, W, net->partial + W * kEdgeF + W, G_(net, *w));
     k_reduce<<<1, 256, 0, st>>>(blocks, 1, stride, 1, net->partial + W * kEdgeF + 2 * W,
                                 G_(net, *b));
+    k_reduce<<<1, 256, 0, st>>>(blocks, 1, stride, kEdgeF,
+                                net->partial + W * kEdgeF + 2 * W + 1, G_(net, *u));
   }
 }
 
@@ -746,6 +775,7 @@ int nn_create(Net* net, int layers, bool train, int64_t n_cap, const NnConsts& k
   add(W, 1, 0);            // ro_c
   add(1, W, W);            // ro_w
   add(1, 1, 0);            // ro_b
+  add(kEdgeF, 1, 0);       // ro_u (FP32: the linear path of the readout)
   net->n_params = off;
   net->n_wb = boff;
   const int64_t N = n_cap, NW = N * W;
@@ -826,8 +856,8 @@ hipError_t nn_forward(Net* net, const DevState& s, hipStream_t st, float* ax, fl
          nullptr, b.P, nullptr, 0);
     gemm(st, n, b.hb, W, nullptr, 0, WB(net, lt(net, l, L_B)), W, nullptr, nullptr, nullptr,
          nullptr, b.Q, nullptr, 0);
-    edge_fwd<0>(net, s, st, b.P, b.Q, lt(net, l, L_C), lt(net, l, L_c), nullptr, nullptr, b.S,
-                b.Sb, nullptr, nullptr);
+    edge_fwd<0>(net, s, st, b.P, b.Q, lt(net, l, L_C), lt(net, l, L_c), nullptr, nullptr,
+                nullptr, b.S, b.Sb, nullptr, nullptr);
     gemm(st, n, b.Sb, W, nullptr, 0, WB(net, lt(net, l, L_D)), W, nullptr, net->deg,
          P_(net, lt(net, l, L_d)), nullptr, b.M, b.Mb, 0);
     gemm(st, n, b.hb, W, b.Mb, W, WB(net, lt(net, l, L_E)), W, P_(net, lt(net, l, L_e)), nullptr,
@@ -843,15 +873,16 @@ hipError_t nn_forward(Net* net, const DevState& s, hipStream_t st, float* ax, fl
        f.Q, nullptr, 0);
   const NnTensor& rw = rt(net, R_w);
   const NnTensor& rb = rt(net, R_b);
-  edge_fwd<1>(net, s, st, f.P, f.Q, rt(net, R_C), rt(net, R_c), &rw, &rb, nullptr, nullptr, ax,
-              ay);
+  const NnTensor& ru = rt(net, R_u);
+  edge_fwd<1>(net, s, st, f.P, f.Q, rt(net, R_C), rt(net, R_c), &rw, &rb, &ru, nullptr, nullptr,
+              ax, ay);
   return hipGetLastError();
 }
 
 hipError_t nn_error_sums(Net* net, int n, const float* ax, const float* ay, const float* tx,
                          const float* ty, hipStream_t st, double* out2) {
   const int blocks = cdiv(n, 256);
-  k_loss<<<blocks, 256, 0, st>>>(n, ax, ay, tx, ty, 0.0f, nullptr, net->red);
+  k_loss<<<blocks, 256, 0, st>>>(n, ax, ay, tx, ty, net->red);
   k_sum2<<<1, 256, 0, st>>>(blocks, net->red, out2);
   return hipGetLastError();
 }
@@ -865,15 +896,16 @@ hipError_t nn_loss_backward(Net* net, const DevState& s, hipStream_t st, const f
   const float* ax = net->a_out;
   const float* ay = net->a_out + net->n_cap;
   const int blocks = cdiv(n, 256);
-  const float inv_scale = 1.0f / ((float)n * net->k.a_s * net->k.a_s);
-  k_loss<<<blocks, 256, 0, st>>>(n, ax, ay, tax, tay, inv_scale, net->da, net->red);
+  k_loss<<<blocks, 256, 0, st>>>(n, ax, ay, tax, tay, net->red);
   k_sum2<<<1, 256, 0, st>>>(blocks, net->red, net->red_out);
+  k_loss_grad<<<blocks, 256, 0, st>>>(n, ax, ay, tax, tay, net->red_out, net->da);
   const int L = net->L;
   // readout
   NnLayerBufs& f = net->lb[L];
   const NnTensor& rw = rt(net, R_w);
   const NnTensor& rb = rt(net, R_b);
-  edge_bwd<1>(net, s, st, f.P, f.Q, rt(net, R_C), rt(net, R_c), &rw, &rb, net->da);
+  const NnTensor& ru = rt(net, R_u);
+  edge_bwd<1>(net, s, st, f.P, f.Q, rt(net, R_C), rt(net, R_c), &rw, &rb, &ru, net->da);
   wgrad(net, st, n, net->dPb, W, f.hb, W, nullptr, 0, rt(net, R_A));
   wgrad(net, st, n, net->dQb, W, f.hb, W, nullptr, 0, rt(net, R_B));
   gemm(st, n, net->dPb, W, nullptr, 0, WBT(net, rt(net, R_A)), W, nullptr, nullptr, nullptr,
@@ -904,7 +936,7 @@ hipError_t nn_loss_backward(Net* net, const DevState& s, hipStream_t st, const f
          nullptr, net->dS, nullptr, 0);
     // S = sum silu(P_i + Q_j + C f + c)
     edge_bwd<0>(net, s, st, b.P, b.Q, lt(net, l, L_C), lt(net, l, L_c), nullptr, nullptr,
-                net->dS);
+                nullptr, net->dS);
     wgrad(net, st, n, net->dPb, W, b.hb, W, nullptr, 0, lt(net, l, L_A));
     wgrad(net, st, n, net->dQb, W, b.hb, W, nullptr, 0, lt(net, l, L_B));
     gemm(st, n, net->dPb, W, nullptr, 0, WBT(net, lt(net, l, L_A)), W, nullptr, nullptr, nullptr,

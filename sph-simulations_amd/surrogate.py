@@ -3,8 +3,8 @@ sph_surrogate_run in libsph_amd.so).
 
 The network, its training kernels and the surrogate-in-the-loop runner are in
 the HIP library; this module builds configurations, initial weights and the
-training data pipeline (solver snapshots kept in device memory, allocated with
-torch and filled by the library).
+training data pipeline (solver snapshots kept in device memory that the library
+allocates and fills).
 """
 from __future__ import annotations
 
@@ -13,8 +13,8 @@ import math
 
 import numpy as np
 
-from .capi import (BC_PERIODIC, SPH_OK, Batch, Context, NnConfig, Params, SphError,
-                   SurrogateStats, lib)
+from .capi import (BC_PERIODIC, SPH_OK, Batch, Context, DeviceBuffer, NnConfig, Params,
+                   SphError, SurrogateStats, lib)
 from . import scenes as _scenes
 
 WIDTH = 128
@@ -32,7 +32,7 @@ def param_layout(width=WIDTH, layers=3):
                 (f"mp{l}_E", (W, 2 * W)), (f"mp{l}_e", (W,)), (f"mp{l}_F", (W, W)),
                 (f"mp{l}_f", (W,))]
     out += [("ro_A", (W, W)), ("ro_B", (W, W)), ("ro_C", (W, EDGE_F)), ("ro_c", (W,)),
-            ("ro_w", (1, W)), ("ro_b", (1,))]
+            ("ro_w", (1, W)), ("ro_b", (1,)), ("ro_u", (EDGE_F,))]
     return out
 
 
@@ -56,8 +56,10 @@ def init_params(layers=3, seed=0, width=WIDTH) -> np.ndarray:
 
 
 def nn_config(params: Params, max_nodes: int, layers=3, train=True, v_scale=None,
-              a_scale=10.0) -> NnConfig:
-    """Network configuration for scenes with the physics of `params`."""
+              a_scale=None) -> NnConfig:
+    """Network configuration for scenes with the physics of `params`: relative
+    velocities scaled by 0.1 c0 and the readout by 0.1 c0^2 / h (the order of one
+    neighbor's pressure acceleration) unless given."""
     c0 = params.c0 if params.c0 > 0 else 10.0
     cfg = NnConfig()
     cfg.width = WIDTH
@@ -69,7 +71,7 @@ def nn_config(params: Params, max_nodes: int, layers=3, train=True, v_scale=None
     cfg.rho0 = params.rho0
     cfg.c0 = c0
     cfg.v_scale = v_scale if v_scale else 0.1 * c0
-    cfg.a_scale = a_scale
+    cfg.a_scale = a_scale if a_scale else 0.1 * c0 * c0 / params.h
     cfg.gravity_x = 0.0
     cfg.gravity_y = params.gravity_y
     cfg.len_x = params.hi[0] - params.lo[0]
@@ -81,7 +83,7 @@ class Network:
     """One network on one GPU (owns an sph_nn handle)."""
 
     def __init__(self, params: Params, max_nodes: int, layers=3, train=True, device=0,
-                 v_scale=None, a_scale=10.0, seed=0):
+                 v_scale=None, a_scale=None, seed=0):
         self._L = lib()
         self.layers = layers
         self.config = nn_config(params, max_nodes, layers, train, v_scale, a_scale)
@@ -190,9 +192,9 @@ def surrogate_run(ctx: Context, net: Network, steps: int, check_every=10, thresh
 # ------------------------------------------------------------------ data
 def training_scenes(n_scenes=64, n_particles=65536, seed=0x5EED0003, box=(4.0, 2.0)):
     """Dam breaks of n_particles each with the 16k dam-break physics (dx =
-    0.25/128) in a `box`: scene s takes a block width nx from {n/1024 ... } the
-    powers of two with nx * dx <= box width / 2 and ny = n / nx rows within the
-    box height (seeded choice), and seeded +-0.01 dx jitter."""
+    0.25/128) in `box`. Scene s picks its block shape (nx x ny = n_particles, nx a
+    power of two, block at most half the box wide and as tall as the box) from
+    uniform(seed)[s], and has its own +-0.01 dx jitter seeded seed + 1 + s."""
     dx = 0.25 / 128
     u = _scenes.uniform(seed, n_scenes)
     shapes = []
@@ -212,36 +214,71 @@ def training_scenes(n_scenes=64, n_particles=65536, seed=0x5EED0003, box=(4.0, 2
     return out
 
 
-def generate_dataset(scenes_list, steps=2000, every=10, device="cuda"):
-    """Advance all scenes together (one batch context) and keep the state every
-    `every` steps: a torch float32 tensor [snapshots, scenes, 4, n] in device
-    memory (rx, ry, vx, vy in particle-id order), snapshot 0 = initial state."""
-    import torch
+class Dataset:
+    """Solver snapshots in device memory: snapshot t of scene s holds rx, ry, vx,
+    vy (particle-id order) of n particles at element offset ((t * S + s) * 4) * n."""
 
+    def __init__(self, snapshots: int, scenes: int, n: int, device: int = 0):
+        self.T, self.S, self.n = int(snapshots), int(scenes), int(n)
+        self.buf = DeviceBuffer(self.T * self.S * 4 * self.n, device)
+
+    def offset(self, t: int, s: int, field: int = 0) -> int:
+        if not (0 <= t < self.T and 0 <= s < self.S and 0 <= field < 4):
+            raise IndexError("snapshot index out of range")
+        return ((t * self.S + s) * 4 + field) * self.n
+
+    def ptr(self, t: int, s: int, field: int = 0) -> int:
+        return self.buf.ptr(self.offset(t, s, field))
+
+    def host(self, t: int, s: int):
+        a = self.buf.to_host(self.offset(t, s), 4 * self.n).reshape(4, self.n)
+        return dict(rx=a[0], ry=a[1], vx=a[2], vy=a[3])
+
+    def close(self):
+        self.buf.close()
+
+
+def generate_dataset(scenes_list, steps=2000, every=10, device=0) -> Dataset:
+    """Advance all scenes together (one batch context) and keep the state every
+    `every` steps; snapshot 0 is the initial state."""
     n = scenes_list[0].n
     if any(sc.n != n for sc in scenes_list):
         raise ValueError("training scenes must have equal particle counts")
-    counts = np.full(len(scenes_list), n, np.int64)
-    b = Batch([sc.params for sc in scenes_list], counts)
+    S = len(scenes_list)
+    counts = np.full(S, n, np.int64)
+    snaps = steps // every + 1
+    data = Dataset(snaps, S, n, device)
+    # the batch exports [rx | ry | vx | vy] of all scenes; scene s of snapshot t
+    # is then copied field by field into its slot
+    stage = DeviceBuffer(4 * S * n, device)
+    b = Batch([sc.params for sc in scenes_list], counts, device)
+    L = lib()
+
+    def keep(t):
+        b.export_state(stage.ptr(0))
+        for s in range(S):
+            for f in range(4):
+                rc = L.sph_device_copy(C.c_void_p(data.ptr(t, s, f)),
+                                       C.c_void_p(stage.ptr(f * S * n + s * n)), n * 4, 2)
+                if rc != SPH_OK:
+                    raise SphError(f"sph_device_copy -> {rc}")
+
     try:
         b.upload(np.concatenate([sc.rx for sc in scenes_list]),
                  np.concatenate([sc.ry for sc in scenes_list]),
                  np.concatenate([sc.vx for sc in scenes_list]),
                  np.concatenate([sc.vy for sc in scenes_list]))
-        snaps = steps // every + 1
-        data = torch.empty((snaps, 4, len(scenes_list) * n), dtype=torch.float32, device=device)
-        torch.cuda.synchronize()
-        b.export_state(data[0].data_ptr())
+        keep(0)
         b.prepare()
         b.set_graph(True)
-        for k in range(1, snaps):
+        for t in range(1, snaps):
             b.step(every)
-            b.export_state(data[k].data_ptr())
+            keep(t)
         b.synchronize()
     finally:
         b.close()
-    # [snapshots, 4, scenes * n] -> [snapshots, scenes, 4, n]
-    return data.view(snaps, 4, len(scenes_list), n).permute(0, 2, 1, 3).contiguous()
+        stage.close()
+    return data
 
 
 class Trainer:
@@ -249,17 +286,17 @@ class Trainer:
     into a plain-SPH context, computes the plain forces (the target) and runs one
     forward + backward + Adam step of the network."""
 
-    def __init__(self, net: Network, params: Params, data, seed=1):
+    def __init__(self, net: Network, params: Params, data: Dataset, seed=1):
         self.net = net
         self.data = data
-        self.n = int(data.shape[-1])
+        self.n = data.n
         self.ctx = Context(params, self.n)
         self.rng = np.random.default_rng(seed)
 
     def load(self, t: int, s: int):
-        x = self.data[t, s]
-        self.ctx.upload_device(self.n, x[0].data_ptr(), x[1].data_ptr(), x[2].data_ptr(),
-                               x[3].data_ptr())
+        d = self.data
+        self.ctx.upload_device(self.n, d.ptr(t, s, 0), d.ptr(t, s, 1), d.ptr(t, s, 2),
+                               d.ptr(t, s, 3))
         self.ctx.prepare()
         self.ctx.forces(1)  # plain SPH accelerations from the fixed-width list
 
@@ -269,8 +306,8 @@ class Trainer:
 
     def train(self, iters: int, lr=1e-3, snapshots=None, scenes=None, lr_final=None,
               log_every=0, log=None):
-        T = self.data.shape[0] if snapshots is None else snapshots
-        S = self.data.shape[1] if scenes is None else scenes
+        T = self.data.T if snapshots is None else snapshots
+        S = self.data.S if scenes is None else scenes
         hist = []
         for it in range(iters):
             t = int(self.rng.integers(0, T))

@@ -94,7 +94,7 @@ def torch_forward(theta, g, W, L):
         U = act(torch.cat([h, M], 1) @ p[q + "E"].T + p[q + "e"])
         h = h + U @ p[q + "F"].T + p[q + "f"]
     zo = (h @ p["ro_A"].T)[recv] + (h @ p["ro_B"].T)[send] + f @ p["ro_C"].T + p["ro_c"]
-    phi = act(zo) @ p["ro_w"][0] + p["ro_b"][0]
+    phi = act(zo) @ p["ro_w"][0] + p["ro_b"][0] + f @ p["ro_u"]
     s = g.cfg["a_scale"] / g.cfg["h"]
     ax = s * torch.zeros(g.n, dtype=torch.float64).index_add(0, recv, dx * phi)
     ay = g.cfg["gy"] + s * torch.zeros(g.n, dtype=torch.float64).index_add(0, recv, dy * phi)
@@ -110,7 +110,8 @@ def test_gradient_vs_torch_autograd():
     loss, grad, a = net.loss_grad(g, target)
     th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
     at = torch_forward(th, g, W, L)
-    lt = ((at - torch.as_tensor(target)) ** 2).sum() / (g.n * g.cfg["a_scale"] ** 2)
+    tt = torch.as_tensor(target)
+    lt = ((at - tt) ** 2).sum() / (tt ** 2).sum()
     lt.backward()
     np.testing.assert_allclose(at.detach().numpy(), a, rtol=1e-12, atol=1e-12)
     assert abs(lt.item() - loss) <= 1e-12 * abs(loss)
