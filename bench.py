@@ -159,6 +159,98 @@ def bench_batched(sph, steps, trial_steps=10):
     }
 
 
+def bench_surrogate(sph, scene, steps, train_iters, check_every=10, threshold=0.02):
+    """Surrogate in the loop on the 1M-particle dam break: a network trained on
+    solver snapshots (eight 65,536-particle dam breaks, 400 steps, one held out)
+    supplies the accelerations; plain SPH is evaluated every `check_every` steps
+    and a window whose RMS relative acceleration error exceeds `threshold` is
+    rolled back and recomputed with plain SPH. Compared with a plain run of the
+    same steps from the same state (throughput and drift)."""
+    sur = importlib.import_module("sph-simulations_amd.surrogate")
+    n = scene.n
+    log(f"surrogate: training ({train_iters} iterations)")
+    trained, rep = sur.train_network(n_scenes=8, n_particles=65536, steps=400, every=10,
+                                     iters=train_iters, log=lambda m: log(f"train {m}"))
+    theta = trained.params()
+    trained.close()
+    net = sur.Network(scene.params, n, layers=3, train=False)
+    net.set_params(theta)
+    out = {"training": rep, "steps": steps, "check_every": check_every, "threshold": threshold}
+    plain = sph.Context(scene.params, n)
+    surr = sph.Context(scene.params, n)
+    for c in (plain, surr):
+        c.upload(scene.rx, scene.ry, scene.vx, scene.vy)
+        c.prepare()
+    plain.set_graph(True)
+    plain.step(2)
+    plain.synchronize()
+    # warm-up of the network path on a throwaway context
+    warm = sph.Context(scene.params, n)
+    warm.upload(scene.rx, scene.ry, scene.vx, scene.vy)
+    warm.prepare()
+    sur.surrogate_run(warm, net, 2, check_every, threshold, 1)
+    warm.close()
+    # surrogate in the loop (window rollback)
+    surr.synchronize()
+    t0 = time.perf_counter()
+    st = sur.surrogate_run(surr, net, steps, check_every, threshold, 1)
+    surr.synchronize()
+    t_s = time.perf_counter() - t0
+    # plain run over the same steps from the same state (graph replay)
+    plain2 = sph.Context(scene.params, n)
+    plain2.upload(scene.rx, scene.ry, scene.vx, scene.vy)
+    plain2.prepare()
+    plain2.set_graph(True)
+    plain2.synchronize()
+    t0 = time.perf_counter()
+    plain2.step(steps)
+    plain2.synchronize()
+    t_p = time.perf_counter() - t0
+    # network-only steps (every check accepted): the surrogate path's best case
+    fast = sph.Context(scene.params, n)
+    fast.upload(scene.rx, scene.ry, scene.vx, scene.vy)
+    fast.prepare()
+    fast.synchronize()
+    t0 = time.perf_counter()
+    st_fast = sur.surrogate_run(fast, net, steps, check_every, 1e30, 0)
+    fast.synchronize()
+    t_f = time.perf_counter() - t0
+    # drift of the surrogate trajectory against the plain one (matched ids)
+    def state(c):
+        d = c.download()
+        c.prepare()
+        f = c.fields()
+        o = np.argsort(d["pid"])
+        return {k: d[k][o].astype(np.float64) for k in ("rx", "ry")} | {
+            "rho": f["rho"][o].astype(np.float64)}
+    a, b, f_ = state(surr), state(plain2), state(fast)
+    h = scene.params.h
+    rho0 = scene.params.rho0
+
+    def drift(x):
+        dpos = np.hypot(x["rx"] - b["rx"], x["ry"] - b["ry"])
+        drho = np.abs(x["rho"] - b["rho"]) / rho0
+        return {"position_rms_over_h": float(np.sqrt(np.mean(dpos ** 2)) / h),
+                "position_max_over_h": float(dpos.max() / h),
+                "density_mean_rel": float(drho.mean()), "density_max_rel": float(drho.max())}
+
+    out.update({
+        "value": n * steps / t_s, "unit": "particle-steps/s", "ms_per_step": t_s * 1e3 / steps,
+        "plain_value": n * steps / t_p, "plain_ms_per_step": t_p * 1e3 / steps,
+        "vs_plain": (n * steps / t_s) / (n * steps / t_p),
+        "fallback_rate": st["failed_checks"] / max(st["checks"], 1),
+        "stats": st,
+        "drift_vs_plain": drift(a),
+        "network_only": {"value": n * steps / t_f, "ms_per_step": t_f * 1e3 / steps,
+                         "vs_plain": t_p / t_f, "mean_error": st_fast["mean_error"],
+                         "max_error": st_fast["max_error"], "drift_vs_plain": drift(f_)},
+    })
+    for c in (plain, surr, plain2, fast):
+        c.close()
+    net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +259,10 @@ def main():
     ap.add_argument("--workload", default="dam_break_1m")
     ap.add_argument("--batched-steps", type=int, default=200,
                     help="steps of the 256-scene batched workload (0 disables)")
+    ap.add_argument("--surrogate-steps", type=int, default=200,
+                    help="steps of the surrogate-in-the-loop comparison (0 disables)")
+    ap.add_argument("--train-iters", type=int, default=1500,
+                    help="training iterations of the surrogate network")
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU work for the cpu_baseline sample (0 disables)")
     args = ap.parse_args()
@@ -213,27 +309,28 @@ def main():
     bstats = ctx.block_stats()
     log(f"block stats {bstats}")
 
-    # timed region: every launch bracketed by hipEvents on the context stream
-    ctx.set_timing(True)
+    # timed region: exactly `steps` graph-replayed steps, synchronised on both sides
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.step(args.steps)
     ctx.synchronize()
     t1 = time.perf_counter()
-    stage = ctx.stage_times()
-    ctx.set_timing(False)
     wall = t1 - t0
     value = n * args.steps / wall
     log(f"timed {args.steps} steps: {wall * 1e3 / args.steps:.4f} ms/step")
 
-    # same steps replayed from the captured hipGraph (no per-launch events)
+    # the same number of steps again with every launch bracketed by hipEvents on
+    # the context stream: per-kernel durations (the roofline) and their sum
+    ctx.set_timing(True)
     ctx.synchronize()
-    g0 = time.perf_counter()
+    e0 = time.perf_counter()
     ctx.step(args.steps)
     ctx.synchronize()
-    g1 = time.perf_counter()
-    graph_ms = (g1 - g0) * 1e3 / args.steps
-    log(f"graph replay: {graph_ms:.4f} ms/step")
+    e1 = time.perf_counter()
+    stage = ctx.stage_times()
+    ctx.set_timing(False)
+    ev_ms = (e1 - e0) * 1e3 / args.steps
+    log(f"per-launch events pass: {ev_ms:.4f} ms/step")
 
     mfrac = ctx.movers() / n
     k = ctx.build_neighbor_lists(False) / n
@@ -266,7 +363,9 @@ def main():
                      * stages[s]["launches"] / args.steps
                      for s in stages if "alg_bytes_per_launch" in stages[s])
     out = {
-        "metric": "particle-steps/s at 1M particles: plain GPU SPH step (surrogate path not built)",
+        "metric": "particle-steps/s at 1M particles: surrogate vs plain SPH, \u22642% accel error",
+        "value_path": "plain GPU SPH (the faster of the two paths within the 2% bound; the "
+                      "surrogate-in-the-loop figures are under 'surrogate')",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -292,13 +391,14 @@ def main():
         "step": {"alg_bytes_per_particle": round(step_bytes, 1),
                  "step_frac_of_hbm": step_bytes * value / (HBM_PEAK_GBS * 1e9),
                  "kernel_ms_per_step": kern_total / args.steps,
-                 "graph_ms_per_step": graph_ms,
-                 "graph_particle_steps_per_s": n / (graph_ms * 1e-3)},
+                 "events_pass_ms_per_step": ev_ms},
         "diagnostics": {"kinetic_energy": m[0], "max_v2": m[1], "inside": int(m[2])},
     }
     ctx.close()
     if args.batched_steps > 0:
         out["batched"] = bench_batched(sph, args.batched_steps)
+    if args.surrogate_steps > 0:
+        out["surrogate"] = bench_surrogate(sph, scene, args.surrogate_steps, args.train_iters)
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(sph, scene, args.cpu_budget)
     else:

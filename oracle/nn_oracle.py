@@ -82,13 +82,16 @@ def dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
+MAX_NEIGHBORS = 32
+
+
 class Graph:
-    """Edges (receiver i, sender j) from a fixed-width list (first min(count, w)
-    slots of each particle) and the per-particle inputs."""
+    """Edges (receiver i, sender j) from a fixed-width list (first
+    min(count, w, 32) slots of each particle) and the per-particle inputs."""
 
     def __init__(self, rx, ry, vx, vy, rho, pr, counts, slots, cfg):
         n, w = slots.shape
-        deg = np.minimum(counts, w)
+        deg = np.minimum(counts, min(w, MAX_NEIGHBORS))
         recv = np.repeat(np.arange(n), deg)
         send = np.concatenate([slots[i, :deg[i]] for i in range(n)]).astype(np.int64) \
             if n else np.zeros(0, np.int64)

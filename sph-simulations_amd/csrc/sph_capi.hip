@@ -1676,7 +1676,9 @@ int sph_nn_loss_grad(void* nn, void* ctx, double* out3) {
   // must be complete (no particle over ell_width neighbors)
   int64_t over = 0;
   NNCHK(h, hipMemsetAsync(c->ovf, 0, sizeof(int64_t), c->stream));
-  NNCHK(h, launch_ell_overflow(c->L, c->s, c->ovf));
+  DevState capped = c->s;  // the network reads at most 32 slots per particle
+  capped.ell_width = std::min(capped.ell_width, 32);
+  NNCHK(h, launch_ell_overflow(c->L, capped, c->ovf));
   NNCHK(h, hipMemcpyAsync(&over, c->ovf, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
   NNCHK(h, hipStreamSynchronize(c->stream));
   if (over != 0) return nn_fail(h, SPH_ERR_STATE, "a neighbor list overflowed ell_width");

@@ -28,6 +28,7 @@ constexpr int kNodesPerWave = 16;         // edge kernels: nodes per wavefront
 constexpr int kEdgeNodesPerBlock = 4 * kNodesPerWave;
 constexpr int kEdgePartial = W * kEdgeF + W + W + 1 + kEdgeF;  // dC, dc, dw, db, du
 constexpr int kRowsPerChunk = 1024;       // weight-gradient / bias-gradient row chunks
+constexpr int kMaxNnNeighbors = 32;       // neighbors per node the network reads
 
 __device__ __forceinline__ uint16_t f2bf(float x) {
   uint32_t u = __float_as_uint(x);
@@ -35,21 +36,34 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-__device__ __forceinline__ float silu_f(float z) { return z / (1.0f + expf(-z)); }
+// SiLU and its derivative from the hardware exp2 / reciprocal approximations
+// (about 1 ulp each; the network is checked against the oracle with tolerances,
+// not bit for bit).
+__device__ __forceinline__ float sigmoid_f(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * z));
+}
+__device__ __forceinline__ float silu_f(float z) { return z * sigmoid_f(z); }
 __device__ __forceinline__ float dsilu_f(float z) {
-  const float s = 1.0f / (1.0f + expf(-z));
+  const float s = sigmoid_f(z);
   return s * (1.0f + z * (1.0f - s));
 }
+__device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 // ------------------------------------------------------------------ GEMM
-// Y[n x Mo] = [X1 | X2] * B^T with B [Mo x (K1 + K2)] bf16 row-major. Workgroup:
-// 128 rows x 128 output columns (blockIdx.y picks the column block); wave w owns
-// rows 32w..32w+31 as four 32x32 MFMA tiles (v_mfma_f32_32x32x16_bf16). Operand
-// layout of that instruction: lane l holds row (l % 32) of A and column (l % 32)
-// of B at k = 8 (l / 32) .. +8; the accumulator register q of lane l is element
-// (8 (q / 4) + 4 (l / 32) + q % 4, l % 32).
+// Y[n x Mo] = [X1 | X2] * B^T with B [Mo x K] bf16 row-major (K = K1 + K2, a
+// compile-time 32, 128 or 256), computed as Y^T = B X^T so that each lane's
+// accumulators are four consecutive output columns of one row (16-byte stores).
+// A workgroup stages its 128-column block of B in LDS once (rows padded by 16
+// bytes against bank conflicts) and walks 128-row tiles; wave w owns rows
+// 32w..32w+31 of a tile and all 128 columns as four 32x32 MFMA tiles
+// (v_mfma_f32_32x32x16_bf16). Operand layout of that instruction: lane l holds
+// row (l % 32) of the A operand (here B's columns) and column (l % 32) of the B
+// operand (here X's rows) at k = 8 (l / 32) .. +8; accumulator register q of
+// lane l is element (8 (q / 4) + 4 (l / 32) + q % 4, l % 32) -- output column
+// and row respectively.
 struct GemmArgs {
   int n, K1, K2, Mo;
   const uint16_t* X1;
@@ -64,48 +78,80 @@ struct GemmArgs {
   int act;                // 0 identity, 1 SiLU (applies to Yb only)
 };
 
+template <int K>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs a) {
+  constexpr int LDB = K + 8;  // padded LDS row (elements)
+  __shared__ __attribute__((aligned(16))) uint16_t sB[128 * LDB];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int row0 = blockIdx.x * 128 + wave * 32;
   const int col0 = blockIdx.y * 128;
-  const int ar = row0 + (lane & 31);
+  for (int idx = threadIdx.x; idx < 128 * (K / 8); idx += 256) {
+    const int r = idx / (K / 8), c8 = idx % (K / 8);
+    *reinterpret_cast<u32x4*>(sB + r * LDB + c8 * 8) =
+        *reinterpret_cast<const u32x4*>(a.B + (int64_t)(col0 + r) * K + c8 * 8);
+  }
+  __syncthreads();
   const int kh = (lane >> 5) * 8;
-  const int K = a.K1 + a.K2;
-  f32x16 acc[4];
+  const int tiles = (a.n + 127) / 128;
+  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int row = tile * 128 + wave * 32 + (lane & 31);  // this lane's output row
+    f32x16 acc[4];
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int q = 0; q < 16; ++q) acc[t][q] = 0.0f;
-  for (int k = 0; k < K; k += 16) {
-    u32x4 av = {0u, 0u, 0u, 0u};
-    if (ar < a.n) {
-      const uint16_t* src = k < a.K1 ? a.X1 + (int64_t)ar * a.K1 + k + kh
-                                     : a.X2 + (int64_t)ar * a.K2 + (k - a.K1) + kh;
-      av = *reinterpret_cast<const u32x4*>(src);
+      for (int q = 0; q < 16; ++q) acc[t][q] = 0.0f;
+    u32x4 xv[K / 16];
+#pragma unroll
+    for (int ks = 0; ks < K / 16; ++ks) {
+      const int k = ks * 16;
+      xv[ks] = u32x4{0u, 0u, 0u, 0u};
+      if (row < a.n)
+        xv[ks] = *reinterpret_cast<const u32x4*>(
+            k < a.K1 ? a.X1 + (int64_t)row * a.K1 + k + kh
+                     : a.X2 + (int64_t)row * a.K2 + (k - a.K1) + kh);
     }
-    const bf16x8 A = as_bf16x8(av);
+#pragma unroll
+    for (int ks = 0; ks < K / 16; ++ks) {
+      const bf16x8 X = as_bf16x8(xv[ks]);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const u32x4 bv =
+            *reinterpret_cast<const u32x4*>(sB + (t * 32 + (lane & 31)) * LDB + ks * 16 + kh);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(bv), X, acc[t], 0, 0, 0);
+      }
+    }
+    if (row >= a.n) continue;
+    const float rs = a.rowscale ? a.rowscale[row] : 0.0f;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const int c = col0 + t * 32 + (lane & 31);
-      const u32x4 bv = *reinterpret_cast<const u32x4*>(a.B + (int64_t)c * K + k + kh);
-      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, as_bf16x8(bv), acc[t], 0, 0, 0);
-    }
-  }
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int col = col0 + t * 32 + (lane & 31);
-    const float bc = a.bias ? a.bias[col] : 0.0f;
-    const float b2 = a.rowscale ? a.bias2[col] : 0.0f;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int row = row0 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
-      if (row >= a.n) continue;
-      float v = acc[t][q] + bc;
-      if (a.rowscale) v += a.rowscale[row] * b2;
-      const int64_t o = (int64_t)row * a.Mo + col;
-      if (a.R) v += a.R[o];
-      if (a.Yf) a.Yf[o] = v;
-      if (a.Yb) a.Yb[o] = f2bf(a.act ? silu_f(v) : v);
+      for (int g = 0; g < 4; ++g) {
+        const int col = col0 + t * 32 + g * 8 + (lane >> 5) * 4;
+        float4 v = make_float4(acc[t][4 * g], acc[t][4 * g + 1], acc[t][4 * g + 2],
+                               acc[t][4 * g + 3]);
+        if (a.bias) {
+          const float4 bc = *reinterpret_cast<const float4*>(a.bias + col);
+          v.x += bc.x; v.y += bc.y; v.z += bc.z; v.w += bc.w;
+        }
+        if (a.rowscale) {
+          const float4 b2 = *reinterpret_cast<const float4*>(a.bias2 + col);
+          v.x += rs * b2.x; v.y += rs * b2.y; v.z += rs * b2.z; v.w += rs * b2.w;
+        }
+        const int64_t o = (int64_t)row * a.Mo + col;
+        if (a.R) {
+          const float4 rr = *reinterpret_cast<const float4*>(a.R + o);
+          v.x += rr.x; v.y += rr.y; v.z += rr.z; v.w += rr.w;
+        }
+        if (a.Yf) *reinterpret_cast<float4*>(a.Yf + o) = v;
+        if (a.Yb) {
+          float4 y = v;
+          if (a.act) {
+            y.x = silu_f(y.x); y.y = silu_f(y.y); y.z = silu_f(y.z); y.w = silu_f(y.w);
+          }
+          const uint32_t lo = (uint32_t)f2bf(y.x) | ((uint32_t)f2bf(y.y) << 16);
+          const uint32_t hi = (uint32_t)f2bf(y.z) | ((uint32_t)f2bf(y.w) << 16);
+          *reinterpret_cast<uint2*>(a.Yb + o) = make_uint2(lo, hi);
+        }
+      }
     }
   }
 }
@@ -258,7 +304,7 @@ __global__ __launch_bounds__(256) void k_node_features(NodeIn in, NnConsts k,
                                                        float* __restrict__ deg) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= in.n) return;
-  const float d = (float)min(in.cnt[i], in.w);
+  const float d = (float)min(in.cnt[i], min(in.w, kMaxNnNeighbors));
   deg[i] = d;
   float f[kNodeF] = {k.rs * (in.rho[i] * k.inv_rho0 - 1.0f), k.ps * in.pr[i], d * 0.125f, 1.0f,
                      0.0f, 0.0f, 0.0f, 0.0f};
@@ -280,24 +326,58 @@ __device__ __forceinline__ void edge_feat(const NnConsts& k, float xi, float yi,
     dx = dx > k.half_x ? dx - k.len_x : (dx < -k.half_x ? dx + k.len_x : dx);
     dy = dy > k.half_y ? dy - k.len_y : (dy < -k.half_y ? dy + k.len_y : dy);
   }
-  const float r = sqrtf(dx * dx + dy * dy);
+  const float r = fast_sqrt(dx * dx + dy * dy);
   const float q = r * k.inv_h;
   const float w = (1.0f - q) * (1.0f - q);
   const float dvx = vxi - vxj, dvy = vyi - vyj;
   f[0] = q;
   f[1] = w;
-  f[2] = w / fmaxf(q, 0.05f);
-  f[3] = (dvx * dx + dvy * dy) * k.inv_vs / fmaxf(r, 0.05f * k.h);
-  f[4] = sqrtf(dvx * dvx + dvy * dvy) * k.inv_vs;
+  f[2] = fast_div(w, fmaxf(q, 0.05f));
+  f[3] = fast_div((dvx * dx + dvy * dy) * k.inv_vs, fmaxf(r, 0.05f * k.h));
+  f[4] = fast_sqrt(dvx * dvx + dvy * dvy) * k.inv_vs;
   f[5] = k.rs * (rhoj * k.inv_rho0 - 1.0f);
   f[6] = k.ps * prj;
   f[7] = f[2] * (k.ps * pri + k.ps * prj);  // the form of the SPH pressure term
 }
 
+// Edge kernels: one wavefront per node, 16 nodes per wavefront in turn. For a
+// node, lane e (< its neighbor count, at most kMaxNnNeighbors) loads neighbor e
+// and computes that edge's invariant features in parallel; the hidden-column
+// phase (lane l owns columns 2l and 2l + 1) then broadcasts each edge's values
+// with readlane and fetches the neighbor rows of 8 edges at a time, so the loads
+// of a node are in flight together instead of one edge after another.
+
+__device__ __forceinline__ float bcast(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// per-lane data of edge `lane` of a node: sender j, features, separation
+struct EdgeLane {
+  int j;
+  float f[kEdgeF];
+  float dx, dy;
+};
+
+template <int BC>
+__device__ __forceinline__ void load_edge_lane(const NodeIn& in, const NnConsts& k, int i, int deg,
+                                               float xi, float yi, float vxi, float vyi,
+                                               float pri, EdgeLane& el) {
+  el.j = i;
+  el.dx = el.dy = 0.0f;
+#pragma unroll
+  for (int q = 0; q < kEdgeF; ++q) el.f[q] = 0.0f;
+  const int lane = threadIdx.x & 63;
+  if (lane < deg) {
+    const int j = in.idx[(int64_t)lane * in.n + i];
+    el.j = j;
+    edge_feat<BC>(k, xi, yi, vxi, vyi, pri, in.rx[j], in.ry[j], in.vx[j], in.vy[j], in.rho[j],
+                  in.pr[j], el.dx, el.dy, el.f);
+  }
+}
+
 // --------------------------------------------------------- edge forward
-// One wavefront per receiver, lane l owns hidden columns 2l and 2l + 1.
 // MODE 0: S_i = sum_j silu(P_i + Q_j + C f_ij + c).
-// MODE 1: a_i = g + (a_s / h) sum_j (r_i - r_j) (w . silu(...) + b).
+// MODE 1: a_i = g + (a_s / h) sum_j (r_i - r_j) (w . silu(...) + b + u . f_ij).
 template <int MODE, int BC>
 __global__ __launch_bounds__(256) void k_edge_fwd(NodeIn in, NnConsts k, const float* __restrict__ P,
                                                   const float* __restrict__ Q,
@@ -311,16 +391,15 @@ __global__ __launch_bounds__(256) void k_edge_fwd(NodeIn in, NnConsts k, const f
                                                   float* __restrict__ ax, float* __restrict__ ay) {
   const int lane = threadIdx.x & 63, wave = wave_id();
   const int c0 = 2 * lane;
-  float C0[kEdgeF], C1[kEdgeF];
+  float C0[kEdgeF], C1[kEdgeF], u[kEdgeF];
 #pragma unroll
   for (int e = 0; e < kEdgeF; ++e) {
     C0[e] = bf2f(Cb[c0 * kEdgeF + e]);
     C1[e] = bf2f(Cb[(c0 + 1) * kEdgeF + e]);
+    u[e] = MODE == 1 ? uvec[e] : 0.0f;
   }
   const float cb0 = cb[c0], cb1 = cb[c0 + 1];
-  float w0 = 0.0f, w1 = 0.0f, b = 0.0f, u[kEdgeF];
-#pragma unroll
-  for (int q = 0; q < kEdgeF; ++q) u[q] = MODE == 1 ? uvec[q] : 0.0f;
+  float w0 = 0.0f, w1 = 0.0f, b = 0.0f;
   if (MODE == 1) {
     w0 = bf2f(wvec[c0]);
     w1 = bf2f(wvec[c0 + 1]);
@@ -332,65 +411,79 @@ __global__ __launch_bounds__(256) void k_edge_fwd(NodeIn in, NnConsts k, const f
     const int i = base + t;
     if (i >= in.n) break;
     const float xi = in.rx[i], yi = in.ry[i], vxi = in.vx[i], vyi = in.vy[i], pri = in.pr[i];
-    const int deg = min(in.cnt[i], in.w);
+    const int deg = min(in.cnt[i], min(in.w, kMaxNnNeighbors));
+    EdgeLane el;
+    load_edge_lane<BC>(in, k, i, deg, xi, yi, vxi, vyi, pri, el);
+    // the linear readout path is per edge: lane-parallel, reduced below
+    float lx = 0.0f, ly = 0.0f;
+    if (MODE == 1 && lane < deg) {
+      float uf = 0.0f;
+#pragma unroll
+      for (int q = 0; q < kEdgeF; ++q) uf += u[q] * el.f[q];
+      lx = el.dx * uf;
+      ly = el.dy * uf;
+    }
     const float2 p = *reinterpret_cast<const float2*>(P + (int64_t)i * W + c0);
-    float s0 = 0.0f, s1 = 0.0f, tx0 = 0.0f, ty0 = 0.0f, tx1 = 0.0f, ty1 = 0.0f, sdx = 0.0f,
-          sdy = 0.0f, lx = 0.0f, ly = 0.0f;
-    for (int e = 0; e < deg; ++e) {
-      const int j = in.idx[(int64_t)e * in.n + i];
-      float dx, dy, f[kEdgeF];
-      edge_feat<BC>(k, xi, yi, vxi, vyi, pri, in.rx[j], in.ry[j], in.vx[j], in.vy[j],
-                    in.rho[j], in.pr[j], dx, dy, f);
-      const float2 qv = *reinterpret_cast<const float2*>(Q + (int64_t)j * W + c0);
-      float z0 = p.x + qv.x + cb0, z1 = p.y + qv.y + cb1;
+    float s0 = 0.0f, s1 = 0.0f, tx0 = 0.0f, ty0 = 0.0f, tx1 = 0.0f, ty1 = 0.0f;
+    for (int e0 = 0; e0 < deg; e0 += 8) {
+      float2 qv[8];
 #pragma unroll
-      for (int q = 0; q < kEdgeF; ++q) {
-        z0 += C0[q] * f[q];
-        z1 += C1[q] * f[q];
+      for (int v = 0; v < 8; ++v) {
+        const int j = __builtin_amdgcn_readlane(el.j, e0 + v);
+        qv[v] = e0 + v < deg ? *reinterpret_cast<const float2*>(Q + (int64_t)j * W + c0)
+                             : make_float2(0.0f, 0.0f);
       }
-      const float a0 = silu_f(z0), a1 = silu_f(z1);
-      if (MODE == 0) {
-        s0 += a0;
-        s1 += a1;
-      } else {
-        tx0 += dx * a0;
-        ty0 += dy * a0;
-        tx1 += dx * a1;
-        ty1 += dy * a1;
-        sdx += dx;
-        sdy += dy;
-        float uf = 0.0f;
 #pragma unroll
-        for (int q = 0; q < kEdgeF; ++q) uf += u[q] * f[q];
-        lx += dx * uf;
-        ly += dy * uf;
+      for (int v = 0; v < 8; ++v) {
+        const int e = e0 + v;
+        if (e >= deg) break;
+        float z0 = p.x + qv[v].x + cb0, z1 = p.y + qv[v].y + cb1;
+#pragma unroll
+        for (int q = 0; q < kEdgeF; ++q) {
+          const float fq = bcast(el.f[q], e);
+          z0 += C0[q] * fq;
+          z1 += C1[q] * fq;
+        }
+        const float a0 = silu_f(z0), a1 = silu_f(z1);
+        if (MODE == 0) {
+          s0 += a0;
+          s1 += a1;
+        } else {
+          const float dx = bcast(el.dx, e), dy = bcast(el.dy, e);
+          tx0 += dx * a0;
+          ty0 += dy * a0;
+          tx1 += dx * a1;
+          ty1 += dy * a1;
+        }
       }
     }
     if (MODE == 0) {
       const int64_t o = (int64_t)i * W + c0;
-      *reinterpret_cast<float2*>(S + o) = make_float2(s0, s1);
-      Sb[o] = f2bf(s0);
-      Sb[o + 1] = f2bf(s1);
+      if (S) *reinterpret_cast<float2*>(S + o) = make_float2(s0, s1);
+      *reinterpret_cast<uint32_t*>(Sb + o) = (uint32_t)f2bf(s0) | ((uint32_t)f2bf(s1) << 16);
     } else {
-      float px = w0 * tx0 + w1 * tx1, py = w0 * ty0 + w1 * ty1;
+      // sum_j d_ij (w . s_ij + b + u . f_ij): w-part per column, b and u parts per edge lane
+      const float bd = lane < deg ? b : 0.0f;
+      float px = w0 * tx0 + w1 * tx1 + bd * el.dx + lx;
+      float py = w0 * ty0 + w1 * ty1 + bd * el.dy + ly;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
         px += __shfl_xor(px, o);
         py += __shfl_xor(py, o);
       }
       if (lane == 0) {
-        ax[i] = k.gx + sc * (px + b * sdx + lx);
-        ay[i] = k.gy + sc * (py + b * sdy + ly);
+        ax[i] = k.gx + sc * px;
+        ay[i] = k.gy + sc * py;
       }
     }
   }
 }
 
 // --------------------------------------------------------- edge backward
-// One wavefront per node k, lane l owns columns 2l, 2l + 1. Receiver part:
-// dz_kj = g_kj * silu'(z_kj) over k's list gives dP_k and the C, c (and readout
-// w, b, u) gradient partials; sender part: dQ_k = sum over receivers i with k in
-// i's list of dz_ik, visited through k's own list (the lists are symmetric).
+// One wavefront per node k. Receiver part: dz_kj = g_kj * silu'(z_kj) over k's
+// list gives dP_k and the C, c (and readout w, b, u) gradient partials; sender
+// part: dQ_k = sum over receivers i with k in i's list of dz_ik, visited through
+// k's own list (the lists are symmetric).
 // MODE 0: g_kj = dS_k. MODE 1: g_kj = (a_s / h) (da_k . (r_k - r_j)) w.
 template <int MODE, int BC>
 __global__ __launch_bounds__(256) void k_edge_bwd(
@@ -402,13 +495,14 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
   __shared__ float s_red[4][kEdgePartial];
   const int lane = threadIdx.x & 63, wave = wave_id();
   const int c0 = 2 * lane;
-  float C0[kEdgeF], C1[kEdgeF], gC0[kEdgeF], gC1[kEdgeF];
+  float C0[kEdgeF], C1[kEdgeF], gC0[kEdgeF], gC1[kEdgeF], gu[kEdgeF];
 #pragma unroll
   for (int e = 0; e < kEdgeF; ++e) {
     C0[e] = bf2f(Cb[c0 * kEdgeF + e]);
     C1[e] = bf2f(Cb[(c0 + 1) * kEdgeF + e]);
     gC0[e] = 0.0f;
     gC1[e] = 0.0f;
+    gu[e] = 0.0f;
   }
   const float cb0 = cb[c0], cb1 = cb[c0 + 1];
   float w0 = 0.0f, w1 = 0.0f;
@@ -416,9 +510,7 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
     w0 = bf2f(wvec[c0]);
     w1 = bf2f(wvec[c0 + 1]);
   }
-  float gc0 = 0.0f, gc1 = 0.0f, gw0 = 0.0f, gw1 = 0.0f, gb = 0.0f, gu[kEdgeF];
-#pragma unroll
-  for (int q = 0; q < kEdgeF; ++q) gu[q] = 0.0f;
+  float gc0 = 0.0f, gc1 = 0.0f, gw0 = 0.0f, gw1 = 0.0f, gb = 0.0f;
   const float sc = k.a_s * k.inv_h;
   const int base = (blockIdx.x * 4 + wave) * kNodesPerWave;
   for (int t = 0; t < kNodesPerWave; ++t) {
@@ -426,92 +518,115 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
     if (n0 >= in.n) break;
     const float xk = in.rx[n0], yk = in.ry[n0], vxk = in.vx[n0], vyk = in.vy[n0],
                 rhok = in.rho[n0], prk = in.pr[n0];
-    const int deg = min(in.cnt[n0], in.w);
+    const int deg = min(in.cnt[n0], min(in.w, kMaxNnNeighbors));
+    // lane e: edge k <- j_e (receiver k) and the reverse edge j_e <- k
+    EdgeLane el;
+    load_edge_lane<BC>(in, k, n0, deg, xk, yk, vxk, vyk, prk, el);
+    float rf[kEdgeF], rdx = 0.0f, rdy = 0.0f, dphi_k = 0.0f, dphi_r = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kEdgeF; ++q) rf[q] = 0.0f;
+    if (lane < deg) {
+      const int j = el.j;
+      edge_feat<BC>(k, in.rx[j], in.ry[j], in.vx[j], in.vy[j], in.pr[j], xk, yk, vxk, vyk, rhok,
+                    prk, rdx, rdy, rf);
+      if (MODE == 1) {
+        dphi_k = sc * (G[2 * (int64_t)n0] * el.dx + G[2 * (int64_t)n0 + 1] * el.dy);
+        dphi_r = sc * (G[2 * (int64_t)j] * rdx + G[2 * (int64_t)j + 1] * rdy);
+#pragma unroll
+        for (int q = 0; q < kEdgeF; ++q) gu[q] += dphi_k * el.f[q];
+        gb += dphi_k;
+      }
+    }
     const float2 pk = *reinterpret_cast<const float2*>(P + (int64_t)n0 * W + c0);
     const float2 qk = *reinterpret_cast<const float2*>(Q + (int64_t)n0 * W + c0);
     float2 gk = make_float2(0.0f, 0.0f);
-    float dak_x = 0.0f, dak_y = 0.0f;
-    if (MODE == 0) {
-      gk = *reinterpret_cast<const float2*>(G + (int64_t)n0 * W + c0);
-    } else {
-      dak_x = G[2 * (int64_t)n0];
-      dak_y = G[2 * (int64_t)n0 + 1];
-    }
+    if (MODE == 0) gk = *reinterpret_cast<const float2*>(G + (int64_t)n0 * W + c0);
     float dp0 = 0.0f, dp1 = 0.0f, dq0 = 0.0f, dq1 = 0.0f;
-    for (int e = 0; e < deg; ++e) {
-      const int j = in.idx[(int64_t)e * in.n + n0];
-      const float xj = in.rx[j], yj = in.ry[j], vxj = in.vx[j], vyj = in.vy[j],
-                  rhoj = in.rho[j], prj = in.pr[j];
-      // k receives from j
-      {
-        float dx, dy, f[kEdgeF];
-        edge_feat<BC>(k, xk, yk, vxk, vyk, prk, xj, yj, vxj, vyj, rhoj, prj, dx, dy, f);
-        const float2 qj = *reinterpret_cast<const float2*>(Q + (int64_t)j * W + c0);
-        float z0 = pk.x + qj.x + cb0, z1 = pk.y + qj.y + cb1;
+    for (int e0 = 0; e0 < deg; e0 += 8) {
+      float2 qj[8], pj[8], gj[8];
 #pragma unroll
-        for (int q = 0; q < kEdgeF; ++q) {
-          z0 += C0[q] * f[q];
-          z1 += C1[q] * f[q];
-        }
-        float g0, g1;
-        if (MODE == 0) {
-          g0 = gk.x;
-          g1 = gk.y;
-        } else {
-          const float dphi = sc * (dak_x * dx + dak_y * dy);
-          gw0 += dphi * silu_f(z0);
-          gw1 += dphi * silu_f(z1);
-          gb += dphi;
-#pragma unroll
-          for (int q = 0; q < kEdgeF; ++q) gu[q] += dphi * f[q];
-          g0 = dphi * w0;
-          g1 = dphi * w1;
-        }
-        const float dz0 = g0 * dsilu_f(z0), dz1 = g1 * dsilu_f(z1);
-        dp0 += dz0;
-        dp1 += dz1;
-        gc0 += dz0;
-        gc1 += dz1;
-#pragma unroll
-        for (int q = 0; q < kEdgeF; ++q) {
-          gC0[q] += dz0 * f[q];
-          gC1[q] += dz1 * f[q];
-        }
+      for (int v = 0; v < 8; ++v) {
+        const int j = __builtin_amdgcn_readlane(el.j, e0 + v);
+        const bool ok = e0 + v < deg;
+        const int64_t o = (int64_t)j * W + c0;
+        qj[v] = ok ? *reinterpret_cast<const float2*>(Q + o) : make_float2(0.0f, 0.0f);
+        pj[v] = ok ? *reinterpret_cast<const float2*>(P + o) : make_float2(0.0f, 0.0f);
+        gj[v] = (MODE == 0 && ok) ? *reinterpret_cast<const float2*>(G + o) : make_float2(0.0f, 0.0f);
       }
-      // j receives from k (the same pair seen from j's list)
-      {
-        float dx, dy, f[kEdgeF];
-        edge_feat<BC>(k, xj, yj, vxj, vyj, prj, xk, yk, vxk, vyk, rhok, prk, dx, dy, f);
-        const float2 pj = *reinterpret_cast<const float2*>(P + (int64_t)j * W + c0);
-        float z0 = pj.x + qk.x + cb0, z1 = pj.y + qk.y + cb1;
 #pragma unroll
-        for (int q = 0; q < kEdgeF; ++q) {
-          z0 += C0[q] * f[q];
-          z1 += C1[q] * f[q];
+      for (int v = 0; v < 8; ++v) {
+        const int e = e0 + v;
+        if (e >= deg) break;
+        // k receives from j
+        {
+          float z0 = pk.x + qj[v].x + cb0, z1 = pk.y + qj[v].y + cb1;
+          float fq[kEdgeF];
+#pragma unroll
+          for (int q = 0; q < kEdgeF; ++q) {
+            fq[q] = bcast(el.f[q], e);
+            z0 += C0[q] * fq[q];
+            z1 += C1[q] * fq[q];
+          }
+          float g0, g1;
+          if (MODE == 0) {
+            g0 = gk.x;
+            g1 = gk.y;
+          } else {
+            const float dphi = bcast(dphi_k, e);
+            gw0 += dphi * silu_f(z0);
+            gw1 += dphi * silu_f(z1);
+            g0 = dphi * w0;
+            g1 = dphi * w1;
+          }
+          const float dz0 = g0 * dsilu_f(z0), dz1 = g1 * dsilu_f(z1);
+          dp0 += dz0;
+          dp1 += dz1;
+          gc0 += dz0;
+          gc1 += dz1;
+#pragma unroll
+          for (int q = 0; q < kEdgeF; ++q) {
+            gC0[q] += dz0 * fq[q];
+            gC1[q] += dz1 * fq[q];
+          }
         }
-        float g0, g1;
-        if (MODE == 0) {
-          const float2 gj = *reinterpret_cast<const float2*>(G + (int64_t)j * W + c0);
-          g0 = gj.x;
-          g1 = gj.y;
-        } else {
-          const float dphi = sc * (G[2 * (int64_t)j] * dx + G[2 * (int64_t)j + 1] * dy);
-          g0 = dphi * w0;
-          g1 = dphi * w1;
+        // j receives from k
+        {
+          float z0 = pj[v].x + qk.x + cb0, z1 = pj[v].y + qk.y + cb1;
+#pragma unroll
+          for (int q = 0; q < kEdgeF; ++q) {
+            const float fq = bcast(rf[q], e);
+            z0 += C0[q] * fq;
+            z1 += C1[q] * fq;
+          }
+          float g0, g1;
+          if (MODE == 0) {
+            g0 = gj[v].x;
+            g1 = gj[v].y;
+          } else {
+            const float dphi = bcast(dphi_r, e);
+            g0 = dphi * w0;
+            g1 = dphi * w1;
+          }
+          dq0 += g0 * dsilu_f(z0);
+          dq1 += g1 * dsilu_f(z1);
         }
-        dq0 += g0 * dsilu_f(z0);
-        dq1 += g1 * dsilu_f(z1);
       }
     }
     const int64_t o = (int64_t)n0 * W + c0;
     *reinterpret_cast<float2*>(dP + o) = make_float2(dp0, dp1);
     *reinterpret_cast<float2*>(dQ + o) = make_float2(dq0, dq1);
-    dPb[o] = f2bf(dp0);
-    dPb[o + 1] = f2bf(dp1);
-    dQb[o] = f2bf(dq0);
-    dQb[o + 1] = f2bf(dq1);
+    *reinterpret_cast<uint32_t*>(dPb + o) = (uint32_t)f2bf(dp0) | ((uint32_t)f2bf(dp1) << 16);
+    *reinterpret_cast<uint32_t*>(dQb + o) = (uint32_t)f2bf(dq0) | ((uint32_t)f2bf(dq1) << 16);
   }
-  // workgroup partials of dC, dc, dw, db (fixed order over the four waves)
+  // workgroup partials of dC, dc, dw, db, du (fixed order over lanes and waves)
+  if (MODE == 1) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      gb += __shfl_xor(gb, o);
+#pragma unroll
+      for (int q = 0; q < kEdgeF; ++q) gu[q] += __shfl_xor(gu[q], o);
+    }
+  }
   float* r = s_red[wave];
 #pragma unroll
   for (int q = 0; q < kEdgeF; ++q) {
@@ -666,7 +781,14 @@ void gemm(hipStream_t st, int n, const uint16_t* X1, int K1, const uint16_t* X2,
           const uint16_t* B, int Mo, const float* bias, const float* rowscale, const float* bias2,
           const float* R, float* Yf, uint16_t* Yb, int act) {
   GemmArgs a{n, K1, K2, Mo, X1, X2, B, bias, rowscale, bias2, R, Yf, Yb, act};
-  k_gemm<<<dim3(cdiv(n, 128), Mo / 128), 256, 0, st>>>(a);
+  // up to 8 workgroups per CU, each walking a few row tiles
+  const dim3 grid(std::min(cdiv(n, 128), 2048), Mo / 128);
+  switch (K1 + K2) {
+    case kNodeK: k_gemm<kNodeK><<<grid, 256, 0, st>>>(a); break;
+    case W: k_gemm<W><<<grid, 256, 0, st>>>(a); break;
+    case 2 * W: k_gemm<2 * W><<<grid, 256, 0, st>>>(a); break;
+    default: break;  // the network uses no other K
+  }
 }
 
 // weight gradient of tensor t (rows Mo, padded K kp = K1 + K2) into grad
@@ -795,7 +917,8 @@ int nn_create(Net* net, int layers, bool train, int64_t n_cap, const NnConsts& k
   A(&net->a1b, NW);
   net->lb.assign(nl, NnLayerBufs{});
   for (auto& b : net->lb) {
-    A(&b.h, NW); A(&b.P, NW); A(&b.Q, NW); A(&b.S, NW); A(&b.M, NW); A(&b.preU, NW);
+    A(&b.h, NW); A(&b.P, NW); A(&b.Q, NW);
+    if (train) A(&b.preU, NW);
     A(&b.hb, NW); A(&b.Sb, NW); A(&b.Mb, NW); A(&b.Ub, NW);
   }
   A(&net->a_out, 2 * N);
@@ -856,10 +979,11 @@ hipError_t nn_forward(Net* net, const DevState& s, hipStream_t st, float* ax, fl
          nullptr, b.P, nullptr, 0);
     gemm(st, n, b.hb, W, nullptr, 0, WB(net, lt(net, l, L_B)), W, nullptr, nullptr, nullptr,
          nullptr, b.Q, nullptr, 0);
+    // only the bf16 copies of S and M are read later (forward and backward)
     edge_fwd<0>(net, s, st, b.P, b.Q, lt(net, l, L_C), lt(net, l, L_c), nullptr, nullptr,
-                nullptr, b.S, b.Sb, nullptr, nullptr);
+                nullptr, nullptr, b.Sb, nullptr, nullptr);
     gemm(st, n, b.Sb, W, nullptr, 0, WB(net, lt(net, l, L_D)), W, nullptr, net->deg,
-         P_(net, lt(net, l, L_d)), nullptr, b.M, b.Mb, 0);
+         P_(net, lt(net, l, L_d)), nullptr, nullptr, b.Mb, 0);
     gemm(st, n, b.hb, W, b.Mb, W, WB(net, lt(net, l, L_E)), W, P_(net, lt(net, l, L_e)), nullptr,
          nullptr, nullptr, net->train ? b.preU : nullptr, b.Ub, 1);
     // h_{l+1} = h_l + F U + f (in place when activations are not kept)

@@ -35,8 +35,8 @@ struct NnTensor {
 };
 
 struct NnLayerBufs {
-  float *h, *P, *Q, *S, *M, *preU;  // fp32 [n x W]
-  uint16_t *hb, *Sb, *Mb, *Ub;      // bf16 copies
+  float *h, *P, *Q, *preU;      // fp32 [n x W] (preU only when training)
+  uint16_t *hb, *Sb, *Mb, *Ub;  // bf16 activations
 };
 
 struct Net {
