@@ -36,6 +36,7 @@
 #include <iostream>
 #include <memory>
 #include <stdexcept>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -637,3 +638,25 @@ inline double MeasureGpu(SoAContainer& c, int k) {
 inline Real ComputeKineticEnergy(SoAContainer& c) { return (Real)MeasureGpu(c, 0); }
 inline Real ComputeMaxVelocity(SoAContainer& c) { return (Real)MeasureGpu(c, 1); }
 inline int ParticlesInDomain(Domain&, SoAContainer& c) { return (int)MeasureGpu(c, 2); }
+
+// The per-step line of the reference driver (app/simulation_configuration.h:152-158),
+// from one FP64 measurement on the device.
+inline std::string StepHeader(int step, SoAContainer& c, Domain&) {
+  void* ctx = c.DeviceContext();
+  if (!ctx) throw sph::Error("no device twin");
+  double m[5];
+  sph::check(sph_measure(ctx, m), ctx, "sph_measure");
+  std::ostringstream out;
+  out << "Step= " << step << "\t N=" << (int)m[2] << "\t E_k=" << (Real)m[0]
+      << "\t |v|_max=" << (Real)m[1] << std::endl;
+  return out.str();
+}
+
+// Time step from the CFL and force criteria (FP64 maxima on the device).
+inline double CflTimeStep(SoAContainer& c, double cfl = 0.25) {
+  void* ctx = c.DeviceContext();
+  if (!ctx) throw sph::Error("no device twin");
+  double out[3];
+  sph::check(sph_cfl(ctx, cfl, out), ctx, "sph_cfl");
+  return out[0];
+}

@@ -163,6 +163,29 @@ int sph_set_dt(void* ctx, double dt);
  * domain, out[3] potential energy, out[4] total mass. Synchronises. */
 int sph_measure(void* ctx, double* out5);
 
+/* Time step from the CFL and force criteria with FP64 maxima over the current
+ * state (velocities and the accelerations of the last force evaluation):
+ *   dt = min(cfl * h / (c0 + max|v|), 0.25 * sqrt(h / max|a|))
+ * (the first term is the fixed dt = cfl h / c0 of a fluid at rest).
+ * out3 = {dt, max|v|, max|a|}. Does not apply it: pass dt to sph_set_dt. The
+ * reference runs with a fixed dt; this backs adaptive stepping. Synchronises. */
+int sph_cfl(void* ctx, double cfl, double* out3);
+
+/* ---- binary snapshots (replace positions_step_<k>.txt of
+ *      core/utils/CheckPointWriter.cpp:12-27, which keeps positions only) ----
+ * Little-endian file: "SPHSNAP1", uint32 version (1), uint32 header bytes (192),
+ * int64 n, int64 step, sph_params (144 bytes), uint32 flags (1: arrays in the
+ * context's sorted order), uint32 reserved, uint64 FNV-1a of the arrays; then
+ * float32 rx[n], ry[n], vx[n], vy[n], uint32 pid[n]. Saving the sorted order
+ * with the ids makes a resumed run continue bit-identically (sph_prepare after
+ * the load re-sorts an already-sorted state). Single-scene contexts only. */
+int sph_snapshot_info(const char* path, sph_params* params, int64_t* n, int64_t* step);
+int sph_snapshot_save(void* ctx, const char* path, int64_t step);
+/* Needs a context created with the file's parameters (compared byte for byte,
+ * except dt, which the load applies) and n <= its capacity; leaves it as after
+ * sph_upload. */
+int sph_snapshot_load(void* ctx, const char* path, int64_t* step);
+
 /* ---- fused pipeline (no reference counterpart; same arithmetic as the stages) ----
  * sph_prepare: find neighbors + density with the fixed-width neighbor list.
  * sph_step: n iterations of [density + forces + integration + keys] -> radix

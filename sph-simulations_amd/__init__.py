@@ -8,9 +8,10 @@ from .capi import (BC_PERIODIC, BC_WALLS, COMPUTE_LISTS, COMPUTE_SORTED, LIB_PAT
                    STEP_FUSED_BLOCK, STEP_PARTICLE, STEP_SPLIT_BLOCK, RESORT_FULL,
                    RESORT_INCREMENTAL, Batch, BatchTuning, Context, GridInfo, Params,
                    SphError, lib)
-from . import scenes
+from . import diagnostics, scenes
+from .capi import snapshot_info
 
-__all__ = ["Batch", "BatchTuning", "Context", "Params", "GridInfo", "SphError", "lib", "scenes", "LIB_PATH",
+__all__ = ["diagnostics", "snapshot_info", "Batch", "BatchTuning", "Context", "Params", "GridInfo", "SphError", "lib", "scenes", "LIB_PATH",
            "ORDER_MORTON", "ORDER_ROWMAJOR", "PHYS_REFERENCE", "PHYS_WCSPH", "BC_WALLS",
            "BC_PERIODIC", "COMPUTE_SORTED", "COMPUTE_LISTS", "STAGES", "STEP_SPLIT_BLOCK",
            "STEP_FUSED_BLOCK", "STEP_PARTICLE", "RESORT_FULL", "RESORT_INCREMENTAL"]

@@ -181,6 +181,10 @@ def lib():
         "sph_batch_autotune": (i32, [vp, P(BatchTuning), i32, i32, P(d), P(i32)]),
         "sph_batch_measure": (i32, [vp, P(d)]),
         "sph_export_state": (i32, [vp, vp]),
+        "sph_cfl": (i32, [vp, d, P(d)]),
+        "sph_snapshot_info": (i32, [C.c_char_p, P(SphParams), P(i64), P(i64)]),
+        "sph_snapshot_save": (i32, [vp, C.c_char_p, i64]),
+        "sph_snapshot_load": (i32, [vp, C.c_char_p, P(i64)]),
         "sph_device_malloc": (i32, [C.c_int, i64, P(vp)]),
         "sph_device_free": (i32, [vp]),
         "sph_device_copy": (i32, [vp, vp, i64, i32]),
@@ -217,7 +221,8 @@ EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_
             "sph_get_device_view", "sph_set_timing", "sph_stage_times", "sph_synchronize",
             "sph_batch_create", "sph_batch_scenes", "sph_batch_scene_info",
             "sph_batch_set_tuning", "sph_batch_get_tuning", "sph_batch_default_candidates",
-            "sph_batch_autotune", "sph_batch_measure", "sph_export_state",
+            "sph_batch_autotune", "sph_batch_measure", "sph_export_state", "sph_cfl",
+            "sph_snapshot_info", "sph_snapshot_save", "sph_snapshot_load",
             "sph_device_malloc", "sph_device_free", "sph_device_copy",
             "sph_upload_device", "sph_nn_create", "sph_nn_destroy", "sph_nn_last_error",
             "sph_nn_num_params", "sph_nn_set_params", "sph_nn_get_params", "sph_nn_get_grad",
@@ -466,6 +471,23 @@ class Context:
         self._chk(self._L.sph_get_ell(self._h, _ptr(cnt), None), "sph_get_ell")
         return cnt
 
+    def cfl(self, cfl=0.25) -> dict:
+        """sph_cfl: dt = min(cfl h / (c0 + max|v|), 0.25 sqrt(h / max|a|)) (FP64 maxima)."""
+        out = np.zeros(3, np.float64)
+        self._chk(self._L.sph_cfl(self._h, float(cfl), out.ctypes.data_as(C.POINTER(C.c_double))),
+                  "sph_cfl")
+        return {"dt": float(out[0]), "v_max": float(out[1]), "a_max": float(out[2])}
+
+    def snapshot_save(self, path: str, step: int = 0):
+        self._chk(self._L.sph_snapshot_save(self._h, os.fsencode(path), int(step)),
+                  "sph_snapshot_save")
+
+    def snapshot_load(self, path: str) -> int:
+        st = C.c_int64()
+        self._chk(self._L.sph_snapshot_load(self._h, os.fsencode(path), C.byref(st)),
+                  "sph_snapshot_load")
+        return int(st.value)
+
     def export_state(self, dev_ptr: int):
         """State (rx | ry | vx | vy, particle-id order) into device memory at dev_ptr."""
         self._chk(self._L.sph_export_state(self._h, C.c_void_p(dev_ptr)), "sph_export_state")
@@ -566,3 +588,19 @@ class Batch(Context):
         self._chk(self._L.sph_batch_measure(self._h, out.ctypes.data_as(C.POINTER(C.c_double))),
                   "sph_batch_measure")
         return out
+
+
+def snapshot_info(path: str):
+    """(Params, n, step) stored in a binary snapshot (sph_snapshot_info)."""
+    L = lib()
+    sp = SphParams()
+    n, st = C.c_int64(), C.c_int64()
+    rc = L.sph_snapshot_info(os.fsencode(path), C.byref(sp), C.byref(n), C.byref(st))
+    if rc != SPH_OK:
+        raise SphError(f"sph_snapshot_info -> {rc}: {L.sph_last_error(None).decode()}")
+    p = Params(lo=(sp.lo[0], sp.lo[1]), hi=(sp.hi[0], sp.hi[1]), h=sp.h, cell_size=sp.cell_size,
+               dt=sp.dt, mass=sp.mass, rho0=sp.rho0, gravity_y=sp.gravity_y, eos_k=sp.eos_k,
+               mu=sp.mu, c0=sp.c0, alpha=sp.alpha, eps_visc=sp.eps_visc, gamma=sp.gamma,
+               physics=sp.physics, boundary=sp.boundary, order=sp.order,
+               ell_width=sp.ell_width)
+    return p, int(n.value), int(st.value)

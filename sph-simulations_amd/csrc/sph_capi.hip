@@ -1811,4 +1811,187 @@ int sph_surrogate_run(void* ctx, void* nn, int32_t n_steps, int32_t check_every,
   return SPH_OK;
 }
 
+int sph_cfl(void* ctx, double cfl, double* out3) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !out3) return SPH_ERR_INVALID;
+  if (!(cfl > 0) || !std::isfinite(cfl)) return fail(c, SPH_ERR_INVALID, "cfl must be positive");
+  if (c->n_scenes) return single_only(c);
+  int rc = check_n(c);
+  if (rc) return rc;
+  double m[2] = {0, 0};
+  HIPCHK(c, launch_max_va(c->L, c->s, c->meas_part, c->meas_out));
+  HIPCHK(c, hipMemcpyAsync(m, c->meas_out, sizeof(m), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double vmax = std::sqrt(m[0]), amax = std::sqrt(m[1]), h = c->prm.h;
+  const double c0 = c->prm.physics == SPH_PHYS_WCSPH ? c->prm.c0 : 0.0;
+  double dt = HUGE_VAL;
+  if (c0 + vmax > 0) dt = std::min(dt, cfl * h / (c0 + vmax));
+  if (amax > 0) dt = std::min(dt, 0.25 * std::sqrt(h / amax));
+  if (!std::isfinite(dt)) dt = c->prm.dt;
+  out3[0] = dt;
+  out3[1] = vmax;
+  out3[2] = amax;
+  return SPH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+#pragma pack(push, 1)
+struct SnapHeader {
+  char magic[8];
+  uint32_t version;
+  uint32_t header_bytes;
+  int64_t n;
+  int64_t step;
+  sph_params params;
+  uint32_t flags;
+  uint32_t reserved;
+  uint64_t checksum;
+};
+#pragma pack(pop)
+static_assert(sizeof(sph_params) == 144, "sph_params layout");
+static_assert(sizeof(SnapHeader) == 192, "snapshot header layout");
+
+constexpr char kSnapMagic[8] = {'S', 'P', 'H', 'S', 'N', 'A', 'P', '1'};
+
+uint64_t fnv1a(uint64_t h, const void* p, size_t bytes) {
+  const unsigned char* c = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < bytes; ++i) {
+    h ^= c[i];
+    h *= 0x100000001b3ull;
+  }
+  return h;
+}
+
+// Reads and validates a header; msg names the problem.
+bool read_header(FILE* f, SnapHeader& hd, std::string& msg) {
+  if (std::fread(&hd, sizeof(hd), 1, f) != 1) {
+    msg = "snapshot: short header";
+    return false;
+  }
+  if (std::memcmp(hd.magic, kSnapMagic, 8) != 0) {
+    msg = "snapshot: not an SPHSNAP1 file";
+    return false;
+  }
+  if (hd.version != 1 || hd.header_bytes != sizeof(SnapHeader)) {
+    msg = "snapshot: unsupported version";
+    return false;
+  }
+  if (hd.n < 0 || hd.n >= ((int64_t)1 << 30)) {
+    msg = "snapshot: particle count out of range";
+    return false;
+  }
+  return true;
+}
+
+bool same_params_but_dt(sph_params a, sph_params b) {
+  a.dt = 0;
+  b.dt = 0;
+  return std::memcmp(&a, &b, sizeof(a)) == 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_snapshot_info(const char* path, sph_params* params, int64_t* n, int64_t* step) {
+  g_create_error.clear();
+  if (!path) return SPH_ERR_INVALID;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) {
+    g_create_error = std::string("snapshot: cannot open ") + path;
+    return SPH_ERR_INVALID;
+  }
+  SnapHeader hd{};
+  std::string msg;
+  const bool ok = read_header(f, hd, msg);
+  std::fclose(f);
+  if (!ok) {
+    g_create_error = msg;
+    return SPH_ERR_INVALID;
+  }
+  if (params) *params = hd.params;
+  if (n) *n = hd.n;
+  if (step) *step = hd.step;
+  return SPH_OK;
+}
+
+int sph_snapshot_save(void* ctx, const char* path, int64_t step) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !path) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
+  const int64_t n = c->s.n;
+  std::vector<float> r(4 * (size_t)n);
+  std::vector<uint32_t> pid((size_t)n);
+  const size_t fb = (size_t)n * sizeof(float);
+  DevState& s = c->s;
+  const float* src[4] = {s.rx, s.ry, s.vx, s.vy};
+  for (int k = 0; k < 4; ++k)
+    HIPCHK(c, hipMemcpyAsync(r.data() + (size_t)k * n, src[k], fb, hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipMemcpyAsync(pid.data(), s.pid, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  SnapHeader hd{};
+  std::memcpy(hd.magic, kSnapMagic, 8);
+  hd.version = 1;
+  hd.header_bytes = sizeof(SnapHeader);
+  hd.n = n;
+  hd.step = step;
+  hd.params = c->prm;
+  hd.flags = 1;
+  uint64_t h = 0xcbf29ce484222325ull;
+  h = fnv1a(h, r.data(), r.size() * sizeof(float));
+  h = fnv1a(h, pid.data(), pid.size() * sizeof(uint32_t));
+  hd.checksum = h;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(c, SPH_ERR_INVALID, std::string("snapshot: cannot create ") + path);
+  bool ok = std::fwrite(&hd, sizeof(hd), 1, f) == 1;
+  ok = ok && (n == 0 || std::fwrite(r.data(), sizeof(float), r.size(), f) == r.size());
+  ok = ok && (n == 0 || std::fwrite(pid.data(), sizeof(uint32_t), pid.size(), f) == pid.size());
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(c, SPH_ERR_INVALID, std::string("snapshot: write failed: ") + path);
+  return SPH_OK;
+}
+
+int sph_snapshot_load(void* ctx, const char* path, int64_t* step) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !path) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(c, SPH_ERR_INVALID, std::string("snapshot: cannot open ") + path);
+  SnapHeader hd{};
+  std::string msg;
+  if (!read_header(f, hd, msg)) {
+    std::fclose(f);
+    return fail(c, SPH_ERR_INVALID, msg);
+  }
+  if (hd.n > c->n_max) {
+    std::fclose(f);
+    return fail(c, SPH_ERR_CAPACITY, "snapshot: more particles than the context holds");
+  }
+  if (!same_params_but_dt(hd.params, c->prm)) {
+    std::fclose(f);
+    return fail(c, SPH_ERR_INVALID, "snapshot: parameters differ from the context's");
+  }
+  const int64_t n = hd.n;
+  std::vector<float> r(4 * (size_t)n);
+  std::vector<uint32_t> pid((size_t)n);
+  bool ok = n == 0 || std::fread(r.data(), sizeof(float), r.size(), f) == r.size();
+  ok = ok && (n == 0 || std::fread(pid.data(), sizeof(uint32_t), pid.size(), f) == pid.size());
+  std::fclose(f);
+  if (!ok) return fail(c, SPH_ERR_INVALID, "snapshot: truncated arrays");
+  uint64_t h = 0xcbf29ce484222325ull;
+  h = fnv1a(h, r.data(), r.size() * sizeof(float));
+  h = fnv1a(h, pid.data(), pid.size() * sizeof(uint32_t));
+  if (h != hd.checksum) return fail(c, SPH_ERR_INVALID, "snapshot: checksum mismatch");
+  int rc = sph_upload(c, n, r.data(), r.data() + n, r.data() + 2 * n, r.data() + 3 * n,
+                      pid.data());
+  if (rc) return rc;
+  if (hd.params.dt != c->prm.dt && (rc = sph_set_dt(c, hd.params.dt))) return rc;
+  if (step) *step = hd.step;
+  return SPH_OK;
+}
+
 }  // extern "C"

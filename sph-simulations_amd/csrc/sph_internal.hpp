@@ -141,6 +141,8 @@ int64_t measure_partial_words(int n);
 hipError_t launch_measure(const Launch& L, const DevState& s, double mass, double gravity_y,
                           double* partials, double* out5);
 hipError_t launch_ell_overflow(const Launch& L, const DevState& s, int64_t* out_count);
+// FP64 max |v|^2 and max |a|^2 of S into out2 (device); partials >= measure_partial_words(n)
+hipError_t launch_max_va(const Launch& L, const DevState& s, double* partials, double* out2);
 
 // ------------------------------------------------------------ batched scenes
 // Per-scene constants of a batch: the scene's grid and physics, and the offset

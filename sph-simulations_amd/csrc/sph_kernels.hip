@@ -422,6 +422,61 @@ __global__ __launch_bounds__(BLK) void k_measure_final(int nparts, int n, double
   }
 }
 
+// FP64 maxima of |v|^2 and |a|^2 for the time-step criteria: per-block maxima,
+// then one block over the partials.
+__global__ __launch_bounds__(BLK) void k_max_va_partial(int n, const float* __restrict__ vx,
+                                                        const float* __restrict__ vy,
+                                                        const float* __restrict__ ax,
+                                                        const float* __restrict__ ay,
+                                                        double* __restrict__ part) {
+  __shared__ double s_v[BLK], s_a[BLK];
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  double v2 = 0, a2 = 0;
+  if (i < n) {
+    const double u = vx[i], v = vy[i], p = ax[i], q = ay[i];
+    v2 = u * u + v * v;
+    a2 = p * p + q * q;
+  }
+  s_v[threadIdx.x] = v2;
+  s_a[threadIdx.x] = a2;
+  __syncthreads();
+  for (int o = BLK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s_v[threadIdx.x] = fmax(s_v[threadIdx.x], s_v[threadIdx.x + o]);
+      s_a[threadIdx.x] = fmax(s_a[threadIdx.x], s_a[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s_v[0];
+    part[2 * blockIdx.x + 1] = s_a[0];
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_max_va_final(int nparts, const double* __restrict__ part,
+                                                      double* __restrict__ out) {
+  __shared__ double s_v[BLK], s_a[BLK];
+  double v = 0, a = 0;
+  for (int b = threadIdx.x; b < nparts; b += BLK) {
+    v = fmax(v, part[2 * b]);
+    a = fmax(a, part[2 * b + 1]);
+  }
+  s_v[threadIdx.x] = v;
+  s_a[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = BLK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s_v[threadIdx.x] = fmax(s_v[threadIdx.x], s_v[threadIdx.x + o]);
+      s_a[threadIdx.x] = fmax(s_a[threadIdx.x], s_a[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s_v[0];
+    out[1] = s_a[0];
+  }
+}
+
 __global__ __launch_bounds__(BLK) void k_ell_overflow(int n, int w,
                                                       const int32_t* __restrict__ cnt,
                                                       unsigned long long* __restrict__ out) {
@@ -593,6 +648,13 @@ hipError_t launch_scan(const Launch& L, const int32_t* in, int64_t* out, int n,
 }
 
 int64_t measure_partial_words(int n) { return 4 * (int64_t)std::max(blocks_for(n), 1); }
+
+hipError_t launch_max_va(const Launch& L, const DevState& s, double* partials, double* out2) {
+  const int nb = std::max(blocks_for(s.n), 1);
+  k_max_va_partial<<<nb, BLK, 0, L.stream>>>(s.n, s.vx, s.vy, s.ax, s.ay, partials);
+  k_max_va_final<<<1, BLK, 0, L.stream>>>(nb, partials, out2);
+  return hipGetLastError();
+}
 
 hipError_t launch_measure(const Launch& L, const DevState& s, double mass, double gravity_y,
                           double* partials, double* out5) {

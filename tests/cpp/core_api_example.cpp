@@ -49,6 +49,9 @@ int main(int argc, char** argv) {
                 ParticlesInDomain(domain, container), ComputeKineticEnergy(container),
                 ComputeMaxVelocity(container));
   }
+  // the same line from the one-measurement helper, and the CFL time step
+  std::fputs(StepHeader(steps - 1, container, domain).c_str(), stdout);
+  std::printf("cfl dt=%.9g\n", CflTimeStep(container, 0.25));
 
   // delta-encode the first particle's (sorted) list and decode it back
   GenericNeighborLists<MortonOrder> vlists{H};

@@ -46,3 +46,31 @@ def test_create_rejects_bad_params_without_touching_gpu(sph):
         sph.Context(p, 100)
     with pytest.raises(sph.SphError, match="n_max"):
         sph.Context(sph.Params(h=1.0), 0)
+
+
+def test_snapshot_header_format_on_cpu(sph, tmp_path):
+    """The documented file layout, written by hand, is read back by
+    sph_snapshot_info (no GPU needed); other files are rejected."""
+    import struct
+    p = sph.scenes.dam_break(8, 8).params
+    c = p.to_c()
+    params_bytes = bytes(memoryview(c))
+    assert len(params_bytes) == 144
+    hdr = b"SPHSNAP1" + struct.pack("<IIqq", 1, 192, 64, 77) + params_bytes + \
+        struct.pack("<IIQ", 1, 0, 0)
+    assert len(hdr) == 192
+    path = str(tmp_path / "h.sphsnap")
+    open(path, "wb").write(hdr)
+    q, n, step = sph.snapshot_info(path)
+    assert (n, step) == (64, 77) and q.h == p.h and q.dt == p.dt and q.gamma == p.gamma
+    open(path, "wb").write(b"NOTASNAP" + hdr[8:])
+    with pytest.raises(sph.SphError):
+        sph.snapshot_info(path)
+    open(path, "wb").write(hdr[:100])
+    with pytest.raises(sph.SphError):
+        sph.snapshot_info(path)
+
+
+def test_step_header_line(sph):
+    line = sph.diagnostics.step_header(12, [0.125, 2.5, 4096, 0.0, 1.0])
+    assert line == "Step= 12\t N=4096\t E_k=0.125\t |v|_max=2.5\n"

@@ -34,6 +34,12 @@ def test_core_api_driver_matches_oracle(sph, orc, tmp_path):
     r = subprocess.run([EXE, str(out), str(steps)], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
+    lines = r.stdout.splitlines()
+    # StepHeader reproduces the printf line of the last step
+    hdr = [l for l in lines if l.startswith(f"Step= {steps - 1}\t")]
+    assert len(hdr) == 2 and hdr[0].split("\t")[1] == hdr[1].split("\t")[1]
+    cfl = [l for l in lines if l.startswith("cfl dt=")]
+    assert len(cfl) == 1 and float(cfl[0].split("=")[1]) > 0
     n, rt, g = read_output(out)
     assert rt == 1  # Codifier round trip
     s = sph.scenes.reference_example()
