@@ -263,6 +263,12 @@ def main():
                     help="steps of the surrogate-in-the-loop comparison (0 disables)")
     ap.add_argument("--train-iters", type=int, default=1500,
                     help="training iterations of the surrogate network")
+    ap.add_argument("--variant", default="",
+                    help="step kernel/re-sort to use without timing the others, e.g. "
+                         "particle/incremental (profiling runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="MiB copied device-to-device before the workload (a copy kernel of "
+                         "known size for scaling HBM counters)")
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU work for the cpu_baseline sample (0 disables)")
     args = ap.parse_args()
@@ -274,6 +280,16 @@ def main():
         sys.exit(2)
 
     sph = importlib.import_module("sph-simulations_amd")
+    if args.calibrate_copy > 0:
+        nf = args.calibrate_copy * (1 << 20) // 4
+        src, dst = sph.capi.DeviceBuffer(nf), sph.capi.DeviceBuffer(nf)
+        rc = sph.lib().sph_device_copy(sph.capi.C.c_void_p(dst.ptr()),
+                                       sph.capi.C.c_void_p(src.ptr()), nf * 4, 2)
+        if rc != 0:
+            raise sph.SphError(f"calibration copy -> {rc}")
+        src.close()
+        dst.close()
+        log(f"calibration copy of {args.calibrate_copy} MiB done")
     scene = sph.scenes.by_name(args.workload)
     n = scene.n
     log(f"workload {scene.name}: n={n}")
@@ -291,17 +307,19 @@ def main():
                "particle": sph.STEP_PARTICLE}
     resorts = {"full": sph.RESORT_FULL, "incremental": sph.RESORT_INCREMENTAL}
     tune = {}
-    for kname, kv in kernels.items():
-        for rname, rv in resorts.items():
-            ctx.set_step_kernel(kv)
-            ctx.set_resort(rv)
-            ctx.step(4)
-            ctx.synchronize()
-            a0 = time.perf_counter()
-            ctx.step(20)
-            ctx.synchronize()
-            tune[f"{kname}/{rname}"] = (time.perf_counter() - a0) * 1e3 / 20
-            log(f"variant {kname}/{rname}: {tune[f'{kname}/{rname}']:.4f} ms/step")
+    combos = [(k, r) for k in kernels for r in resorts]
+    if args.variant:
+        combos = [tuple(args.variant.split("/"))]
+    for kname, rname in combos:
+        ctx.set_step_kernel(kernels[kname])
+        ctx.set_resort(resorts[rname])
+        ctx.step(4)
+        ctx.synchronize()
+        a0 = time.perf_counter()
+        ctx.step(20)
+        ctx.synchronize()
+        tune[f"{kname}/{rname}"] = (time.perf_counter() - a0) * 1e3 / 20
+        log(f"variant {kname}/{rname}: {tune[f'{kname}/{rname}']:.4f} ms/step")
     best_combo = min(tune, key=tune.get)
     best, best_resort = best_combo.split("/")
     ctx.set_step_kernel(kernels[best])

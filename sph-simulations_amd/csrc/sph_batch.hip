@@ -17,7 +17,7 @@ __global__ __launch_bounds__(kMaxTile) void k_bkeys(const TileDev* __restrict__ 
                                                     const float* __restrict__ rx,
                                                     const float* __restrict__ ry,
                                                     uint64_t* __restrict__ keys) {
-  const TileDev t = tiles[blockIdx.x];
+  const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int i = t.begin + (int)threadIdx.x;
   if (i >= t.end) return;
   const DevGrid g = scenes[t.scene].g;
@@ -33,7 +33,7 @@ __global__ __launch_bounds__(kMaxTile) void k_bdensity(
     int32_t* __restrict__ ell_count, int32_t* __restrict__ ell_idx, int ell_w,
     float* __restrict__ rho_out, float* __restrict__ pr_out, float* __restrict__ p_out) {
   extern __shared__ int s_list[];  // kEllStage x blockDim.x
-  const TileDev t = tiles[blockIdx.x];
+  const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int i = t.begin + (int)threadIdx.x;
   if (i >= t.end) return;
   const SceneDev& S = scenes[t.scene];
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
     const int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ ax_out,
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
     float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out) {
-  const TileDev t = tiles[blockIdx.x];
+  const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int i = t.begin + (int)threadIdx.x;
   if (i >= t.end) return;
   const SceneDev& S = scenes[t.scene];

@@ -13,6 +13,17 @@
 namespace sphamd {
 
 constexpr int kWave = 64;
+constexpr int kXcds = 8;  // accelerator dies of the MI355X, each with its own L2
+
+// Workgroups are dealt to the XCDs round-robin (workgroup b runs on die b % 8).
+// Renumbering so that die x processes a contiguous range of tiles keeps tiles
+// that read each other's particles (neighbors in Morton order) on one L2. A
+// bijection of [0, nb); not used by kernels whose tiles wait on predecessors.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int x = b % kXcds, k = b / kXcds;
+  const int q = nb / kXcds, r = nb % kXcds;
+  return x * q + min(x, r) + k;
+}
 
 struct DevGrid {
   float lo_x, lo_y, cs_x, cs_y;

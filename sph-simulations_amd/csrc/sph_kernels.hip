@@ -125,7 +125,7 @@ __global__ __launch_bounds__(BLK) void k_density(
     const int32_t* __restrict__ csr_idx, int32_t* __restrict__ ell_count,
     int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ rho_out,
     float* __restrict__ pr_out, float* __restrict__ p_out) {
-  const int i = blockIdx.x * BLK + threadIdx.x;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
   const float xi = rx[i], yi = ry[i];
   float acc = density_term(c.h2, 0.0f);
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(BLK) void k_forces(
     const int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ ax_out,
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
     float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out) {
-  const int i = blockIdx.x * BLK + threadIdx.x;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
   const float xi = rx[i], yi = ry[i], vxi = vx[i], vyi = vy[i], rhoi = rho[i], pri = pr[i];
   Acc a{0.f, 0.f, 0.f, 0.f};
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(BLK) void k_csr_count(int n, DevGrid g, DevPhys c,
                                                    const float* __restrict__ ry,
                                                    const int2* __restrict__ cells,
                                                    int32_t* __restrict__ count) {
-  const int i = blockIdx.x * BLK + threadIdx.x;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
   int k = 0;
   for_each_neighbor<BC, SELF>(g, c, rx, ry, cells, n, i, rx[i], ry[i],
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(BLK) void k_csr_fill(int n, DevGrid g, DevPhys c,
                                                   const int2* __restrict__ cells,
                                                   const int64_t* __restrict__ off,
                                                   int32_t* __restrict__ idx) {
-  const int i = blockIdx.x * BLK + threadIdx.x;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
   int64_t o = off[i];
   for_each_neighbor<BC, SELF>(g, c, rx, ry, cells, n, i, rx[i], ry[i],

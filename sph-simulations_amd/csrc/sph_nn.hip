@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void k_edge_fwd(NodeIn in, NnConsts k, const f
     b = bo[0];
   }
   const float sc = k.a_s * k.inv_h;
-  const int base = (blockIdx.x * 4 + wave) * kNodesPerWave;
+  const int base = (xcd_tile(blockIdx.x, gridDim.x) * 4 + wave) * kNodesPerWave;
   for (int t = 0; t < kNodesPerWave; ++t) {
     const int i = base + t;
     if (i >= in.n) break;
@@ -512,7 +512,7 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
   }
   float gc0 = 0.0f, gc1 = 0.0f, gw0 = 0.0f, gw1 = 0.0f, gb = 0.0f;
   const float sc = k.a_s * k.inv_h;
-  const int base = (blockIdx.x * 4 + wave) * kNodesPerWave;
+  const int base = (xcd_tile(blockIdx.x, gridDim.x) * 4 + wave) * kNodesPerWave;
   for (int t = 0; t < kNodesPerWave; ++t) {
     const int n0 = base + t;
     if (n0 >= in.n) break;
