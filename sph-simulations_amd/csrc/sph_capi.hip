@@ -356,8 +356,15 @@ int incremental_resort(Ctx* c) {
     StageTimer t(c, SPH_T_SORT_HIST);
     HIPCHK(c, launch_movers(c->L, s, r, c->sb.keys_in));
   }
+  {
+    // up to kSmallSortCap movers: one workgroup sorts them in LDS; the radix
+    // sort below then finds the count too small and returns at once
+    StageTimer t(c, SPH_T_SORT_PASS);
+    HIPCHK(c, launch_small_sort(c->L, r));
+  }
   SortBuffers mb = c->sb;
   mb.n_dev = r.n_movers;
+  mb.skip_upto = kSmallSortCap;
   mb.vals_in = r.midx;
   mb.keys_in = r.mkey;
   mb.keys_tmp = r.ktmp;
@@ -556,8 +563,7 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
         (rc = dev_alloc(c, &r.ktmp, N)) || (rc = dev_alloc(c, &r.keys_alt, N)) ||
         (rc = dev_alloc(c, &r.flags, resort_flag_words((int)N))) ||
         (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
-        (rc = dev_alloc(c, &r.lookback, resort_tiles((int)N))) ||
-        (rc = dev_alloc(c, &r.ticket, 1)) || (rc = dev_alloc(c, &r.n_movers, 1)))
+        (rc = dev_alloc(c, &r.n_movers, 1)))
       return bail(rc);
   }
   b.blocks = s.blocks;
@@ -578,9 +584,7 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
   e = hipMemsetAsync(b.hist, 0, kMaxPasses * 256 * sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(b.status, 0, sizeof(uint32_t), c->stream);
   c->rb.status = b.status;
-  if (e == hipSuccess)
-    e = hipMemsetAsync(c->rb.lookback, 0, resort_tiles((int)N) * sizeof(uint32_t), c->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(c->rb.ticket, 0, sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->rb.n_movers, 0, sizeof(int), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, n_keys);
   if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

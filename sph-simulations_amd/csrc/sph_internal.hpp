@@ -11,6 +11,7 @@
 namespace sphamd {
 
 constexpr int kMaxPasses = 8;  // 64-bit keys, 8-bit digits
+constexpr int kSmallSortCap = 4096;  // movers sorted in LDS by one workgroup
 
 struct SortBuffers {
   // n passed to the launchers sizes the grids; when n_dev is set the kernels
@@ -35,6 +36,8 @@ struct SortBuffers {
   int clear_n;
   uint64_t n_keys;
   uint32_t* status;  // nonzero after a look-back gave up (never expected)
+  // with n_dev set: do nothing when *n_dev <= skip_upto (a smaller sort ran)
+  int skip_upto = 0;
   // Morton blocks of 2^block_bits x 2^block_bits cells (0: no block table)
   int2* blocks;
   int block_bits;
@@ -87,8 +90,6 @@ struct ResortBuffers {
   uint64_t* ktmp;                  // sort scratch
   unsigned long long* flags;       // mover bit per particle, one word per wave
   uint32_t* tile_base;             // movers before each 256-particle tile
-  uint32_t* lookback;              // per tile, cleared by k_merge
-  uint32_t* ticket;
   int* n_movers;
   uint32_t* status;
   uint64_t* keys_alt;              // second sorted-key buffer (ping-pong)
@@ -124,6 +125,8 @@ int64_t resort_flag_words(int n);
 int64_t resort_tiles(int n);
 hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers& r,
                          const uint64_t* knew);
+// movers (count <= kSmallSortCap) sorted into skey / sidx by one workgroup
+hipError_t launch_small_sort(const Launch& L, const ResortBuffers& r);
 hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
                         uint64_t* keys_dst);
 // cell and block ranges of the sorted keys s.keys_sorted; clears the histogram

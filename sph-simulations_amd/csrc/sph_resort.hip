@@ -7,10 +7,12 @@
 // unchanged keys. The new order is the stable merge of the two sorted sequences
 // by (key, previous index) -- exactly the order a stable sort of all keys gives.
 //
-//   k_movers  flag movers, compact them in index order (windowed decoupled
-//             look-back over 256-particle tiles), record per-tile mover counts,
-//             and clear the cell / block ranges of the previous order
-//   (radix sort of the movers, element count read on the device)
+//   k_mover_flags    flag movers per 256-particle tile, count them, clear the
+//                    cell / block ranges of the previous order
+//   k_tile_scan      exclusive scan of the tile counts (one workgroup)
+//   k_mover_compact  movers in previous-index order
+//   k_small_sort     up to kSmallSortCap movers sorted in LDS (one workgroup);
+//                    above that the radix sort runs on the device-side count
 //   k_merge   final slot of every particle: stayer j -> (stayers before j) +
 //             (movers ordered before it); mover r -> r + (stayers ordered before
 //             it); scatter r, v, pid, key and the permutation
@@ -42,89 +44,126 @@ __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t ia, uint64_t kb,
   return ka < kb || (ka == kb && ia < ib);
 }
 
-__global__ __launch_bounds__(TW) void k_movers(
+// Movers of each tile: one ballot word per wavefront and the tile's count (into
+// tile_base, scanned in place by k_tile_scan). Also clears the cell / block
+// ranges of the previous order (k_ranges writes the new ones).
+__global__ __launch_bounds__(TW) void k_mover_flags(
     int n, const uint64_t* __restrict__ knew, const uint64_t* __restrict__ kold,
     int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
-    uint64_t* __restrict__ mkey, uint32_t* __restrict__ midx,
-    unsigned long long* __restrict__ flags, uint32_t* __restrict__ tile_base,
-    uint32_t* __restrict__ lookback, uint32_t* __restrict__ ticket, int* __restrict__ n_movers,
-    uint32_t* __restrict__ status) {
-  __shared__ uint32_t s_base;
+    unsigned long long* __restrict__ flags, uint32_t* __restrict__ tile_count) {
   __shared__ uint32_t s_wcnt[TW / kWave];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  // Tiles in workgroup-id order: a 1-D grid is dispatched in id order on each
-  // XCD, so the lowest unfinished tile is always resident and every look-back
-  // wait ends (a same-address ticket counter would serialise 4096 atomics).
-  const uint32_t tile = blockIdx.x;
-  (void)ticket;
-  const int i = (int)tile * TW + tid;
+  const int tile = blockIdx.x;
+  const int i = tile * TW + tid;
   const bool live = i < n;
   const uint64_t kn = live ? knew[i] : 0, ko = live ? kold[i] : 0;
   if (live) {
-    // ranges of the previous order: cleared here, rewritten by k_ranges
     const uint64_t kp = i > 0 ? kold[i - 1] : ~ko;
     if (ko != kp && ko < n_keys) cells[ko] = make_int2(-1, -1);
     if (block_bits > 0 && (ko >> (2 * block_bits)) != (kp >> (2 * block_bits)) && ko < n_keys)
       blocks[ko >> (2 * block_bits)] = make_int2(-1, -1);
   }
-  const bool mv = live && kn != ko;
-  const unsigned long long bal = __ballot(mv);
+  const unsigned long long bal = __ballot(live && kn != ko);
   if (lane == 0) {
     s_wcnt[wave] = (uint32_t)__popcll(bal);
     flags[(size_t)tile * (TW / kWave) + wave] = bal;
   }
   __syncthreads();
-  // Tile prefix by a windowed look-back: wave 0 reads the 64 closest predecessor
-  // words at once, sums back to the nearest inclusive prefix, and only re-polls
-  // while a predecessor inside that span has not published yet.
-  if (wave == 0) {
-    uint32_t cnt = 0;
-    for (int w = 0; w < TW / kWave; ++w) cnt += s_wcnt[w];
-    uint32_t excl = 0;
-    if (tile == 0) {
-      if (lane == 0) st_agent(&lookback[0], FLAG_PREFIX | cnt);
-    } else {
-      if (lane == 0) st_agent(&lookback[tile], FLAG_AGG | cnt);
-      int top = (int)tile - 1;
-      uint32_t polls = 0;
-      while (true) {
-        const int t = top - lane;
-        const uint32_t w = t >= 0 ? ld_agent(&lookback[t]) : FLAG_PREFIX;
-        const uint32_t f = w & ~COUNT_MASK;
-        const unsigned long long pm = __ballot(f == FLAG_PREFIX);
-        const unsigned long long zm = __ballot(f == 0);
-        const int first_p = pm ? __ffsll((long long)pm) - 1 : kWave;
-        const int first_z = zm ? __ffsll((long long)zm) - 1 : kWave;
-        if (first_z < first_p) {
-          if (++polls < MAX_POLLS) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-          }
-          if (lane == 0) atomicOr(status, 2u);  // a predecessor never published
-          break;
-        }
-        uint32_t v = lane <= first_p ? (w & COUNT_MASK) : 0u;
+  if (tid == 0) {
+    uint32_t c = 0;
+    for (int w = 0; w < TW / kWave; ++w) c += s_wcnt[w];
+    tile_count[tile] = c;
+  }
+}
+
+// In-place exclusive scan of the per-tile mover counts (one workgroup, fixed
+// order) and the total.
+constexpr int SCAN_TW = 1024;
+__global__ __launch_bounds__(SCAN_TW) void k_tile_scan(int tiles, uint32_t* __restrict__ tile_base,
+                                                       int* __restrict__ n_movers) {
+  __shared__ uint32_t s_w[SCAN_TW / kWave];
+  __shared__ uint32_t s_carry;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < tiles; base += SCAN_TW) {
+    const int t = base + tid;
+    const uint32_t v = t < tiles ? tile_base[t] : 0u;
+    uint32_t x = v;
 #pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        excl += v;
-        if (first_p < kWave) break;
-        top -= kWave;
-      }
-      if (lane == 0) st_agent(&lookback[tile], FLAG_PREFIX | (excl + cnt));
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
     }
-    if (lane == 0) {
-      s_base = excl;
-      tile_base[tile] = excl;
-      if (tile == gridDim.x - 1) *n_movers = (int)(excl + cnt);
-    }
+    if (lane == kWave - 1) s_w[wave] = x;
+    __syncthreads();
+    uint32_t off = s_carry;
+    for (int w = 0; w < wave; ++w) off += s_w[w];
+    if (t < tiles) tile_base[t] = off + x - v;
+    __syncthreads();
+    if (tid == SCAN_TW - 1) s_carry = off + x;
+    __syncthreads();
+  }
+  if (tid == 0) *n_movers = (int)s_carry;
+}
+
+// Movers in previous-index order: (new key, previous index).
+__global__ __launch_bounds__(TW) void k_mover_compact(
+    int n, const uint64_t* __restrict__ knew, const unsigned long long* __restrict__ flags,
+    const uint32_t* __restrict__ tile_base, uint64_t* __restrict__ mkey,
+    uint32_t* __restrict__ midx) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int tile = blockIdx.x;
+  const int i = tile * TW + tid;
+  const unsigned long long fw = flags[(size_t)tile * (TW / kWave) + wave];
+  if (i >= n || !((fw >> lane) & 1ull)) return;
+  uint32_t r = tile_base[tile];
+  for (int w = 0; w < wave; ++w) r += (uint32_t)__popcll(flags[(size_t)tile * (TW / kWave) + w]);
+  r += (uint32_t)__popcll(fw & ((1ull << lane) - 1ull));
+  mkey[r] = knew[i];
+  midx[r] = (uint32_t)i;
+}
+
+// Movers sorted by (key, previous index) in one workgroup: a bitonic network in
+// LDS over the next power of two (sentinels sort last). Does nothing when the
+// count exceeds SMALL_CAP: the radix sort handles those.
+constexpr int SMALL_TW = 1024;
+__global__ __launch_bounds__(SMALL_TW) void k_small_sort(const int* __restrict__ n_dev,
+                                                         const uint64_t* __restrict__ mkey,
+                                                         const uint32_t* __restrict__ midx,
+                                                         uint64_t* __restrict__ skey,
+                                                         uint32_t* __restrict__ sidx) {
+  __shared__ uint64_t s_k[kSmallSortCap];
+  __shared__ uint32_t s_i[kSmallSortCap];
+  const int n = *n_dev;
+  if (n <= 0 || n > kSmallSortCap) return;
+  int m = 1;
+  while (m < n) m <<= 1;
+  for (int t = threadIdx.x; t < m; t += SMALL_TW) {
+    s_k[t] = t < n ? mkey[t] : ~0ull;
+    s_i[t] = t < n ? midx[t] : ~0u;
   }
   __syncthreads();
-  if (mv) {
-    uint32_t r = s_base;
-    for (int w = 0; w < wave; ++w) r += s_wcnt[w];
-    r += (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-    mkey[r] = kn;
-    midx[r] = (uint32_t)i;
+  for (int size = 2; size <= m; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < (m >> 1); t += SMALL_TW) {
+        const int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const uint64_t ka = s_k[lo], kb = s_k[hi];
+        const uint32_t ia = s_i[lo], ib = s_i[hi];
+        if (pair_less(kb, ib, ka, ia) == up) {
+          s_k[lo] = kb;
+          s_k[hi] = ka;
+          s_i[lo] = ib;
+          s_i[hi] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = threadIdx.x; t < n; t += SMALL_TW) {
+    skey[t] = s_k[t];
+    sidx[t] = s_i[t];
   }
 }
 
@@ -190,8 +229,7 @@ __global__ __launch_bounds__(TW) void k_merge(
     const float* __restrict__ tvy, const uint32_t* __restrict__ pid_src,
     float* __restrict__ rx, float* __restrict__ ry, float* __restrict__ vx,
     float* __restrict__ vy, uint32_t* __restrict__ pid_dst, uint64_t* __restrict__ keys_dst,
-    uint32_t* __restrict__ perm, uint32_t* __restrict__ movers_lookback,
-    uint32_t* __restrict__ movers_ticket) {
+    uint32_t* __restrict__ perm) {
   __shared__ uint64_t s_k[WIN];
   __shared__ uint32_t s_i[WIN];
   __shared__ int s_lo, s_hi;
@@ -199,9 +237,6 @@ __global__ __launch_bounds__(TW) void k_merge(
   const int n_mov = *n_movers_p;
   if ((int)blockIdx.x < tiles) {
     const int tile = blockIdx.x;
-    // reset the look-back state k_movers used, for the next step
-    if (tid == 0) movers_lookback[tile] = 0;
-    if (tile == 0 && tid == 0) *movers_ticket = 0;
     const int j = tile * TW + tid;
     const bool live = j < n;
     const unsigned long long fw = flags[(size_t)tile * (TW / kWave) + wave];
@@ -262,10 +297,16 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
                          const uint64_t* knew) {
   if (s.n <= 0) return hipSuccess;
   const int tiles = (int)resort_tiles(s.n);
-  k_movers<<<tiles, TW, 0, L.stream>>>(s.n, knew, s.keys_sorted, s.cells, s.blocks,
-                                       s.block_bits, (uint64_t)s.n_keys, r.mkey, r.midx,
-                                       r.flags, r.tile_base, r.lookback, r.ticket, r.n_movers,
-                                       r.status);
+  k_mover_flags<<<tiles, TW, 0, L.stream>>>(s.n, knew, s.keys_sorted, s.cells, s.blocks,
+                                            s.block_bits, (uint64_t)s.n_keys, r.flags,
+                                            r.tile_base);
+  k_tile_scan<<<1, SCAN_TW, 0, L.stream>>>(tiles, r.tile_base, r.n_movers);
+  k_mover_compact<<<tiles, TW, 0, L.stream>>>(s.n, knew, r.flags, r.tile_base, r.mkey, r.midx);
+  return hipGetLastError();
+}
+
+hipError_t launch_small_sort(const Launch& L, const ResortBuffers& r) {
+  k_small_sort<<<1, SMALL_TW, 0, L.stream>>>(r.n_movers, r.mkey, r.midx, r.skey, r.sidx);
   return hipGetLastError();
 }
 
@@ -275,8 +316,7 @@ hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers&
   const int tiles = (int)resort_tiles(s.n);
   k_merge<<<2 * tiles, TW, 0, L.stream>>>(
       s.n, tiles, s.keys_sorted, r.flags, r.tile_base, r.skey, r.sidx, r.n_movers, s.tx, s.ty,
-      s.tvx, s.tvy, s.pid, s.rx, s.ry, s.vx, s.vy, s.pid_alt, keys_dst, s.perm, r.lookback,
-      r.ticket);
+      s.tvx, s.tvy, s.pid, s.rx, s.ry, s.vx, s.vy, s.pid_alt, keys_dst, s.perm);
   return hipGetLastError();
 }
 

@@ -390,3 +390,28 @@ def test_capacity_and_state_errors(sph):
             ctx.density_pressure()
         with pytest.raises(sph.SphError, match="prepare"):
             ctx.step(1)
+
+
+def test_incremental_resort_many_movers(sph, orc):
+    """More movers than one workgroup sorts in LDS: the device-counted radix sort
+    takes over; the permutation is still the oracle's stable sort."""
+    s = sph.scenes.dam_break(100, 100, seed=12)
+    rng = np.random.default_rng(3)
+    s.vx = rng.normal(0, 40.0, s.n).astype(np.float32)  # ~0.5 cell per step
+    s.vy = rng.normal(0, 40.0, s.n).astype(np.float32)
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.prepare()
+        k_before = ctx.sorted_keys()
+        ctx.step(1)
+        after = ctx.download()
+        perm = ctx.permutation()
+        m = ctx.movers()
+    note(f"[gpu] many movers: {m} of {s.n}")
+    assert m > 4096
+    x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
+    x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
+    k_new = orc.keys(s.params, x_prev, y_prev)
+    assert m == int(np.sum(k_new != k_before))
+    operm, _ = orc.stable_sort(k_new)
+    assert np.array_equal(perm, operm)
