@@ -32,9 +32,9 @@ def log(msg):
 
 
 def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers_frac=0.0):
-    """Algorithmic HBM bytes per particle of one launch of each kernel (FP32
+    """Algorithmic HBM bytes per particle of one launch of each stage (FP32
     state, u64 keys, u32 indices; neighbor and halo re-reads counted as cache
-    hits)."""
+    hits). None for a stage without a model."""
     if variant == "split_block":
         # density: R r 8, W rho,pr 8; force: R r,v,rho,pr 24, W a 8, T r,v 16, key 8
         phys = {"approximate_density": 16.0, "force_computation": 56.0}
@@ -57,15 +57,35 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
     else:
         f = movers_frac
         sort = {
-            # movers: R new + old keys 16, clear old ranges 8 occ, W flags ~0.1, movers 12 f;
-            # mover histogram: R 8 f  (mean of the two launches)
-            "sort_histogram": (16.0 + 8.0 * occ + 12.0 * f + 8.0 * f) / 2,
-            "sort_pass": (20.0 + 24.0 * (passes - 1)) / passes * f,
-            # merge: R old key 8, T r,v 16, pid 4; W r,v 16, pid 4, key 8, perm 4;
-            # ranges: R key 8, W 8 occ  (mean of the two launches)
-            "neighbor_search": (60.0 + 8.0 + 8.0 * occ) / 2,
+            # flags: R new + old keys 16, clear old ranges 8 occ; compaction of the
+            # movers R key 8, W key+idx 12; their sort R 12 + W 12
+            "movers": 16.0 + 8.0 * occ + 20.0 * f + 24.0 * f,
+            # R old key 8, T r,v 16, pid 4; W r,v 16, pid 4, key 8, perm 4
+            "merge": 60.0,
+            # ranges: R key 8, W 8 occ
+            "neighbor_search": 8.0 + 8.0 * occ,
         }
-    return {**phys, **sort}[stage]
+    return {**phys, **sort}.get(stage)
+
+
+STAGE_KERNEL = {"approximate_density": "k_density", "force_computation": "k_forces",
+                "movers": "k_mover_flags", "merge": "k_merge", "neighbor_search": "k_ranges"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at the 1M workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel),
+    or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_dam_break_1m.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            for item in line.split()[1:]:
+                name, _, val = item.partition("=")
+                if name == kernel:
+                    return int(val)
+    return None
 
 
 def cpu_baseline(sph, scene, budget_s):
@@ -131,12 +151,15 @@ def bench_batched(sph, steps, trial_steps=10):
             continue
         per = ms / launches
         bpp = alg_bytes_per_particle(name, k, occ, passes, "particle", resort, mfrac)
-        stages[name] = {"launches": launches, "us_per_launch": round(per * 1e3, 3),
-                        "alg_bytes_per_launch": int(bpp * n),
-                        "achieved_GBs": round(bpp * n / (per * 1e-3) / 1e9, 1)}
-        step_bytes += bpp * launches / ev_steps
+        stages[name] = {"launches": launches, "us_per_launch": round(per * 1e3, 3)}
         kern_ms += ms
-    dom = max(stages, key=lambda s: stages[s]["us_per_launch"] * stages[s]["launches"])
+        if bpp is None:
+            continue
+        stages[name]["alg_bytes_per_launch"] = int(bpp * n)
+        stages[name]["achieved_GBs"] = round(bpp * n / (per * 1e-3) / 1e9, 1)
+        step_bytes += bpp * launches / ev_steps
+    dom = max((s for s in stages if "achieved_GBs" in stages[s]),
+              key=lambda s: stages[s]["us_per_launch"] * stages[s]["launches"])
     ctx.close()
     return {
         "workload": b.name, "scenes": int(len(b.counts)), "particles": n,
@@ -326,6 +349,13 @@ def main():
     ctx.set_resort(resorts[best_resort])
     bstats = ctx.block_stats()
     log(f"block stats {bstats}")
+    # movers per step (particles whose cell changed) over a short sample
+    mv = []
+    for _ in range(20):
+        ctx.step(1)
+        mv.append(ctx.movers())
+    mover_stats = {"min": int(min(mv)), "mean": float(np.mean(mv)), "max": int(max(mv))}
+    log(f"movers per step {mover_stats}")
 
     # timed region: exactly `steps` graph-replayed steps, synchronised on both sides
     ctx.synchronize()
@@ -365,12 +395,11 @@ def main():
         per_launch_ms = ms / launches
         entry = {"launches": launches, "ms_total": round(ms, 4),
                  "us_per_launch": round(per_launch_ms * 1e3, 3)}
-        try:
-            b = alg_bytes_per_particle(name, k, occ, passes, best, best_resort, mfrac) * n
+        bpp = alg_bytes_per_particle(name, k, occ, passes, best, best_resort, mfrac)
+        if bpp is not None:
+            b = bpp * n
             entry["alg_bytes_per_launch"] = int(b)
             entry["achieved_GBs"] = round(b / (per_launch_ms * 1e-3) / 1e9, 1)
-        except KeyError:
-            pass
         stages[name] = entry
         kern_total += ms
     dom_name = max((s for s in stages if "alg_bytes_per_launch" in stages[s]),
@@ -399,12 +428,13 @@ def main():
                    "h": scene.params.h, "dt": scene.params.dt, "physics": "wcsph",
                    "mean_neighbors": round(k, 3), "occupied_cells_per_particle": round(occ, 4),
                    "radix_passes": passes, "step_kernel": best, "resort": best_resort,
-                   "movers_last_step": ctx.movers(),
+                   "movers_last_step": ctx.movers(), "movers_per_step": mover_stats,
                    "step_kernel_ms": {k2: round(v, 4) for k2, v in tune.items()},
                    "block_stats": bstats},
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL.get(dom_name, dom_name),
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(STAGE_KERNEL.get(dom_name, dom_name))},
         "stages": stages,
         "step": {"alg_bytes_per_particle": round(step_bytes, 1),
                  "step_frac_of_hbm": step_bytes * value / (HBM_PEAK_GBS * 1e9),

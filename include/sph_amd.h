@@ -107,7 +107,7 @@ typedef struct sph_device_view {
  * bracketed by hipEvents on the context stream when timing is on. */
 enum sph_stage {
   SPH_T_INTEGRATION = 0,   /* standalone integration */
-  SPH_T_NEIGHBOR_SEARCH,   /* cell keys; permutation + cell ranges */
+  SPH_T_NEIGHBOR_SEARCH,   /* cell keys; permutation (full sort) + cell ranges */
   SPH_T_NEIGHBOR_LIST,     /* CSR build (count, scan, fill) */
   SPH_T_DENSITY,           /* density + EOS (+ fixed-width neighbor list) */
   SPH_T_FORCE,             /* forces (+ integration and next keys inside sph_step) */
@@ -115,6 +115,8 @@ enum sph_stage {
   SPH_T_SORT_PASS,         /* radix sort: one onesweep pass (several per sort) */
   SPH_T_BLOCK_STEP,        /* fused density + forces + integration + keys (sph_step) */
   SPH_T_MEASURE,
+  SPH_T_MOVERS,            /* incremental re-sort: mover flags, tile prefix, mover sort */
+  SPH_T_MERGE,             /* incremental re-sort: merge movers and stayers (permutation) */
   SPH_T_COUNT
 };
 

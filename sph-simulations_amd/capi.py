@@ -23,7 +23,8 @@ COMPUTE_SORTED, COMPUTE_LISTS = 0, 1
 STEP_SPLIT_BLOCK, STEP_FUSED_BLOCK, STEP_PARTICLE = 0, 1, 2
 RESORT_FULL, RESORT_INCREMENTAL = 0, 1
 STAGES = ("integration", "neighbor_search", "neighbor_list", "approximate_density",
-          "force_computation", "sort_histogram", "sort_pass", "block_step", "measure")
+          "force_computation", "sort_histogram", "sort_pass", "block_step", "measure",
+          "movers", "merge")
 
 
 class SphError(RuntimeError):

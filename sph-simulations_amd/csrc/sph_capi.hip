@@ -347,41 +347,17 @@ int find_neighbors(Ctx* c) {
 }
 
 // Re-sort of the integrated state T (keys in sb.keys_in) against the previous
-// sorted order: movers -> radix sort of the movers -> merge/scatter T -> S ->
-// ranges. Same permutation as sort_keys + gather.
+// sorted order: mover flags -> sorted movers (one launch) -> merge/scatter
+// T -> S -> ranges. Same permutation as sort_keys + gather.
 int incremental_resort(Ctx* c) {
   DevState& s = c->s;
   ResortBuffers& r = c->rb;
   {
-    StageTimer t(c, SPH_T_SORT_HIST);
-    HIPCHK(c, launch_movers(c->L, s, r, c->sb.keys_in));
+    StageTimer t(c, SPH_T_MOVERS);
+    HIPCHK(c, launch_movers(c->L, s, r, c->sb.keys_in, c->grid.key_bits));
   }
   {
-    // up to kSmallSortCap movers: one workgroup sorts them in LDS; the radix
-    // sort below then finds the count too small and returns at once
-    StageTimer t(c, SPH_T_SORT_PASS);
-    HIPCHK(c, launch_small_sort(c->L, r));
-  }
-  SortBuffers mb = c->sb;
-  mb.n_dev = r.n_movers;
-  mb.skip_upto = kSmallSortCap;
-  mb.vals_in = r.midx;
-  mb.keys_in = r.mkey;
-  mb.keys_tmp = r.ktmp;
-  mb.keys_out = r.skey;
-  mb.vals_out = r.sidx;
-  mb.clear_n = 0;
-  mb.block_bits = 0;
-  {
-    StageTimer t(c, SPH_T_SORT_HIST);
-    HIPCHK(c, radix_sort_hist(mb, s.n, c->grid.key_bits, c->stream));
-  }
-  for (int p = 0; p < sort_passes(c->grid.key_bits); ++p) {
-    StageTimer t(c, SPH_T_SORT_PASS);
-    HIPCHK(c, radix_sort_pass(mb, s.n, c->grid.key_bits, p, c->stream));
-  }
-  {
-    StageTimer t(c, SPH_T_NEIGHBOR_SEARCH);
+    StageTimer t(c, SPH_T_MERGE);
     HIPCHK(c, launch_merge(c->L, s, r, r.keys_alt));
   }
   std::swap(s.keys_sorted, r.keys_alt);
@@ -563,7 +539,8 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
         (rc = dev_alloc(c, &r.ktmp, N)) || (rc = dev_alloc(c, &r.keys_alt, N)) ||
         (rc = dev_alloc(c, &r.flags, resort_flag_words((int)N))) ||
         (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
-        (rc = dev_alloc(c, &r.n_movers, 1)))
+        (rc = dev_alloc(c, &r.vtmp, N)) || (rc = dev_alloc(c, &r.counts, kMoverGrid * 256)) ||
+        (rc = dev_alloc(c, &r.bar, 1)) || (rc = dev_alloc(c, &r.n_movers, 1)))
       return bail(rc);
   }
   b.blocks = s.blocks;
@@ -585,6 +562,7 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
   if (e == hipSuccess) e = hipMemsetAsync(b.status, 0, sizeof(uint32_t), c->stream);
   c->rb.status = b.status;
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.n_movers, 0, sizeof(int), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->rb.bar, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, n_keys);
   if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

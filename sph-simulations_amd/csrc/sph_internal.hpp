@@ -12,6 +12,7 @@ namespace sphamd {
 
 constexpr int kMaxPasses = 8;  // 64-bit keys, 8-bit digits
 constexpr int kSmallSortCap = 4096;  // movers sorted in LDS by one workgroup
+constexpr int kMoverGrid = 64;       // workgroups of the mover sort (all resident)
 
 struct SortBuffers {
   // n passed to the launchers sizes the grids; when n_dev is set the kernels
@@ -36,8 +37,6 @@ struct SortBuffers {
   int clear_n;
   uint64_t n_keys;
   uint32_t* status;  // nonzero after a look-back gave up (never expected)
-  // with n_dev set: do nothing when *n_dev <= skip_upto (a smaller sort ran)
-  int skip_upto = 0;
   // Morton blocks of 2^block_bits x 2^block_bits cells (0: no block table)
   int2* blocks;
   int block_bits;
@@ -88,6 +87,9 @@ struct ResortBuffers {
   uint64_t* skey;                  // movers sorted by (key, previous index)
   uint32_t* sidx;
   uint64_t* ktmp;                  // sort scratch
+  uint32_t* vtmp;
+  uint32_t* counts;                // [kMoverGrid][256] digit counts
+  uint32_t* bar;                   // grid barrier counter
   unsigned long long* flags;       // mover bit per particle, one word per wave
   uint32_t* tile_base;             // movers before each 256-particle tile
   int* n_movers;
@@ -123,10 +125,9 @@ int block_step_grid(int64_t n_blocks, int cus);
 // incremental re-sort: movers -> (sort movers) -> merge -> ranges
 int64_t resort_flag_words(int n);
 int64_t resort_tiles(int n);
+// flags, tile prefix and the movers sorted into skey / sidx (two launches)
 hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers& r,
-                         const uint64_t* knew);
-// movers (count <= kSmallSortCap) sorted into skey / sidx by one workgroup
-hipError_t launch_small_sort(const Launch& L, const ResortBuffers& r);
+                         const uint64_t* knew, int key_bits);
 hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
                         uint64_t* keys_dst);
 // cell and block ranges of the sorted keys s.keys_sorted; clears the histogram

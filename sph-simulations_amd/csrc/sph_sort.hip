@@ -77,12 +77,9 @@ __global__ __launch_bounds__(WG) void k_hist(const uint64_t* __restrict__ keys, 
                                              const uint64_t* __restrict__ old_sorted,
                                              int old_n, int2* __restrict__ cells,
                                              uint64_t n_keys, int2* __restrict__ blocks,
-                                             int block_bits, int skip_upto) {
+                                             int block_bits) {
   __shared__ uint32_t s_h[kMaxPasses * RADIX];
-  if (n_dev) {
-    n = *n_dev;  // count known on the device only (incremental re-sort)
-    if (n <= skip_upto) return;
-  }
+  if (n_dev) n = *n_dev;  // count known on the device only
   for (int i = threadIdx.x; i < passes * RADIX; i += WG) s_h[i] = 0;
   __syncthreads();
   const int gtid = blockIdx.x * WG + threadIdx.x, gsize = gridDim.x * WG;
@@ -137,7 +134,7 @@ __global__ __launch_bounds__(WG) void k_onesweep(int n, const int* __restrict__ 
                                                  const uint32_t* __restrict__ hist,
                                                  uint32_t* __restrict__ lookback,
                                                  uint32_t* __restrict__ tickets,
-                                                 uint32_t* __restrict__ status, int skip_upto) {
+                                                 uint32_t* __restrict__ status) {
   __shared__ uint64_t s_keys[TILE];
   __shared__ uint32_t s_vals[TILE];
   __shared__ uint32_t s_whist[WAVES * RADIX];
@@ -147,10 +144,7 @@ __global__ __launch_bounds__(WG) void k_onesweep(int n, const int* __restrict__ 
   __shared__ uint32_t s_tile;
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  if (n_dev) {
-    n = *n_dev;
-    if (n <= skip_upto) return;  // sorted elsewhere (small count)
-  }
+  if (n_dev) n = *n_dev;
   // With a device-side count the grid is a small persistent pool that keeps taking
   // tickets until every tile is done; otherwise one tile per workgroup.
   for (;;) {
@@ -291,8 +285,7 @@ hipError_t radix_sort_hist(const SortBuffers& b, int n, int key_bits, hipStream_
   // `passes` look-back arrays laid out back to back with stride lookback_words_per_pass(n)
   k_hist<<<hist_blocks, WG, 0, stream>>>(b.keys_in, n, b.n_dev, passes, b.hist, b.lookback,
                                          lookback_words_per_pass(n), b.tickets, b.keys_out,
-                                         b.clear_n, b.cells, b.n_keys, b.blocks, b.block_bits,
-                                         b.skip_upto);
+                                         b.clear_n, b.cells, b.n_keys, b.blocks, b.block_bits);
   return hipGetLastError();
 }
 
@@ -309,7 +302,7 @@ hipError_t radix_sort_pass(const SortBuffers& b, int n, int key_bits, int p,
   const int grid = b.n_dev ? min(tiles, b.pool) : tiles;
   k_onesweep<<<grid, WG, 0, stream>>>(n, b.n_dev, p * RB, p, kin, vin, kout, vout, b.hist,
                                        b.lookback + (size_t)p * lb_words, b.tickets,
-                                       b.status, b.skip_upto);
+                                       b.status);
   return hipGetLastError();
 }
 

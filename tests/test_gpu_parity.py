@@ -392,13 +392,15 @@ def test_capacity_and_state_errors(sph):
             ctx.step(1)
 
 
-def test_incremental_resort_many_movers(sph, orc):
-    """More movers than one workgroup sorts in LDS: the device-counted radix sort
-    takes over; the permutation is still the oracle's stable sort."""
-    s = sph.scenes.dam_break(100, 100, seed=12)
+@pytest.mark.parametrize("side,sigma", [(100, 40.0), (512, 400.0)])
+def test_incremental_resort_many_movers(sph, orc, side, sigma):
+    """More movers than one workgroup sorts in LDS: the grid-wide radix sort of
+    the mover-sort kernel takes over (one and several 1024-element sub-tiles per
+    workgroup); the permutation is still the oracle's stable sort."""
+    s = sph.scenes.dam_break(side, side, seed=12)
     rng = np.random.default_rng(3)
-    s.vx = rng.normal(0, 40.0, s.n).astype(np.float32)  # ~0.5 cell per step
-    s.vy = rng.normal(0, 40.0, s.n).astype(np.float32)
+    s.vx = rng.normal(0, sigma, s.n).astype(np.float32)  # ~0.5 to 5 cells per step
+    s.vy = rng.normal(0, sigma, s.n).astype(np.float32)
     with sph.Context(s.params, s.n) as ctx:
         ctx.upload(s.rx, s.ry, s.vx, s.vy)
         ctx.prepare()
