@@ -195,9 +195,10 @@ int sph_snapshot_load(void* ctx, const char* path, int64_t* step);
  *   state before the last integration (sph_density_pressure + sph_forces refresh).
  * Reference-equivalent run of S iterations (simulation_configuration.h:297-387):
  *   sph_prepare; sph_step(S-1); sph_density_pressure(SORTED); sph_forces(SORTED).
- * With Morton order one iteration is three kernels kinds: a fused block kernel
- * (density + EOS + forces + integration + next keys over occupied 8x8-cell
- * blocks staged in LDS), the radix sort, and the permutation + ranges kernel. */
+ * By default (SPH_STEP_PARTICLE, SPH_RESORT_INCREMENTAL) one iteration is: the
+ * density kernel (density + EOS + fixed-width list), the force kernel (forces +
+ * integration + next keys), the incremental re-sort (mover flags, tile scan,
+ * mover compaction, mover sort, merge) and the cell-range kernel. */
 int sph_prepare(void* ctx);
 int sph_step(void* ctx, int32_t n_steps);
 /* Capture one fused iteration in a hipGraph and replay it in sph_step. */
