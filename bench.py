@@ -31,10 +31,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers_frac=0.0):
+def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers_frac=0.0,
+                           flags_in_force=False):
     """Algorithmic HBM bytes per particle of one launch of each stage (FP32
     state, u64 keys, u32 indices; neighbor and halo re-reads counted as cache
-    hits). None for a stage without a model."""
+    hits). flags_in_force: the force kernel writes the re-sort's mover flags
+    (reading the previous key). None for a stage without a model."""
     if variant == "split_block":
         # density: R r 8, W rho,pr 8; force: R r,v,rho,pr 24, W a 8, T r,v 16, key 8
         phys = {"approximate_density": 16.0, "force_computation": 56.0}
@@ -44,7 +46,9 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
     else:
         # density: R r 8, W rho,pr 8, count 4, list 4k;
         # force: R r,v 16, rho,pr 8, count 4, list 4k, W a 8, T 16, key 8
-        phys = {"approximate_density": 20.0 + 4.0 * k, "force_computation": 60.0 + 4.0 * k}
+        # (+ R previous key 8, W one flag bit when it flags the movers)
+        phys = {"approximate_density": 20.0 + 4.0 * k,
+                "force_computation": 60.0 + 4.0 * k + (8.125 if flags_in_force else 0.0)}
     if resort == "full":
         sort = {
             # R new keys (8) + previous sorted keys (8); clear previous ranges (8/occ. cell)
@@ -57,11 +61,12 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
     else:
         f = movers_frac
         sort = {
-            # flags: R new + old keys 16, clear old ranges 8 occ; compaction of the
-            # movers R key 8, W key+idx 12; their sort R 12 + W 12
-            "movers": 16.0 + 8.0 * occ + 20.0 * f + 24.0 * f,
-            # R old key 8, T r,v 16, pid 4; W r,v 16, pid 4, key 8, perm 4
-            "merge": 60.0,
+            # flags R 1/8 (+ R new and old keys 16, W flags when no kernel before
+            # wrote them); movers R key 8, W sorted key+idx 12
+            "movers": 0.125 + (0.0 if flags_in_force else 16.125) + 20.0 * f,
+            # R old key 8, T r,v 16, pid 4; W r,v 16, pid 4, key 8, perm 4;
+            # clear the old ranges 8 occ
+            "merge": 60.0 + 8.0 * occ,
             # ranges: R key 8, W 8 occ
             "neighbor_search": 8.0 + 8.0 * occ,
         }
@@ -69,7 +74,7 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
 
 
 STAGE_KERNEL = {"approximate_density": "k_density", "force_computation": "k_forces",
-                "movers": "k_mover_flags", "merge": "k_merge", "neighbor_search": "k_ranges"}
+                "movers": "k_movers", "merge": "k_merge", "neighbor_search": "k_ranges"}
 
 
 def measured_traffic(kernel):
@@ -395,7 +400,8 @@ def main():
         per_launch_ms = ms / launches
         entry = {"launches": launches, "ms_total": round(ms, 4),
                  "us_per_launch": round(per_launch_ms * 1e3, 3)}
-        bpp = alg_bytes_per_particle(name, k, occ, passes, best, best_resort, mfrac)
+        bpp = alg_bytes_per_particle(name, k, occ, passes, best, best_resort, mfrac,
+                                     best == "particle" and best_resort == "incremental")
         if bpp is not None:
             b = bpp * n
             entry["alg_bytes_per_launch"] = int(b)
@@ -406,7 +412,8 @@ def main():
                    key=lambda s: stages[s]["ms_total"])
     dom = stages[dom_name]
     achieved = dom["achieved_GBs"]
-    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes, best, best_resort, mfrac)
+    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes, best, best_resort, mfrac,
+                                            best == "particle" and best_resort == "incremental")
                      * stages[s]["launches"] / args.steps
                      for s in stages if "alg_bytes_per_launch" in stages[s])
     out = {

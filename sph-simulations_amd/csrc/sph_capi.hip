@@ -349,12 +349,13 @@ int find_neighbors(Ctx* c) {
 // Re-sort of the integrated state T (keys in sb.keys_in) against the previous
 // sorted order: mover flags -> sorted movers (one launch) -> merge/scatter
 // T -> S -> ranges. Same permutation as sort_keys + gather.
-int incremental_resort(Ctx* c) {
+// flags_ready: the producer of the new keys wrote the mover flags.
+int incremental_resort(Ctx* c, bool flags_ready) {
   DevState& s = c->s;
   ResortBuffers& r = c->rb;
   {
     StageTimer t(c, SPH_T_MOVERS);
-    HIPCHK(c, launch_movers(c->L, s, r, c->sb.keys_in, c->grid.key_bits));
+    HIPCHK(c, launch_movers(c->L, s, r, c->sb.keys_in, c->grid.key_bits, flags_ready));
   }
   {
     StageTimer t(c, SPH_T_MERGE);
@@ -371,7 +372,7 @@ int incremental_resort(Ctx* c) {
   return SPH_OK;
 }
 
-int resort_integrated(Ctx* c);
+int resort_integrated(Ctx* c, bool flags_ready = false);
 
 // One fused iteration: forces(ELL) + integrate + keys -> sort -> gather T->S
 // + ranges -> density + EOS + ELL.
@@ -400,21 +401,26 @@ int fused_iteration(Ctx* c) {
     HIPCHK(c, launch_block_step(c->L, s, c->sb.keys_in, c->block_grid, c->step_kernel, nullptr,
                                 2));
   } else {
-    // row-major order: density with the fixed-width list, then forces from it
+    // per-particle kernels: density with the fixed-width list, then forces from it
     {
       StageTimer t(c, SPH_T_DENSITY);
       HIPCHK(c, launch_density(c->L, s, NB_ELL, false));
     }
-    StageTimer t(c, SPH_T_FORCE);
-    HIPCHK(c, launch_forces(c->L, s, NB_ELL, true, c->sb.keys_in));
+    // the force kernel also flags the movers for the incremental re-sort
+    const bool inc = c->resort == SPH_RESORT_INCREMENTAL;
+    {
+      StageTimer t(c, SPH_T_FORCE);
+      HIPCHK(c, launch_forces(c->L, s, NB_ELL, true, c->sb.keys_in, inc ? c->rb.flags : nullptr));
+    }
+    return resort_integrated(c, inc);
   }
   return resort_integrated(c);
 }
 
 // Re-sort the integrated twin T (keys in sb.keys_in) into S with new ranges.
-int resort_integrated(Ctx* c) {
+int resort_integrated(Ctx* c, bool flags_ready) {
   DevState& s = c->s;
-  if (c->resort == SPH_RESORT_INCREMENTAL) return incremental_resort(c);
+  if (c->resort == SPH_RESORT_INCREMENTAL) return incremental_resort(c, flags_ready);
   int rc = sort_keys(c);
   if (rc) return rc;
   {
@@ -539,6 +545,8 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
         (rc = dev_alloc(c, &r.ktmp, N)) || (rc = dev_alloc(c, &r.keys_alt, N)) ||
         (rc = dev_alloc(c, &r.flags, resort_flag_words((int)N))) ||
         (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
+        (rc = dev_alloc(c, &r.tile_lo, resort_tiles((int)N) + 1)) ||
+        (rc = dev_alloc(c, &s.tile_first, resort_tiles((int)N))) ||
         (rc = dev_alloc(c, &r.vtmp, N)) || (rc = dev_alloc(c, &r.counts, kMoverGrid * 256)) ||
         (rc = dev_alloc(c, &r.bar, 1)) || (rc = dev_alloc(c, &r.n_movers, 1)))
       return bail(rc);

@@ -25,6 +25,17 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x * q + min(x, r) + k;
 }
 
+// Element j of a 4-byte array whose base is wave-uniform (a kernel argument),
+// addressed by a 32-bit byte offset so that the load takes the base in scalar
+// registers and only the offset in a vector register. Contexts hold fewer than
+// 2^30 particles, so 4 j fits in 32 bits.
+template <class T>
+__device__ __forceinline__ T ld4(const T* __restrict__ base, int j) {
+  static_assert(sizeof(T) == 4, "4-byte elements");
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     ((uint32_t)j << 2));
+}
+
 struct DevGrid {
   float lo_x, lo_y, cs_x, cs_y;
   int32_t nx, ny;
@@ -41,14 +52,17 @@ struct DevPhys {
 
 // Spread the low 32 bits of v to the even bit positions of a 64-bit word
 // (Morton dilation; x -> even bits, y -> odd bits as in MortonOrder.h:117-126).
-__device__ __forceinline__ uint64_t dilate(uint32_t v) {
-  uint64_t x = v;
-  x = (x | (x << 16)) & 0x0000ffff0000ffffull;
-  x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
-  x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
-  x = (x | (x << 2)) & 0x3333333333333333ull;
-  x = (x | (x << 1)) & 0x5555555555555555ull;
+// Bits 0..15 of v to the even bits of a 32-bit word.
+__device__ __forceinline__ uint32_t dilate16(uint32_t v) {
+  uint32_t x = v & 0xffffu;
+  x = (x | (x << 8)) & 0x00ff00ffu;
+  x = (x | (x << 4)) & 0x0f0f0f0fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
   return x;
+}
+__device__ __forceinline__ uint64_t dilate(uint32_t v) {
+  return (uint64_t)dilate16(v) | ((uint64_t)dilate16(v >> 16) << 32);
 }
 
 __device__ __forceinline__ uint64_t cell_key(const DevGrid& g, int cx, int cy) {
@@ -222,6 +236,14 @@ __device__ __forceinline__ void for_each_neighbor(const DevGrid& g, const DevPhy
 }
 
 constexpr int kEllStage = 16;  // list slots per thread staged in LDS before the store
+#ifndef SPH_CAND_BATCH
+#define SPH_CAND_BATCH 8
+#endif
+#ifndef SPH_LIST_BATCH
+#define SPH_LIST_BATCH 8
+#endif
+constexpr int kCandBatch = SPH_CAND_BATCH;  // stencil candidates loaded together (density)
+constexpr int kListBatch = SPH_LIST_BATCH;  // list slots loaded together (forces)
 
 // Cell ranges of the 3x3 stencil of position (x, y) in visiting order (x offset
 // outer, y offset inner); cells outside a walled box and empty cells give the
@@ -231,22 +253,42 @@ __device__ __forceinline__ void stencil_ranges(const DevGrid& g, const int2* __r
                                                float x, float y, int2* rng) {
   const int cx = cell_coord(x, g.lo_x, g.cs_x, g.nx);
   const int cy = cell_coord(y, g.lo_y, g.cs_y, g.ny);
+  // Key = (x part) + (y part) in both orders (the Morton parts have disjoint
+  // bits). Cells outside a walled box are loaded at a clamped, valid key and
+  // masked afterwards, so the nine loads carry no branch between them.
+  uint64_t kx[3], ky[3];
+  bool okx[3], oky[3];
 #pragma unroll
-  for (int ox = -1; ox <= 1; ++ox) {
-#pragma unroll
-    for (int oy = -1; oy <= 1; ++oy) {
-      int cxj = cx + ox, cyj = cy + oy;
-      bool ok = true;
-      if (BC == 1) {
-        cxj = cxj < 0 ? cxj + g.nx : (cxj >= g.nx ? cxj - g.nx : cxj);
-        cyj = cyj < 0 ? cyj + g.ny : (cyj >= g.ny ? cyj - g.ny : cyj);
-      } else {
-        ok = cxj >= 0 && cxj < g.nx && cyj >= 0 && cyj < g.ny;
-      }
-      int2 se = ok ? cells[cell_key(g, cxj, cyj)] : make_int2(-1, -1);
-      if (se.x < 0 || se.y < 0) se = make_int2(0, 0);
-      rng[(ox + 1) * 3 + (oy + 1)] = se;
+  for (int o = 0; o < 3; ++o) {
+    int cxj = cx + o - 1, cyj = cy + o - 1;
+    if (BC == 1) {
+      cxj = cxj < 0 ? cxj + g.nx : (cxj >= g.nx ? cxj - g.nx : cxj);
+      cyj = cyj < 0 ? cyj + g.ny : (cyj >= g.ny ? cyj - g.ny : cyj);
+      okx[o] = oky[o] = true;
+    } else {
+      okx[o] = cxj >= 0 && cxj < g.nx;
+      oky[o] = cyj >= 0 && cyj < g.ny;
+      cxj = min(max(cxj, 0), g.nx - 1);
+      cyj = min(max(cyj, 0), g.ny - 1);
     }
+    if (g.order != 0) {
+      kx[o] = (uint64_t)cxj;
+      ky[o] = (uint64_t)cyj * (uint64_t)g.nx;
+    } else if (g.nx <= 65536 && g.ny <= 65536) {  // coordinates fit 16 bits
+      kx[o] = dilate16((uint32_t)cxj);
+      ky[o] = (uint64_t)dilate16((uint32_t)cyj) << 1;
+    } else {
+      kx[o] = dilate((uint32_t)cxj);
+      ky[o] = dilate((uint32_t)cyj) << 1;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) rng[q] = cells[kx[q / 3] + ky[q % 3]];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const int2 se = rng[q];
+    const bool ok = okx[q / 3] & oky[q % 3] & (se.x >= 0) & (se.y >= 0);
+    rng[q] = ok ? se : make_int2(0, 0);
   }
 }
 
@@ -268,14 +310,42 @@ __device__ __forceinline__ float density_ell(const DevGrid& g, const DevPhys& c,
   float acc = density_term(c.h2, 0.0f);
   int2 rng[9];
   stencil_ranges<BC>(g, cells, xi, yi, rng);
-  int cnt = 0;
+  // The candidates of the nine cells form one sequence in stencil order: slot t
+  // of it is index t + shift[q] for the last cell q with first[q] <= t. Walking
+  // it kCandBatch slots at a time lets every slot's position loads be in flight
+  // together (slots past the end load particle i itself, a valid address).
+  int first[9], shift[9];
+  int total = 0;
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
-    const int je = min(rng[q].y, n);
-    for (int j = rng[q].x; j < je; ++j) {
-      if (j == i) continue;
+    const int len = max(min(rng[q].y, n) - rng[q].x, 0);
+    first[q] = total;
+    shift[q] = rng[q].x - total;
+    total += len;
+  }
+  int cnt = 0;
+  for (int t0 = 0; t0 < total; t0 += kCandBatch) {
+    int jb[kCandBatch];
+    float xb[kCandBatch], yb[kCandBatch];
+#pragma unroll
+    for (int u = 0; u < kCandBatch; ++u) {
+      const int t = t0 + u;
+      int s = shift[0];
+#pragma unroll
+      for (int q = 1; q < 9; ++q) s = t >= first[q] ? shift[q] : s;
+      jb[u] = t < total ? t + s : i;
+    }
+#pragma unroll
+    for (int u = 0; u < kCandBatch; ++u) {
+      xb[u] = ld4(rx, jb[u]);
+      yb[u] = ld4(ry, jb[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kCandBatch; ++u) {
+      const int j = jb[u];
+      if (t0 + u >= total || j == i) continue;
       float dx, dy;
-      separation<BC>(c, xi, yi, rx[j], ry[j], dx, dy);
+      separation<BC>(c, xi, yi, xb[u], yb[u], dx, dy);
       const float r2 = dx * dx + dy * dy;
       if (r2 > c.h2) continue;
       acc += density_term(c.h2, r2);
@@ -309,21 +379,45 @@ __device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
   auto pair = [&](int j, float dx, float dy, float r2) {
     force_pair<PHYS>(c, dx, dy, r2, rhoi, rho[j], pri, pr[j], vxi, vyi, vx[j], vy[j], a);
   };
-  auto from_list = [&](int j) {
-    float dx, dy;
-    separation<BC>(c, xi, yi, rx[j], ry[j], dx, dy);
-    pair(j, dx, dy, dx * dx + dy * dy);
-  };
+  // Slot loads are unconditional (clamped to the list width, so always inside
+  // the buffer; slots past the count are replaced by i before any use): the
+  // first batch's slots load together with the count. Then kListBatch slots at
+  // a time, every neighbor load of the batch issued before the first use
+  // (particle i itself for unused slots); the sums still run in slot order.
   const int cnt = ell_count[i];
+  int jl[kListBatch];
+#pragma unroll
+  for (int u = 0; u < kListBatch; ++u) jl[u] = ell_idx[(int64_t)min(u, ell_w - 1) * n + i];
+  auto batch = [&](int k0) {
+    float xb[kListBatch], yb[kListBatch], ub[kListBatch], wb[kListBatch], db[kListBatch],
+        pb[kListBatch];
+#pragma unroll
+    for (int u = 0; u < kListBatch; ++u) {
+      const int j = k0 + u < cnt ? jl[u] : i;
+      xb[u] = ld4(rx, j);
+      yb[u] = ld4(ry, j);
+      ub[u] = ld4(vx, j);
+      wb[u] = ld4(vy, j);
+      db[u] = ld4(rho, j);
+      pb[u] = ld4(pr, j);
+    }
+#pragma unroll
+    for (int u = 0; u < kListBatch; ++u) {
+      if (k0 + u >= cnt) continue;
+      float dx, dy;
+      separation<BC>(c, xi, yi, xb[u], yb[u], dx, dy);
+      force_pair<PHYS>(c, dx, dy, dx * dx + dy * dy, rhoi, db[u], pri, pb[u], vxi, vyi, ub[u],
+                       wb[u], a);
+    }
+  };
   if (cnt <= ell_w) {
-    // first 8 slots fetched up front so their neighbor loads can all be in flight
-    int jl[8];
+    batch(0);
+    for (int k0 = kListBatch; k0 < cnt; k0 += kListBatch) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) jl[k] = k < cnt ? ell_idx[(int64_t)k * n + i] : i;
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (k < cnt) from_list(jl[k]);
-    for (int k = 8; k < cnt; ++k) from_list(ell_idx[(int64_t)k * n + i]);
+      for (int u = 0; u < kListBatch; ++u)
+        jl[u] = ell_idx[(int64_t)min(k0 + u, ell_w - 1) * n + i];
+      batch(k0);
+    }
   } else {
     for_each_neighbor<BC>(g, c, rx, ry, cells, n, i, xi, yi, pair);
   }

@@ -13,6 +13,7 @@ namespace sphamd {
 constexpr int kMaxPasses = 8;  // 64-bit keys, 8-bit digits
 constexpr int kSmallSortCap = 4096;  // movers sorted in LDS by one workgroup
 constexpr int kMoverGrid = 64;       // workgroups of the mover sort (all resident)
+constexpr int kResortTile = 256;     // particles per tile of the incremental re-sort
 
 struct SortBuffers {
   // n passed to the launchers sizes the grids; when n_dev is set the kernels
@@ -71,6 +72,7 @@ struct DevState {
   int64_t* csr_off;
   int32_t* csr_idx;
   uint32_t* hist;  // the sort's histogram, cleared by the permutation kernel
+  uint64_t* tile_first;  // key of every kResortTile-th sorted slot (with the ranges)
   // Morton blocks (square groups of 2^block_bits cells per side, a contiguous key
   // range): particle range per block, -1 when empty
   int block_bits;   // 0 when the order has no block structure (row-major)
@@ -92,6 +94,7 @@ struct ResortBuffers {
   uint32_t* bar;                   // grid barrier counter
   unsigned long long* flags;       // mover bit per particle, one word per wave
   uint32_t* tile_base;             // movers before each 256-particle tile
+  uint32_t* tile_lo;               // [tiles + 1] sorted movers before each tile's first particle
   int* n_movers;
   uint32_t* status;
   uint64_t* keys_alt;              // second sorted-key buffer (ping-pong)
@@ -113,8 +116,11 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
                          float* dst_rx, float* dst_ry, float* dst_vx, float* dst_vy,
                          uint32_t* dst_pid, float* dst_ax, float* dst_ay, int hist_words);
 hipError_t launch_density(const Launch& L, const DevState& s, int nb_mode, bool write_p);
+// fused: also integrate into T and write the next keys; mover_flags (fused
+// only, may be null): one ballot word per 64 particles, bit set where the next
+// key differs from s.keys_sorted (the input of the incremental re-sort)
 hipError_t launch_forces(const Launch& L, const DevState& s, int nb_mode, bool fused,
-                         uint64_t* keys_out);
+                         uint64_t* keys_out, unsigned long long* mover_flags = nullptr);
 hipError_t launch_integrate(const Launch& L, const DevState& s);
 // density + forces + integration + next keys over LDS-staged Morton blocks
 // (variant: enum sph_step_kernel); stats (may be null): 3 counters per kernel;
@@ -125,9 +131,10 @@ int block_step_grid(int64_t n_blocks, int cus);
 // incremental re-sort: movers -> (sort movers) -> merge -> ranges
 int64_t resort_flag_words(int n);
 int64_t resort_tiles(int n);
-// flags, tile prefix and the movers sorted into skey / sidx (two launches)
+// tile prefixes, the movers sorted into skey / sidx and the merge windows (one
+// launch; a second one first when the producer of knew wrote no mover flags)
 hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers& r,
-                         const uint64_t* knew, int key_bits);
+                         const uint64_t* knew, int key_bits, bool flags_ready);
 hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
                         uint64_t* keys_dst);
 // cell and block ranges of the sorted keys s.keys_sorted; clears the histogram

@@ -35,22 +35,29 @@ __global__ __launch_bounds__(BLK) void k_keys(int n, DevGrid g, const float* __r
 // run starts are found with one wave64 ballot of "key differs from the previous
 // key"; a run ends where the next one starts, so the end mask is the start mask
 // shifted by one lane (the last lane compares with the next wave's first key).
-// The same is done one level up for Morton blocks. Returns i < n.
+// The same is done one level up for Morton blocks. The three key loads are
+// unconditional (clamped into [0, n), n >= 1) so that they issue together.
+// Also keeps the key of every kResortTile-th slot for the incremental re-sort.
+// Returns i < n.
 __device__ __forceinline__ bool write_ranges(int i, int n, const uint64_t* __restrict__ keys,
                                              int2* __restrict__ cells, int64_t n_keys,
-                                             int2* __restrict__ blocks, int block_bits) {
+                                             int2* __restrict__ blocks, int block_bits,
+                                             uint64_t* __restrict__ tile_first) {
   const bool live = i < n;
-  uint64_t k = live ? keys[i] : ~0ull;
+  const int ic = min(i, n - 1);
+  const uint64_t kc = keys[ic], kpc = keys[max(ic - 1, 0)], knc = keys[min(ic + 1, n - 1)];
+  uint64_t k = live ? kc : ~0ull;
+  if (live && i % kResortTile == 0) tile_first[i / kResortTile] = k;
   const int lane = threadIdx.x & (kWave - 1);
   uint64_t prev = __shfl_up(k, 1);
-  if (lane == 0) prev = (i > 0 && live) ? keys[i - 1] : ~k;
+  if (lane == 0) prev = (i > 0 && live) ? kpc : ~k;
   const bool is_start = live && (i == 0 || k != prev);
   const uint64_t starts = __ballot(is_start);
   bool is_end;
   if (lane < kWave - 1) {
     is_end = live && (i == n - 1 || ((starts >> (lane + 1)) & 1ull));
   } else {
-    is_end = live && (i == n - 1 || k != keys[i + 1]);
+    is_end = live && (i == n - 1 || k != knc);
   }
   if (block_bits > 0) {
     // the same boundary detection one level up: Morton blocks of 4^block_bits cells
@@ -61,7 +68,7 @@ __device__ __forceinline__ bool write_ranges(int i, int n, const uint64_t* __res
     if (lane < kWave - 1)
       b_end = live && (i == n - 1 || ((bstarts >> (lane + 1)) & 1ull));
     else
-      b_end = live && (i == n - 1 || (k >> sh) != (keys[i + 1] >> sh));
+      b_end = live && (i == n - 1 || (k >> sh) != (knc >> sh));
     if (live && k < (uint64_t)n_keys) {
       const int bk = (int)(k >> sh);
       if (b_start) blocks[bk].x = i;
@@ -86,11 +93,11 @@ __global__ __launch_bounds__(BLK) void k_gather(
     float* __restrict__ dvx, float* __restrict__ dvy, uint32_t* __restrict__ dpid,
     float* __restrict__ dax, float* __restrict__ day, int2* __restrict__ cells,
     int64_t n_keys, uint32_t* __restrict__ hist, int hist_words, int2* __restrict__ blocks,
-    int block_bits) {
+    int block_bits, uint64_t* __restrict__ tile_first) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
-  const bool live = write_ranges(i, n, keys, cells, n_keys, blocks, block_bits);
+  const bool live = write_ranges(i, n, keys, cells, n_keys, blocks, block_bits, tile_first);
   if (!live) return;
   const uint32_t j = min(perm[i], (uint32_t)(n - 1));  // in bounds even if perm were bad
   drx[i] = srx[j];
@@ -109,11 +116,12 @@ __global__ __launch_bounds__(BLK) void k_gather(
 __global__ __launch_bounds__(BLK) void k_ranges(int n, const uint64_t* __restrict__ keys,
                                                 int2* __restrict__ cells, int64_t n_keys,
                                                 int2* __restrict__ blocks, int block_bits,
-                                                uint32_t* __restrict__ hist, int hist_words) {
+                                                uint32_t* __restrict__ hist, int hist_words,
+                                                uint64_t* __restrict__ tile_first) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
-  write_ranges(i, n, keys, cells, n_keys, blocks, block_bits);
+  write_ranges(i, n, keys, cells, n_keys, blocks, block_bits, tile_first);
 }
 
 // Summation density + EOS. NB: stencil traversal, CSR rows, or stencil with the
@@ -162,10 +170,12 @@ __global__ __launch_bounds__(BLK) void k_forces(
     const int32_t* __restrict__ csr_idx, const int32_t* __restrict__ ell_count,
     const int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ ax_out,
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
-    float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out) {
+    float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out,
+    const uint64_t* __restrict__ kold, unsigned long long* __restrict__ mflags) {
   const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
   const float xi = rx[i], yi = ry[i], vxi = vx[i], vyi = vy[i], rhoi = rho[i], pri = pr[i];
+  const uint64_t ko = (FUSED && mflags) ? kold[i] : 0ull;
   Acc a{0.f, 0.f, 0.f, 0.f};
   auto pair = [&](int j, float dx, float dy, float r2) {
     force_pair<PHYS>(c, dx, dy, r2, rhoi, rho[j], pri, pr[j], vxi, vyi, vx[j], vy[j], a);
@@ -194,8 +204,15 @@ __global__ __launch_bounds__(BLK) void k_forces(
     ty[i] = y;
     tvx[i] = u;
     tvy[i] = v;
-    keys_out[i] = cell_key(g, cell_coord(x, g.lo_x, g.cs_x, g.nx),
-                           cell_coord(y, g.lo_y, g.cs_y, g.ny));
+    const uint64_t kn = cell_key(g, cell_coord(x, g.lo_x, g.cs_x, g.nx),
+                                 cell_coord(y, g.lo_y, g.cs_y, g.ny));
+    keys_out[i] = kn;
+    if (mflags) {
+      // particles whose key changes (tiles are BLK-aligned, so wave w of the
+      // tile covers flag word i / 64; waves past n write none)
+      const unsigned long long bal = __ballot(kn != ko);
+      if ((threadIdx.x & (kWave - 1)) == 0) mflags[i / kWave] = bal;
+    }
   }
 }
 
@@ -509,14 +526,15 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
     k_gather<<<blocks_for(s.n), BLK, 0, L.stream>>>(
         s.n, s.perm, s.keys_sorted, src_rx, src_ry, src_vx, src_vy, src_pid, src_ax, src_ay,
         dst_rx, dst_ry, dst_vx, dst_vy, dst_pid, dst_ax, dst_ay, s.cells, s.n_keys, s.hist,
-        hist_words, s.blocks, s.block_bits);
+        hist_words, s.blocks, s.block_bits, s.tile_first);
   return hipGetLastError();
 }
 
 hipError_t launch_ranges(const Launch& L, const DevState& s, int hist_words) {
   if (s.n > 0)
     k_ranges<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, s.keys_sorted, s.cells, s.n_keys,
-                                                    s.blocks, s.block_bits, s.hist, hist_words);
+                                                    s.blocks, s.block_bits, s.hist, hist_words,
+                                                    s.tile_first);
   return hipGetLastError();
 }
 
@@ -549,34 +567,36 @@ hipError_t launch_density(const Launch& L, const DevState& s, int nb, bool write
 }
 
 template <int PHYS, int BC, int NB, bool FUSED>
-static hipError_t forces_go(const Launch& L, const DevState& s, uint64_t* keys_out) {
+static hipError_t forces_go(const Launch& L, const DevState& s, uint64_t* keys_out,
+                            unsigned long long* mflags) {
   k_forces<PHYS, BC, NB, FUSED><<<blocks_for(s.n), BLK, 0, L.stream>>>(
       s.n, L.grid, L.phys, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.csr_off, s.csr_idx,
-      s.ell_count, s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out);
+      s.ell_count, s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out,
+      s.keys_sorted, FUSED ? mflags : nullptr);
   return hipGetLastError();
 }
 
 template <int PHYS, int BC>
 static hipError_t forces_dispatch(const Launch& L, const DevState& s, int nb, bool fused,
-                                  uint64_t* keys_out) {
+                                  uint64_t* keys_out, unsigned long long* mflags) {
   if (s.n <= 0) return hipSuccess;
   if (fused) {
-    if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, true>(L, s, keys_out);
-    if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, true>(L, s, keys_out);
-    return forces_go<PHYS, BC, NB_STENCIL, true>(L, s, keys_out);
+    if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, true>(L, s, keys_out, mflags);
+    if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, true>(L, s, keys_out, mflags);
+    return forces_go<PHYS, BC, NB_STENCIL, true>(L, s, keys_out, mflags);
   }
-  if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, false>(L, s, keys_out);
-  if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, false>(L, s, keys_out);
-  return forces_go<PHYS, BC, NB_STENCIL, false>(L, s, keys_out);
+  if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, false>(L, s, keys_out, nullptr);
+  if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, false>(L, s, keys_out, nullptr);
+  return forces_go<PHYS, BC, NB_STENCIL, false>(L, s, keys_out, nullptr);
 }
 
 hipError_t launch_forces(const Launch& L, const DevState& s, int nb, bool fused,
-                         uint64_t* keys_out) {
+                         uint64_t* keys_out, unsigned long long* mover_flags) {
   const int ph = L.phys.physics, bc = L.phys.boundary;
-  if (ph == 0 && bc == 0) return forces_dispatch<0, 0>(L, s, nb, fused, keys_out);
-  if (ph == 0 && bc == 1) return forces_dispatch<0, 1>(L, s, nb, fused, keys_out);
-  if (ph == 1 && bc == 0) return forces_dispatch<1, 0>(L, s, nb, fused, keys_out);
-  return forces_dispatch<1, 1>(L, s, nb, fused, keys_out);
+  if (ph == 0 && bc == 0) return forces_dispatch<0, 0>(L, s, nb, fused, keys_out, mover_flags);
+  if (ph == 0 && bc == 1) return forces_dispatch<0, 1>(L, s, nb, fused, keys_out, mover_flags);
+  if (ph == 1 && bc == 0) return forces_dispatch<1, 0>(L, s, nb, fused, keys_out, mover_flags);
+  return forces_dispatch<1, 1>(L, s, nb, fused, keys_out, mover_flags);
 }
 
 hipError_t launch_integrate(const Launch& L, const DevState& s) {

@@ -7,16 +7,21 @@
 // unchanged keys. The new order is the stable merge of the two sorted sequences
 // by (key, previous index) -- exactly the order a stable sort of all keys gives.
 //
-//   k_mover_flags    flag movers per 256-particle tile, count them, clear the
-//                    cell / block ranges of the previous order
-//   k_tile_scan      exclusive scan of the tile counts (one workgroup)
-//   k_mover_compact  movers in previous-index order
-//   k_mover_sort     movers sorted by (key, previous index): one workgroup in
-//                    LDS for up to kSmallSortCap movers, a grid-wide radix
-//                    sort above that (one launch either way)
-//   k_merge   final slot of every particle: stayer j -> (stayers before j) +
-//             (movers ordered before it); mover r -> r + (stayers ordered before
-//             it); scatter r, v, pid, key and the permutation
+// Mover flags: one ballot word per 64 particles of the previous order, written
+// by the force kernel of the step (or by k_mover_flags for other producers).
+//
+//   k_movers  one launch: every workgroup counts the movers (so none waits for
+//             another to decide the path); workgroup 0 writes the 256-particle
+//             tile prefixes and compacts the movers in previous-index order. Up
+//             to kSmallSortCap movers it radix-sorts them in LDS; above that the
+//             grid runs an LSD radix sort with bounded grid barriers. Ends with,
+//             per merge tile, the number of sorted movers before the tile's
+//             first particle.
+//   k_merge   one workgroup per tile of the previous order: clears the tile's
+//             old cell / block ranges, places its stayers (rank among the
+//             movers of the tile's window, searched in LDS) and the movers of
+//             its window (rank among the tile's keys, searched in LDS); scatters
+//             r, v, pid, key and the permutation
 //   k_ranges  cell start/end and Morton block ranges of the new order
 #include <hip/hip_runtime.h>
 
@@ -26,18 +31,11 @@ namespace sphamd {
 
 namespace {
 
-constexpr int TW = 256;  // particles per tile / threads per workgroup
-constexpr uint32_t FLAG_AGG = 1u << 30;
-constexpr uint32_t FLAG_PREFIX = 2u << 30;
-constexpr uint32_t COUNT_MASK = (1u << 30) - 1;
-constexpr uint32_t MAX_POLLS = 1u << 18;
-constexpr int WIN = 512;  // movers a merge tile may hold in LDS
+constexpr int TW = kResortTile;  // particles per tile (four flag words)
+constexpr int WIN = 512;  // sorted movers a merge tile holds in LDS at once
 
 __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // (ka, ia) < (kb, ib) lexicographically
@@ -45,106 +43,38 @@ __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t ia, uint64_t kb,
   return ka < kb || (ka == kb && ia < ib);
 }
 
-// Movers of each tile: one ballot word per wavefront and the tile's count (into
-// tile_base, scanned in place by k_tile_scan). Also clears the cell / block
-// ranges of the previous order (k_ranges writes the new ones).
-__global__ __launch_bounds__(TW) void k_mover_flags(
-    int n, const uint64_t* __restrict__ knew, const uint64_t* __restrict__ kold,
-    int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
-    unsigned long long* __restrict__ flags, uint32_t* __restrict__ tile_count,
-    uint32_t* __restrict__ bar) {
-  __shared__ uint32_t s_wcnt[TW / kWave];
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const int tile = blockIdx.x;
-  if (tile == 0 && tid == 0) *bar = 0;  // the mover sort's grid barrier
-  const int i = tile * TW + tid;
+// Loads that a lane predicate would guard are issued unconditionally at a
+// clamped index inside the buffer and masked afterwards: a load under a branch
+// is waited for before the branch closes, which serialises a run of them.
+
+// Flag word w of the previous order (zero past the last particle; n >= 1).
+__device__ __forceinline__ unsigned long long flag_word(const unsigned long long* flags, int w,
+                                                        int n) {
+  const unsigned long long v = flags[min(w, (n - 1) / kWave)];
+  return (int64_t)w * kWave < n ? v : 0ull;
+}
+
+// Mover flags for producers of the new keys that do not write them.
+__global__ __launch_bounds__(TW) void k_mover_flags(int n, const uint64_t* __restrict__ knew,
+                                                    const uint64_t* __restrict__ kold,
+                                                    unsigned long long* __restrict__ flags) {
+  const int i = blockIdx.x * TW + threadIdx.x;
   const bool live = i < n;
-  const uint64_t kn = live ? knew[i] : 0, ko = live ? kold[i] : 0;
-  if (live) {
-    const uint64_t kp = i > 0 ? kold[i - 1] : ~ko;
-    if (ko != kp && ko < n_keys) cells[ko] = make_int2(-1, -1);
-    if (block_bits > 0 && (ko >> (2 * block_bits)) != (kp >> (2 * block_bits)) && ko < n_keys)
-      blocks[ko >> (2 * block_bits)] = make_int2(-1, -1);
-  }
-  const unsigned long long bal = __ballot(live && kn != ko);
-  if (lane == 0) {
-    s_wcnt[wave] = (uint32_t)__popcll(bal);
-    flags[(size_t)tile * (TW / kWave) + wave] = bal;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t c = 0;
-    for (int w = 0; w < TW / kWave; ++w) c += s_wcnt[w];
-    tile_count[tile] = c;
-  }
+  const unsigned long long bal = __ballot(live && knew[i] != kold[i]);
+  if ((threadIdx.x & (kWave - 1)) == 0) flags[i / kWave] = bal;
 }
 
-// In-place exclusive scan of the per-tile mover counts (one workgroup, fixed
-// order) and the total.
-constexpr int SCAN_TW = 1024;
-__global__ __launch_bounds__(SCAN_TW) void k_tile_scan(int tiles, uint32_t* __restrict__ tile_base,
-                                                       int* __restrict__ n_movers) {
-  __shared__ uint32_t s_w[SCAN_TW / kWave];
-  __shared__ uint32_t s_carry;
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  if (tid == 0) s_carry = 0;
-  __syncthreads();
-  for (int base = 0; base < tiles; base += SCAN_TW) {
-    const int t = base + tid;
-    const uint32_t v = t < tiles ? tile_base[t] : 0u;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o);
-      if (lane >= o) x += y;
-    }
-    if (lane == kWave - 1) s_w[wave] = x;
-    __syncthreads();
-    uint32_t off = s_carry;
-    for (int w = 0; w < wave; ++w) off += s_w[w];
-    if (t < tiles) tile_base[t] = off + x - v;
-    __syncthreads();
-    if (tid == SCAN_TW - 1) s_carry = off + x;
-    __syncthreads();
-  }
-  if (tid == 0) *n_movers = (int)s_carry;
-}
-
-// Movers in previous-index order: (new key, previous index), one thread per
-// particle.
-__global__ __launch_bounds__(TW) void k_mover_compact(
-    int n, const uint64_t* __restrict__ knew, const unsigned long long* __restrict__ flags,
-    const uint32_t* __restrict__ tile_base, uint64_t* __restrict__ mkey,
-    uint32_t* __restrict__ midx) {
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const int tile = blockIdx.x;
-  const int i = tile * TW + tid;
-  const unsigned long long fw = flags[(size_t)tile * (TW / kWave) + wave];
-  if (i >= n || !((fw >> lane) & 1ull)) return;
-  uint32_t r = tile_base[tile];
-  for (int w = 0; w < wave; ++w) r += (uint32_t)__popcll(flags[(size_t)tile * (TW / kWave) + w]);
-  r += (uint32_t)__popcll(fw & ((1ull << lane) - 1ull));
-  mkey[r] = knew[i];
-  midx[r] = (uint32_t)i;
-}
-
-// ------------------------------------------------------------ mover sort
-// The compacted movers sorted by (new key, previous index) into skey / sidx, in
-// one launch of kMoverGrid workgroups: with at most kSmallSortCap movers
-// workgroup 0 sorts them in LDS with a bitonic network (the others return at
-// once); above that all workgroups run an LSD radix sort of 8-bit digits (per
-// pass: digit counts per workgroup chunk, then a stable scatter in
-// 1024-element sub-tiles with wave64 ballot ranking, a grid barrier after each
-// phase). The grid is small enough to be resident at once; every barrier wait
-// is bounded and reports to the status word if it gives up.
-constexpr int MS_TW = 1024;  // threads per mover-sort workgroup
+// ------------------------------------------------------------------ movers
+constexpr int MS_TW = 1024;  // threads per k_movers workgroup
 constexpr int MS_WAVES = MS_TW / kWave;
 // the grid radix path reads every workgroup's 256 digit counts into the LDS of
-// the small-sort key buffers
+// the small-sort key buffers; the LDS sort gives each thread 4 elements
 static_assert(kMoverGrid * 256 <= 4 * kSmallSortCap, "digit counts exceed the LDS arrays");
+static_assert(4 * MS_TW == kSmallSortCap && MS_WAVES == 16, "LDS sort layout");
 constexpr uint32_t MS_POLLS = 1u << 20;
 
-// Grid-wide barrier on a monotonic counter (zeroed by k_mover_flags every step).
+// Grid-wide barrier on a monotonic counter (zero at kernel start: k_merge
+// resets it every step).
 __device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t& gen, uint32_t* status,
                                              int* s_ok) {
   __syncthreads();
@@ -168,6 +98,31 @@ __device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t& gen, uint3
   __syncthreads();
   ++gen;
   return *s_ok != 0;
+}
+
+// Exclusive scan of one value per thread over the workgroup; *total gets the
+// sum. scratch: MS_WAVES words. Ends with a workgroup barrier.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* scratch,
+                                                    uint32_t* total) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) scratch[wave] = x;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < MS_WAVES; ++w) {
+    const uint32_t c = scratch[w];
+    off += w < wave ? c : 0u;
+    tot += c;
+  }
+  *total = tot;
+  __syncthreads();
+  return off + x - v;
 }
 
 // In-place exclusive scan of 256 digit counts by the first four wavefronts
@@ -209,86 +164,295 @@ __device__ __forceinline__ uint32_t wave_digit_rank(uint32_t d, bool valid,
   return (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
 
-__global__ __launch_bounds__(MS_TW) void k_mover_sort(
-    int passes, const int* __restrict__ n_movers, uint64_t* __restrict__ mkey,
-    uint32_t* __restrict__ midx, uint64_t* __restrict__ ktmp, uint32_t* __restrict__ vtmp,
-    uint64_t* __restrict__ skey, uint32_t* __restrict__ sidx, uint32_t* __restrict__ counts,
-    uint32_t* __restrict__ bar, uint32_t* __restrict__ status) {
+// Number of sorted movers r in [lo, hi) with (keys[r], idx[r]) < (k, i).
+__device__ __forceinline__ int rank_in_sorted(const uint64_t* keys, const uint32_t* idx, int lo,
+                                              int hi, uint64_t k, uint32_t i) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (pair_less(keys[mid], idx[mid], k, i)) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves (no LDS
+// round trips): shifts by 1, 2, 4 and 8 lanes inside each row of 16, then row 1
+// and 3 add the last lane of the row before, then rows 2 and 3 add lane 31.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  int x = (int)v;
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return (uint32_t)x;
+}
+
+// Flag words per thread and per batch of the ordered compaction.
+constexpr int MS_WPT = 16;
+constexpr int MS_BATCH = MS_WPT * MS_TW;  // 16384 words: 1,048,576 particles
+
+// Exclusive prefix over the waves of each digit's per-wave counts (threads
+// 0..255, one digit each; the 16 counts are loaded before any is replaced) and
+// the digit totals into tot.
+__device__ __forceinline__ void wave_digit_prefix(uint32_t* wc, uint32_t* tot) {
+  const int tid = threadIdx.x;
+  if (tid < 256) {
+    uint32_t c[MS_WAVES];
+#pragma unroll
+    for (int w = 0; w < MS_WAVES; ++w) c[w] = wc[w * 256 + tid];
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < MS_WAVES; ++w) {
+      wc[w * 256 + tid] = run;
+      run += c[w];
+    }
+    tot[tid] = run;
+  }
+}
+
+__global__ __launch_bounds__(MS_TW) void k_movers(
+    int n, int tiles, int passes, const uint64_t* __restrict__ knew,
+    const uint64_t* __restrict__ tfirst, const unsigned long long* __restrict__ flags,
+    uint32_t* __restrict__ tile_base, uint32_t* __restrict__ tile_lo, int* __restrict__ n_movers,
+    uint64_t* __restrict__ mkey, uint32_t* __restrict__ midx, uint64_t* __restrict__ ktmp,
+    uint32_t* __restrict__ vtmp, uint64_t* __restrict__ skey, uint32_t* __restrict__ sidx,
+    uint32_t* __restrict__ counts, uint32_t* __restrict__ bar, uint32_t* __restrict__ status) {
   // small case: two key/value buffers (ping-pong) of kSmallSortCap each
   __shared__ uint64_t s_k[2 * kSmallSortCap];
   __shared__ uint32_t s_i[2 * kSmallSortCap];
-  __shared__ uint32_t s_wc[MS_WAVES * 256];  // per-wave digit counts -> offsets
+  // digit counts: u16 [4 * MS_WAVES][256] per segment in the LDS sort, u32
+  // [MS_WAVES][256] per wave in the grid sort
+  __shared__ uint32_t s_wc[2 * MS_WAVES * 256];
   __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_tot[256];
+  __shared__ uint32_t s_w[MS_WAVES];
+  __shared__ uint32_t s_gt[4 * 256];  // LDS sort: per digit, offsets of 4 segment groups
   __shared__ int s_ok;
   const int tid = threadIdx.x, wave = tid / kWave;
-  const uint32_t total = (uint32_t)*n_movers;
-  if (total <= (uint32_t)kSmallSortCap) {
-    if (blockIdx.x != 0) return;
-    // LSD radix sort in LDS: per 8-bit pass, digit counts of all elements give
-    // each digit's start; elements are then placed in order, 1024 at a time,
-    // at start + (same digit earlier in this round) + (same digit earlier in
-    // the wave) -- stable, so ties keep their (key, previous index) order
-    const int m = (int)total;
-    for (int t = tid; t < m; t += MS_TW) {
-      s_k[t] = mkey[t];
-      s_i[t] = midx[t];
+  const int words = 4 * tiles;
+
+  // 1. Workgroup 0: tile prefixes and the movers' previous indices in order
+  // (into midx, and into s_i while they fit the LDS sort). Per batch each wave
+  // takes MS_WPT * 64 consecutive words (lane l: words q * 64 + l, coalesced);
+  // the prefixes come from wave scans plus the waves' offsets. The other
+  // workgroups only count the movers, to choose the same path.
+  uint32_t total = 0;
+  if (blockIdx.x == 0) {
+    const int lane = tid & (kWave - 1);
+    // each word's prefix within its wave, [wave][q][lane] in the key buffers
+    uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_k);
+    static_assert(MS_BATCH * 4 <= (int)sizeof(s_k), "word prefixes exceed the key buffers");
+    for (int b0 = 0; b0 < words; b0 += MS_BATCH) {
+      const int wbase = b0 + wave * (MS_WPT * kWave);
+      uint32_t* pre = s_pre + wave * (MS_WPT * kWave) + lane;  // pre[q * kWave]
+      unsigned long long wd[MS_WPT];
+#pragma unroll
+      for (int q = 0; q < MS_WPT; ++q) wd[q] = flag_word(flags, wbase + q * kWave + lane, n);
+      uint32_t wtot = 0;
+#pragma unroll
+      for (int q = 0; q < MS_WPT; ++q) {
+        const uint32_t c = (uint32_t)__popcll(wd[q]);
+        const uint32_t x = wave_incl_scan(c);
+        pre[q * kWave] = wtot + x - c;
+        wtot += (uint32_t)__builtin_amdgcn_readlane((int)x, kWave - 1);
+      }
+      if (lane == 0) s_w[wave] = wtot;
+      __syncthreads();
+      uint32_t woff = total, btot = 0;
+#pragma unroll
+      for (int w = 0; w < MS_WAVES; ++w) {
+        const uint32_t c = s_w[w];
+        woff += w < wave ? c : 0u;
+        btot += c;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < MS_WPT; ++q) {
+        const int w = wbase + q * kWave + lane;
+        uint32_t pos = woff + pre[q * kWave];
+        if ((w & 3) == 0 && (w >> 2) < tiles) tile_base[w >> 2] = pos;
+        unsigned long long bits = wd[q];
+        while (bits) {
+          const uint32_t i = (uint32_t)w * kWave + (uint32_t)(__ffsll((long long)bits) - 1);
+          midx[pos] = i;
+          if (pos < (uint32_t)kSmallSortCap) s_i[pos] = i;
+          ++pos;
+          bits &= bits - 1ull;
+        }
+      }
+      total += btot;
     }
+    if (tid == 0) {
+      *n_movers = (int)total;
+      tile_lo[0] = 0;
+      tile_lo[tiles] = total;
+    }
+    __syncthreads();
+  } else {
+    uint32_t cnt = 0;
+    for (int b0 = 0; b0 < words; b0 += MS_BATCH) {
+      unsigned long long wd[MS_WPT];
+#pragma unroll
+      for (int q = 0; q < MS_WPT; ++q) wd[q] = flag_word(flags, b0 + q * MS_TW + tid, n);
+#pragma unroll
+      for (int q = 0; q < MS_WPT; ++q) cnt += (uint32_t)__popcll(wd[q]);
+    }
+    (void)block_excl_scan(cnt, s_w, &total);
+  }
+  const bool small = total <= (uint32_t)kSmallSortCap;
+  if (small && blockIdx.x != 0) return;
+
+  if (small) {
+    // Workgroup 0: the keys in one round of loads, then an LSD radix sort in
+    // LDS with 8-bit digits. Element e = q * MS_TW + tid (q < 4) belongs to
+    // segment (q, wave) of 64 consecutive elements; per pass each element's
+    // slot is (start of its digit) + (same digit in earlier segments) + (same
+    // digit earlier in its segment) -- stable, so ties keep their (key,
+    // previous index) order. The per-segment counts also give the digit totals.
+    const int m = (int)total;
+    const int mc = max(m - 1, 0);
+    // this thread's run of merge tiles and their first keys, loaded together
+    // with the movers' keys (used after the sort)
+    const int tpt = (tiles + MS_TW - 1) / MS_TW;
+    const int ta = max(1, tid * tpt), tb = min(tiles, tid * tpt + tpt);
+    uint64_t qk0[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) qk0[q] = tfirst[min(ta + q, tb - 1)];
+    {
+      uint32_t gi[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = q * MS_TW + tid;
+        const uint32_t v = s_i[min(e, mc)];
+        gi[q] = e < m ? v : 0u;
+      }
+      uint64_t gk[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) gk[q] = knew[gi[q]];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q * MS_TW + tid < m) s_k[q * MS_TW + tid] = gk[q];
+    }
+    uint16_t* s_seg = reinterpret_cast<uint16_t*>(s_wc);  // [4 * MS_WAVES][256]
+    const int lane = tid & (kWave - 1);
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
     int cur = 0;
     for (int p = 0; p < passes; ++p) {
       const int shift = 8 * p;
-      uint64_t* kin = s_k + cur * kSmallSortCap;
-      uint32_t* vin = s_i + cur * kSmallSortCap;
+      const uint64_t* kin = s_k + cur * kSmallSortCap;
+      const uint32_t* vin = s_i + cur * kSmallSortCap;
       uint64_t* kout = s_k + (cur ^ 1) * kSmallSortCap;
       uint32_t* vout = s_i + (cur ^ 1) * kSmallSortCap;
-      if (tid < 256) s_base[tid] = 0;
+      for (int q = tid; q < 2 * MS_WAVES * 256; q += MS_TW) s_wc[q] = 0;
       __syncthreads();
-      for (int t = tid; t < m; t += MS_TW) atomicAdd(&s_base[(uint32_t)(kin[t] >> shift) & 255u], 1u);
+      uint64_t key[4];
+      uint32_t val[4], dig[4], wrank[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = q * MS_TW + tid;
+        const bool valid = e < m;
+        const uint64_t kq = kin[min(e, mc)];
+        const uint32_t vq = vin[min(e, mc)];
+        key[q] = valid ? kq : 0ull;
+        val[q] = valid ? vq : 0u;
+        dig[q] = (uint32_t)(key[q] >> shift) & 255u;
+        unsigned long long same = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const unsigned long long bb = __ballot((dig[q] >> b) & 1u);
+          same &= ((dig[q] >> b) & 1u) ? bb : ~bb;
+        }
+        wrank[q] = (uint32_t)__popcll(same & lt_mask);
+        if (valid && wrank[q] == 0)
+          s_seg[(q * MS_WAVES + wave) * 256 + dig[q]] = (uint16_t)__popcll(same);
+      }
+      __syncthreads();
+      // per digit d and group g of 16 segments (thread g * 256 + d): exclusive
+      // prefix inside the group (counts loaded before any is replaced)
+      {
+        const int d = tid & 255, g = tid >> 8;
+        uint32_t c[MS_WAVES];
+#pragma unroll
+        for (int k = 0; k < MS_WAVES; ++k) c[k] = s_seg[(g * MS_WAVES + k) * 256 + d];
+        uint32_t run = 0;
+#pragma unroll
+        for (int k = 0; k < MS_WAVES; ++k) {
+          s_seg[(g * MS_WAVES + k) * 256 + d] = (uint16_t)run;
+          run += c[k];
+        }
+        s_gt[g * 256 + d] = run;
+      }
+      __syncthreads();
+      // per digit: the groups' offsets and the digit total
+      if (tid < 256) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const uint32_t c = s_gt[g * 256 + tid];
+          s_gt[g * 256 + tid] = run;
+          run += c;
+        }
+        s_base[tid] = run;
+      }
       __syncthreads();
       digit_excl_scan(s_base, s_tot);
-
-      for (int base = 0; base < m; base += MS_TW) {
-        for (int q = tid; q < MS_WAVES * 256; q += MS_TW) s_wc[q] = 0;
-        __syncthreads();
-        const int e = base + tid;
-        const bool valid = e < m;
-        const uint64_t key = valid ? kin[e] : 0ull;
-        const uint32_t val = valid ? vin[e] : 0u;
-        const uint32_t d = (uint32_t)(key >> shift) & 255u;
-        unsigned long long same;
-        const uint32_t wrank = wave_digit_rank(d, valid, &same);
-        if (valid && wrank == 0) s_wc[wave * 256 + d] = (uint32_t)__popcll(same);
-        __syncthreads();
-        if (tid < 256) {
-          uint32_t run = 0;
-          for (int w = 0; w < MS_WAVES; ++w) {
-            const uint32_t c = s_wc[w * 256 + tid];
-            s_wc[w * 256 + tid] = run;
-            run += c;
-          }
-          s_tot[tid] = run;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q * MS_TW + tid < m) {
+          const uint32_t pos = s_base[dig[q]] + s_gt[q * 256 + dig[q]] +
+                               s_seg[(q * MS_WAVES + wave) * 256 + dig[q]] + wrank[q];
+          kout[pos] = key[q];
+          vout[pos] = val[q];
         }
-        __syncthreads();
-        if (valid) {
-          const uint32_t pos = s_base[d] + s_wc[wave * 256 + d] + wrank;
-          kout[pos] = key;
-          vout[pos] = val;
-        }
-        __syncthreads();
-        if (tid < 256) s_base[tid] += s_tot[tid];
-        __syncthreads();
       }
+      __syncthreads();
       cur ^= 1;
     }
+    const uint64_t* sk = s_k + cur * kSmallSortCap;
+    const uint32_t* si = s_i + cur * kSmallSortCap;
     for (int t = tid; t < m; t += MS_TW) {
-      skey[t] = s_k[cur * kSmallSortCap + t];
-      sidx[t] = s_i[cur * kSmallSortCap + t];
+      skey[t] = sk[t];
+      sidx[t] = si[t];
+    }
+    // Window starts of the merge tiles: movers before each tile's first
+    // particle, non-decreasing in the tile. Each thread takes its run of tiles:
+    // a lower bound for the first (trip count set by m alone: the answer stays
+    // in [base, base + len]), then a forward walk.
+    int r = 0;
+    for (int t0 = ta; t0 < tb; t0 += 4) {
+      uint64_t qk[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) qk[q] = t0 == ta ? qk0[q] : tfirst[min(t0 + q, tb - 1)];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int t = t0 + q;
+        if (t >= tb) break;
+        const uint32_t qi = (uint32_t)t * TW;
+        if (t == ta) {
+          int base = 0;
+          for (int len = m; len > 1;) {
+            const int half = len >> 1;
+            if (pair_less(sk[base + half - 1], si[base + half - 1], qk[q], qi)) base += half;
+            len -= half;
+          }
+          r = (m > 0 && pair_less(sk[base], si[base], qk[q], qi)) ? base + 1 : base;
+        } else {
+          while (r < m && pair_less(sk[r], si[r], qk[q], qi)) ++r;
+        }
+        tile_lo[t] = (uint32_t)r;
+      }
     }
     return;
   }
+
   // ---- more movers than one workgroup sorts: LSD radix sort over the grid
+  // (the indices in midx from workgroup 0; the first pass reads the keys
+  // through them)
   uint32_t gen = 0;
-  uint32_t* s_cnt = s_wc;  // [MS_WAVES][256] per-wave digit counts
+  if (!grid_barrier(bar, gen, status, &s_ok)) return;
+  uint32_t* s_cnt = s_wc;    // [MS_WAVES][256] per-wave digit counts
   uint32_t* s_off = s_base;  // [256] running digit offsets
   const int nb = gridDim.x;
   const int chunk = (int)((total + (uint32_t)(nb * MS_TW) - 1) / (uint32_t)(nb * MS_TW)) * MS_TW;
@@ -297,16 +461,18 @@ __global__ __launch_bounds__(MS_TW) void k_mover_sort(
   const int lane = tid & (kWave - 1);
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
   for (int p = 0; p < passes; ++p) {
-    const uint64_t* kin = p == 0 ? mkey : ((p & 1) ? ktmp : mkey);
-    const uint32_t* vin = p == 0 ? midx : ((p & 1) ? vtmp : midx);
+    // pass 0 reads the keys through midx; then ktmp / mkey alternate
+    const uint64_t* kin = (p & 1) ? ktmp : mkey;
+    const uint32_t* vin = (p == 0) ? midx : ((p & 1) ? vtmp : midx);
     uint64_t* kout = p == passes - 1 ? skey : ((p & 1) ? mkey : ktmp);
     uint32_t* vout = p == passes - 1 ? sidx : ((p & 1) ? midx : vtmp);
+    auto key_of = [&](int e) { return p == 0 ? knew[vin[e]] : kin[e]; };
     const int shift = 8 * p;
     // A: digit counts of this workgroup's chunk
     if (tid < 256) s_off[tid] = 0;
     __syncthreads();
     for (int e = c0 + tid; e < c1; e += MS_TW)
-      atomicAdd(&s_off[(uint32_t)(kin[e] >> shift) & 255u], 1u);
+      atomicAdd(&s_off[(uint32_t)(key_of(e) >> shift) & 255u], 1u);
     __syncthreads();
     if (tid < 256) counts[(size_t)blockIdx.x * 256 + tid] = s_off[tid];
     if (!grid_barrier(bar, gen, status, &s_ok)) return;
@@ -338,7 +504,7 @@ __global__ __launch_bounds__(MS_TW) void k_mover_sort(
       __syncthreads();
       const int e = base + tid;
       const bool valid = e < c1;
-      const uint64_t key = valid ? kin[e] : 0ull;
+      const uint64_t key = valid ? key_of(e) : 0ull;
       const uint32_t val = valid ? vin[e] : 0u;
       const uint32_t d = (uint32_t)(key >> shift) & 255u;
       unsigned long long same = __ballot(valid);
@@ -350,16 +516,7 @@ __global__ __launch_bounds__(MS_TW) void k_mover_sort(
       const uint32_t wrank = (uint32_t)__popcll(same & lt_mask);
       if (valid && wrank == 0) s_cnt[wave * 256 + d] = (uint32_t)__popcll(same);
       __syncthreads();
-      // per digit: exclusive prefix over the waves, and the sub-tile total
-      if (tid < 256) {
-        uint32_t run = 0;
-        for (int w = 0; w < MS_WAVES; ++w) {
-          const uint32_t c = s_cnt[w * 256 + tid];
-          s_cnt[w * 256 + tid] = run;
-          run += c;
-        }
-        s_tot[tid] = run;
-      }
+      wave_digit_prefix(s_cnt, s_tot);
       __syncthreads();
       if (valid) {
         const uint32_t pos = s_off[d] + s_cnt[wave * 256 + d] + wrank;
@@ -372,126 +529,117 @@ __global__ __launch_bounds__(MS_TW) void k_mover_sort(
     }
     if (!grid_barrier(bar, gen, status, &s_ok)) return;
   }
+  // window starts of the merge tiles (binary searches in the sorted movers)
+  for (int t = 1 + (int)blockIdx.x * MS_TW + tid; t < tiles; t += nb * MS_TW)
+    tile_lo[t] = (uint32_t)rank_in_sorted(skey, sidx, 0, (int)total, tfirst[t],
+                                          (uint32_t)t * TW);
 }
 
-// Movers among the previous-order indices [0, p) (p <= n).
-__device__ __forceinline__ uint32_t movers_before(int p, int n, int n_mov,
-                                                  const unsigned long long* flags,
-                                                  const uint32_t* tile_base) {
-  if (p >= n) return (uint32_t)n_mov;
-  const int tile = p / TW, within = p % TW;
-  uint32_t r = tile_base[tile];
-  const int w = within / kWave, l = within % kWave;
-  for (int k = 0; k < w; ++k) r += (uint32_t)__popcll(flags[(size_t)tile * (TW / kWave) + k]);
-  r += (uint32_t)__popcll(flags[(size_t)tile * (TW / kWave) + w] & ((1ull << l) - 1ull));
-  return r;
-}
-
-// Number of previous-order particles j with (kold[j], j) < (k, idx): the
-// sequence (kold[j], j) is strictly increasing in j.
-__device__ __forceinline__ int rank_in_previous(const uint64_t* kold, int n, uint64_t k,
-                                                uint32_t idx) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (pair_less(kold[mid], (uint32_t)mid, k, idx)) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// Number of sorted movers r in [lo, hi) with (skey[r], sidx[r]) < (k, idx).
-__device__ __forceinline__ int rank_in_movers(const uint64_t* skey, const uint32_t* sidx, int lo,
-                                              int hi, uint64_t k, uint32_t idx) {
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (pair_less(skey[mid], sidx[mid], k, idx)) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ void scatter(int pos, int src, uint64_t key, const float* tx,
-                                        const float* ty, const float* tvx, const float* tvy,
-                                        const uint32_t* pid_src, float* rx, float* ry,
-                                        float* vx, float* vy, uint32_t* pid_dst,
-                                        uint64_t* keys_dst, uint32_t* perm) {
-  rx[pos] = tx[src];
-  ry[pos] = ty[src];
-  vx[pos] = tvx[src];
-  vy[pos] = tvy[src];
-  pid_dst[pos] = pid_src[src];
-  keys_dst[pos] = key;
-  perm[pos] = (uint32_t)src;
-}
-
-// Grid: ceil(n / TW) workgroups for the previous order (stayers), then as many
-// again for the sorted movers (those past the device-side mover count exit).
+// ------------------------------------------------------------------- merge
+// One workgroup per tile of the previous order [j0, j0 + TW). Its window of
+// sorted movers [lo, hi) holds the movers ordered after the tile's first
+// particle and before the next tile's first particle (tile 0: from the first
+// mover), so every mover belongs to exactly one window.
+//   stayer j:  slot = (j - movers before j) + lo + (window movers before it)
+//   mover r:   slot = r + (stayers before it), where the previous-order
+//              particles before it are j0 + (tile particles before it)
 __global__ __launch_bounds__(TW) void k_merge(
-    int n, int tiles, const uint64_t* __restrict__ kold,
-    const unsigned long long* __restrict__ flags, const uint32_t* __restrict__ tile_base,
+    int n, const uint64_t* __restrict__ kold, const unsigned long long* __restrict__ flags,
+    const uint32_t* __restrict__ tile_base, const uint32_t* __restrict__ tile_lo,
     const uint64_t* __restrict__ skey, const uint32_t* __restrict__ sidx,
-    const int* __restrict__ n_movers_p, const float* __restrict__ tx,
-    const float* __restrict__ ty, const float* __restrict__ tvx,
-    const float* __restrict__ tvy, const uint32_t* __restrict__ pid_src,
-    float* __restrict__ rx, float* __restrict__ ry, float* __restrict__ vx,
-    float* __restrict__ vy, uint32_t* __restrict__ pid_dst, uint64_t* __restrict__ keys_dst,
-    uint32_t* __restrict__ perm) {
-  __shared__ uint64_t s_k[WIN];
-  __shared__ uint32_t s_i[WIN];
-  __shared__ int s_lo, s_hi;
+    const float* __restrict__ tx, const float* __restrict__ ty, const float* __restrict__ tvx,
+    const float* __restrict__ tvy, const uint32_t* __restrict__ pid_src, float* __restrict__ rx,
+    float* __restrict__ ry, float* __restrict__ vx, float* __restrict__ vy,
+    uint32_t* __restrict__ pid_dst, uint64_t* __restrict__ keys_dst, uint32_t* __restrict__ perm,
+    int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
+    uint32_t* __restrict__ bar) {
+  __shared__ uint64_t s_tk[TW];  // the tile's previous keys (all ones past n)
+  __shared__ uint64_t s_mk[WIN];
+  __shared__ uint32_t s_mi[WIN];
+  __shared__ uint32_t s_wpre[TW / kWave + 1];  // movers of the tile before each wave
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const int n_mov = *n_movers_p;
-  if ((int)blockIdx.x < tiles) {
-    const int tile = blockIdx.x;
-    const int j = tile * TW + tid;
-    const bool live = j < n;
-    const unsigned long long fw = flags[(size_t)tile * (TW / kWave) + wave];
-    const bool mv = live && ((fw >> lane) & 1ull);
-    // window of sorted movers that can fall between this tile's first and last
-    // previous-order elements
-    const int j0 = tile * TW, j1 = min(n, j0 + TW) - 1;
-    if (tid == 0) s_lo = rank_in_movers(skey, sidx, 0, n_mov, kold[j0], (uint32_t)j0);
-    if (tid == 1) s_hi = rank_in_movers(skey, sidx, 0, n_mov, kold[j1], (uint32_t)j1);
-    __syncthreads();
-    const int lo = s_lo, hi = s_hi;
-    const bool windowed = hi - lo <= WIN;
-    if (windowed)
-      for (int r = lo + tid; r < hi; r += TW) {
-        s_k[r - lo] = skey[r];
-        s_i[r - lo] = sidx[r];
-      }
+  const int tile = blockIdx.x;
+  const int j0 = tile * TW, j = j0 + tid;
+  const bool live = j < n;
+  // every load that depends only on j, issued together
+  const unsigned long long fw = flag_word(flags, j0 / kWave + wave, n);
+  const uint32_t tb = tile_base[tile];
+  const int lo = (int)tile_lo[tile], hi = (int)tile_lo[tile + 1];
+  const int jc = min(j, n - 1);
+  const uint64_t kc = kold[jc], kpc = kold[max(jc - 1, 0)];
+  const uint64_t k = live ? kc : ~0ull;
+  const uint64_t kp = (live && j > 0) ? kpc : ~k;
+  const float x = tx[jc], y = ty[jc], u = tvx[jc], v = tvy[jc];
+  const uint32_t id = pid_src[jc];
+  if (tile == 0 && tid == 0) *bar = 0;  // the next k_movers' grid barrier
+  // the previous order's ranges (k_ranges writes the new ones)
+  if (live && k != kp && k < n_keys) {
+    cells[k] = make_int2(-1, -1);
+    if (block_bits > 0 && (k >> (2 * block_bits)) != (kp >> (2 * block_bits)))
+      blocks[k >> (2 * block_bits)] = make_int2(-1, -1);
+  }
+  s_tk[tid] = k;
+  if (lane == 0) s_wpre[wave + 1] = (uint32_t)__popcll(fw);
+  __syncthreads();
+  if (tid == 0) {
+    s_wpre[0] = 0;
+    for (int w = 1; w <= TW / kWave; ++w) s_wpre[w] += s_wpre[w - 1];
+  }
+  __syncthreads();
+  const bool mv = live && ((fw >> lane) & 1ull);
+  int before_movers = 0;  // window movers ordered before stayer j
+  for (int c0 = lo; c0 < hi; c0 += WIN) {
+    const int len = min(WIN, hi - c0);
+    for (int r = tid; r < len; r += TW) {
+      s_mk[r] = skey[c0 + r];
+      s_mi[r] = sidx[c0 + r];
+    }
     __syncthreads();
     if (live && !mv) {
-      const uint64_t k = kold[j];
-      uint32_t before = tile_base[tile];
-      for (int w = 0; w < wave; ++w)
-        before += (uint32_t)__popcll(flags[(size_t)tile * (TW / kWave) + w]);
-      before += (uint32_t)__popcll(fw & ((1ull << lane) - 1ull));
-      int b;
-      if (windowed) {
-        int a = 0, z = hi - lo;
-        while (a < z) {
-          const int mid = (a + z) >> 1;
-          if (pair_less(s_k[mid], s_i[mid], k, (uint32_t)j)) a = mid + 1;
-          else z = mid;
-        }
-        b = lo + a;
-      } else {
-        b = rank_in_movers(skey, sidx, lo, hi, k, (uint32_t)j);
+      int a = 0, z = len;
+      while (a < z) {
+        const int mid = (a + z) >> 1;
+        if (pair_less(s_mk[mid], s_mi[mid], k, (uint32_t)j)) a = mid + 1;
+        else z = mid;
       }
-      const int pos = (j - (int)before) + b;
-      scatter(pos, j, k, tx, ty, tvx, tvy, pid_src, rx, ry, vx, vy, pid_dst, keys_dst, perm);
+      before_movers += a;
     }
-  } else {
-    const int r = ((int)blockIdx.x - tiles) * TW + tid;
-    if (r >= n_mov) return;
-    const uint64_t k = skey[r];
-    const uint32_t src = sidx[r];
-    const int p = rank_in_previous(kold, n, k, src);
-    const int stayers_before = p - (int)movers_before(p, n, n_mov, flags, tile_base);
-    scatter(r + stayers_before, (int)src, k, tx, ty, tvx, tvy, pid_src, rx, ry, vx, vy, pid_dst,
-            keys_dst, perm);
+    for (int r = tid; r < len; r += TW) {
+      const uint64_t km = s_mk[r];
+      const uint32_t src = s_mi[r];
+      // tile particles ordered before the mover (keys past n compare greater)
+      int a = 0, z = TW;
+      while (a < z) {
+        const int mid = (a + z) >> 1;
+        if (pair_less(s_tk[mid], (uint32_t)(j0 + mid), km, src)) a = mid + 1;
+        else z = mid;
+      }
+      // movers among the tile particles [j0, j0 + a)
+      const int w = a / kWave, l = a % kWave;
+      uint32_t mb = tb + s_wpre[w];
+      if (w < TW / kWave)
+        mb += (uint32_t)__popcll(flag_word(flags, j0 / kWave + w, n) & ((1ull << l) - 1ull));
+      const int pos = (c0 + r) + (j0 + a - (int)mb);
+      rx[pos] = tx[src];
+      ry[pos] = ty[src];
+      vx[pos] = tvx[src];
+      vy[pos] = tvy[src];
+      pid_dst[pos] = pid_src[src];
+      keys_dst[pos] = km;
+      perm[pos] = src;
+    }
+    __syncthreads();
+  }
+  if (live && !mv) {
+    const uint32_t mb = tb + s_wpre[wave] + (uint32_t)__popcll(fw & ((1ull << lane) - 1ull));
+    const int pos = (j - (int)mb) + lo + before_movers;
+    rx[pos] = x;
+    ry[pos] = y;
+    vx[pos] = u;
+    vy[pos] = v;
+    pid_dst[pos] = id;
+    keys_dst[pos] = k;
+    perm[pos] = (uint32_t)j;
   }
 }
 
@@ -501,19 +649,17 @@ int64_t resort_flag_words(int n) { return ((int64_t)n + TW - 1) / TW * (TW / kWa
 int64_t resort_tiles(int n) { return ((int64_t)n + TW - 1) / TW; }
 
 hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers& r,
-                         const uint64_t* knew, int key_bits) {
+                         const uint64_t* knew, int key_bits, bool flags_ready) {
   if (s.n <= 0) return hipSuccess;
   const int tiles = (int)resort_tiles(s.n);
   const int passes = (key_bits + 7) / 8;
   if (passes < 1 || passes > kMaxPasses) return hipErrorInvalidValue;
-  k_mover_flags<<<tiles, TW, 0, L.stream>>>(s.n, knew, s.keys_sorted, s.cells, s.blocks,
-                                            s.block_bits, (uint64_t)s.n_keys, r.flags,
-                                            r.tile_base, r.bar);
-  k_tile_scan<<<1, SCAN_TW, 0, L.stream>>>(tiles, r.tile_base, r.n_movers);
-  k_mover_compact<<<tiles, TW, 0, L.stream>>>(s.n, knew, r.flags, r.tile_base, r.mkey, r.midx);
-  k_mover_sort<<<kMoverGrid, MS_TW, 0, L.stream>>>(passes, r.n_movers, r.mkey, r.midx, r.ktmp,
-                                                   r.vtmp, r.skey, r.sidx, r.counts, r.bar,
-                                                   r.status);
+  if (!flags_ready)
+    k_mover_flags<<<tiles, TW, 0, L.stream>>>(s.n, knew, s.keys_sorted, r.flags);
+  k_movers<<<kMoverGrid, MS_TW, 0, L.stream>>>(s.n, tiles, passes, knew, s.tile_first, r.flags,
+                                               r.tile_base, r.tile_lo, r.n_movers, r.mkey,
+                                               r.midx, r.ktmp, r.vtmp, r.skey, r.sidx, r.counts,
+                                               r.bar, r.status);
   return hipGetLastError();
 }
 
@@ -521,9 +667,10 @@ hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers&
                         uint64_t* keys_dst) {
   if (s.n <= 0) return hipSuccess;
   const int tiles = (int)resort_tiles(s.n);
-  k_merge<<<2 * tiles, TW, 0, L.stream>>>(
-      s.n, tiles, s.keys_sorted, r.flags, r.tile_base, r.skey, r.sidx, r.n_movers, s.tx, s.ty,
-      s.tvx, s.tvy, s.pid, s.rx, s.ry, s.vx, s.vy, s.pid_alt, keys_dst, s.perm);
+  k_merge<<<tiles, TW, 0, L.stream>>>(s.n, s.keys_sorted, r.flags, r.tile_base, r.tile_lo,
+                                      r.skey, r.sidx, s.tx, s.ty, s.tvx, s.tvy, s.pid, s.rx, s.ry,
+                                      s.vx, s.vy, s.pid_alt, keys_dst, s.perm, s.cells, s.blocks,
+                                      s.block_bits, (uint64_t)s.n_keys, r.bar);
   return hipGetLastError();
 }
 
