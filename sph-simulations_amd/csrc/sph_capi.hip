@@ -547,6 +547,7 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
         (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
         (rc = dev_alloc(c, &r.tile_lo, resort_tiles((int)N) + 1)) ||
         (rc = dev_alloc(c, &s.tile_first, resort_tiles((int)N))) ||
+        (rc = dev_alloc(c, &s.tile_count, resort_tiles((int)N))) ||
         (rc = dev_alloc(c, &r.vtmp, N)) || (rc = dev_alloc(c, &r.counts, kMoverGrid * 256)) ||
         (rc = dev_alloc(c, &r.bar, 1)) || (rc = dev_alloc(c, &r.n_movers, 1)))
       return bail(rc);

@@ -73,6 +73,7 @@ struct DevState {
   int32_t* csr_idx;
   uint32_t* hist;  // the sort's histogram, cleared by the permutation kernel
   uint64_t* tile_first;  // key of every kResortTile-th sorted slot (with the ranges)
+  uint32_t* tile_count;  // per kResortTile tile, zeroed with the ranges (re-sort counters)
   // Morton blocks (square groups of 2^block_bits cells per side, a contiguous key
   // range): particle range per block, -1 when empty
   int block_bits;   // 0 when the order has no block structure (row-major)

@@ -37,17 +37,22 @@ __global__ __launch_bounds__(BLK) void k_keys(int n, DevGrid g, const float* __r
 // shifted by one lane (the last lane compares with the next wave's first key).
 // The same is done one level up for Morton blocks. The three key loads are
 // unconditional (clamped into [0, n), n >= 1) so that they issue together.
-// Also keeps the key of every kResortTile-th slot for the incremental re-sort.
+// Also keeps the key of every kResortTile-th slot, and clears the tile's counter,
+// for the incremental re-sort.
 // Returns i < n.
 __device__ __forceinline__ bool write_ranges(int i, int n, const uint64_t* __restrict__ keys,
                                              int2* __restrict__ cells, int64_t n_keys,
                                              int2* __restrict__ blocks, int block_bits,
-                                             uint64_t* __restrict__ tile_first) {
+                                             uint64_t* __restrict__ tile_first,
+                                             uint32_t* __restrict__ tile_count) {
   const bool live = i < n;
   const int ic = min(i, n - 1);
   const uint64_t kc = keys[ic], kpc = keys[max(ic - 1, 0)], knc = keys[min(ic + 1, n - 1)];
   uint64_t k = live ? kc : ~0ull;
-  if (live && i % kResortTile == 0) tile_first[i / kResortTile] = k;
+  if (live && i % kResortTile == 0) {
+    tile_first[i / kResortTile] = k;
+    tile_count[i / kResortTile] = 0;
+  }
   const int lane = threadIdx.x & (kWave - 1);
   uint64_t prev = __shfl_up(k, 1);
   if (lane == 0) prev = (i > 0 && live) ? kpc : ~k;
@@ -93,11 +98,12 @@ __global__ __launch_bounds__(BLK) void k_gather(
     float* __restrict__ dvx, float* __restrict__ dvy, uint32_t* __restrict__ dpid,
     float* __restrict__ dax, float* __restrict__ day, int2* __restrict__ cells,
     int64_t n_keys, uint32_t* __restrict__ hist, int hist_words, int2* __restrict__ blocks,
-    int block_bits, uint64_t* __restrict__ tile_first) {
+    int block_bits, uint64_t* __restrict__ tile_first, uint32_t* __restrict__ tile_count) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
-  const bool live = write_ranges(i, n, keys, cells, n_keys, blocks, block_bits, tile_first);
+  const bool live =
+      write_ranges(i, n, keys, cells, n_keys, blocks, block_bits, tile_first, tile_count);
   if (!live) return;
   const uint32_t j = min(perm[i], (uint32_t)(n - 1));  // in bounds even if perm were bad
   drx[i] = srx[j];
@@ -117,11 +123,12 @@ __global__ __launch_bounds__(BLK) void k_ranges(int n, const uint64_t* __restric
                                                 int2* __restrict__ cells, int64_t n_keys,
                                                 int2* __restrict__ blocks, int block_bits,
                                                 uint32_t* __restrict__ hist, int hist_words,
-                                                uint64_t* __restrict__ tile_first) {
+                                                uint64_t* __restrict__ tile_first,
+                                                uint32_t* __restrict__ tile_count) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
-  write_ranges(i, n, keys, cells, n_keys, blocks, block_bits, tile_first);
+  write_ranges(i, n, keys, cells, n_keys, blocks, block_bits, tile_first, tile_count);
 }
 
 // Summation density + EOS. NB: stencil traversal, CSR rows, or stencil with the
@@ -526,7 +533,7 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
     k_gather<<<blocks_for(s.n), BLK, 0, L.stream>>>(
         s.n, s.perm, s.keys_sorted, src_rx, src_ry, src_vx, src_vy, src_pid, src_ax, src_ay,
         dst_rx, dst_ry, dst_vx, dst_vy, dst_pid, dst_ax, dst_ay, s.cells, s.n_keys, s.hist,
-        hist_words, s.blocks, s.block_bits, s.tile_first);
+        hist_words, s.blocks, s.block_bits, s.tile_first, s.tile_count);
   return hipGetLastError();
 }
 
@@ -534,7 +541,7 @@ hipError_t launch_ranges(const Launch& L, const DevState& s, int hist_words) {
   if (s.n > 0)
     k_ranges<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, s.keys_sorted, s.cells, s.n_keys,
                                                     s.blocks, s.block_bits, s.hist, hist_words,
-                                                    s.tile_first);
+                                                    s.tile_first, s.tile_count);
   return hipGetLastError();
 }
 

@@ -13,10 +13,11 @@
 //   k_movers  one launch: every workgroup counts the movers (so none waits for
 //             another to decide the path); workgroup 0 writes the 256-particle
 //             tile prefixes and compacts the movers in previous-index order. Up
-//             to kSmallSortCap movers it radix-sorts them in LDS; above that the
-//             grid runs an LSD radix sort with bounded grid barriers. Ends with,
-//             per merge tile, the number of sorted movers before the tile's
-//             first particle.
+//             to kSmallSortCap movers it groups them by the merge tile they fall
+//             in and ranks them inside each group; above that the grid runs an
+//             LSD radix sort with bounded grid barriers. Either way it ends with
+//             the movers sorted by (key, previous index) and, per merge tile,
+//             the number of movers before the tile's first particle.
 //   k_merge   one workgroup per tile of the previous order: clears the tile's
 //             old cell / block ranges, places its stayers (rank among the
 //             movers of the tile's window, searched in LDS) and the movers of
@@ -70,7 +71,7 @@ constexpr int MS_WAVES = MS_TW / kWave;
 // the grid radix path reads every workgroup's 256 digit counts into the LDS of
 // the small-sort key buffers; the LDS sort gives each thread 4 elements
 static_assert(kMoverGrid * 256 <= 4 * kSmallSortCap, "digit counts exceed the LDS arrays");
-static_assert(4 * MS_TW == kSmallSortCap && MS_WAVES == 16, "LDS sort layout");
+static_assert(4 * MS_TW == kSmallSortCap, "four movers per thread in the LDS path");
 constexpr uint32_t MS_POLLS = 1u << 20;
 
 // Grid-wide barrier on a monotonic counter (zero at kernel start: k_merge
@@ -149,21 +150,6 @@ __device__ __forceinline__ void digit_excl_scan(uint32_t* v, uint32_t* scratch) 
   __syncthreads();
 }
 
-// Stable rank of this lane's digit d among the valid lanes of its wave with a
-// lower lane id and the same digit; *same gets the mask of those lanes' group.
-__device__ __forceinline__ uint32_t wave_digit_rank(uint32_t d, bool valid,
-                                                    unsigned long long* same) {
-  const int lane = threadIdx.x & (kWave - 1);
-  unsigned long long m = __ballot(valid);
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const unsigned long long bb = __ballot((d >> b) & 1u);
-    m &= ((d >> b) & 1u) ? bb : ~bb;
-  }
-  *same = m;
-  return (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-}
-
 // Number of sorted movers r in [lo, hi) with (keys[r], idx[r]) < (k, i).
 __device__ __forceinline__ int rank_in_sorted(const uint64_t* keys, const uint32_t* idx, int lo,
                                               int hi, uint64_t k, uint32_t i) {
@@ -188,6 +174,10 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
   return (uint32_t)x;
 }
+
+// Merge tiles whose movers the LDS path counts in LDS (and whose first keys it
+// searches there directly; above, every S-th one).
+constexpr int kTileTab = 4096;
 
 // Flag words per thread and per batch of the ordered compaction.
 constexpr int MS_WPT = 16;
@@ -214,8 +204,9 @@ __device__ __forceinline__ void wave_digit_prefix(uint32_t* wc, uint32_t* tot) {
 
 __global__ __launch_bounds__(MS_TW) void k_movers(
     int n, int tiles, int passes, const uint64_t* __restrict__ knew,
-    const uint64_t* __restrict__ tfirst, const unsigned long long* __restrict__ flags,
-    uint32_t* __restrict__ tile_base, uint32_t* __restrict__ tile_lo, int* __restrict__ n_movers,
+    const uint64_t* __restrict__ tfirst, uint32_t* __restrict__ tcount,
+    const unsigned long long* __restrict__ flags, uint32_t* __restrict__ tile_base,
+    uint32_t* __restrict__ tile_lo, int* __restrict__ n_movers,
     uint64_t* __restrict__ mkey, uint32_t* __restrict__ midx, uint64_t* __restrict__ ktmp,
     uint32_t* __restrict__ vtmp, uint64_t* __restrict__ skey, uint32_t* __restrict__ sidx,
     uint32_t* __restrict__ counts, uint32_t* __restrict__ bar, uint32_t* __restrict__ status) {
@@ -228,7 +219,7 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
   __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_tot[256];
   __shared__ uint32_t s_w[MS_WAVES];
-  __shared__ uint32_t s_gt[4 * 256];  // LDS sort: per digit, offsets of 4 segment groups
+  __shared__ uint32_t s_bk[kTileTab];  // movers per merge tile (up to kTileTab tiles)
   __shared__ int s_ok;
   const int tid = threadIdx.x, wave = tid / kWave;
   const int words = 4 * tiles;
@@ -305,144 +296,125 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
   if (small && blockIdx.x != 0) return;
 
   if (small) {
-    // Workgroup 0: the keys in one round of loads, then an LSD radix sort in
-    // LDS with 8-bit digits. Element e = q * MS_TW + tid (q < 4) belongs to
-    // segment (q, wave) of 64 consecutive elements; per pass each element's
-    // slot is (start of its digit) + (same digit in earlier segments) + (same
-    // digit earlier in its segment) -- stable, so ties keep their (key,
-    // previous index) order. The per-segment counts also give the digit totals.
+    // Workgroup 0. A mover belongs to the merge tile t whose first particle
+    // (tile_first[t], 256 t) is the last one ordered before it (tile 0 if
+    // none). The movers are grouped by tile -- a counter per tile gives each
+    // its slot, the counts' exclusive scan gives the tiles' window starts --
+    // and ranked by counting inside a group: the sorted order by (key,
+    // previous index) without a comparison sort of all movers.
     const int m = (int)total;
     const int mc = max(m - 1, 0);
-    // this thread's run of merge tiles and their first keys, loaded together
-    // with the movers' keys (used after the sort)
-    const int tpt = (tiles + MS_TW - 1) / MS_TW;
-    const int ta = max(1, tid * tpt), tb = min(tiles, tid * tpt + tpt);
-    uint64_t qk0[4];
+    const bool lds_count = tiles <= kTileTab;  // counters in LDS, else tcount
+    // tile-first keys of every S-th tile (all of them when tiles <= kTileTab)
+    const int S = (tiles + kTileTab - 1) / kTileTab;
+    const int ne = (tiles + S - 1) / S;
+    uint64_t* s_tab = reinterpret_cast<uint64_t*>(s_wc);
+    static_assert(kTileTab * 8 <= (int)sizeof(s_wc), "tile table exceeds the count buffer");
+    if (lds_count)
+      for (int t = tid; t < tiles; t += MS_TW) s_bk[t] = 0;
+    // the movers' keys and the table, one round of loads
+    uint32_t gi[4];
+    uint64_t gk[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) qk0[q] = tfirst[min(ta + q, tb - 1)];
+    for (int q = 0; q < 4; ++q) {
+      const int e = q * MS_TW + tid;
+      const uint32_t v = s_i[min(e, mc)];
+      gi[q] = e < m ? v : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) gk[q] = knew[gi[q]];
     {
-      uint32_t gi[4];
+      uint64_t tv[kTileTab / MS_TW];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = q * MS_TW + tid;
-        const uint32_t v = s_i[min(e, mc)];
-        gi[q] = e < m ? v : 0u;
-      }
-      uint64_t gk[4];
+      for (int q = 0; q < kTileTab / MS_TW; ++q)
+        tv[q] = tfirst[(size_t)min(q * MS_TW + tid, ne - 1) * S];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) gk[q] = knew[gi[q]];
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (q * MS_TW + tid < m) s_k[q * MS_TW + tid] = gk[q];
+      for (int q = 0; q < kTileTab / MS_TW; ++q)
+        if (q * MS_TW + tid < ne) s_tab[q * MS_TW + tid] = tv[q];
     }
-    uint16_t* s_seg = reinterpret_cast<uint16_t*>(s_wc);  // [4 * MS_WAVES][256]
-    const int lane = tid & (kWave - 1);
-    const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    int cur = 0;
-    for (int p = 0; p < passes; ++p) {
-      const int shift = 8 * p;
-      const uint64_t* kin = s_k + cur * kSmallSortCap;
-      const uint32_t* vin = s_i + cur * kSmallSortCap;
-      uint64_t* kout = s_k + (cur ^ 1) * kSmallSortCap;
-      uint32_t* vout = s_i + (cur ^ 1) * kSmallSortCap;
-      for (int q = tid; q < 2 * MS_WAVES * 256; q += MS_TW) s_wc[q] = 0;
-      __syncthreads();
-      uint64_t key[4];
-      uint32_t val[4], dig[4], wrank[4];
+    __syncthreads();
+    // owner tile and slot: table entries ordered before the mover (lower bound
+    // with a trip count set by ne alone, four movers in lockstep), then the
+    // tiles between two table entries
+    int own[4];
+    uint32_t slot[4];
+    {
+      int base[4] = {0, 0, 0, 0};
+      for (int len = ne; len > 1;) {
+        const int half = len >> 1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int mid = base[q] + half - 1;
+          if (pair_less(s_tab[mid], (uint32_t)mid * S * TW, gk[q], gi[q])) base[q] += half;
+        }
+        len -= half;
+      }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int e = q * MS_TW + tid;
-        const bool valid = e < m;
-        const uint64_t kq = kin[min(e, mc)];
-        const uint32_t vq = vin[min(e, mc)];
-        key[q] = valid ? kq : 0ull;
-        val[q] = valid ? vq : 0u;
-        dig[q] = (uint32_t)(key[q] >> shift) & 255u;
-        unsigned long long same = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-          const unsigned long long bb = __ballot((dig[q] >> b) & 1u);
-          same &= ((dig[q] >> b) & 1u) ? bb : ~bb;
-        }
-        wrank[q] = (uint32_t)__popcll(same & lt_mask);
-        if (valid && wrank[q] == 0)
-          s_seg[(q * MS_WAVES + wave) * 256 + dig[q]] = (uint16_t)__popcll(same);
-      }
-      __syncthreads();
-      // per digit d and group g of 16 segments (thread g * 256 + d): exclusive
-      // prefix inside the group (counts loaded before any is replaced)
-      {
-        const int d = tid & 255, g = tid >> 8;
-        uint32_t c[MS_WAVES];
-#pragma unroll
-        for (int k = 0; k < MS_WAVES; ++k) c[k] = s_seg[(g * MS_WAVES + k) * 256 + d];
-        uint32_t run = 0;
-#pragma unroll
-        for (int k = 0; k < MS_WAVES; ++k) {
-          s_seg[(g * MS_WAVES + k) * 256 + d] = (uint16_t)run;
-          run += c[k];
-        }
-        s_gt[g * 256 + d] = run;
-      }
-      __syncthreads();
-      // per digit: the groups' offsets and the digit total
-      if (tid < 256) {
-        uint32_t run = 0;
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const uint32_t c = s_gt[g * 256 + tid];
-          s_gt[g * 256 + tid] = run;
-          run += c;
-        }
-        s_base[tid] = run;
-      }
-      __syncthreads();
-      digit_excl_scan(s_base, s_tot);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (q * MS_TW + tid < m) {
-          const uint32_t pos = s_base[dig[q]] + s_gt[q * 256 + dig[q]] +
-                               s_seg[(q * MS_WAVES + wave) * 256 + dig[q]] + wrank[q];
-          kout[pos] = key[q];
-          vout[pos] = val[q];
-        }
-      }
-      __syncthreads();
-      cur ^= 1;
-    }
-    const uint64_t* sk = s_k + cur * kSmallSortCap;
-    const uint32_t* si = s_i + cur * kSmallSortCap;
-    for (int t = tid; t < m; t += MS_TW) {
-      skey[t] = sk[t];
-      sidx[t] = si[t];
-    }
-    // Window starts of the merge tiles: movers before each tile's first
-    // particle, non-decreasing in the tile. Each thread takes its run of tiles:
-    // a lower bound for the first (trip count set by m alone: the answer stays
-    // in [base, base + len]), then a forward walk.
-    int r = 0;
-    for (int t0 = ta; t0 < tb; t0 += 4) {
-      uint64_t qk[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) qk[q] = t0 == ta ? qk0[q] : tfirst[min(t0 + q, tb - 1)];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int t = t0 + q;
-        if (t >= tb) break;
-        const uint32_t qi = (uint32_t)t * TW;
-        if (t == ta) {
-          int base = 0;
-          for (int len = m; len > 1;) {
-            const int half = len >> 1;
-            if (pair_less(sk[base + half - 1], si[base + half - 1], qk[q], qi)) base += half;
-            len -= half;
+        if (pair_less(s_tab[base[q]], (uint32_t)base[q] * S * TW, gk[q], gi[q])) ++base[q];
+        const int c = base[q] - 1;  // last table entry before the mover, -1 if none
+        int o = max(c, 0) * S;
+        if (c >= 0)
+          for (int k = 1; k < S; ++k) {
+            const int t = c * S + k;
+            if (t < tiles && pair_less(tfirst[t], (uint32_t)t * TW, gk[q], gi[q])) o = t;
           }
-          r = (m > 0 && pair_less(sk[base], si[base], qk[q], qi)) ? base + 1 : base;
-        } else {
-          while (r < m && pair_less(sk[r], si[r], qk[q], qi)) ++r;
-        }
-        tile_lo[t] = (uint32_t)r;
+        own[q] = o;
+        slot[q] = 0;
+        if (q * MS_TW + tid < m)
+          slot[q] = lds_count ? atomicAdd(&s_bk[o], 1u) : atomicAdd(&tcount[o], 1u);
       }
+    }
+    __syncthreads();
+    // window starts: exclusive scan of the counts (in place), four tiles per
+    // thread per round
+    uint32_t carry = 0;
+    for (int t0 = 0; t0 < tiles; t0 += 4 * MS_TW) {
+      uint32_t c[4], sum = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int t = t0 + 4 * tid + k, tc = min(t, tiles - 1);
+        const uint32_t v = lds_count ? s_bk[tc] : ld_agent(&tcount[tc]);
+        c[k] = t < tiles ? v : 0u;
+        sum += c[k];
+      }
+      uint32_t rtot;
+      uint32_t run = carry + block_excl_scan(sum, s_w, &rtot);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int t = t0 + 4 * tid + k;
+        if (t < tiles) {
+          tile_lo[t] = run;
+          if (lds_count) s_bk[t] = run;
+          else tcount[t] = run;
+        }
+        run += c[k];
+      }
+      carry += rtot;
+    }
+    __syncthreads();
+    // group placement, then the rank inside the group
+    uint64_t* s_gk = s_k + kSmallSortCap;
+    uint32_t* s_gi = s_i + kSmallSortCap;
+    uint32_t st[4], en[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int o = own[q];
+      st[q] = lds_count ? s_bk[o] : ld_agent(&tcount[o]);
+      en[q] = o + 1 < tiles ? (lds_count ? s_bk[o + 1] : ld_agent(&tcount[o + 1])) : (uint32_t)m;
+      if (q * MS_TW + tid < m) {
+        s_gk[st[q] + slot[q]] = gk[q];
+        s_gi[st[q] + slot[q]] = gi[q];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q * MS_TW + tid >= m) continue;
+      uint32_t rank = st[q];
+      for (uint32_t u = st[q]; u < en[q]; ++u) rank += pair_less(s_gk[u], s_gi[u], gk[q], gi[q]);
+      skey[rank] = gk[q];
+      sidx[rank] = gi[q];
     }
     return;
   }
@@ -656,7 +628,8 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
   if (passes < 1 || passes > kMaxPasses) return hipErrorInvalidValue;
   if (!flags_ready)
     k_mover_flags<<<tiles, TW, 0, L.stream>>>(s.n, knew, s.keys_sorted, r.flags);
-  k_movers<<<kMoverGrid, MS_TW, 0, L.stream>>>(s.n, tiles, passes, knew, s.tile_first, r.flags,
+  k_movers<<<kMoverGrid, MS_TW, 0, L.stream>>>(s.n, tiles, passes, knew, s.tile_first,
+                                               s.tile_count, r.flags,
                                                r.tile_base, r.tile_lo, r.n_movers, r.mkey,
                                                r.midx, r.ktmp, r.vtmp, r.skey, r.sidx, r.counts,
                                                r.bar, r.status);
