@@ -36,6 +36,16 @@ __device__ __forceinline__ T ld4(const T* __restrict__ base, int j) {
                                      ((uint32_t)j << 2));
 }
 
+// Pins loaded values at this point. A value used only under a lane predicate
+// would otherwise have its load moved into that branch, where it is waited for
+// before the branch closes -- one load in flight at a time. After a batch of
+// loads, pinning them keeps the whole batch in flight together.
+__device__ __forceinline__ void pin(float v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void pin(int v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void pin(unsigned v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void pin(unsigned long long v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void pin(unsigned long v) { asm volatile("" ::"v"(v)); }
+
 struct DevGrid {
   float lo_x, lo_y, cs_x, cs_y;
   int32_t nx, ny;
@@ -150,8 +160,9 @@ __device__ __forceinline__ void force_pair(const DevPhys& c, float dx, float dy,
       float vr = dvx * dx + dvy * dy;
       float pij = 0.0f;
       if (vr < 0.0f) {
-        float mu = (c.h * vr) / (r2 + c.eps_h2);
-        pij = (c.neg_alpha_c0 * mu) / ((rho_i + rho_j) * 0.5f);
+        // Monaghan: -alpha c0 mu_ij / rho_ij with mu_ij = h vr / (r^2 + eps h^2),
+        // rho_ij = (rho_i + rho_j) / 2, as one quotient
+        pij = (c.neg_alpha_c0 * (c.h * vr)) / ((r2 + c.eps_h2) * ((rho_i + rho_j) * 0.5f));
       }
       float coef = ((c.c_pres * (s * s)) / r) * ((pr_i + pr_j) + pij);
       a.fpx += coef * dx;
@@ -237,10 +248,10 @@ __device__ __forceinline__ void for_each_neighbor(const DevGrid& g, const DevPhy
 
 constexpr int kEllStage = 16;  // list slots per thread staged in LDS before the store
 #ifndef SPH_CAND_BATCH
-#define SPH_CAND_BATCH 8
+#define SPH_CAND_BATCH 4
 #endif
 #ifndef SPH_LIST_BATCH
-#define SPH_LIST_BATCH 8
+#define SPH_LIST_BATCH 4
 #endif
 constexpr int kCandBatch = SPH_CAND_BATCH;  // stencil candidates loaded together (density)
 constexpr int kListBatch = SPH_LIST_BATCH;  // list slots loaded together (forces)
@@ -342,6 +353,11 @@ __device__ __forceinline__ float density_ell(const DevGrid& g, const DevPhys& c,
     }
 #pragma unroll
     for (int u = 0; u < kCandBatch; ++u) {
+      pin(xb[u]);
+      pin(yb[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kCandBatch; ++u) {
       const int j = jb[u];
       if (t0 + u >= total || j == i) continue;
       float dx, dy;
@@ -400,6 +416,15 @@ __device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
       wb[u] = ld4(vy, j);
       db[u] = ld4(rho, j);
       pb[u] = ld4(pr, j);
+    }
+#pragma unroll
+    for (int u = 0; u < kListBatch; ++u) {
+      pin(xb[u]);
+      pin(yb[u]);
+      pin(ub[u]);
+      pin(wb[u]);
+      pin(db[u]);
+      pin(pb[u]);
     }
 #pragma unroll
     for (int u = 0; u < kListBatch; ++u) {

@@ -362,16 +362,21 @@ template <int BC>
 __device__ __forceinline__ void load_edge_lane(const NodeIn& in, const NnConsts& k, int i, int deg,
                                                float xi, float yi, float vxi, float vyi,
                                                float pri, EdgeLane& el) {
-  el.j = i;
-  el.dx = el.dy = 0.0f;
-#pragma unroll
-  for (int q = 0; q < kEdgeF; ++q) el.f[q] = 0.0f;
+  // Every load is unconditional (the slot clamped into the list, the node
+  // itself standing in for a missing neighbor), so that they issue together;
+  // lanes past the degree are zeroed afterwards.
   const int lane = threadIdx.x & 63;
-  if (lane < deg) {
-    const int j = in.idx[(int64_t)lane * in.n + i];
-    el.j = j;
-    edge_feat<BC>(k, xi, yi, vxi, vyi, pri, in.rx[j], in.ry[j], in.vx[j], in.vy[j], in.rho[j],
-                  in.pr[j], el.dx, el.dy, el.f);
+  const bool ok = lane < deg;
+  const int jl = in.idx[(int64_t)min(lane, in.w - 1) * in.n + i];
+  const int j = ok ? jl : i;
+  el.j = j;
+  const float xj = in.rx[j], yj = in.ry[j], vxj = in.vx[j], vyj = in.vy[j], rhoj = in.rho[j],
+              prj = in.pr[j];
+  edge_feat<BC>(k, xi, yi, vxi, vyi, pri, xj, yj, vxj, vyj, rhoj, prj, el.dx, el.dy, el.f);
+  if (!ok) {
+    el.dx = el.dy = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kEdgeF; ++q) el.f[q] = 0.0f;
   }
 }
 
@@ -426,12 +431,18 @@ __global__ __launch_bounds__(256) void k_edge_fwd(NodeIn in, NnConsts k, const f
     const float2 p = *reinterpret_cast<const float2*>(P + (int64_t)i * W + c0);
     float s0 = 0.0f, s1 = 0.0f, tx0 = 0.0f, ty0 = 0.0f, tx1 = 0.0f, ty1 = 0.0f;
     for (int e0 = 0; e0 < deg; e0 += 8) {
+      // rows of 8 edges, loaded unconditionally (a missing edge's sender is the
+      // node itself) and pinned so that they are in flight together
       float2 qv[8];
 #pragma unroll
       for (int v = 0; v < 8; ++v) {
-        const int j = __builtin_amdgcn_readlane(el.j, e0 + v);
-        qv[v] = e0 + v < deg ? *reinterpret_cast<const float2*>(Q + (int64_t)j * W + c0)
-                             : make_float2(0.0f, 0.0f);
+        const int j = __builtin_amdgcn_readlane(el.j, min(e0 + v, 63));
+        qv[v] = *reinterpret_cast<const float2*>(Q + (int64_t)j * W + c0);
+      }
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        pin(qv[v].x);
+        pin(qv[v].y);
       }
 #pragma unroll
       for (int v = 0; v < 8; ++v) {
@@ -525,13 +536,27 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
     float rf[kEdgeF], rdx = 0.0f, rdy = 0.0f, dphi_k = 0.0f, dphi_r = 0.0f;
 #pragma unroll
     for (int q = 0; q < kEdgeF; ++q) rf[q] = 0.0f;
-    if (lane < deg) {
+    {
+      // unconditional loads (el.j is the node itself past the degree), masked
+      const bool ok = lane < deg;
       const int j = el.j;
-      edge_feat<BC>(k, in.rx[j], in.ry[j], in.vx[j], in.vy[j], in.pr[j], xk, yk, vxk, vyk, rhok,
-                    prk, rdx, rdy, rf);
+      const float xj = in.rx[j], yj = in.ry[j], vxj = in.vx[j], vyj = in.vy[j], prj = in.pr[j];
+      float gjx = 0.0f, gjy = 0.0f, gkx = 0.0f, gky = 0.0f;
       if (MODE == 1) {
-        dphi_k = sc * (G[2 * (int64_t)n0] * el.dx + G[2 * (int64_t)n0 + 1] * el.dy);
-        dphi_r = sc * (G[2 * (int64_t)j] * rdx + G[2 * (int64_t)j + 1] * rdy);
+        gkx = G[2 * (int64_t)n0];
+        gky = G[2 * (int64_t)n0 + 1];
+        gjx = G[2 * (int64_t)j];
+        gjy = G[2 * (int64_t)j + 1];
+      }
+      edge_feat<BC>(k, xj, yj, vxj, vyj, prj, xk, yk, vxk, vyk, rhok, prk, rdx, rdy, rf);
+      if (!ok) {
+        rdx = rdy = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kEdgeF; ++q) rf[q] = 0.0f;
+      }
+      if (MODE == 1 && ok) {
+        dphi_k = sc * (gkx * el.dx + gky * el.dy);
+        dphi_r = sc * (gjx * rdx + gjy * rdy);
 #pragma unroll
         for (int q = 0; q < kEdgeF; ++q) gu[q] += dphi_k * el.f[q];
         gb += dphi_k;
@@ -546,12 +571,22 @@ __global__ __launch_bounds__(256) void k_edge_bwd(
       float2 qj[8], pj[8], gj[8];
 #pragma unroll
       for (int v = 0; v < 8; ++v) {
-        const int j = __builtin_amdgcn_readlane(el.j, e0 + v);
-        const bool ok = e0 + v < deg;
+        const int j = __builtin_amdgcn_readlane(el.j, min(e0 + v, 63));
         const int64_t o = (int64_t)j * W + c0;
-        qj[v] = ok ? *reinterpret_cast<const float2*>(Q + o) : make_float2(0.0f, 0.0f);
-        pj[v] = ok ? *reinterpret_cast<const float2*>(P + o) : make_float2(0.0f, 0.0f);
-        gj[v] = (MODE == 0 && ok) ? *reinterpret_cast<const float2*>(G + o) : make_float2(0.0f, 0.0f);
+        qj[v] = *reinterpret_cast<const float2*>(Q + o);
+        pj[v] = *reinterpret_cast<const float2*>(P + o);
+        gj[v] = MODE == 0 ? *reinterpret_cast<const float2*>(G + o) : make_float2(0.0f, 0.0f);
+      }
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        pin(qj[v].x);
+        pin(qj[v].y);
+        pin(pj[v].x);
+        pin(pj[v].y);
+        if (MODE == 0) {
+          pin(gj[v].x);
+          pin(gj[v].y);
+        }
       }
 #pragma unroll
       for (int v = 0; v < 8; ++v) {
