@@ -27,7 +27,8 @@ constexpr int W = kNnWidth;
 constexpr int kNodesPerWave = 16;         // edge kernels: nodes per wavefront
 constexpr int kEdgeNodesPerBlock = 4 * kNodesPerWave;
 constexpr int kEdgePartial = W * kEdgeF + W + W + 1 + kEdgeF;  // dC, dc, dw, db, du
-constexpr int kRowsPerChunk = 1024;       // weight-gradient / bias-gradient row chunks
+constexpr int kRowsPerChunk = 256;        // weight-gradient row chunks (one partial each)
+constexpr int kColRows = 64;              // bias-gradient (column sum) row chunks
 constexpr int kMaxNnNeighbors = 32;       // neighbors per node the network reads
 
 __device__ __forceinline__ uint16_t f2bf(float x) {
@@ -221,32 +222,89 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
 }
 
 // out[m * kr + k] = sum_p partial[p][m][k] (k < kr; partial rows have kp
-// entries), fixed order over p.
+// entries). G threads per output sum the parts p = g, g + G, ... and a tree
+// over g in LDS combines them: a fixed order, so the result is reproducible.
+template <int G>
 __global__ __launch_bounds__(256) void k_reduce(int parts, int rows, int kp, int kr,
                                                 const float* __restrict__ partial,
                                                 float* __restrict__ out) {
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= rows * kr) return;
-  const int mr = idx / kr, kk = idx % kr;
-  const int64_t stride = (int64_t)rows * kp;
-  float s = 0.0f;
-  for (int p = 0; p < parts; ++p) s += partial[p * stride + (int64_t)mr * kp + kk];
-  out[idx] = s;
+  constexpr int OB = 256 / G;  // outputs per workgroup
+  __shared__ float s_acc[256];
+  const int ol = threadIdx.x % OB, g = threadIdx.x / OB;
+  const int idx = blockIdx.x * OB + ol;
+  const bool live = idx < rows * kr;
+  float acc = 0.0f;
+  if (live) {
+    const int mr = idx / kr, kk = idx % kr;
+    const int64_t stride = (int64_t)rows * kp;
+    const float* src = partial + (int64_t)mr * kp + kk;
+#pragma unroll 4
+    for (int p = g; p < parts; p += G) acc += src[p * stride];
+  }
+  s_acc[threadIdx.x] = acc;  // [g][ol]
+  __syncthreads();
+#pragma unroll
+  for (int h = G / 2; h > 0; h >>= 1) {
+    if (g < h) s_acc[threadIdx.x] += s_acc[threadIdx.x + h * OB];
+    __syncthreads();
+  }
+  if (g == 0 && live) out[idx] = s_acc[ol];
 }
 
-// Column sums of Y [n x Mo] fp32 (optionally weighted by rowscale) per row chunk.
+// Column sums of Y [n x Mo] fp32 (optionally weighted by rowscale) per chunk of
+// kColRows rows: thread t takes column t % Mo and every (256 / Mo)-th row of
+// the chunk (Mo <= 256), a fixed-order tree in LDS combines the row groups.
 __global__ __launch_bounds__(256) void k_colsum(int n, int Mo, const float* __restrict__ Y,
                                                 const float* __restrict__ rowscale,
                                                 float* __restrict__ partial) {
-  const int r_begin = blockIdx.x * kRowsPerChunk;
-  const int r_end = min(n, r_begin + kRowsPerChunk);
-  for (int c = threadIdx.x; c < Mo; c += 256) {
-    float s = 0.0f;
-    for (int r = r_begin; r < r_end; ++r) {
+  __shared__ float s_acc[256];
+  const int r_begin = blockIdx.x * kColRows;
+  const int r_end = min(n, r_begin + kColRows);
+  const int groups = 256 / Mo;
+  const int c = threadIdx.x % Mo, rg = threadIdx.x / Mo;
+  float acc = 0.0f;
+  if (rg < groups) {
+#pragma unroll 4
+    for (int r = r_begin + rg; r < r_end; r += groups) {
       const float v = Y[(int64_t)r * Mo + c];
-      s += rowscale ? rowscale[r] * v : v;
+      acc += rowscale ? rowscale[r] * v : v;
     }
-    partial[(int64_t)blockIdx.x * Mo + c] = s;
+  }
+  s_acc[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = groups / 2; h > 0; h >>= 1) {
+    if (rg < h) s_acc[threadIdx.x] += s_acc[threadIdx.x + h * Mo];
+    __syncthreads();
+  }
+  if (rg == 0) partial[(int64_t)blockIdx.x * Mo + c] = s_acc[threadIdx.x];
+}
+
+// Threads per output of k_reduce for a number of parts: about eight parts each.
+int reduce_group(int parts) {
+  int g = 1;
+  while (g < 256 && g * 8 < parts) g <<= 1;
+  return g;
+}
+
+void launch_reduce(hipStream_t st, int parts, int rows, int kp, int kr, const float* partial,
+                   float* out) {
+  const int g = reduce_group(parts);
+  const int64_t outs = (int64_t)rows * kr;
+  const int blocks = (int)((outs * g + 255) / 256);
+  switch (g) {
+#define SPH_REDUCE_CASE(G) \
+  case G: k_reduce<G><<<blocks, 256, 0, st>>>(parts, rows, kp, kr, partial, out); break;
+    SPH_REDUCE_CASE(1)
+    SPH_REDUCE_CASE(2)
+    SPH_REDUCE_CASE(4)
+    SPH_REDUCE_CASE(8)
+    SPH_REDUCE_CASE(16)
+    SPH_REDUCE_CASE(32)
+    SPH_REDUCE_CASE(64)
+    SPH_REDUCE_CASE(128)
+    SPH_REDUCE_CASE(256)
+#undef SPH_REDUCE_CASE
+    default: break;
   }
 }
 
@@ -833,15 +891,14 @@ void wgrad(Net* net, hipStream_t st, int n, const uint16_t* dY, int Mo, const ui
   const int K = K1 + K2;
   WgradArgs a{n, Mo, K1, K2, dY, X1, X2, net->partial};
   k_wgrad<<<dim3(chunks, Mo / 128, cdiv(K, 128)), 256, 0, st>>>(a);
-  k_reduce<<<cdiv((int64_t)t.rows * t.cols, 256), 256, 0, st>>>(chunks, Mo, K, t.cols,
-                                                                net->partial, G_(net, t));
+  launch_reduce(st, chunks, Mo, K, t.cols, net->partial, G_(net, t));
 }
 
 void colsum(Net* net, hipStream_t st, int n, int Mo, const float* Y, const float* rowscale,
             const NnTensor& t) {
-  const int chunks = cdiv(n, kRowsPerChunk);
+  const int chunks = cdiv(n, kColRows);
   k_colsum<<<chunks, 256, 0, st>>>(n, Mo, Y, rowscale, net->partial);
-  k_reduce<<<cdiv(Mo, 256), 256, 0, st>>>(chunks, 1, Mo, Mo, net->partial, G_(net, t));
+  launch_reduce(st, chunks, 1, Mo, Mo, net->partial, G_(net, t));
 }
 
 NodeIn node_in(const DevState& s) {
@@ -883,15 +940,13 @@ void edge_bwd(Net* net, const DevState& s, hipStream_t st, const float* P, const
                                                 net->partial);
   // partial rows: [dC (W x 8) | dc (W) | dw (W) | db | du (8)]
   const int stride = kEdgePartial;
-  k_reduce<<<cdiv(W * kEdgeF, 256), 256, 0, st>>>(blocks, 1, stride, W * kEdgeF, net->partial,
-                                                  G_(net, C));
-  k_reduce<<<1, 256, 0, st>>>(blocks, 1, stride, W, net->partial + W * kEdgeF, G_(net, c));
+  launch_reduce(st, blocks, 1, stride, W * kEdgeF, net->partial, G_(net, C));
+  launch_reduce(st, blocks, 1, stride, W, net->partial + W * kEdgeF, G_(net, c));
   if (w) {
-    k_reduce<<<1, 256, 0, st>>>(blocks, 1, stride, W, net->partial + W * kEdgeF + W, G_(net, *w));
-    k_reduce<<<1, 256, 0, st>>>(blocks, 1, stride, 1, net->partial + W * kEdgeF + 2 * W,
-                                G_(net, *b));
-    k_reduce<<<1, 256, 0, st>>>(blocks, 1, stride, kEdgeF,
-                                net->partial + W * kEdgeF + 2 * W + 1, G_(net, *u));
+    launch_reduce(st, blocks, 1, stride, W, net->partial + W * kEdgeF + W, G_(net, *w));
+    launch_reduce(st, blocks, 1, stride, 1, net->partial + W * kEdgeF + 2 * W, G_(net, *b));
+    launch_reduce(st, blocks, 1, stride, kEdgeF, net->partial + W * kEdgeF + 2 * W + 1,
+                  G_(net, *u));
   }
 }
 
@@ -964,7 +1019,8 @@ int nn_create(Net* net, int layers, bool train, int64_t n_cap, const NnConsts& k
     A(&net->da, 2 * N);
     const int64_t chunks = cdiv(N, kRowsPerChunk);
     const int64_t eb = cdiv(N, kEdgeNodesPerBlock);
-    net->partial_cap = std::max(chunks * W * 2 * W, eb * kEdgePartial);
+    net->partial_cap = std::max({(int64_t)chunks * W * 2 * W, (int64_t)eb * kEdgePartial,
+                                 (int64_t)cdiv(N, kColRows) * 2 * W});
     A(&net->partial, net->partial_cap);
   }
   net->red_cap = 2 * (int64_t)cdiv(N, 256);
