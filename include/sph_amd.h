@@ -315,6 +315,10 @@ int sph_nn_set_params(void* nn, const float* host, int64_t count);
 int sph_nn_get_params(void* nn, float* host, int64_t count);
 int sph_nn_get_grad(void* nn, float* host, int64_t count); /* last sph_nn_loss_grad */
 int sph_nn_reset_optimizer(void* nn);
+/* Global gradient-norm limit applied by sph_nn_adam before the update (the
+ * gradient is scaled to norm max_norm when it is larger; 0, the default, turns
+ * it off). The norm is computed in FP64 on the device. */
+int sph_nn_set_clip(void* nn, double max_norm);
 /* Predicted accelerations of the context's sorted particles; ax, ay (host, n
  * each, sorted order) may be NULL. Needs the fixed-width list (sph_prepare or
  * sph_density_pressure on a batch / after sph_prepare). */

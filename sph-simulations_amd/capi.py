@@ -198,6 +198,7 @@ def lib():
         "sph_nn_get_params": (i32, [vp, vp, i64]),
         "sph_nn_get_grad": (i32, [vp, vp, i64]),
         "sph_nn_reset_optimizer": (i32, [vp]),
+        "sph_nn_set_clip": (i32, [vp, d]),
         "sph_nn_forward": (i32, [vp, vp, vp, vp]),
         "sph_nn_loss_grad": (i32, [vp, vp, P(d)]),
         "sph_nn_adam": (i32, [vp, vp, d, d, d, d]),
@@ -227,7 +228,7 @@ EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_
             "sph_device_malloc", "sph_device_free", "sph_device_copy",
             "sph_upload_device", "sph_nn_create", "sph_nn_destroy", "sph_nn_last_error",
             "sph_nn_num_params", "sph_nn_set_params", "sph_nn_get_params", "sph_nn_get_grad",
-            "sph_nn_reset_optimizer", "sph_nn_forward", "sph_nn_loss_grad", "sph_nn_adam",
+            "sph_nn_reset_optimizer", "sph_nn_set_clip", "sph_nn_forward", "sph_nn_loss_grad", "sph_nn_adam",
             "sph_nn_train_step", "sph_surrogate_run")
 
 

@@ -1641,6 +1641,15 @@ int sph_nn_reset_optimizer(void* nn) {
   return SPH_OK;
 }
 
+int sph_nn_set_clip(void* nn, double max_norm) {
+  NnH* h = as_nn(nn);
+  if (!h) return SPH_ERR_INVALID;
+  if (!(max_norm >= 0) || !std::isfinite(max_norm))
+    return nn_fail(h, SPH_ERR_INVALID, "gradient-norm limit must be finite and >= 0");
+  h->net.clip = (float)max_norm;
+  return SPH_OK;
+}
+
 int sph_nn_forward(void* nn, void* ctx, float* ax, float* ay) {
   NnH* h = as_nn(nn);
   Ctx* c = as_ctx(ctx);

@@ -333,14 +333,57 @@ __global__ __launch_bounds__(256) void k_wcopy(int rows, int kr, int kp, const f
   wbt[(int64_t)kk * rows + mr] = b;
 }
 
+// Sum of squares of the gradient in FP64: per-workgroup partials (grid
+// stride), then one workgroup adds them in a fixed order.
+constexpr int kNormBlocks = 256;
+__global__ __launch_bounds__(256) void k_sq_partial(int64_t count, const float* __restrict__ g,
+                                                    double* __restrict__ part) {
+  __shared__ double s[256];
+  double a = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * 256) {
+    const double x = g[i];
+    a += x * x;
+  }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void k_sq_final(int parts, const double* __restrict__ part,
+                                                  double* __restrict__ out) {
+  __shared__ double s[256];
+  double a = 0.0;
+  for (int p = threadIdx.x; p < parts; p += 256) a += part[p];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// Adam on FP32 master weights. With g2 (the gradient's squared norm, on the
+// device) the gradient is first scaled to a global norm of at most clip.
 __global__ __launch_bounds__(256) void k_adam(int64_t count, float* __restrict__ w,
                                               const float* __restrict__ g,
                                               float* __restrict__ m, float* __restrict__ v,
                                               float lr, float b1, float b2, float eps,
-                                              float c1, float c2) {
+                                              float c1, float c2, const double* __restrict__ g2,
+                                              float clip) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= count) return;
-  const float gi = g[i];
+  float scale = 1.0f;
+  if (g2) {
+    const double nrm = sqrt(*g2);
+    if (nrm > (double)clip) scale = (float)((double)clip / nrm);
+  }
+  const float gi = g[i] * scale;
   const float mi = b1 * m[i] + (1.0f - b1) * gi;
   const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
   m[i] = mi;
@@ -1026,6 +1069,7 @@ int nn_create(Net* net, int layers, bool train, int64_t n_cap, const NnConsts& k
   net->red_cap = 2 * (int64_t)cdiv(N, 256);
   A(&net->red, net->red_cap);
   A(&net->red_out, 4);
+  if (train) A(&net->gnorm, kNormBlocks + 1);
   if (e == hipSuccess) e = hipMemset(net->m, 0, off * sizeof(float));
   if (e == hipSuccess) e = hipMemset(net->v, 0, off * sizeof(float));
   if (e == hipSuccess) e = hipMemset(net->grad, 0, off * sizeof(float));
@@ -1175,8 +1219,15 @@ hipError_t nn_adam(Net* net, hipStream_t st, float lr, float b1, float b2, float
   net->adam_t += 1;
   const float c1 = 1.0f - (float)std::pow((double)b1, net->adam_t);
   const float c2 = 1.0f - (float)std::pow((double)b2, net->adam_t);
+  const double* g2 = nullptr;
+  if (net->clip > 0.0f) {
+    k_sq_partial<<<kNormBlocks, 256, 0, st>>>(net->n_params, net->grad, net->gnorm);
+    k_sq_final<<<1, 256, 0, st>>>(kNormBlocks, net->gnorm, net->gnorm + kNormBlocks);
+    g2 = net->gnorm + kNormBlocks;
+  }
   k_adam<<<cdiv(net->n_params, 256), 256, 0, st>>>(net->n_params, net->theta, net->grad, net->m,
-                                                   net->v, lr, b1, b2, eps, c1, c2);
+                                                   net->v, lr, b1, b2, eps, c1, c2, g2,
+                                                   net->clip);
   return nn_sync_weights(net, st);
 }
 

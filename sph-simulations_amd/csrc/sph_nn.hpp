@@ -66,6 +66,8 @@ struct Net {
   double* red = nullptr;        // FP64 reduction scratch
   int64_t red_cap = 0;
   double* red_out = nullptr;    // 4 doubles
+  double* gnorm = nullptr;      // gradient-norm partials + the squared norm (train only)
+  float clip = 0.0f;            // global gradient-norm limit of Adam (0: none)
   std::vector<void*> allocs;
   std::string err;
   int device = 0;

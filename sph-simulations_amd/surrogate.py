@@ -140,6 +140,10 @@ class Network:
     def reset_optimizer(self):
         self._chk(self._L.sph_nn_reset_optimizer(self._h), "sph_nn_reset_optimizer")
 
+    def set_clip(self, max_norm: float):
+        """Global gradient-norm limit of the Adam step (0 turns it off)."""
+        self._chk(self._L.sph_nn_set_clip(self._h, float(max_norm)), "sph_nn_set_clip")
+
     def forward(self, ctx: Context):
         n = ctx.n
         ax = np.empty(n, np.float32)
@@ -348,9 +352,10 @@ class Trainer:
 
 
 def train_network(n_scenes=8, n_particles=65536, steps=400, every=10, iters=1500, lr=2e-3,
-                  lr_final=1e-4, layers=3, seed=0x5EED0003, held_out=1, log=None):
+                  lr_final=1e-4, layers=3, seed=0x5EED0003, held_out=1, log=None, clip=1.0):
     """Generate solver snapshots of `n_scenes` dam breaks, train a network on all
-    but the last `held_out` scenes and evaluate it on those. Returns (network,
+    but the last `held_out` scenes and evaluate it on those. `clip` limits the
+    gradient's global norm in each Adam step (0: no limit). Returns (network,
     report); the caller closes the network."""
     import time
 
@@ -359,6 +364,7 @@ def train_network(n_scenes=8, n_particles=65536, steps=400, every=10, iters=1500
     data = generate_dataset(scs, steps=steps, every=every)
     t_data = time.perf_counter() - t0
     net = Network(scs[0].params, n_particles, layers=layers, train=True)
+    net.set_clip(clip)
     tr = Trainer(net, scs[0].params, data, seed=seed)
     train_scenes = n_scenes - held_out
     pairs = [(t, s) for s in range(train_scenes, n_scenes)
@@ -372,7 +378,7 @@ def train_network(n_scenes=8, n_particles=65536, steps=400, every=10, iters=1500
     report = {
         "scenes": n_scenes, "particles_per_scene": n_particles, "solver_steps": steps,
         "snapshot_every": every, "snapshots": data.T * data.S, "held_out_scenes": held_out,
-        "iterations": iters, "batch_particles": n_particles, "lr": [lr, lr_final],
+        "iterations": iters, "batch_particles": n_particles, "lr": [lr, lr_final], "clip": clip,
         "data_s": round(t_data, 3), "train_s": round(t_train, 3),
         "ms_per_iteration": round(t_train * 1e3 / max(iters, 1), 3),
         "train_rms_rel_last50": float(np.mean(hist[-50:])) if hist else None,

@@ -6,7 +6,7 @@
 Defaults are the full training configuration: 64 dam-break scenes of 65,536
 particles, 2,000 plain-SPH steps each, a snapshot every 10 steps kept in device
 memory (13.4 GB), batches of one 65,536-particle snapshot, Adam from lr 2e-3
-decaying to 1e-4. The last scene is held out for evaluation. Prints one JSON
+decaying to 1e-4 with the gradient's global norm limited to 1. The last scene is held out for evaluation. Prints one JSON
 line with the training report.
 """
 import argparse
@@ -29,6 +29,7 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--lr-final", type=float, default=1e-4)
     ap.add_argument("--layers", type=int, default=3)
+    ap.add_argument("--clip", type=float, default=1.0, help="gradient-norm limit (0: none)")
     ap.add_argument("--seed", type=lambda s: int(s, 0), default=0x5EED0003)
     ap.add_argument("--out", default="")
     args = ap.parse_args()
@@ -37,7 +38,7 @@ def main():
     net, rep = sur.train_network(n_scenes=args.scenes, n_particles=args.particles,
                                  steps=args.steps, every=args.every, iters=args.iters,
                                  lr=args.lr, lr_final=args.lr_final, layers=args.layers,
-                                 seed=args.seed, log=log)
+                                 seed=args.seed, log=log, clip=args.clip)
     if args.out:
         net.save(args.out)
         rep["saved"] = args.out

@@ -281,3 +281,39 @@ def test_network_errors_and_save_load(sph, sur, tmp_path):
         with pytest.raises(sph.SphError):
             net2.loss_grad(c2)
     ctx.close()
+
+
+def test_adam_step_with_gradient_norm_limit(sph, sur):
+    """Two Adam steps on the device against the same steps on the host from the
+    device's gradients: the first without a limit, the second with the global
+    norm limited to a quarter of the gradient's. The moments then mix a full and
+    a scaled gradient, so the result shows whether (and how) the scaling ran."""
+    scene = sph.scenes.dam_break(32, 32, seed=3)
+    ctx = state_after(sph, scene, 0)
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+    with sur.Network(scene.params, scene.n, train=True, seed=5) as net:
+        theta = random_theta(sur, net, 11).astype(np.float64)
+        net.set_params(theta.astype(np.float32))
+        theta = net.params().astype(np.float64)
+        net.reset_optimizer()
+        m = np.zeros_like(theta)
+        v = np.zeros_like(theta)
+        for t, frac in ((1, 0.0), (2, 0.25)):
+            net.loss_grad(ctx)
+            g = net.grad().astype(np.float64)
+            gn = float(np.sqrt(np.sum(g * g)))
+            net.set_clip(frac * gn)
+            net.adam(ctx, lr, b1, b2, eps)
+            gs = g * (frac if frac > 0 else 1.0)
+            m = b1 * m + (1 - b1) * gs
+            v = b2 * v + (1 - b2) * gs * gs
+            theta = theta - lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+            got = net.params().astype(np.float64)
+            note(f"[adam] step {t} limit {frac} x |g| ({gn:.3e}): "
+                 f"max |device - host| {np.abs(got - theta).max():.3e}")
+            assert np.allclose(got, theta, rtol=1e-5, atol=2e-6), t
+            theta = got
+        with pytest.raises(sph.SphError):
+            net.set_clip(-1.0)
+        net.set_clip(0.0)
+    ctx.close()
