@@ -187,18 +187,21 @@ def bench_batched(sph, steps, trial_steps=10):
     }
 
 
-def bench_surrogate(sph, scene, steps, train_iters, check_every=10, threshold=0.02):
+def bench_surrogate(sph, scene, steps, train_iters, train_scenes=16, train_steps=1000,
+                    check_every=10, threshold=0.02):
     """Surrogate in the loop on the 1M-particle dam break: a network trained on
-    solver snapshots (eight 65,536-particle dam breaks, 400 steps, one held out)
-    supplies the accelerations; plain SPH is evaluated every `check_every` steps
+    solver snapshots (`train_scenes` 65,536-particle dam breaks of `train_steps`
+    steps, a snapshot every 10, the last scene held out; Adam with a gradient-norm
+    limit) supplies the accelerations; plain SPH is evaluated every `check_every` steps
     and a window whose RMS relative acceleration error exceeds `threshold` is
     rolled back and recomputed with plain SPH. Compared with a plain run of the
     same steps from the same state (throughput and drift)."""
     sur = importlib.import_module("sph-simulations_amd.surrogate")
     n = scene.n
     log(f"surrogate: training ({train_iters} iterations)")
-    trained, rep = sur.train_network(n_scenes=8, n_particles=65536, steps=400, every=10,
-                                     iters=train_iters, log=lambda m: log(f"train {m}"))
+    trained, rep = sur.train_network(n_scenes=train_scenes, n_particles=65536, steps=train_steps,
+                                     every=10, iters=train_iters,
+                                     log=lambda m: log(f"train {m}"))
     theta = trained.params()
     trained.close()
     net = sur.Network(scene.params, n, layers=3, train=False)
@@ -289,8 +292,12 @@ def main():
                     help="steps of the 256-scene batched workload (0 disables)")
     ap.add_argument("--surrogate-steps", type=int, default=200,
                     help="steps of the surrogate-in-the-loop comparison (0 disables)")
-    ap.add_argument("--train-iters", type=int, default=1500,
+    ap.add_argument("--train-iters", type=int, default=4000,
                     help="training iterations of the surrogate network")
+    ap.add_argument("--train-scenes", type=int, default=16,
+                    help="65,536-particle dam breaks generating the training snapshots")
+    ap.add_argument("--train-steps", type=int, default=1000,
+                    help="plain-SPH steps of each training scene")
     ap.add_argument("--variant", default="",
                     help="step kernel/re-sort to use without timing the others, e.g. "
                          "particle/incremental (profiling runs)")
@@ -453,7 +460,8 @@ def main():
     if args.batched_steps > 0:
         out["batched"] = bench_batched(sph, args.batched_steps)
     if args.surrogate_steps > 0:
-        out["surrogate"] = bench_surrogate(sph, scene, args.surrogate_steps, args.train_iters)
+        out["surrogate"] = bench_surrogate(sph, scene, args.surrogate_steps, args.train_iters,
+                                           args.train_scenes, args.train_steps)
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(sph, scene, args.cpu_budget)
     else:
