@@ -361,8 +361,9 @@ inline void force_pair(const Consts<R>& c, R dx, R dy, R r2, R rho_i, R rho_j, R
       R vr = dvx * dx + dvy * dy;
       R pij = (R)0;
       if (vr < (R)0) {
-        R mu = (c.h * vr) / (r2 + c.eps_h2);
-        pij = (c.neg_alpha_c0 * mu) / ((rho_i + rho_j) * (R)0.5);
+        // Monaghan: -alpha c0 mu_ij / rho_ij with mu_ij = h vr / (r^2 + eps h^2),
+        // rho_ij = (rho_i + rho_j) / 2, as one quotient
+        pij = (c.neg_alpha_c0 * (c.h * vr)) / ((r2 + c.eps_h2) * ((rho_i + rho_j) * (R)0.5));
       }
       R coef = ((c.c_pres * (s * s)) / r) * ((pr_i + pr_j) + pij);
       a.fpx += coef * dx;

@@ -32,7 +32,7 @@ __global__ __launch_bounds__(kMaxTile) void k_bdensity(
     const float* __restrict__ rx, const float* __restrict__ ry, const int2* __restrict__ cells,
     int32_t* __restrict__ ell_count, int32_t* __restrict__ ell_idx, int ell_w,
     float* __restrict__ rho_out, float* __restrict__ pr_out, float* __restrict__ p_out) {
-  extern __shared__ int s_list[];  // kEllStage x blockDim.x
+  extern __shared__ int s_list[];  // kEllLds x blockDim.x
   const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int i = t.begin + (int)threadIdx.x;
   if (i >= t.end) return;
@@ -102,7 +102,7 @@ hipError_t launch_batch_keys(const Launch& L, const BatchLaunch& B, const DevSta
 
 template <int PHYS, int BC>
 static void bdensity_go(const Launch& L, const BatchLaunch& B, const DevState& s) {
-  const size_t lds = (size_t)kEllStage * B.tile * sizeof(int);
+  const size_t lds = (size_t)kEllLds * B.tile * sizeof(int);
   k_bdensity<PHYS, BC><<<B.n_tiles, B.tile, lds, L.stream>>>(
       s.n, B.tiles, B.scenes, s.rx, s.ry, s.cells, s.ell_count, s.ell_idx, s.ell_width, s.rho,
       s.pr, s.pres);

@@ -246,7 +246,8 @@ __device__ __forceinline__ void for_each_neighbor(const DevGrid& g, const DevPhy
   }
 }
 
-constexpr int kEllStage = 16;  // list slots per thread staged in LDS before the store
+constexpr int kEllStage = 8;   // list slots per thread staged in LDS before the store
+constexpr int kEllLds = kEllStage + 9;  // LDS words per thread: staged slots + 9 cell shifts
 #ifndef SPH_CAND_BATCH
 #define SPH_CAND_BATCH 4
 #endif
@@ -307,9 +308,10 @@ __device__ __forceinline__ void stencil_ranges(const DevGrid& g, const int2* __r
 // then the neighbors in stencil order), writing the neighbor indices (self
 // excluded) to the column-major fixed-width list: slot k of i at ell_idx[k*n+i].
 // The first kEllStage slots go through stage[k * stride] (this thread's LDS
-// column) and are stored slot by slot at the end, so a wavefront's stores of one
-// slot are consecutive addresses. Returns the sum; ell_count[i] gets the count
-// (which may exceed ell_w: the force pass then walks the stencil instead).
+// column of kEllLds words) and are stored slot by slot at the end, so a
+// wavefront's stores of one slot are consecutive addresses. Returns the sum;
+// ell_count[i] gets the count (which may exceed ell_w: the force pass then walks
+// the stencil instead).
 template <int BC>
 __device__ __forceinline__ float density_ell(const DevGrid& g, const DevPhys& c,
                                              const float* __restrict__ rx,
@@ -334,41 +336,70 @@ __device__ __forceinline__ float density_ell(const DevGrid& g, const DevPhys& c,
     shift[q] = rng[q].x - total;
     total += len;
   }
+  // Cell of slot t: the number q of cells 1..8 with first[q] <= t; the slot's
+  // index is t + shift[q], the shifts in this thread's LDS column. With fewer
+  // than 128 candidates in every lane of the wave, q comes from a byte-wise
+  // compare of t with first[1..8] packed four to a word (bit 7 of each byte
+  // difference) and two popcounts; otherwise from eight compares. Slot indices
+  // are computed for every slot of a batch and masked afterwards.
+  int* shift_col = stage + kEllStage * stride;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) shift_col[q * stride] = shift[q];
+  const bool packed = __all(total < 128);
+  const uint32_t f_lo = (uint32_t)(first[1] & 255) | ((uint32_t)(first[2] & 255) << 8) |
+                        ((uint32_t)(first[3] & 255) << 16) | ((uint32_t)(first[4] & 255) << 24);
+  const uint32_t f_hi = (uint32_t)(first[5] & 255) | ((uint32_t)(first[6] & 255) << 8) |
+                        ((uint32_t)(first[7] & 255) << 16) | ((uint32_t)(first[8] & 255) << 24);
+  const int f1 = first[1], f2 = first[2], f3 = first[3], f4 = first[4], f5 = first[5],
+            f6 = first[6], f7 = first[7], f8 = first[8];
   int cnt = 0;
-  for (int t0 = 0; t0 < total; t0 += kCandBatch) {
-    int jb[kCandBatch];
-    float xb[kCandBatch], yb[kCandBatch];
+  auto walk = [&](auto cell_of) {
+    for (int t0 = 0; t0 < total; t0 += kCandBatch) {
+      int jb[kCandBatch];
+      float xb[kCandBatch], yb[kCandBatch];
+      int sh[kCandBatch];
 #pragma unroll
-    for (int u = 0; u < kCandBatch; ++u) {
-      const int t = t0 + u;
-      int s = shift[0];
+      for (int u = 0; u < kCandBatch; ++u) sh[u] = shift_col[cell_of(t0 + u) * stride];
 #pragma unroll
-      for (int q = 1; q < 9; ++q) s = t >= first[q] ? shift[q] : s;
-      jb[u] = t < total ? t + s : i;
+      for (int u = 0; u < kCandBatch; ++u) pin(sh[u]);
+#pragma unroll
+      for (int u = 0; u < kCandBatch; ++u) jb[u] = t0 + u < total ? t0 + u + sh[u] : i;
+#pragma unroll
+      for (int u = 0; u < kCandBatch; ++u) {
+        xb[u] = ld4(rx, jb[u]);
+        yb[u] = ld4(ry, jb[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kCandBatch; ++u) {
+        pin(xb[u]);
+        pin(yb[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kCandBatch; ++u) {
+        const int j = jb[u];
+        if (t0 + u >= total || j == i) continue;
+        float dx, dy;
+        separation<BC>(c, xi, yi, xb[u], yb[u], dx, dy);
+        const float r2 = dx * dx + dy * dy;
+        if (r2 > c.h2) continue;
+        acc += density_term(c.h2, r2);
+        if (cnt < kEllStage) stage[cnt * stride] = j;
+        else if (cnt < ell_w) ell_idx[(int64_t)cnt * n + i] = j;
+        ++cnt;
+      }
     }
-#pragma unroll
-    for (int u = 0; u < kCandBatch; ++u) {
-      xb[u] = ld4(rx, jb[u]);
-      yb[u] = ld4(ry, jb[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < kCandBatch; ++u) {
-      pin(xb[u]);
-      pin(yb[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < kCandBatch; ++u) {
-      const int j = jb[u];
-      if (t0 + u >= total || j == i) continue;
-      float dx, dy;
-      separation<BC>(c, xi, yi, xb[u], yb[u], dx, dy);
-      const float r2 = dx * dx + dy * dy;
-      if (r2 > c.h2) continue;
-      acc += density_term(c.h2, r2);
-      if (cnt < kEllStage) stage[cnt * stride] = j;
-      else if (cnt < ell_w) ell_idx[(int64_t)cnt * n + i] = j;
-      ++cnt;
-    }
+  };
+  if (packed) {
+    walk([&](int t) {
+      const uint32_t tb = ((uint32_t)t | 0x80u) * 0x01010101u;
+      return __builtin_popcount((tb - f_lo) & 0x80808080u) +
+             __builtin_popcount((tb - f_hi) & 0x80808080u);
+    });
+  } else {
+    walk([&](int t) {
+      return (t >= f1) + (t >= f2) + (t >= f3) + (t >= f4) + (t >= f5) + (t >= f6) + (t >= f7) +
+             (t >= f8);
+    });
   }
   ell_count[i] = cnt;
   const int staged = min(cnt, min(kEllStage, ell_w));

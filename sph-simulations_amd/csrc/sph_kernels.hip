@@ -155,7 +155,7 @@ __global__ __launch_bounds__(BLK) void k_density(
     for_each_neighbor<BC>(g, c, rx, ry, cells, n, i, xi, yi,
                           [&](int, float, float, float r2) { acc += density_term(c.h2, r2); });
   } else {
-    __shared__ int s_list[kEllStage * BLK];
+    __shared__ int s_list[kEllLds * BLK];
     acc = density_ell<BC>(g, c, rx, ry, cells, n, i, xi, yi, s_list + threadIdx.x, BLK,
                           ell_count, ell_idx, ell_w);
   }
