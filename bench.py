@@ -35,8 +35,9 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
                            flags_in_force=False):
     """Algorithmic HBM bytes per particle of one launch of each stage (FP32
     state, u64 keys, u32 indices; neighbor and halo re-reads counted as cache
-    hits). flags_in_force: the force kernel writes the re-sort's mover flags
-    (reading the previous key). None for a stage without a model."""
+    hits). flags_in_force: the force kernel reports the re-sort's movers
+    (reading the previous key; flag bits and the mover list). None for a stage
+    without a model."""
     if variant == "split_block":
         # density: R r 8, W rho,pr 8; force: R r,v,rho,pr 24, W a 8, T r,v 16, key 8
         phys = {"approximate_density": 16.0, "force_computation": 56.0}
@@ -46,9 +47,11 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
     else:
         # density: R r 8, W rho,pr 8, count 4, list 4k;
         # force: R r,v 16, rho,pr 8, count 4, list 4k, W a 8, T 16, key 8
-        # (+ R previous key 8, W one flag bit when it flags the movers)
+        # (+ R previous key 8, W one flag bit and the movers' list entries
+        # 12 f when it reports the movers)
         phys = {"approximate_density": 20.0 + 4.0 * k,
-                "force_computation": 60.0 + 4.0 * k + (8.125 if flags_in_force else 0.0)}
+                "force_computation": 60.0 + 4.0 * k
+                + ((8.125 + 12.0 * movers_frac) if flags_in_force else 0.0)}
     if resort == "full":
         sort = {
             # R new keys (8) + previous sorted keys (8); clear previous ranges (8/occ. cell)
@@ -61,9 +64,12 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
     else:
         f = movers_frac
         sort = {
-            # flags R 1/8 (+ R new and old keys 16, W flags when no kernel before
-            # wrote them); movers R key 8, W sorted key+idx 12
-            "movers": 0.125 + (0.0 if flags_in_force else 16.125) + 20.0 * f,
+            # up to 4096 movers: R tile-first keys 8 and W tile prefixes and
+            # window starts 8 per 256-particle tile; R list key+idx 12, W sorted
+            # key+idx 12 per mover (+ R new and old keys 16, W flag bit and list
+            # 12 f when no kernel before reported the movers)
+            "movers": 16.0 / 256.0 + 24.0 * f
+            + (0.0 if flags_in_force else 16.125 + 12.0 * f),
             # R old key 8, T r,v 16, pid 4; W r,v 16, pid 4, key 8, perm 4;
             # clear the old ranges 8 occ
             "merge": 60.0 + 8.0 * occ,

@@ -410,7 +410,8 @@ int fused_iteration(Ctx* c) {
     const bool inc = c->resort == SPH_RESORT_INCREMENTAL;
     {
       StageTimer t(c, SPH_T_FORCE);
-      HIPCHK(c, launch_forces(c->L, s, NB_ELL, true, c->sb.keys_in, inc ? c->rb.flags : nullptr));
+      HIPCHK(c, launch_forces(c->L, s, NB_ELL, true, c->sb.keys_in,
+                              inc ? mover_sink(c->rb) : MoverSink{}));
     }
     return resort_integrated(c, inc);
   }
@@ -544,6 +545,9 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
         (rc = dev_alloc(c, &r.skey, N)) || (rc = dev_alloc(c, &r.sidx, N)) ||
         (rc = dev_alloc(c, &r.ktmp, N)) || (rc = dev_alloc(c, &r.keys_alt, N)) ||
         (rc = dev_alloc(c, &r.flags, resort_flag_words((int)N))) ||
+        (rc = dev_alloc(c, &r.lcount, kListSegs * kListStride)) ||
+        (rc = dev_alloc(c, &r.lkey, kListSegs * kSmallSortCap)) ||
+        (rc = dev_alloc(c, &r.lidx, kListSegs * kSmallSortCap)) ||
         (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
         (rc = dev_alloc(c, &r.tile_lo, resort_tiles((int)N) + 1)) ||
         (rc = dev_alloc(c, &s.tile_first, resort_tiles((int)N))) ||
@@ -572,6 +576,8 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
   c->rb.status = b.status;
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.n_movers, 0, sizeof(int), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.bar, 0, sizeof(uint32_t), c->stream);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(c->rb.lcount, 0, kListSegs * kListStride * sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, n_keys);
   if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -966,7 +972,7 @@ int sph_forces(void* ctx, int32_t compute) {
     return fail(c, SPH_ERR_INVALID, "unknown compute type");
   }
   StageTimer t(c, SPH_T_FORCE);
-  HIPCHK(c, launch_forces(c->L, c->s, nb, false, nullptr));
+  HIPCHK(c, launch_forces(c->L, c->s, nb, false, nullptr, MoverSink{}));
   return SPH_OK;
 }
 
@@ -1763,7 +1769,7 @@ int sph_surrogate_run(void* ctx, void* nn, int32_t n_steps, int32_t check_every,
       if (q == k - 1) {
         {
           StageTimer tm(c, SPH_T_FORCE);
-          HIPCHK(c, launch_forces(c->L, s, NB_ELL, false, nullptr));
+          HIPCHK(c, launch_forces(c->L, s, NB_ELL, false, nullptr, MoverSink{}));
         }
         double sums[2] = {0, 0};
         HIPCHK(c, nn_error_sums(&h->net, s.n, s.tax, s.tay, s.ax, s.ay, c->stream, d_sums));

@@ -14,6 +14,8 @@ constexpr int kMaxPasses = 8;  // 64-bit keys, 8-bit digits
 constexpr int kSmallSortCap = 4096;  // movers sorted in LDS by one workgroup
 constexpr int kMoverGrid = 64;       // workgroups of the mover sort (all resident)
 constexpr int kResortTile = 256;     // particles per tile of the incremental re-sort
+constexpr int kListSegs = 8;         // mover list segments (one per XCD)
+constexpr int kListStride = 32;      // words between segment counters (one 128-byte line each)
 
 struct SortBuffers {
   // n passed to the launchers sizes the grids; when n_dev is set the kernels
@@ -94,12 +96,51 @@ struct ResortBuffers {
   uint32_t* counts;                // [kMoverGrid][256] digit counts
   uint32_t* bar;                   // grid barrier counter
   unsigned long long* flags;       // mover bit per particle, one word per wave
+  uint32_t* lcount;                // [kListSegs * kListStride] movers listed per segment
+  uint64_t* lkey;                  // [kListSegs][kSmallSortCap] listed movers' new keys
+  uint32_t* lidx;                  // and previous indices (any order)
   uint32_t* tile_base;             // movers before each 256-particle tile
   uint32_t* tile_lo;               // [tiles + 1] sorted movers before each tile's first particle
   int* n_movers;
   uint32_t* status;
   uint64_t* keys_alt;              // second sorted-key buffer (ping-pong)
 };
+
+// Where the producer of the new keys reports the movers: the flag words and,
+// per segment, a counter and up to kSmallSortCap (previous index, new key)
+// entries. flags == nullptr: report nothing.
+struct MoverSink {
+  unsigned long long* flags = nullptr;
+  uint32_t* count = nullptr;
+  uint64_t* key = nullptr;
+  uint32_t* idx = nullptr;
+};
+
+inline MoverSink mover_sink(const ResortBuffers& r) {
+  return MoverSink{r.flags, r.lcount, r.lkey, r.lidx};
+}
+
+// One wave's part of the report, called by all its active lanes (the wave's
+// particles are the 64 indices from a multiple of 64, its first lane active):
+// the wave's flag word, then one atomic on the segment's counter reserves the
+// slots of its movers. Entries past kSmallSortCap are not
+// written; the consumer then has more than kSmallSortCap movers in total and
+// takes them from the flag words.
+__device__ __forceinline__ void record_movers(const MoverSink& m, int i, bool moved, uint64_t kn,
+                                              int seg) {
+  const unsigned long long bal = __ballot(moved);
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  if (lane == 0) m.flags[i / kWave] = bal;
+  if (bal == 0ull) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(&m.count[seg * kListStride], (uint32_t)__popcll(bal));
+  base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+  const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+  if (moved && slot < (uint32_t)kSmallSortCap) {
+    m.key[seg * kSmallSortCap + slot] = kn;
+    m.idx[seg * kSmallSortCap + slot] = (uint32_t)i;
+  }
+}
 
 struct Launch {
   hipStream_t stream;
@@ -117,11 +158,11 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
                          float* dst_rx, float* dst_ry, float* dst_vx, float* dst_vy,
                          uint32_t* dst_pid, float* dst_ax, float* dst_ay, int hist_words);
 hipError_t launch_density(const Launch& L, const DevState& s, int nb_mode, bool write_p);
-// fused: also integrate into T and write the next keys; mover_flags (fused
-// only, may be null): one ballot word per 64 particles, bit set where the next
-// key differs from s.keys_sorted (the input of the incremental re-sort)
+// fused: also integrate into T and write the next keys; movers (fused only,
+// flags may be null): the particles whose next key differs from s.keys_sorted
+// (the input of the incremental re-sort), as flag words and the mover list
 hipError_t launch_forces(const Launch& L, const DevState& s, int nb_mode, bool fused,
-                         uint64_t* keys_out, unsigned long long* mover_flags = nullptr);
+                         uint64_t* keys_out, MoverSink movers = MoverSink{});
 hipError_t launch_integrate(const Launch& L, const DevState& s);
 // density + forces + integration + next keys over LDS-staged Morton blocks
 // (variant: enum sph_step_kernel); stats (may be null): 3 counters per kernel;
@@ -133,7 +174,7 @@ int block_step_grid(int64_t n_blocks, int cus);
 int64_t resort_flag_words(int n);
 int64_t resort_tiles(int n);
 // tile prefixes, the movers sorted into skey / sidx and the merge windows (one
-// launch; a second one first when the producer of knew wrote no mover flags)
+// launch; a second one first when the producer of knew reported no movers)
 hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers& r,
                          const uint64_t* knew, int key_bits, bool flags_ready);
 hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,

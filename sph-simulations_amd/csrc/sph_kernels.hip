@@ -178,11 +178,11 @@ __global__ __launch_bounds__(BLK) void k_forces(
     const int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ ax_out,
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
     float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out,
-    const uint64_t* __restrict__ kold, unsigned long long* __restrict__ mflags) {
+    const uint64_t* __restrict__ kold, MoverSink movers) {
   const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
   const float xi = rx[i], yi = ry[i], vxi = vx[i], vyi = vy[i], rhoi = rho[i], pri = pr[i];
-  const uint64_t ko = (FUSED && mflags) ? kold[i] : 0ull;
+  const uint64_t ko = (FUSED && movers.flags) ? kold[i] : 0ull;
   Acc a{0.f, 0.f, 0.f, 0.f};
   auto pair = [&](int j, float dx, float dy, float r2) {
     force_pair<PHYS>(c, dx, dy, r2, rhoi, rho[j], pri, pr[j], vxi, vyi, vx[j], vy[j], a);
@@ -214,12 +214,10 @@ __global__ __launch_bounds__(BLK) void k_forces(
     const uint64_t kn = cell_key(g, cell_coord(x, g.lo_x, g.cs_x, g.nx),
                                  cell_coord(y, g.lo_y, g.cs_y, g.ny));
     keys_out[i] = kn;
-    if (mflags) {
-      // particles whose key changes (tiles are BLK-aligned, so wave w of the
-      // tile covers flag word i / 64; waves past n write none)
-      const unsigned long long bal = __ballot(kn != ko);
-      if ((threadIdx.x & (kWave - 1)) == 0) mflags[i / kWave] = bal;
-    }
+    // particles whose key changes (tiles are BLK-aligned, so wave w of the
+    // tile covers flag word i / 64; waves past n write none); list segment by
+    // the XCD the hardware places the workgroup on
+    if (movers.flags) record_movers(movers, i, kn != ko, kn, blockIdx.x & (kListSegs - 1));
   }
 }
 
@@ -575,35 +573,35 @@ hipError_t launch_density(const Launch& L, const DevState& s, int nb, bool write
 
 template <int PHYS, int BC, int NB, bool FUSED>
 static hipError_t forces_go(const Launch& L, const DevState& s, uint64_t* keys_out,
-                            unsigned long long* mflags) {
+                            MoverSink movers) {
   k_forces<PHYS, BC, NB, FUSED><<<blocks_for(s.n), BLK, 0, L.stream>>>(
       s.n, L.grid, L.phys, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.csr_off, s.csr_idx,
       s.ell_count, s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out,
-      s.keys_sorted, FUSED ? mflags : nullptr);
+      s.keys_sorted, FUSED ? movers : MoverSink{});
   return hipGetLastError();
 }
 
 template <int PHYS, int BC>
 static hipError_t forces_dispatch(const Launch& L, const DevState& s, int nb, bool fused,
-                                  uint64_t* keys_out, unsigned long long* mflags) {
+                                  uint64_t* keys_out, MoverSink movers) {
   if (s.n <= 0) return hipSuccess;
   if (fused) {
-    if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, true>(L, s, keys_out, mflags);
-    if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, true>(L, s, keys_out, mflags);
-    return forces_go<PHYS, BC, NB_STENCIL, true>(L, s, keys_out, mflags);
+    if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, true>(L, s, keys_out, movers);
+    if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, true>(L, s, keys_out, movers);
+    return forces_go<PHYS, BC, NB_STENCIL, true>(L, s, keys_out, movers);
   }
-  if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, false>(L, s, keys_out, nullptr);
-  if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, false>(L, s, keys_out, nullptr);
-  return forces_go<PHYS, BC, NB_STENCIL, false>(L, s, keys_out, nullptr);
+  if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, false>(L, s, keys_out, MoverSink{});
+  if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, false>(L, s, keys_out, MoverSink{});
+  return forces_go<PHYS, BC, NB_STENCIL, false>(L, s, keys_out, MoverSink{});
 }
 
 hipError_t launch_forces(const Launch& L, const DevState& s, int nb, bool fused,
-                         uint64_t* keys_out, unsigned long long* mover_flags) {
+                         uint64_t* keys_out, MoverSink movers) {
   const int ph = L.phys.physics, bc = L.phys.boundary;
-  if (ph == 0 && bc == 0) return forces_dispatch<0, 0>(L, s, nb, fused, keys_out, mover_flags);
-  if (ph == 0 && bc == 1) return forces_dispatch<0, 1>(L, s, nb, fused, keys_out, mover_flags);
-  if (ph == 1 && bc == 0) return forces_dispatch<1, 0>(L, s, nb, fused, keys_out, mover_flags);
-  return forces_dispatch<1, 1>(L, s, nb, fused, keys_out, mover_flags);
+  if (ph == 0 && bc == 0) return forces_dispatch<0, 0>(L, s, nb, fused, keys_out, movers);
+  if (ph == 0 && bc == 1) return forces_dispatch<0, 1>(L, s, nb, fused, keys_out, movers);
+  if (ph == 1 && bc == 0) return forces_dispatch<1, 0>(L, s, nb, fused, keys_out, movers);
+  return forces_dispatch<1, 1>(L, s, nb, fused, keys_out, movers);
 }
 
 hipError_t launch_integrate(const Launch& L, const DevState& s) {

@@ -7,17 +7,22 @@
 // unchanged keys. The new order is the stable merge of the two sorted sequences
 // by (key, previous index) -- exactly the order a stable sort of all keys gives.
 //
-// Mover flags: one ballot word per 64 particles of the previous order, written
-// by the force kernel of the step (or by k_mover_flags for other producers).
+// The movers' report (record_movers): one ballot word per 64 particles of the
+// previous order, and lists of the movers' (previous index, new key) in eight
+// segments, each filled in any order through one atomic counter per wave with
+// movers; written by the force kernel of the step (or by k_mover_flags for other
+// producers) and read by k_movers; k_merge zeroes the counters.
 //
-//   k_movers  one launch: every workgroup counts the movers (so none waits for
-//             another to decide the path); workgroup 0 writes the 256-particle
-//             tile prefixes and compacts the movers in previous-index order. Up
-//             to kSmallSortCap movers it groups them by the merge tile they fall
-//             in and ranks them inside each group; above that the grid runs an
-//             LSD radix sort with bounded grid barriers. Either way it ends with
-//             the movers sorted by (key, previous index) and, per merge tile,
-//             the number of movers before the tile's first particle.
+//   k_movers  one launch: every workgroup reads the list counters (so none
+//             waits for another to decide the path). Up to kSmallSortCap movers
+//             workgroup 0 takes them from the lists, groups them by the merge
+//             tile they fall in and ranks them inside each group, counting the
+//             tile prefixes on the way; above that it compacts the movers from
+//             the flag words in previous-index order and the grid runs an LSD
+//             radix sort with bounded grid barriers. Either way it ends with
+//             the movers sorted by (key, previous index), the 256-particle tile
+//             prefixes and, per merge tile, the number of movers before the
+//             tile's first particle.
 //   k_merge   one workgroup per tile of the previous order: clears the tile's
 //             old cell / block ranges, places its stayers (rank among the
 //             movers of the tile's window, searched in LDS) and the movers of
@@ -55,14 +60,15 @@ __device__ __forceinline__ unsigned long long flag_word(const unsigned long long
   return (int64_t)w * kWave < n ? v : 0ull;
 }
 
-// Mover flags for producers of the new keys that do not write them.
+// The movers' report for producers of the new keys that do not write it.
 __global__ __launch_bounds__(TW) void k_mover_flags(int n, const uint64_t* __restrict__ knew,
                                                     const uint64_t* __restrict__ kold,
-                                                    unsigned long long* __restrict__ flags) {
+                                                    MoverSink movers) {
   const int i = blockIdx.x * TW + threadIdx.x;
   const bool live = i < n;
-  const unsigned long long bal = __ballot(live && knew[i] != kold[i]);
-  if ((threadIdx.x & (kWave - 1)) == 0) flags[i / kWave] = bal;
+  const int ic = min(i, n - 1);
+  const uint64_t kn = knew[ic], ko = kold[ic];
+  record_movers(movers, i, live && kn != ko, kn, blockIdx.x & (kListSegs - 1));
 }
 
 // ------------------------------------------------------------------ movers
@@ -205,105 +211,49 @@ __device__ __forceinline__ void wave_digit_prefix(uint32_t* wc, uint32_t* tot) {
 __global__ __launch_bounds__(MS_TW) void k_movers(
     int n, int tiles, int passes, const uint64_t* __restrict__ knew,
     const uint64_t* __restrict__ tfirst, uint32_t* __restrict__ tcount,
-    const unsigned long long* __restrict__ flags, uint32_t* __restrict__ tile_base,
-    uint32_t* __restrict__ tile_lo, int* __restrict__ n_movers,
+    const unsigned long long* __restrict__ flags, const uint32_t* __restrict__ lcount,
+    const uint64_t* __restrict__ lkey, const uint32_t* __restrict__ lidx,
+    uint32_t* __restrict__ tile_base, uint32_t* __restrict__ tile_lo, int* __restrict__ n_movers,
     uint64_t* __restrict__ mkey, uint32_t* __restrict__ midx, uint64_t* __restrict__ ktmp,
     uint32_t* __restrict__ vtmp, uint64_t* __restrict__ skey, uint32_t* __restrict__ sidx,
     uint32_t* __restrict__ counts, uint32_t* __restrict__ bar, uint32_t* __restrict__ status) {
-  // small case: two key/value buffers (ping-pong) of kSmallSortCap each
+  // key buffers: the word prefixes of the compaction and the digit counts of
+  // the grid sort; the second halves hold the movers grouped by merge tile
   __shared__ uint64_t s_k[2 * kSmallSortCap];
-  __shared__ uint32_t s_i[2 * kSmallSortCap];
-  // digit counts: u16 [4 * MS_WAVES][256] per segment in the LDS sort, u32
-  // [MS_WAVES][256] per wave in the grid sort
+  __shared__ uint32_t s_i[kSmallSortCap];
+  // digit counts: u32 [MS_WAVES][256] per wave in the grid sort; the tile-first
+  // key table of the grouping
   __shared__ uint32_t s_wc[2 * MS_WAVES * 256];
   __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_tot[256];
   __shared__ uint32_t s_w[MS_WAVES];
-  __shared__ uint32_t s_bk[kTileTab];  // movers per merge tile (up to kTileTab tiles)
+  __shared__ uint32_t s_bk[kTileTab];  // packed counters per merge tile (up to kTileTab tiles)
   __shared__ int s_ok;
   const int tid = threadIdx.x, wave = tid / kWave;
   const int words = 4 * tiles;
 
-  // 1. Workgroup 0: tile prefixes and the movers' previous indices in order
-  // (into midx, and into s_i while they fit the LDS sort). Per batch each wave
-  // takes MS_WPT * 64 consecutive words (lane l: words q * 64 + l, coalesced);
-  // the prefixes come from wave scans plus the waves' offsets. The other
-  // workgroups only count the movers, to choose the same path.
-  uint32_t total = 0;
-  if (blockIdx.x == 0) {
-    const int lane = tid & (kWave - 1);
-    // each word's prefix within its wave, [wave][q][lane] in the key buffers
-    uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_k);
-    static_assert(MS_BATCH * 4 <= (int)sizeof(s_k), "word prefixes exceed the key buffers");
-    for (int b0 = 0; b0 < words; b0 += MS_BATCH) {
-      const int wbase = b0 + wave * (MS_WPT * kWave);
-      uint32_t* pre = s_pre + wave * (MS_WPT * kWave) + lane;  // pre[q * kWave]
-      unsigned long long wd[MS_WPT];
+  // movers listed per segment by the producer (their sum is the number of set
+  // flag bits; the lists are whole when it is at most kSmallSortCap)
+  uint32_t seg_lo[kListSegs + 1];
+  seg_lo[0] = 0;
 #pragma unroll
-      for (int q = 0; q < MS_WPT; ++q) wd[q] = flag_word(flags, wbase + q * kWave + lane, n);
-      uint32_t wtot = 0;
-#pragma unroll
-      for (int q = 0; q < MS_WPT; ++q) {
-        const uint32_t c = (uint32_t)__popcll(wd[q]);
-        const uint32_t x = wave_incl_scan(c);
-        pre[q * kWave] = wtot + x - c;
-        wtot += (uint32_t)__builtin_amdgcn_readlane((int)x, kWave - 1);
-      }
-      if (lane == 0) s_w[wave] = wtot;
-      __syncthreads();
-      uint32_t woff = total, btot = 0;
-#pragma unroll
-      for (int w = 0; w < MS_WAVES; ++w) {
-        const uint32_t c = s_w[w];
-        woff += w < wave ? c : 0u;
-        btot += c;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < MS_WPT; ++q) {
-        const int w = wbase + q * kWave + lane;
-        uint32_t pos = woff + pre[q * kWave];
-        if ((w & 3) == 0 && (w >> 2) < tiles) tile_base[w >> 2] = pos;
-        unsigned long long bits = wd[q];
-        while (bits) {
-          const uint32_t i = (uint32_t)w * kWave + (uint32_t)(__ffsll((long long)bits) - 1);
-          midx[pos] = i;
-          if (pos < (uint32_t)kSmallSortCap) s_i[pos] = i;
-          ++pos;
-          bits &= bits - 1ull;
-        }
-      }
-      total += btot;
-    }
-    if (tid == 0) {
-      *n_movers = (int)total;
-      tile_lo[0] = 0;
-      tile_lo[tiles] = total;
-    }
-    __syncthreads();
-  } else {
-    uint32_t cnt = 0;
-    for (int b0 = 0; b0 < words; b0 += MS_BATCH) {
-      unsigned long long wd[MS_WPT];
-#pragma unroll
-      for (int q = 0; q < MS_WPT; ++q) wd[q] = flag_word(flags, b0 + q * MS_TW + tid, n);
-#pragma unroll
-      for (int q = 0; q < MS_WPT; ++q) cnt += (uint32_t)__popcll(wd[q]);
-    }
-    (void)block_excl_scan(cnt, s_w, &total);
-  }
+  for (int g = 0; g < kListSegs; ++g) seg_lo[g + 1] = seg_lo[g] + lcount[g * kListStride];
+  const uint32_t total = seg_lo[kListSegs];
   const bool small = total <= (uint32_t)kSmallSortCap;
   if (small && blockIdx.x != 0) return;
 
   if (small) {
     // Workgroup 0. A mover belongs to the merge tile t whose first particle
     // (tile_first[t], 256 t) is the last one ordered before it (tile 0 if
-    // none). The movers are grouped by tile -- a counter per tile gives each
-    // its slot, the counts' exclusive scan gives the tiles' window starts --
-    // and ranked by counting inside a group: the sorted order by (key,
-    // previous index) without a comparison sort of all movers.
+    // none). One counter word per tile counts, in its low half, the movers
+    // whose previous index lies in the tile and, in its high half, the movers
+    // the tile owns; the exclusive scan of the words gives the tile prefixes
+    // and the window starts at once (both at most kSmallSortCap < 2^16, so the
+    // halves never carry into each other). The movers are grouped by owner
+    // tile (the high-half counter gives the slot) and ranked by counting
+    // inside a group: the sorted order by (key, previous index) without a
+    // comparison sort, whatever the order of the lists.
     const int m = (int)total;
-    const int mc = max(m - 1, 0);
     const bool lds_count = tiles <= kTileTab;  // counters in LDS, else tcount
     // tile-first keys of every S-th tile (all of them when tiles <= kTileTab)
     const int S = (tiles + kTileTab - 1) / kTileTab;
@@ -312,17 +262,21 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
     static_assert(kTileTab * 8 <= (int)sizeof(s_wc), "tile table exceeds the count buffer");
     if (lds_count)
       for (int t = tid; t < tiles; t += MS_TW) s_bk[t] = 0;
-    // the movers' keys and the table, one round of loads
+    // the movers and the table, one round of loads (entry e of the lists is
+    // entry e - seg_lo[g] of the segment g holding it; clamped when e >= m)
     uint32_t gi[4];
     uint64_t gk[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = q * MS_TW + tid;
-      const uint32_t v = s_i[min(e, mc)];
-      gi[q] = e < m ? v : 0u;
-    }
+      const uint32_t ec = (uint32_t)min(e, max(m - 1, 0));
+      int g = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) gk[q] = knew[gi[q]];
+      for (int h = 1; h < kListSegs; ++h) g += ec >= seg_lo[h] ? 1 : 0;
+      const int at = g * kSmallSortCap + (int)(ec - seg_lo[g]);
+      gi[q] = lidx[at];
+      gk[q] = lkey[at];
+    }
     {
       uint64_t tv[kTileTab / MS_TW];
 #pragma unroll
@@ -361,13 +315,22 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
           }
         own[q] = o;
         slot[q] = 0;
-        if (q * MS_TW + tid < m)
-          slot[q] = lds_count ? atomicAdd(&s_bk[o], 1u) : atomicAdd(&tcount[o], 1u);
+        if (q * MS_TW + tid < m) {
+          const int home = (int)(gi[q] / (uint32_t)TW);
+          if (lds_count) {
+            slot[q] = atomicAdd(&s_bk[o], 1u << 16) >> 16;
+            atomicAdd(&s_bk[home], 1u);
+          } else {
+            slot[q] = atomicAdd(&tcount[o], 1u << 16) >> 16;
+            atomicAdd(&tcount[home], 1u);
+          }
+        }
       }
     }
     __syncthreads();
-    // window starts: exclusive scan of the counts (in place), four tiles per
-    // thread per round
+    // exclusive scan of the counter words (in place), four tiles per thread
+    // per round: tile prefixes from the low halves, window starts from the
+    // high halves
     uint32_t carry = 0;
     for (int t0 = 0; t0 < tiles; t0 += 4 * MS_TW) {
       uint32_t c[4], sum = 0;
@@ -384,7 +347,8 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
       for (int k = 0; k < 4; ++k) {
         const int t = t0 + 4 * tid + k;
         if (t < tiles) {
-          tile_lo[t] = run;
+          tile_base[t] = run & 0xffffu;
+          tile_lo[t] = run >> 16;
           if (lds_count) s_bk[t] = run;
           else tcount[t] = run;
         }
@@ -392,16 +356,23 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
       }
       carry += rtot;
     }
+    if (tid == 0) {
+      *n_movers = m;
+      tile_lo[tiles] = (uint32_t)m;
+    }
     __syncthreads();
     // group placement, then the rank inside the group
     uint64_t* s_gk = s_k + kSmallSortCap;
-    uint32_t* s_gi = s_i + kSmallSortCap;
+    uint32_t* s_gi = s_i;
     uint32_t st[4], en[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int o = own[q];
-      st[q] = lds_count ? s_bk[o] : ld_agent(&tcount[o]);
-      en[q] = o + 1 < tiles ? (lds_count ? s_bk[o + 1] : ld_agent(&tcount[o + 1])) : (uint32_t)m;
+      const uint32_t w0 = lds_count ? s_bk[o] : ld_agent(&tcount[o]);
+      const uint32_t w1 =
+          o + 1 < tiles ? (lds_count ? s_bk[o + 1] : ld_agent(&tcount[o + 1])) : (uint32_t)m << 16;
+      st[q] = w0 >> 16;
+      en[q] = w1 >> 16;
       if (q * MS_TW + tid < m) {
         s_gk[st[q] + slot[q]] = gk[q];
         s_gi[st[q] + slot[q]] = gi[q];
@@ -417,6 +388,62 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
       sidx[rank] = gi[q];
     }
     return;
+  }
+
+  // ---- more movers than one workgroup groups: workgroup 0 writes the tile
+  // prefixes and compacts the movers in previous-index order into midx. Per
+  // batch each wave takes MS_WPT * 64 consecutive flag words (lane l: words
+  // q * 64 + l, coalesced); the prefixes come from wave scans plus the waves'
+  // offsets.
+  if (blockIdx.x == 0) {
+    const int lane = tid & (kWave - 1);
+    // each word's prefix within its wave, [wave][q][lane] in the key buffers
+    uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_k);
+    static_assert(MS_BATCH * 4 <= (int)sizeof(s_k), "word prefixes exceed the key buffers");
+    uint32_t done = 0;
+    for (int b0 = 0; b0 < words; b0 += MS_BATCH) {
+      const int wbase = b0 + wave * (MS_WPT * kWave);
+      uint32_t* pre = s_pre + wave * (MS_WPT * kWave) + lane;  // pre[q * kWave]
+      unsigned long long wd[MS_WPT];
+#pragma unroll
+      for (int q = 0; q < MS_WPT; ++q) wd[q] = flag_word(flags, wbase + q * kWave + lane, n);
+      uint32_t wtot = 0;
+#pragma unroll
+      for (int q = 0; q < MS_WPT; ++q) {
+        const uint32_t c = (uint32_t)__popcll(wd[q]);
+        const uint32_t x = wave_incl_scan(c);
+        pre[q * kWave] = wtot + x - c;
+        wtot += (uint32_t)__builtin_amdgcn_readlane((int)x, kWave - 1);
+      }
+      if (lane == 0) s_w[wave] = wtot;
+      __syncthreads();
+      uint32_t woff = done, btot = 0;
+#pragma unroll
+      for (int w = 0; w < MS_WAVES; ++w) {
+        const uint32_t c = s_w[w];
+        woff += w < wave ? c : 0u;
+        btot += c;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < MS_WPT; ++q) {
+        const int w = wbase + q * kWave + lane;
+        uint32_t pos = woff + pre[q * kWave];
+        if ((w & 3) == 0 && (w >> 2) < tiles) tile_base[w >> 2] = pos;
+        unsigned long long bits = wd[q];
+        while (bits) {
+          midx[pos] = (uint32_t)w * kWave + (uint32_t)(__ffsll((long long)bits) - 1);
+          ++pos;
+          bits &= bits - 1ull;
+        }
+      }
+      done += btot;
+    }
+    if (tid == 0) {
+      *n_movers = (int)total;
+      tile_lo[0] = 0;
+      tile_lo[tiles] = total;
+    }
   }
 
   // ---- more movers than one workgroup sorts: LSD radix sort over the grid
@@ -524,7 +551,7 @@ __global__ __launch_bounds__(TW) void k_merge(
     float* __restrict__ ry, float* __restrict__ vx, float* __restrict__ vy,
     uint32_t* __restrict__ pid_dst, uint64_t* __restrict__ keys_dst, uint32_t* __restrict__ perm,
     int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
-    uint32_t* __restrict__ bar) {
+    uint32_t* __restrict__ bar, uint32_t* __restrict__ lcount) {
   __shared__ uint64_t s_tk[TW];  // the tile's previous keys (all ones past n)
   __shared__ uint64_t s_mk[WIN];
   __shared__ uint32_t s_mi[WIN];
@@ -544,6 +571,8 @@ __global__ __launch_bounds__(TW) void k_merge(
   const float x = tx[jc], y = ty[jc], u = tvx[jc], v = tvy[jc];
   const uint32_t id = pid_src[jc];
   if (tile == 0 && tid == 0) *bar = 0;  // the next k_movers' grid barrier
+  // the next step's mover lists (k_movers, which read the counters, has ended)
+  if (tile == 0 && tid < kListSegs) lcount[tid * kListStride] = 0;
   // the previous order's ranges (k_ranges writes the new ones)
   if (live && k != kp && k < n_keys) {
     cells[k] = make_int2(-1, -1);
@@ -627,9 +656,9 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
   const int passes = (key_bits + 7) / 8;
   if (passes < 1 || passes > kMaxPasses) return hipErrorInvalidValue;
   if (!flags_ready)
-    k_mover_flags<<<tiles, TW, 0, L.stream>>>(s.n, knew, s.keys_sorted, r.flags);
+    k_mover_flags<<<tiles, TW, 0, L.stream>>>(s.n, knew, s.keys_sorted, mover_sink(r));
   k_movers<<<kMoverGrid, MS_TW, 0, L.stream>>>(s.n, tiles, passes, knew, s.tile_first,
-                                               s.tile_count, r.flags,
+                                               s.tile_count, r.flags, r.lcount, r.lkey, r.lidx,
                                                r.tile_base, r.tile_lo, r.n_movers, r.mkey,
                                                r.midx, r.ktmp, r.vtmp, r.skey, r.sidx, r.counts,
                                                r.bar, r.status);
@@ -643,7 +672,7 @@ hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers&
   k_merge<<<tiles, TW, 0, L.stream>>>(s.n, s.keys_sorted, r.flags, r.tile_base, r.tile_lo,
                                       r.skey, r.sidx, s.tx, s.ty, s.tvx, s.tvy, s.pid, s.rx, s.ry,
                                       s.vx, s.vy, s.pid_alt, keys_dst, s.perm, s.cells, s.blocks,
-                                      s.block_bits, (uint64_t)s.n_keys, r.bar);
+                                      s.block_bits, (uint64_t)s.n_keys, r.bar, r.lcount);
   return hipGetLastError();
 }
 

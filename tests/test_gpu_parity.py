@@ -392,6 +392,35 @@ def test_capacity_and_state_errors(sph):
             ctx.step(1)
 
 
+def test_incremental_resort_listed_movers_many_tiles(sph, orc):
+    """At most 4096 movers among more than 4096 re-sort tiles (1.44 M particles,
+    5625 tiles of 256): the movers come from the force kernel's lists and the
+    per-tile counters live in global memory instead of LDS; the permutation is
+    still the oracle's stable sort and the mover count the number of changed
+    keys."""
+    s = sph.scenes.dam_break(1200, 1200, box=(4.0, 4.0), seed=5)
+    rng = np.random.default_rng(11)
+    pick = rng.choice(s.n, 1000, replace=False)  # plus about 1,700 slow movers
+    s.vx[pick] = rng.normal(0, 400.0, pick.size).astype(np.float32)  # several cells per step
+    s.vy[pick] = rng.normal(0, 400.0, pick.size).astype(np.float32)
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.prepare()
+        k_before = ctx.sorted_keys()
+        ctx.step(1)
+        after = ctx.download()
+        perm = ctx.permutation()
+        m = ctx.movers()
+    note(f"[gpu] listed movers, {s.n} particles: {m}")
+    assert 0 < m <= 4096
+    x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
+    x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
+    k_new = orc.keys(s.params, x_prev, y_prev)
+    assert m == int(np.sum(k_new != k_before))
+    operm, _ = orc.stable_sort(k_new)
+    assert np.array_equal(perm, operm)
+
+
 @pytest.mark.parametrize("side,sigma", [(100, 40.0), (512, 400.0)])
 def test_incremental_resort_many_movers(sph, orc, side, sigma):
     """More movers than one workgroup sorts in LDS: the grid-wide radix sort of
