@@ -107,17 +107,26 @@ __device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t& gen, uint3
   return *s_ok != 0;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves (no LDS
+// round trips): shifts by 1, 2, 4 and 8 lanes inside each row of 16, then row 1
+// and 3 add the last lane of the row before, then rows 2 and 3 add lane 31.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  int x = (int)v;
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return (uint32_t)x;
+}
+
 // Exclusive scan of one value per thread over the workgroup; *total gets the
 // sum. scratch: MS_WAVES words. Ends with a workgroup barrier.
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* scratch,
                                                     uint32_t* total) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  uint32_t x = v;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
+  const uint32_t x = wave_incl_scan(v);
   if (lane == kWave - 1) scratch[wave] = x;
   __syncthreads();
   uint32_t off = 0, tot = 0;
@@ -167,27 +176,13 @@ __device__ __forceinline__ int rank_in_sorted(const uint64_t* keys, const uint32
   return lo;
 }
 
-// Inclusive prefix sum over the 64 lanes of a wave with DPP moves (no LDS
-// round trips): shifts by 1, 2, 4 and 8 lanes inside each row of 16, then row 1
-// and 3 add the last lane of the row before, then rows 2 and 3 add lane 31.
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  int x = (int)v;
-  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
-  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
-  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
-  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
-  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
-  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
-  return (uint32_t)x;
-}
-
 // Merge tiles whose movers the LDS path counts in LDS (and whose first keys it
 // searches there directly; above, every S-th one).
 constexpr int kTileTab = 4096;
 
 // Flag words per thread and per batch of the ordered compaction.
-constexpr int MS_WPT = 16;
-constexpr int MS_BATCH = MS_WPT * MS_TW;  // 16384 words: 1,048,576 particles
+constexpr int MS_WPT = 8;
+constexpr int MS_BATCH = MS_WPT * MS_TW;  // 8192 words: 524,288 particles
 
 // Exclusive prefix over the waves of each digit's per-wave counts (threads
 // 0..255, one digit each; the 16 counts are loaded before any is replaced) and
@@ -208,6 +203,30 @@ __device__ __forceinline__ void wave_digit_prefix(uint32_t* wc, uint32_t* tot) {
   }
 }
 
+// List entries per segment that workgroup 0 reads before the counts are known
+// (group q of thread t: segment 2 q + t / kSpec, entry t % kSpec).
+constexpr int kSpec = kSmallSortCap / kListSegs;
+static_assert(kSpec * kListSegs == 4 * MS_TW && MS_TW == 2 * kSpec, "four list groups per thread");
+
+// (key, previous index) order of the listed movers: one word key << 32 | index
+// when the keys have at most 32 bits (one compare), else the pair.
+template <bool WIDE>
+struct Ord;
+template <>
+struct Ord<false> {
+  uint64_t w;
+  __device__ static Ord make(uint64_t k, uint32_t i) { return Ord{(k << 32) | i}; }
+  __device__ bool operator<(const Ord& o) const { return w < o.w; }
+};
+template <>
+struct Ord<true> {
+  uint64_t k;
+  uint32_t i;
+  __device__ static Ord make(uint64_t k, uint32_t i) { return Ord{k, i}; }
+  __device__ bool operator<(const Ord& o) const { return pair_less(k, i, o.k, o.i); }
+};
+
+template <bool WIDE>
 __global__ __launch_bounds__(MS_TW) void k_movers(
     int n, int tiles, int passes, const uint64_t* __restrict__ knew,
     const uint64_t* __restrict__ tfirst, uint32_t* __restrict__ tcount,
@@ -217,12 +236,13 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
     uint64_t* __restrict__ mkey, uint32_t* __restrict__ midx, uint64_t* __restrict__ ktmp,
     uint32_t* __restrict__ vtmp, uint64_t* __restrict__ skey, uint32_t* __restrict__ sidx,
     uint32_t* __restrict__ counts, uint32_t* __restrict__ bar, uint32_t* __restrict__ status) {
+  using O = Ord<WIDE>;
   // key buffers: the word prefixes of the compaction and the digit counts of
   // the grid sort; the second halves hold the movers grouped by merge tile
   __shared__ uint64_t s_k[2 * kSmallSortCap];
   __shared__ uint32_t s_i[kSmallSortCap];
   // digit counts: u32 [MS_WAVES][256] per wave in the grid sort; the tile-first
-  // key table of the grouping
+  // table of the grouping
   __shared__ uint32_t s_wc[2 * MS_WAVES * 256];
   __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_tot[256];
@@ -233,11 +253,33 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
   const int words = 4 * tiles;
 
   // movers listed per segment by the producer (their sum is the number of set
-  // flag bits; the lists are whole when it is at most kSmallSortCap)
+  // flag bits; the lists are whole when it is at most kSmallSortCap).
+  // Workgroup 0 loads in the same round the tile-first keys of every S-th tile
+  // (all of them when tiles <= kTileTab) and the first kSpec entries of every
+  // list.
+  uint32_t cnt[kListSegs];
+#pragma unroll
+  for (int g = 0; g < kListSegs; ++g) cnt[g] = lcount[g * kListStride];
+  const int S = (tiles + kTileTab - 1) / kTileTab;
+  const int ne = (tiles + S - 1) / S;
+  uint64_t tv[kTileTab / MS_TW];
+  uint32_t gi[4];
+  uint64_t gk[4];
+  if (blockIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < kTileTab / MS_TW; ++q)
+      tv[q] = tfirst[(size_t)min(q * MS_TW + tid, ne - 1) * S];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int at = (2 * q + tid / kSpec) * kSmallSortCap + tid % kSpec;
+      gi[q] = lidx[at];
+      gk[q] = lkey[at];
+    }
+  }
   uint32_t seg_lo[kListSegs + 1];
   seg_lo[0] = 0;
 #pragma unroll
-  for (int g = 0; g < kListSegs; ++g) seg_lo[g + 1] = seg_lo[g] + lcount[g * kListStride];
+  for (int g = 0; g < kListSegs; ++g) seg_lo[g + 1] = seg_lo[g] + cnt[g];
   const uint32_t total = seg_lo[kListSegs];
   const bool small = total <= (uint32_t)kSmallSortCap;
   if (small && blockIdx.x != 0) return;
@@ -255,67 +297,75 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
     // comparison sort, whatever the order of the lists.
     const int m = (int)total;
     const bool lds_count = tiles <= kTileTab;  // counters in LDS, else tcount
-    // tile-first keys of every S-th tile (all of them when tiles <= kTileTab)
-    const int S = (tiles + kTileTab - 1) / kTileTab;
-    const int ne = (tiles + S - 1) / S;
+    bool spec = true;
+#pragma unroll
+    for (int g = 0; g < kListSegs; ++g) spec = spec && cnt[g] <= (uint32_t)kSpec;
+    bool valid[4];
+    if (spec) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) valid[q] = (uint32_t)(tid % kSpec) < cnt[2 * q + tid / kSpec];
+    } else {
+      // a list longer than kSpec: entry e of the concatenated lists is entry
+      // e - seg_lo[g] of the segment g holding it (clamped when e >= m)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = q * MS_TW + tid;
+        const uint32_t ec = (uint32_t)min(e, max(m - 1, 0));
+        int g = 0;
+#pragma unroll
+        for (int h = 1; h < kListSegs; ++h) g += ec >= seg_lo[h] ? 1 : 0;
+        const int at = g * kSmallSortCap + (int)(ec - seg_lo[g]);
+        gi[q] = lidx[at];
+        gk[q] = lkey[at];
+        valid[q] = e < m;
+      }
+    }
     uint64_t* s_tab = reinterpret_cast<uint64_t*>(s_wc);
     static_assert(kTileTab * 8 <= (int)sizeof(s_wc), "tile table exceeds the count buffer");
     if (lds_count)
       for (int t = tid; t < tiles; t += MS_TW) s_bk[t] = 0;
-    // the movers and the table, one round of loads (entry e of the lists is
-    // entry e - seg_lo[g] of the segment g holding it; clamped when e >= m)
-    uint32_t gi[4];
-    uint64_t gk[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kTileTab / MS_TW; ++q) {
       const int e = q * MS_TW + tid;
-      const uint32_t ec = (uint32_t)min(e, max(m - 1, 0));
-      int g = 0;
-#pragma unroll
-      for (int h = 1; h < kListSegs; ++h) g += ec >= seg_lo[h] ? 1 : 0;
-      const int at = g * kSmallSortCap + (int)(ec - seg_lo[g]);
-      gi[q] = lidx[at];
-      gk[q] = lkey[at];
+      if (e < ne) {
+        if constexpr (WIDE) s_tab[e] = tv[q];
+        else s_tab[e] = O::make(tv[q], (uint32_t)(e * S * TW)).w;
+      }
     }
-    {
-      uint64_t tv[kTileTab / MS_TW];
-#pragma unroll
-      for (int q = 0; q < kTileTab / MS_TW; ++q)
-        tv[q] = tfirst[(size_t)min(q * MS_TW + tid, ne - 1) * S];
-#pragma unroll
-      for (int q = 0; q < kTileTab / MS_TW; ++q)
-        if (q * MS_TW + tid < ne) s_tab[q * MS_TW + tid] = tv[q];
-    }
+    auto tab = [&](int e) {
+      if constexpr (WIDE) return O::make(s_tab[e], (uint32_t)(e * S * TW));
+      else return O{s_tab[e]};
+    };
     __syncthreads();
     // owner tile and slot: table entries ordered before the mover (lower bound
-    // with a trip count set by ne alone, four movers in lockstep), then the
-    // tiles between two table entries
-    int own[4];
-    uint32_t slot[4];
-    {
+    // with a trip count set by ne alone, four movers in lockstep; waves
+    // without a mover skip it), then the tiles between two table entries
+    O mo[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) mo[q] = O::make(gk[q], gi[q]);
+    int own[4] = {0, 0, 0, 0};
+    uint32_t slot[4] = {0, 0, 0, 0};
+    if (__any(valid[0] || valid[1] || valid[2] || valid[3])) {
       int base[4] = {0, 0, 0, 0};
       for (int len = ne; len > 1;) {
         const int half = len >> 1;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int mid = base[q] + half - 1;
-          if (pair_less(s_tab[mid], (uint32_t)mid * S * TW, gk[q], gi[q])) base[q] += half;
-        }
+        for (int q = 0; q < 4; ++q)
+          if (tab(base[q] + half - 1) < mo[q]) base[q] += half;
         len -= half;
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        if (pair_less(s_tab[base[q]], (uint32_t)base[q] * S * TW, gk[q], gi[q])) ++base[q];
+        if (tab(base[q]) < mo[q]) ++base[q];
         const int c = base[q] - 1;  // last table entry before the mover, -1 if none
         int o = max(c, 0) * S;
         if (c >= 0)
           for (int k = 1; k < S; ++k) {
             const int t = c * S + k;
-            if (t < tiles && pair_less(tfirst[t], (uint32_t)t * TW, gk[q], gi[q])) o = t;
+            if (t < tiles && O::make(tfirst[t], (uint32_t)t * TW) < mo[q]) o = t;
           }
         own[q] = o;
-        slot[q] = 0;
-        if (q * MS_TW + tid < m) {
+        if (valid[q]) {
           const int home = (int)(gi[q] / (uint32_t)TW);
           if (lds_count) {
             slot[q] = atomicAdd(&s_bk[o], 1u << 16) >> 16;
@@ -330,7 +380,7 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
     __syncthreads();
     // exclusive scan of the counter words (in place), four tiles per thread
     // per round: tile prefixes from the low halves, window starts from the
-    // high halves
+    // high halves (written to global memory at the end when in LDS)
     uint32_t carry = 0;
     for (int t0 = 0; t0 < tiles; t0 += 4 * MS_TW) {
       uint32_t c[4], sum = 0;
@@ -347,18 +397,17 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
       for (int k = 0; k < 4; ++k) {
         const int t = t0 + 4 * tid + k;
         if (t < tiles) {
-          tile_base[t] = run & 0xffffu;
-          tile_lo[t] = run >> 16;
-          if (lds_count) s_bk[t] = run;
-          else tcount[t] = run;
+          if (lds_count) {
+            s_bk[t] = run;
+          } else {
+            tcount[t] = run;
+            tile_base[t] = run & 0xffffu;
+            tile_lo[t] = run >> 16;
+          }
         }
         run += c[k];
       }
       carry += rtot;
-    }
-    if (tid == 0) {
-      *n_movers = m;
-      tile_lo[tiles] = (uint32_t)m;
     }
     __syncthreads();
     // group placement, then the rank inside the group
@@ -373,19 +422,36 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
           o + 1 < tiles ? (lds_count ? s_bk[o + 1] : ld_agent(&tcount[o + 1])) : (uint32_t)m << 16;
       st[q] = w0 >> 16;
       en[q] = w1 >> 16;
-      if (q * MS_TW + tid < m) {
-        s_gk[st[q] + slot[q]] = gk[q];
-        s_gi[st[q] + slot[q]] = gi[q];
+      if (valid[q]) {
+        if constexpr (WIDE) {
+          s_gk[st[q] + slot[q]] = gk[q];
+          s_gi[st[q] + slot[q]] = gi[q];
+        } else {
+          s_gk[st[q] + slot[q]] = mo[q].w;
+        }
       }
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (q * MS_TW + tid >= m) continue;
+      if (!valid[q]) continue;
       uint32_t rank = st[q];
-      for (uint32_t u = st[q]; u < en[q]; ++u) rank += pair_less(s_gk[u], s_gi[u], gk[q], gi[q]);
+      for (uint32_t u = st[q]; u < en[q]; ++u) {
+        if constexpr (WIDE) rank += O::make(s_gk[u], s_gi[u]) < mo[q];
+        else rank += O{s_gk[u]} < mo[q];
+      }
       skey[rank] = gk[q];
       sidx[rank] = gi[q];
+    }
+    if (lds_count)
+      for (int t = tid; t < tiles; t += MS_TW) {
+        const uint32_t w = s_bk[t];
+        tile_base[t] = w & 0xffffu;
+        tile_lo[t] = w >> 16;
+      }
+    if (tid == 0) {
+      *n_movers = m;
+      tile_lo[tiles] = (uint32_t)m;
     }
     return;
   }
@@ -657,11 +723,12 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
   if (passes < 1 || passes > kMaxPasses) return hipErrorInvalidValue;
   if (!flags_ready)
     k_mover_flags<<<tiles, TW, 0, L.stream>>>(s.n, knew, s.keys_sorted, mover_sink(r));
-  k_movers<<<kMoverGrid, MS_TW, 0, L.stream>>>(s.n, tiles, passes, knew, s.tile_first,
-                                               s.tile_count, r.flags, r.lcount, r.lkey, r.lidx,
-                                               r.tile_base, r.tile_lo, r.n_movers, r.mkey,
-                                               r.midx, r.ktmp, r.vtmp, r.skey, r.sidx, r.counts,
-                                               r.bar, r.status);
+  // keys of at most 32 bits: (key, index) in one word
+  auto go = passes <= 4 ? k_movers<false> : k_movers<true>;
+  go<<<kMoverGrid, MS_TW, 0, L.stream>>>(s.n, tiles, passes, knew, s.tile_first, s.tile_count,
+                                         r.flags, r.lcount, r.lkey, r.lidx, r.tile_base,
+                                         r.tile_lo, r.n_movers, r.mkey, r.midx, r.ktmp, r.vtmp,
+                                         r.skey, r.sidx, r.counts, r.bar, r.status);
   return hipGetLastError();
 }
 
