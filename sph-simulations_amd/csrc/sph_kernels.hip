@@ -118,17 +118,48 @@ __global__ __launch_bounds__(BLK) void k_gather(
 }
 
 // Cell and block ranges of an order produced by the incremental re-sort; clears
-// the digit histogram its mover sort used.
+// the digit histogram its mover sort used. Each thread takes kRangesPer slots a
+// grid stride apart, every key it needs (its own, the one before and the one
+// after) loaded in one round: few enough waves that they all run at once.
+constexpr int kRangesPer = 4;
+
 __global__ __launch_bounds__(BLK) void k_ranges(int n, const uint64_t* __restrict__ keys,
                                                 int2* __restrict__ cells, int64_t n_keys,
                                                 int2* __restrict__ blocks, int block_bits,
                                                 uint32_t* __restrict__ hist, int hist_words,
                                                 uint64_t* __restrict__ tile_first,
                                                 uint32_t* __restrict__ tile_count) {
-  const int i = blockIdx.x * BLK + threadIdx.x;
+  const int stride = gridDim.x * BLK;
+  const int i0 = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
-  write_ranges(i, n, keys, cells, n_keys, blocks, block_bits, tile_first, tile_count);
+  uint64_t kc[kRangesPer], kp[kRangesPer], kn[kRangesPer];
+#pragma unroll
+  for (int q = 0; q < kRangesPer; ++q) {
+    const int ic = min(i0 + q * stride, n - 1);
+    kc[q] = keys[ic];
+    kp[q] = keys[max(ic - 1, 0)];
+    kn[q] = keys[min(ic + 1, n - 1)];
+  }
+  const int sh = 2 * block_bits;
+#pragma unroll
+  for (int q = 0; q < kRangesPer; ++q) {
+    const int i = i0 + q * stride;
+    if (i >= n) break;
+    const uint64_t k = kc[q];
+    if (i % kResortTile == 0) {
+      tile_first[i / kResortTile] = k;
+      tile_count[i / kResortTile] = 0;
+    }
+    if (k >= (uint64_t)n_keys) continue;
+    if (i == 0 || k != kp[q]) cells[k].x = i;
+    if (i == n - 1 || k != kn[q]) cells[k].y = i + 1;
+    if (block_bits > 0) {
+      const int bk = (int)(k >> sh);
+      if (i == 0 || (k >> sh) != (kp[q] >> sh)) blocks[bk].x = i;
+      if (i == n - 1 || (k >> sh) != (kn[q] >> sh)) blocks[bk].y = i + 1;
+    }
+  }
 }
 
 // Summation density + EOS. NB: stencil traversal, CSR rows, or stencil with the
@@ -537,9 +568,9 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
 
 hipError_t launch_ranges(const Launch& L, const DevState& s, int hist_words) {
   if (s.n > 0)
-    k_ranges<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, s.keys_sorted, s.cells, s.n_keys,
-                                                    s.blocks, s.block_bits, s.hist, hist_words,
-                                                    s.tile_first, s.tile_count);
+    k_ranges<<<(blocks_for(s.n) + kRangesPer - 1) / kRangesPer, BLK, 0, L.stream>>>(
+        s.n, s.keys_sorted, s.cells, s.n_keys, s.blocks, s.block_bits, s.hist, hist_words,
+        s.tile_first, s.tile_count);
   return hipGetLastError();
 }
 

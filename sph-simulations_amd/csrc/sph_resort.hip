@@ -622,6 +622,7 @@ __global__ __launch_bounds__(TW) void k_merge(
   __shared__ uint64_t s_mk[WIN];
   __shared__ uint32_t s_mi[WIN];
   __shared__ uint32_t s_wpre[TW / kWave + 1];  // movers of the tile before each wave
+  __shared__ unsigned long long s_fw[TW / kWave];  // the tile's flag words
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int tile = blockIdx.x;
   const int j0 = tile * TW, j = j0 + tid;
@@ -646,7 +647,10 @@ __global__ __launch_bounds__(TW) void k_merge(
       blocks[k >> (2 * block_bits)] = make_int2(-1, -1);
   }
   s_tk[tid] = k;
-  if (lane == 0) s_wpre[wave + 1] = (uint32_t)__popcll(fw);
+  if (lane == 0) {
+    s_wpre[wave + 1] = (uint32_t)__popcll(fw);
+    s_fw[wave] = fw;
+  }
   __syncthreads();
   if (tid == 0) {
     s_wpre[0] = 0;
@@ -674,6 +678,10 @@ __global__ __launch_bounds__(TW) void k_merge(
     for (int r = tid; r < len; r += TW) {
       const uint64_t km = s_mk[r];
       const uint32_t src = s_mi[r];
+      // the mover's state, in flight during the search
+      const uint32_t sc = min(src, (uint32_t)(n - 1));
+      const float mx = tx[sc], my = ty[sc], mu = tvx[sc], mv = tvy[sc];
+      const uint32_t mid_ = pid_src[sc];
       // tile particles ordered before the mover (keys past n compare greater)
       int a = 0, z = TW;
       while (a < z) {
@@ -684,14 +692,13 @@ __global__ __launch_bounds__(TW) void k_merge(
       // movers among the tile particles [j0, j0 + a)
       const int w = a / kWave, l = a % kWave;
       uint32_t mb = tb + s_wpre[w];
-      if (w < TW / kWave)
-        mb += (uint32_t)__popcll(flag_word(flags, j0 / kWave + w, n) & ((1ull << l) - 1ull));
+      if (w < TW / kWave) mb += (uint32_t)__popcll(s_fw[w] & ((1ull << l) - 1ull));
       const int pos = (c0 + r) + (j0 + a - (int)mb);
-      rx[pos] = tx[src];
-      ry[pos] = ty[src];
-      vx[pos] = tvx[src];
-      vy[pos] = tvy[src];
-      pid_dst[pos] = pid_src[src];
+      rx[pos] = mx;
+      ry[pos] = my;
+      vx[pos] = mu;
+      vy[pos] = mv;
+      pid_dst[pos] = mid_;
       keys_dst[pos] = km;
       perm[pos] = src;
     }
