@@ -109,29 +109,41 @@ struct ResortBuffers {
 // Where the producer of the new keys reports the movers: the flag words and,
 // per segment, a counter and up to kSmallSortCap (previous index, new key)
 // entries. flags == nullptr: report nothing.
+// prev: the previous re-sort's mover count; above kSmallSortCap the producer
+// lists nothing (most likely the total exceeds the cap again) and marks the
+// lists full instead (list_full), saving the atomics.
 struct MoverSink {
   unsigned long long* flags = nullptr;
   uint32_t* count = nullptr;
   uint64_t* key = nullptr;
   uint32_t* idx = nullptr;
+  const int* prev = nullptr;
 };
 
 inline MoverSink mover_sink(const ResortBuffers& r) {
-  return MoverSink{r.flags, r.lcount, r.lkey, r.lidx};
+  return MoverSink{r.flags, r.lcount, r.lkey, r.lidx, r.n_movers};
+}
+
+// Whether this launch lists the movers (uniform); if not, one thread marks the
+// lists full so that the consumer takes the movers from the flag words.
+__device__ __forceinline__ bool list_movers(const MoverSink& m) {
+  const bool listing = *m.prev <= kSmallSortCap;
+  if (!listing && blockIdx.x == 0 && threadIdx.x == 0) m.count[0] = (uint32_t)kSmallSortCap + 1u;
+  return listing;
 }
 
 // One wave's part of the report, called by all its active lanes (the wave's
 // particles are the 64 indices from a multiple of 64, its first lane active):
 // the wave's flag word, then one atomic on the segment's counter reserves the
-// slots of its movers. Entries past kSmallSortCap are not
-// written; the consumer then has more than kSmallSortCap movers in total and
-// takes them from the flag words.
+// slots of its movers. Entries past kSmallSortCap are not written; the consumer
+// then has more than kSmallSortCap movers in total and takes them from the flag
+// words.
 __device__ __forceinline__ void record_movers(const MoverSink& m, int i, bool moved, uint64_t kn,
-                                              int seg) {
+                                              int seg, bool listing) {
   const unsigned long long bal = __ballot(moved);
   const int lane = (int)(threadIdx.x & (kWave - 1));
   if (lane == 0) m.flags[i / kWave] = bal;
-  if (bal == 0ull) return;
+  if (bal == 0ull || !listing) return;
   uint32_t base = 0;
   if (lane == 0) base = atomicAdd(&m.count[seg * kListStride], (uint32_t)__popcll(bal));
   base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);

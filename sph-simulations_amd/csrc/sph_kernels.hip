@@ -248,7 +248,8 @@ __global__ __launch_bounds__(BLK) void k_forces(
     // particles whose key changes (tiles are BLK-aligned, so wave w of the
     // tile covers flag word i / 64; waves past n write none); list segment by
     // the XCD the hardware places the workgroup on
-    if (movers.flags) record_movers(movers, i, kn != ko, kn, blockIdx.x & (kListSegs - 1));
+    if (movers.flags)
+      record_movers(movers, i, kn != ko, kn, blockIdx.x & (kListSegs - 1), list_movers(movers));
   }
 }
 

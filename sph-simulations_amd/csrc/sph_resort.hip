@@ -60,15 +60,42 @@ __device__ __forceinline__ unsigned long long flag_word(const unsigned long long
   return (int64_t)w * kWave < n ? v : 0ull;
 }
 
-// The movers' report for producers of the new keys that do not write it.
-__global__ __launch_bounds__(TW) void k_mover_flags(int n, const uint64_t* __restrict__ knew,
-                                                    const uint64_t* __restrict__ kold,
-                                                    MoverSink movers) {
-  const int i = blockIdx.x * TW + threadIdx.x;
-  const bool live = i < n;
+// The movers' report for producers of the new keys that do not write it: the
+// flag words, and the lists with one atomic per workgroup of MF_TW threads (the
+// waves' counts summed in LDS), since such producers may have many movers
+// (list_movers: none at all after a step with more than kSmallSortCap).
+constexpr int MF_TW = 1024;
+__global__ __launch_bounds__(MF_TW) void k_mover_flags(int n, int words,
+                                                       const uint64_t* __restrict__ knew,
+                                                       const uint64_t* __restrict__ kold,
+                                                       MoverSink m) {
+  __shared__ uint32_t s_cnt[MF_TW / kWave];
+  __shared__ uint32_t s_base;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int i = blockIdx.x * MF_TW + tid;
   const int ic = min(i, n - 1);
   const uint64_t kn = knew[ic], ko = kold[ic];
-  record_movers(movers, i, live && kn != ko, kn, blockIdx.x & (kListSegs - 1));
+  const bool moved = i < n && kn != ko;
+  const unsigned long long bal = __ballot(moved);
+  if (lane == 0 && i / kWave < words) m.flags[i / kWave] = bal;
+  if (!list_movers(m)) return;
+  if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t tot = 0;
+    for (int w = 0; w < MF_TW / kWave; ++w) tot += s_cnt[w];
+    s_base = tot ? atomicAdd(&m.count[(blockIdx.x & (kListSegs - 1)) * kListStride], tot)
+                 : (uint32_t)kSmallSortCap;
+  }
+  __syncthreads();
+  uint32_t base = s_base;
+  for (int w = 0; w < wave; ++w) base += s_cnt[w];
+  const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+  const int seg = blockIdx.x & (kListSegs - 1);
+  if (moved && slot < (uint32_t)kSmallSortCap) {
+    m.key[seg * kSmallSortCap + slot] = kn;
+    m.idx[seg * kSmallSortCap + slot] = (uint32_t)i;
+  }
 }
 
 // ------------------------------------------------------------------ movers
@@ -181,8 +208,8 @@ __device__ __forceinline__ int rank_in_sorted(const uint64_t* keys, const uint32
 constexpr int kTileTab = 4096;
 
 // Flag words per thread and per batch of the ordered compaction.
-constexpr int MS_WPT = 8;
-constexpr int MS_BATCH = MS_WPT * MS_TW;  // 8192 words: 524,288 particles
+constexpr int MS_WPT = 12;
+constexpr int MS_BATCH = MS_WPT * MS_TW;  // 12288 words: 786,432 particles
 
 // Exclusive prefix over the waves of each digit's per-wave counts (threads
 // 0..255, one digit each; the 16 counts are loaded before any is replaced) and
@@ -200,6 +227,68 @@ __device__ __forceinline__ void wave_digit_prefix(uint32_t* wc, uint32_t* tot) {
       run += c[w];
     }
     tot[tid] = run;
+  }
+}
+
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+
+// Workgroup 0 of the radix path: tile prefixes and the movers' previous
+// indices in order, into midx. Per batch each wave takes MS_WPT * 64
+// consecutive flag words (lane l: words q * 64 + l, coalesced); the prefixes
+// come from wave scans plus the waves' offsets. s_pre: MS_BATCH words; s_w:
+// MS_WAVES words (both LDS).
+__device__ __forceinline__ void compact_movers(int n, int tiles,
+                                            const unsigned long long* __restrict__ flags,
+                                            uint32_t* __restrict__ tile_base,
+                                            uint32_t* __restrict__ tile_lo,
+                                            int* __restrict__ n_movers,
+                                            uint32_t* __restrict__ midx, lds_u32* s_pre,
+                                            lds_u32* s_w) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int words = 4 * tiles;
+  uint32_t done = 0;
+  for (int b0 = 0; b0 < words; b0 += MS_BATCH) {
+    const int wbase = b0 + wave * (MS_WPT * kWave);
+    lds_u32* pre = s_pre + wave * (MS_WPT * kWave) + lane;  // pre[q * kWave]
+    unsigned long long wd[MS_WPT];
+#pragma unroll
+    for (int q = 0; q < MS_WPT; ++q) wd[q] = flag_word(flags, wbase + q * kWave + lane, n);
+    uint32_t wtot = 0;
+#pragma unroll
+    for (int q = 0; q < MS_WPT; ++q) {
+      const uint32_t c = (uint32_t)__popcll(wd[q]);
+      const uint32_t x = wave_incl_scan(c);
+      pre[q * kWave] = wtot + x - c;
+      wtot += (uint32_t)__builtin_amdgcn_readlane((int)x, kWave - 1);
+    }
+    if (lane == 0) s_w[wave] = wtot;
+    __syncthreads();
+    uint32_t woff = done, btot = 0;
+#pragma unroll
+    for (int w = 0; w < MS_WAVES; ++w) {
+      const uint32_t c = s_w[w];
+      woff += w < wave ? c : 0u;
+      btot += c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MS_WPT; ++q) {
+      const int w = wbase + q * kWave + lane;
+      uint32_t pos = woff + pre[q * kWave];
+      if ((w & 3) == 0 && (w >> 2) < tiles) tile_base[w >> 2] = pos;
+      unsigned long long bits = wd[q];
+      while (bits) {
+        midx[pos] = (uint32_t)w * kWave + (uint32_t)(__ffsll((long long)bits) - 1);
+        ++pos;
+        bits &= bits - 1ull;
+      }
+    }
+    done += btot;
+  }
+  if (tid == 0) {
+    *n_movers = (int)done;
+    tile_lo[0] = 0;
+    tile_lo[tiles] = done;
   }
 }
 
@@ -280,8 +369,8 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
   seg_lo[0] = 0;
 #pragma unroll
   for (int g = 0; g < kListSegs; ++g) seg_lo[g + 1] = seg_lo[g] + cnt[g];
-  const uint32_t total = seg_lo[kListSegs];
-  const bool small = total <= (uint32_t)kSmallSortCap;
+  const uint32_t listed = seg_lo[kListSegs];
+  const bool small = listed <= (uint32_t)kSmallSortCap;
   if (small && blockIdx.x != 0) return;
 
   if (small) {
@@ -295,7 +384,7 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
     // tile (the high-half counter gives the slot) and ranked by counting
     // inside a group: the sorted order by (key, previous index) without a
     // comparison sort, whatever the order of the lists.
-    const int m = (int)total;
+    const int m = (int)listed;  // exact: no counter passed kSmallSortCap
     const bool lds_count = tiles <= kTileTab;  // counters in LDS, else tcount
     bool spec = true;
 #pragma unroll
@@ -457,66 +546,20 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
   }
 
   // ---- more movers than one workgroup groups: workgroup 0 writes the tile
-  // prefixes and compacts the movers in previous-index order into midx. Per
-  // batch each wave takes MS_WPT * 64 consecutive flag words (lane l: words
-  // q * 64 + l, coalesced); the prefixes come from wave scans plus the waves'
-  // offsets.
-  if (blockIdx.x == 0) {
-    const int lane = tid & (kWave - 1);
-    // each word's prefix within its wave, [wave][q][lane] in the key buffers
-    uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_k);
-    static_assert(MS_BATCH * 4 <= (int)sizeof(s_k), "word prefixes exceed the key buffers");
-    uint32_t done = 0;
-    for (int b0 = 0; b0 < words; b0 += MS_BATCH) {
-      const int wbase = b0 + wave * (MS_WPT * kWave);
-      uint32_t* pre = s_pre + wave * (MS_WPT * kWave) + lane;  // pre[q * kWave]
-      unsigned long long wd[MS_WPT];
-#pragma unroll
-      for (int q = 0; q < MS_WPT; ++q) wd[q] = flag_word(flags, wbase + q * kWave + lane, n);
-      uint32_t wtot = 0;
-#pragma unroll
-      for (int q = 0; q < MS_WPT; ++q) {
-        const uint32_t c = (uint32_t)__popcll(wd[q]);
-        const uint32_t x = wave_incl_scan(c);
-        pre[q * kWave] = wtot + x - c;
-        wtot += (uint32_t)__builtin_amdgcn_readlane((int)x, kWave - 1);
-      }
-      if (lane == 0) s_w[wave] = wtot;
-      __syncthreads();
-      uint32_t woff = done, btot = 0;
-#pragma unroll
-      for (int w = 0; w < MS_WAVES; ++w) {
-        const uint32_t c = s_w[w];
-        woff += w < wave ? c : 0u;
-        btot += c;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < MS_WPT; ++q) {
-        const int w = wbase + q * kWave + lane;
-        uint32_t pos = woff + pre[q * kWave];
-        if ((w & 3) == 0 && (w >> 2) < tiles) tile_base[w >> 2] = pos;
-        unsigned long long bits = wd[q];
-        while (bits) {
-          midx[pos] = (uint32_t)w * kWave + (uint32_t)(__ffsll((long long)bits) - 1);
-          ++pos;
-          bits &= bits - 1ull;
-        }
-      }
-      done += btot;
-    }
-    if (tid == 0) {
-      *n_movers = (int)total;
-      tile_lo[0] = 0;
-      tile_lo[tiles] = total;
-    }
-  }
+  // prefixes and compacts the movers in previous-index order into midx
+  static_assert(MS_BATCH * 4 <= (int)sizeof(s_k), "word prefixes exceed the key buffers");
+  if (blockIdx.x == 0)
+    compact_movers(n, tiles, flags, tile_base, tile_lo, n_movers, midx,
+                   (lds_u32*)(s_k), (lds_u32*)(s_w));
 
   // ---- more movers than one workgroup sorts: LSD radix sort over the grid
   // (the indices in midx from workgroup 0; the first pass reads the keys
   // through them)
   uint32_t gen = 0;
   if (!grid_barrier(bar, gen, status, &s_ok)) return;
+  // the exact number of movers, counted from the flag words by workgroup 0 (the
+  // list counters may stop short of it once past kSmallSortCap)
+  const uint32_t total = ld_agent(reinterpret_cast<const uint32_t*>(n_movers));
   uint32_t* s_cnt = s_wc;    // [MS_WAVES][256] per-wave digit counts
   uint32_t* s_off = s_base;  // [256] running digit offsets
   const int nb = gridDim.x;
@@ -729,7 +772,8 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
   const int passes = (key_bits + 7) / 8;
   if (passes < 1 || passes > kMaxPasses) return hipErrorInvalidValue;
   if (!flags_ready)
-    k_mover_flags<<<tiles, TW, 0, L.stream>>>(s.n, knew, s.keys_sorted, mover_sink(r));
+    k_mover_flags<<<(s.n + MF_TW - 1) / MF_TW, MF_TW, 0, L.stream>>>(
+        s.n, (int)resort_flag_words(s.n), knew, s.keys_sorted, mover_sink(r));
   // keys of at most 32 bits: (key, index) in one word
   auto go = passes <= 4 ? k_movers<false> : k_movers<true>;
   go<<<kMoverGrid, MS_TW, 0, L.stream>>>(s.n, tiles, passes, knew, s.tile_first, s.tile_count,

@@ -425,7 +425,8 @@ def test_incremental_resort_listed_movers_many_tiles(sph, orc):
 def test_incremental_resort_many_movers(sph, orc, side, sigma):
     """More movers than one workgroup sorts in LDS: the grid-wide radix sort of
     the mover-sort kernel takes over (one and several 1024-element sub-tiles per
-    workgroup); the permutation is still the oracle's stable sort."""
+    workgroup), two steps in a row; the permutation is still the oracle's
+    stable sort and the mover count exact."""
     s = sph.scenes.dam_break(side, side, seed=12)
     rng = np.random.default_rng(3)
     s.vx = rng.normal(0, sigma, s.n).astype(np.float32)  # ~0.5 to 5 cells per step
@@ -433,16 +434,19 @@ def test_incremental_resort_many_movers(sph, orc, side, sigma):
     with sph.Context(s.params, s.n) as ctx:
         ctx.upload(s.rx, s.ry, s.vx, s.vy)
         ctx.prepare()
-        k_before = ctx.sorted_keys()
-        ctx.step(1)
-        after = ctx.download()
-        perm = ctx.permutation()
-        m = ctx.movers()
-    note(f"[gpu] many movers: {m} of {s.n}")
-    assert m > 4096
-    x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
-    x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
-    k_new = orc.keys(s.params, x_prev, y_prev)
-    assert m == int(np.sum(k_new != k_before))
-    operm, _ = orc.stable_sort(k_new)
-    assert np.array_equal(perm, operm)
+        # the second step follows one with more than 4096 movers: the force
+        # kernel then lists none and marks the lists full
+        for step in range(2):
+            k_before = ctx.sorted_keys()
+            ctx.step(1)
+            after = ctx.download()
+            perm = ctx.permutation()
+            m = ctx.movers()
+            note(f"[gpu] many movers, step {step}: {m} of {s.n}")
+            assert m > 4096
+            x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
+            x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
+            k_new = orc.keys(s.params, x_prev, y_prev)
+            assert m == int(np.sum(k_new != k_before))
+            operm, _ = orc.stable_sort(k_new)
+            assert np.array_equal(perm, operm)
