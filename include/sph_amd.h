@@ -115,7 +115,8 @@ enum sph_stage {
   SPH_T_SORT_PASS,         /* radix sort: one onesweep pass (several per sort) */
   SPH_T_BLOCK_STEP,        /* fused density + forces + integration + keys (sph_step) */
   SPH_T_MEASURE,
-  SPH_T_MOVERS,            /* incremental re-sort: mover flags, tile prefix, mover sort */
+  SPH_T_MOVERS,            /* incremental re-sort: mover report (when the step kernel
+                              writes none), tile prefixes, movers sorted */
   SPH_T_MERGE,             /* incremental re-sort: merge movers and stayers (permutation) */
   SPH_T_COUNT
 };
@@ -197,8 +198,9 @@ int sph_snapshot_load(void* ctx, const char* path, int64_t* step);
  *   sph_prepare; sph_step(S-1); sph_density_pressure(SORTED); sph_forces(SORTED).
  * By default (SPH_STEP_PARTICLE, SPH_RESORT_INCREMENTAL) one iteration is: the
  * density kernel (density + EOS + fixed-width list), the force kernel (forces +
- * integration + next keys), the incremental re-sort (mover flags, tile scan,
- * mover compaction, mover sort, merge) and the cell-range kernel. */
+ * integration + next keys + the movers' flags and lists), the incremental
+ * re-sort (the movers grouped and ranked by merge tile, then the merge) and the
+ * cell-range kernel. */
 int sph_prepare(void* ctx);
 int sph_step(void* ctx, int32_t n_steps);
 /* Capture one fused iteration in a hipGraph and replay it in sph_step. */
