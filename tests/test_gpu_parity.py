@@ -278,6 +278,40 @@ def test_incremental_resort_movers_and_permutation(sph, orc, all_scenes, name):
     note(f"[gpu] {name}: {m} movers of {s.n}")
 
 
+@pytest.mark.parametrize("name", ["dam_break_16k", "random_clusters", "periodic_9k",
+                                  "packed_cluster"])
+def test_incremental_resort_cell_ranges(sph, orc, all_scenes, name):
+    """Cell ranges after incremental re-sorts, read straight from the table the
+    step leaves (no sort in between), against the oracle's ranges of the sorted
+    keys: every occupied cell's (start, end) and -1 for every other cell.
+    packed_cluster puts 2,600 particles in a 2h x 2h square, so one cell's run
+    spans several 256-particle merge tiles."""
+    if name == "packed_cluster":
+        s = sph.scenes.dam_break(60, 60, seed=6, name="packed")
+        rng = np.random.default_rng(8)
+        h = s.params.h
+        s.rx = (0.3 + rng.random(s.n) * 2 * h).astype(np.float32)
+        s.ry = (0.3 + rng.random(s.n) * 2 * h).astype(np.float32)
+        s.rx[:1000] = (0.5 + rng.random(1000) * 30 * h).astype(np.float32)
+    else:
+        s = all_scenes[name]
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.prepare()
+        n_keys = int(ctx.grid().n_keys)
+        for step in range(3):
+            ctx.step(1)
+            sk = ctx.sorted_keys()
+            st, en = ctx.cell_ranges()
+            ost, oen = orc.cell_ranges(sk, n_keys)
+            note(f"[gpu] {name} step {step}: {ctx.movers()} movers, "
+                 f"{int(np.sum(ost >= 0))} occupied cells, largest {int((oen - ost).max())}")
+            assert np.all(sk[1:] >= sk[:-1])
+            assert np.array_equal(st, ost) and np.array_equal(en, oen)
+            if name == "packed_cluster" and step == 0:
+                assert int((oen - ost).max()) > 256
+
+
 def test_block_stats_counts_capacity_fallbacks(sph, all_scenes):
     s = all_scenes["dense_cluster"]
     with sph.Context(s.params, s.n) as ctx:
@@ -411,8 +445,13 @@ def test_incremental_resort_listed_movers_many_tiles(sph, orc):
         after = ctx.download()
         perm = ctx.permutation()
         m = ctx.movers()
+        sk = ctx.sorted_keys()
+        st, en = ctx.cell_ranges()
+        n_keys = int(ctx.grid().n_keys)
     note(f"[gpu] listed movers, {s.n} particles: {m}")
     assert 0 < m <= 4096
+    ost, oen = orc.cell_ranges(sk, n_keys)
+    assert np.array_equal(st, ost) and np.array_equal(en, oen)
     x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
     x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
     k_new = orc.keys(s.params, x_prev, y_prev)
@@ -448,5 +487,8 @@ def test_incremental_resort_many_movers(sph, orc, side, sigma):
             x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
             k_new = orc.keys(s.params, x_prev, y_prev)
             assert m == int(np.sum(k_new != k_before))
-            operm, _ = orc.stable_sort(k_new)
+            operm, sk = orc.stable_sort(k_new)
             assert np.array_equal(perm, operm)
+            st, en = ctx.cell_ranges()
+            ost, oen = orc.cell_ranges(sk, int(ctx.grid().n_keys))
+            assert np.array_equal(st, ost) and np.array_equal(en, oen)
