@@ -2,7 +2,8 @@
 // launch per stage. Every workgroup covers one tile of one scene's contiguous
 // segment of the sorted order (the tile table is static because scene segments
 // never move), so the scene's constants are workgroup-uniform and come in
-// through scalar loads; the per-particle arithmetic is the single-scene code.
+// through scalar loads; the per-particle arithmetic is the single-scene code,
+// neighbor data gathered from the packed copies (x, y, vx, vy) and (rho, pr).
 #include <hip/hip_runtime.h>
 
 #include "sph_internal.hpp"
@@ -31,7 +32,8 @@ __global__ __launch_bounds__(kMaxTile) void k_bdensity(
     int n, const TileDev* __restrict__ tiles, const SceneDev* __restrict__ scenes,
     const float* __restrict__ rx, const float* __restrict__ ry, const int2* __restrict__ cells,
     int32_t* __restrict__ ell_count, int32_t* __restrict__ ell_idx, int ell_w,
-    float* __restrict__ rho_out, float* __restrict__ pr_out, float* __restrict__ p_out) {
+    float* __restrict__ rho_out, float* __restrict__ pr_out, float* __restrict__ p_out,
+    const float4* __restrict__ xyuv, float2* __restrict__ rp_out) {
   extern __shared__ int s_list[];  // kEllLds x blockDim.x
   const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int i = t.begin + (int)threadIdx.x;
@@ -40,14 +42,15 @@ __global__ __launch_bounds__(kMaxTile) void k_bdensity(
   const DevGrid g = S.g;
   const DevPhys c = S.c;
   const float xi = rx[i], yi = ry[i];
-  const float acc = density_ell<BC>(g, c, rx, ry, cells + S.base, n, i, xi, yi,
-                                    s_list + threadIdx.x, (int)blockDim.x, ell_count, ell_idx,
-                                    ell_w);
+  const float acc = density_ell<BC, true>(g, c, rx, ry, cells + S.base, n, i, xi, yi,
+                                          s_list + threadIdx.x, (int)blockDim.x, ell_count,
+                                          ell_idx, ell_w, xyuv);
   float rho, p, pr;
   eos<PHYS>(c, acc, rho, p, pr);
   rho_out[i] = rho;
   pr_out[i] = pr;
   if (p_out) p_out[i] = p;
+  rp_out[i] = make_float2(rho, pr);
 }
 
 template <int PHYS, int BC>
@@ -58,7 +61,8 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
     const int2* __restrict__ cells, const int32_t* __restrict__ ell_count,
     const int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ ax_out,
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
-    float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out) {
+    float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out,
+    const float4* __restrict__ xyuv, const float2* __restrict__ rp) {
   const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int i = t.begin + (int)threadIdx.x;
   if (i >= t.end) return;
@@ -67,8 +71,9 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
   const DevPhys c = S.c;
   const uint64_t base = S.base;
   const float xi = rx[i], yi = ry[i], vxi = vx[i], vyi = vy[i], rhoi = rho[i], pri = pr[i];
-  const Acc a = forces_ell<PHYS, BC>(g, c, rx, ry, vx, vy, rho, pr, cells + base, ell_count,
-                                     ell_idx, ell_w, n, i, xi, yi, vxi, vyi, rhoi, pri);
+  const Acc a = forces_ell<PHYS, BC, true>(g, c, rx, ry, vx, vy, rho, pr, cells + base,
+                                           ell_count, ell_idx, ell_w, n, i, xi, yi, vxi, vyi,
+                                           rhoi, pri, xyuv, rp);
   float axi, ayi;
   force_finish<PHYS>(c, a, axi, ayi);
   ax_out[i] = axi;
@@ -105,7 +110,7 @@ static void bdensity_go(const Launch& L, const BatchLaunch& B, const DevState& s
   const size_t lds = (size_t)kEllLds * B.tile * sizeof(int);
   k_bdensity<PHYS, BC><<<B.n_tiles, B.tile, lds, L.stream>>>(
       s.n, B.tiles, B.scenes, s.rx, s.ry, s.cells, s.ell_count, s.ell_idx, s.ell_width, s.rho,
-      s.pr, s.pres);
+      s.pr, s.pres, s.xyuv, s.rp);
 }
 
 hipError_t launch_batch_density(const Launch& L, const BatchLaunch& B, const DevState& s) {
@@ -124,7 +129,7 @@ static void bforces_go(const Launch& L, const BatchLaunch& B, const DevState& s,
                        uint64_t* keys_out) {
   k_bforces<PHYS, BC><<<B.n_tiles, B.tile, 0, L.stream>>>(
       s.n, B.tiles, B.scenes, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.ell_count,
-      s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out);
+      s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out, s.xyuv, s.rp);
 }
 
 hipError_t launch_batch_forces(const Launch& L, const BatchLaunch& B, const DevState& s,

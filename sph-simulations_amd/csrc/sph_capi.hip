@@ -401,17 +401,19 @@ int fused_iteration(Ctx* c) {
     HIPCHK(c, launch_block_step(c->L, s, c->sb.keys_in, c->block_grid, c->step_kernel, nullptr,
                                 2));
   } else {
-    // per-particle kernels: density with the fixed-width list, then forces from it
+    // per-particle kernels: density with the fixed-width list, then forces from
+    // it; neighbor data gathered from the packed copy s.xyuv (kept by the
+    // re-sorts, made by sph_step on entry) and s.rp (written by the density)
     {
       StageTimer t(c, SPH_T_DENSITY);
-      HIPCHK(c, launch_density(c->L, s, NB_ELL, false));
+      HIPCHK(c, launch_density(c->L, s, NB_ELL, false, true));
     }
     // the force kernel also flags the movers for the incremental re-sort
     const bool inc = c->resort == SPH_RESORT_INCREMENTAL;
     {
       StageTimer t(c, SPH_T_FORCE);
       HIPCHK(c, launch_forces(c->L, s, NB_ELL, true, c->sb.keys_in,
-                              inc ? mover_sink(c->rb) : MoverSink{}));
+                              inc ? mover_sink(c->rb) : MoverSink{}, true));
     }
     return resort_integrated(c, inc);
   }
@@ -519,6 +521,7 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
   for (float** a : farrs)
     if ((rc = dev_alloc(c, a, N))) return bail(rc);
   if ((rc = dev_alloc(c, &s.pid, N)) || (rc = dev_alloc(c, &s.pid_alt, N)) ||
+      (rc = dev_alloc(c, &s.xyuv, N)) || (rc = dev_alloc(c, &s.rp, N)) ||
       (rc = dev_alloc(c, &s.cells, n_keys)) ||
       (rc = dev_alloc(c, &s.ell_count, N)) ||
       (rc = dev_alloc(c, &s.ell_idx, N * s.ell_width)) ||
@@ -926,6 +929,7 @@ int sph_density_pressure(void* ctx, int32_t compute) {
     if (compute != SPH_COMPUTE_SORTED && compute != SPH_COMPUTE_LISTS)
       return fail(c, SPH_ERR_INVALID, "unknown compute type");
     StageTimer t(c, SPH_T_DENSITY);
+    HIPCHK(c, launch_pack(c->L, c->s));  // the state may have changed since the sort
     HIPCHK(c, launch_batch_density(c->L, c->B, c->s));
     c->density_ell = true;
     return SPH_OK;
@@ -956,6 +960,7 @@ int sph_forces(void* ctx, int32_t compute) {
     if (!c->density_ell) return fail(c, SPH_ERR_STATE, "call sph_density_pressure first");
     // the fused kernel also writes the integrated twin and keys into scratch
     StageTimer t(c, SPH_T_FORCE);
+    HIPCHK(c, launch_pack(c->L, c->s));  // the state may have changed since the sort
     HIPCHK(c, launch_batch_forces(c->L, c->B, c->s, c->sb.keys_in));
     return SPH_OK;
   }
@@ -1062,6 +1067,9 @@ int sph_step(void* ctx, int32_t n_steps) {
   int rc = check_n(c);
   if (rc) return rc;
   if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_prepare first");
+  // the packed copy of the sorted state the step kernels gather from (other
+  // calls may have changed the state since the last re-sort wrote it)
+  HIPCHK(c, launch_pack(c->L, c->s));
   int done = 0;
   if (c->use_graph && !c->timing) {
     if (n_steps >= 2 && !graph_matches(c) && (rc = capture_graph(c))) return rc;

@@ -36,6 +36,20 @@ __device__ __forceinline__ T ld4(const T* __restrict__ base, int j) {
                                      ((uint32_t)j << 2));
 }
 
+// 8 and 16 bytes of element j of a packed array (32-bit byte offsets, as ld4).
+__device__ __forceinline__ float2 ld8(const float2* __restrict__ base, int j) {
+  return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(base) +
+                                          ((uint32_t)j << 3));
+}
+__device__ __forceinline__ float2 ld8_of16(const float4* __restrict__ base, int j) {
+  return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(base) +
+                                          ((uint32_t)j << 4));
+}
+__device__ __forceinline__ float4 ld16(const float4* __restrict__ base, int j) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) +
+                                          ((uint32_t)j << 4));
+}
+
 // Pins loaded values at this point. A value used only under a lane predicate
 // would otherwise have its load moved into that branch, where it is waited for
 // before the branch closes -- one load in flight at a time. After a batch of
@@ -312,14 +326,15 @@ __device__ __forceinline__ void stencil_ranges(const DevGrid& g, const int2* __r
 // wavefront's stores of one slot are consecutive addresses. Returns the sum;
 // ell_count[i] gets the count (which may exceed ell_w: the force pass then walks
 // the stencil instead).
-template <int BC>
+template <int BC, bool PACKED = false>
 __device__ __forceinline__ float density_ell(const DevGrid& g, const DevPhys& c,
                                              const float* __restrict__ rx,
                                              const float* __restrict__ ry,
                                              const int2* __restrict__ cells, int n, int i,
                                              float xi, float yi, int* stage, int stride,
                                              int32_t* __restrict__ ell_count,
-                                             int32_t* __restrict__ ell_idx, int ell_w) {
+                                             int32_t* __restrict__ ell_idx, int ell_w,
+                                             const float4* __restrict__ xyuv = nullptr) {
   float acc = density_term(c.h2, 0.0f);
   int2 rng[9];
   stencil_ranges<BC>(g, cells, xi, yi, rng);
@@ -366,8 +381,14 @@ __device__ __forceinline__ float density_ell(const DevGrid& g, const DevPhys& c,
       for (int u = 0; u < kCandBatch; ++u) jb[u] = t0 + u < total ? t0 + u + sh[u] : i;
 #pragma unroll
       for (int u = 0; u < kCandBatch; ++u) {
-        xb[u] = ld4(rx, jb[u]);
-        yb[u] = ld4(ry, jb[u]);
+        if constexpr (PACKED) {
+          const float2 q = ld8_of16(xyuv, jb[u]);
+          xb[u] = q.x;
+          yb[u] = q.y;
+        } else {
+          xb[u] = ld4(rx, jb[u]);
+          yb[u] = ld4(ry, jb[u]);
+        }
       }
 #pragma unroll
       for (int u = 0; u < kCandBatch; ++u) {
@@ -409,7 +430,7 @@ __device__ __forceinline__ float density_ell(const DevGrid& g, const DevPhys& c,
 
 // Pressure (+ viscous) sum of sorted particle i over its fixed-width list, or
 // over the stencil when the list overflowed.
-template <int PHYS, int BC>
+template <int PHYS, int BC, bool PACKED = false>
 __device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
                                           const float* __restrict__ rx,
                                           const float* __restrict__ ry,
@@ -421,7 +442,9 @@ __device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
                                           const int32_t* __restrict__ ell_count,
                                           const int32_t* __restrict__ ell_idx, int ell_w, int n,
                                           int i, float xi, float yi, float vxi, float vyi,
-                                          float rhoi, float pri) {
+                                          float rhoi, float pri,
+                                          const float4* __restrict__ xyuv = nullptr,
+                                          const float2* __restrict__ rp = nullptr) {
   Acc a{0.f, 0.f, 0.f, 0.f};
   auto pair = [&](int j, float dx, float dy, float r2) {
     force_pair<PHYS>(c, dx, dy, r2, rhoi, rho[j], pri, pr[j], vxi, vyi, vx[j], vy[j], a);
@@ -441,12 +464,23 @@ __device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
 #pragma unroll
     for (int u = 0; u < kListBatch; ++u) {
       const int j = k0 + u < cnt ? jl[u] : i;
-      xb[u] = ld4(rx, j);
-      yb[u] = ld4(ry, j);
-      ub[u] = ld4(vx, j);
-      wb[u] = ld4(vy, j);
-      db[u] = ld4(rho, j);
-      pb[u] = ld4(pr, j);
+      if constexpr (PACKED) {
+        const float4 q = ld16(xyuv, j);
+        const float2 r = ld8(rp, j);
+        xb[u] = q.x;
+        yb[u] = q.y;
+        ub[u] = q.z;
+        wb[u] = q.w;
+        db[u] = r.x;
+        pb[u] = r.y;
+      } else {
+        xb[u] = ld4(rx, j);
+        yb[u] = ld4(ry, j);
+        ub[u] = ld4(vx, j);
+        wb[u] = ld4(vy, j);
+        db[u] = ld4(rho, j);
+        pb[u] = ld4(pr, j);
+      }
     }
 #pragma unroll
     for (int u = 0; u < kListBatch; ++u) {

@@ -63,6 +63,8 @@ struct DevState {
   float *tx, *ty, *tvx, *tvy;
   float *ax, *ay, *tax, *tay;
   float *rho, *pr, *pres;
+  float4* xyuv;  // sorted state packed per particle (x, y, vx, vy) for neighbor gathers
+  float2* rp;    // (rho, pr) per particle, for neighbor gathers
   uint32_t *pid, *pid_alt;
   uint64_t* keys_sorted;
   uint32_t* perm;
@@ -169,12 +171,18 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
                          const uint32_t* src_pid, const float* src_ax, const float* src_ay,
                          float* dst_rx, float* dst_ry, float* dst_vx, float* dst_vy,
                          uint32_t* dst_pid, float* dst_ax, float* dst_ay, int hist_words);
-hipError_t launch_density(const Launch& L, const DevState& s, int nb_mode, bool write_p);
+// packed (NB_ELL only): neighbor positions from s.xyuv, and (rho, pr) written
+// to s.rp as well
+hipError_t launch_density(const Launch& L, const DevState& s, int nb_mode, bool write_p,
+                          bool packed = false);
+// s.xyuv from the sorted state
+hipError_t launch_pack(const Launch& L, const DevState& s);
 // fused: also integrate into T and write the next keys; movers (fused only,
 // flags may be null): the particles whose next key differs from s.keys_sorted
 // (the input of the incremental re-sort), as flag words and the mover list
 hipError_t launch_forces(const Launch& L, const DevState& s, int nb_mode, bool fused,
-                         uint64_t* keys_out, MoverSink movers = MoverSink{});
+                         uint64_t* keys_out, MoverSink movers = MoverSink{},
+                         bool packed = false);
 hipError_t launch_integrate(const Launch& L, const DevState& s);
 // density + forces + integration + next keys over LDS-staged Morton blocks
 // (variant: enum sph_step_kernel); stats (may be null): 3 counters per kernel;

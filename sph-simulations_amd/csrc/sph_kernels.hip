@@ -98,7 +98,8 @@ __global__ __launch_bounds__(BLK) void k_gather(
     float* __restrict__ dvx, float* __restrict__ dvy, uint32_t* __restrict__ dpid,
     float* __restrict__ dax, float* __restrict__ day, int2* __restrict__ cells,
     int64_t n_keys, uint32_t* __restrict__ hist, int hist_words, int2* __restrict__ blocks,
-    int block_bits, uint64_t* __restrict__ tile_first, uint32_t* __restrict__ tile_count) {
+    int block_bits, uint64_t* __restrict__ tile_first, uint32_t* __restrict__ tile_count,
+    float4* __restrict__ xyuv) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
@@ -106,10 +107,12 @@ __global__ __launch_bounds__(BLK) void k_gather(
       write_ranges(i, n, keys, cells, n_keys, blocks, block_bits, tile_first, tile_count);
   if (!live) return;
   const uint32_t j = min(perm[i], (uint32_t)(n - 1));  // in bounds even if perm were bad
-  drx[i] = srx[j];
-  dry[i] = sry[j];
-  dvx[i] = svx[j];
-  dvy[i] = svy[j];
+  const float x = srx[j], y = sry[j], u = svx[j], v = svy[j];
+  drx[i] = x;
+  dry[i] = y;
+  dvx[i] = u;
+  dvy[i] = v;
+  xyuv[i] = make_float4(x, y, u, v);
   dpid[i] = spid[j];
   if (sax) {
     dax[i] = sax[j];
@@ -170,7 +173,8 @@ __global__ __launch_bounds__(BLK) void k_density(
     const int2* __restrict__ cells, const int64_t* __restrict__ csr_off,
     const int32_t* __restrict__ csr_idx, int32_t* __restrict__ ell_count,
     int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ rho_out,
-    float* __restrict__ pr_out, float* __restrict__ p_out) {
+    float* __restrict__ pr_out, float* __restrict__ p_out, const float4* __restrict__ xyuv,
+    float2* __restrict__ rp_out) {
   const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
   const float xi = rx[i], yi = ry[i];
@@ -187,14 +191,29 @@ __global__ __launch_bounds__(BLK) void k_density(
                           [&](int, float, float, float r2) { acc += density_term(c.h2, r2); });
   } else {
     __shared__ int s_list[kEllLds * BLK];
-    acc = density_ell<BC>(g, c, rx, ry, cells, n, i, xi, yi, s_list + threadIdx.x, BLK,
-                          ell_count, ell_idx, ell_w);
+    if (xyuv)
+      acc = density_ell<BC, true>(g, c, rx, ry, cells, n, i, xi, yi, s_list + threadIdx.x, BLK,
+                                  ell_count, ell_idx, ell_w, xyuv);
+    else
+      acc = density_ell<BC>(g, c, rx, ry, cells, n, i, xi, yi, s_list + threadIdx.x, BLK,
+                            ell_count, ell_idx, ell_w);
   }
   float rho, p, pr;
   eos<PHYS>(c, acc, rho, p, pr);
   rho_out[i] = rho;
   pr_out[i] = pr;
   if (p_out) p_out[i] = p;
+  if (rp_out) rp_out[i] = make_float2(rho, pr);
+}
+
+// The sorted state packed per particle (x, y, vx, vy) for the neighbor gathers.
+__global__ __launch_bounds__(BLK) void k_pack(int n, const float* __restrict__ rx,
+                                              const float* __restrict__ ry,
+                                              const float* __restrict__ vx,
+                                              const float* __restrict__ vy,
+                                              float4* __restrict__ xyuv) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i < n) xyuv[i] = make_float4(rx[i], ry[i], vx[i], vy[i]);
 }
 
 // Accelerations. FUSED additionally integrates (symplectic Euler) into the T
@@ -209,7 +228,8 @@ __global__ __launch_bounds__(BLK) void k_forces(
     const int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ ax_out,
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
     float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out,
-    const uint64_t* __restrict__ kold, MoverSink movers) {
+    const uint64_t* __restrict__ kold, MoverSink movers, const float4* __restrict__ xyuv,
+    const float2* __restrict__ rp) {
   const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
   const float xi = rx[i], yi = ry[i], vxi = vx[i], vyi = vy[i], rhoi = rho[i], pri = pr[i];
@@ -226,8 +246,12 @@ __global__ __launch_bounds__(BLK) void k_forces(
   if (NB == NB_CSR) {
     for (int64_t k = csr_off[i]; k < csr_off[i + 1]; ++k) from_list(csr_idx[k]);
   } else if (NB == NB_ELL) {
-    a = forces_ell<PHYS, BC>(g, c, rx, ry, vx, vy, rho, pr, cells, ell_count, ell_idx, ell_w, n,
-                             i, xi, yi, vxi, vyi, rhoi, pri);
+    if (xyuv)
+      a = forces_ell<PHYS, BC, true>(g, c, rx, ry, vx, vy, rho, pr, cells, ell_count, ell_idx,
+                                     ell_w, n, i, xi, yi, vxi, vyi, rhoi, pri, xyuv, rp);
+    else
+      a = forces_ell<PHYS, BC>(g, c, rx, ry, vx, vy, rho, pr, cells, ell_count, ell_idx, ell_w,
+                               n, i, xi, yi, vxi, vyi, rhoi, pri);
   } else {
     for_each_neighbor<BC>(g, c, rx, ry, cells, n, i, xi, yi, pair);
   }
@@ -563,7 +587,7 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
     k_gather<<<blocks_for(s.n), BLK, 0, L.stream>>>(
         s.n, s.perm, s.keys_sorted, src_rx, src_ry, src_vx, src_vy, src_pid, src_ax, src_ay,
         dst_rx, dst_ry, dst_vx, dst_vy, dst_pid, dst_ax, dst_ay, s.cells, s.n_keys, s.hist,
-        hist_words, s.blocks, s.block_bits, s.tile_first, s.tile_count);
+        hist_words, s.blocks, s.block_bits, s.tile_first, s.tile_count, s.xyuv);
   return hipGetLastError();
 }
 
@@ -576,64 +600,78 @@ hipError_t launch_ranges(const Launch& L, const DevState& s, int hist_words) {
 }
 
 template <int PHYS, int BC>
-static hipError_t density_dispatch(const Launch& L, const DevState& s, int nb, bool write_p) {
+static hipError_t density_dispatch(const Launch& L, const DevState& s, int nb, bool write_p,
+                                   bool packed) {
   float* p = write_p ? s.pres : nullptr;
   const int nb_ = s.n > 0 ? blocks_for(s.n) : 0;
   if (nb_ == 0) return hipSuccess;
   if (nb == NB_CSR)
     k_density<PHYS, BC, NB_CSR><<<nb_, BLK, 0, L.stream>>>(
         s.n, L.grid, L.phys, s.rx, s.ry, s.cells, s.csr_off, s.csr_idx, s.ell_count, s.ell_idx,
-        s.ell_width, s.rho, s.pr, p);
+        s.ell_width, s.rho, s.pr, p, nullptr, nullptr);
   else if (nb == NB_ELL)
     k_density<PHYS, BC, NB_ELL><<<nb_, BLK, 0, L.stream>>>(
         s.n, L.grid, L.phys, s.rx, s.ry, s.cells, s.csr_off, s.csr_idx, s.ell_count, s.ell_idx,
-        s.ell_width, s.rho, s.pr, p);
+        s.ell_width, s.rho, s.pr, p, packed ? s.xyuv : nullptr, packed ? s.rp : nullptr);
   else
     k_density<PHYS, BC, NB_STENCIL><<<nb_, BLK, 0, L.stream>>>(
         s.n, L.grid, L.phys, s.rx, s.ry, s.cells, s.csr_off, s.csr_idx, s.ell_count, s.ell_idx,
-        s.ell_width, s.rho, s.pr, p);
+        s.ell_width, s.rho, s.pr, p, nullptr, nullptr);
   return hipGetLastError();
 }
 
-hipError_t launch_density(const Launch& L, const DevState& s, int nb, bool write_p) {
+hipError_t launch_density(const Launch& L, const DevState& s, int nb, bool write_p,
+                          bool packed) {
   const int ph = L.phys.physics, bc = L.phys.boundary;
-  if (ph == 0 && bc == 0) return density_dispatch<0, 0>(L, s, nb, write_p);
-  if (ph == 0 && bc == 1) return density_dispatch<0, 1>(L, s, nb, write_p);
-  if (ph == 1 && bc == 0) return density_dispatch<1, 0>(L, s, nb, write_p);
-  return density_dispatch<1, 1>(L, s, nb, write_p);
+  if (ph == 0 && bc == 0) return density_dispatch<0, 0>(L, s, nb, write_p, packed);
+  if (ph == 0 && bc == 1) return density_dispatch<0, 1>(L, s, nb, write_p, packed);
+  if (ph == 1 && bc == 0) return density_dispatch<1, 0>(L, s, nb, write_p, packed);
+  return density_dispatch<1, 1>(L, s, nb, write_p, packed);
+}
+
+hipError_t launch_pack(const Launch& L, const DevState& s) {
+  if (s.n > 0)
+    k_pack<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, s.rx, s.ry, s.vx, s.vy, s.xyuv);
+  return hipGetLastError();
 }
 
 template <int PHYS, int BC, int NB, bool FUSED>
 static hipError_t forces_go(const Launch& L, const DevState& s, uint64_t* keys_out,
-                            MoverSink movers) {
+                            MoverSink movers, bool packed) {
   k_forces<PHYS, BC, NB, FUSED><<<blocks_for(s.n), BLK, 0, L.stream>>>(
       s.n, L.grid, L.phys, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.csr_off, s.csr_idx,
       s.ell_count, s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out,
-      s.keys_sorted, FUSED ? movers : MoverSink{});
+      s.keys_sorted, FUSED ? movers : MoverSink{}, packed && NB == NB_ELL ? s.xyuv : nullptr,
+      packed && NB == NB_ELL ? s.rp : nullptr);
   return hipGetLastError();
 }
 
 template <int PHYS, int BC>
 static hipError_t forces_dispatch(const Launch& L, const DevState& s, int nb, bool fused,
-                                  uint64_t* keys_out, MoverSink movers) {
+                                  uint64_t* keys_out, MoverSink movers, bool packed) {
   if (s.n <= 0) return hipSuccess;
   if (fused) {
-    if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, true>(L, s, keys_out, movers);
-    if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, true>(L, s, keys_out, movers);
-    return forces_go<PHYS, BC, NB_STENCIL, true>(L, s, keys_out, movers);
+    if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, true>(L, s, keys_out, movers, packed);
+    if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, true>(L, s, keys_out, movers, false);
+    return forces_go<PHYS, BC, NB_STENCIL, true>(L, s, keys_out, movers, false);
   }
-  if (nb == NB_ELL) return forces_go<PHYS, BC, NB_ELL, false>(L, s, keys_out, MoverSink{});
-  if (nb == NB_CSR) return forces_go<PHYS, BC, NB_CSR, false>(L, s, keys_out, MoverSink{});
-  return forces_go<PHYS, BC, NB_STENCIL, false>(L, s, keys_out, MoverSink{});
+  if (nb == NB_ELL)
+    return forces_go<PHYS, BC, NB_ELL, false>(L, s, keys_out, MoverSink{}, packed);
+  if (nb == NB_CSR)
+    return forces_go<PHYS, BC, NB_CSR, false>(L, s, keys_out, MoverSink{}, false);
+  return forces_go<PHYS, BC, NB_STENCIL, false>(L, s, keys_out, MoverSink{}, false);
 }
 
 hipError_t launch_forces(const Launch& L, const DevState& s, int nb, bool fused,
-                         uint64_t* keys_out, MoverSink movers) {
+                         uint64_t* keys_out, MoverSink movers, bool packed) {
   const int ph = L.phys.physics, bc = L.phys.boundary;
-  if (ph == 0 && bc == 0) return forces_dispatch<0, 0>(L, s, nb, fused, keys_out, movers);
-  if (ph == 0 && bc == 1) return forces_dispatch<0, 1>(L, s, nb, fused, keys_out, movers);
-  if (ph == 1 && bc == 0) return forces_dispatch<1, 0>(L, s, nb, fused, keys_out, movers);
-  return forces_dispatch<1, 1>(L, s, nb, fused, keys_out, movers);
+  if (ph == 0 && bc == 0)
+    return forces_dispatch<0, 0>(L, s, nb, fused, keys_out, movers, packed);
+  if (ph == 0 && bc == 1)
+    return forces_dispatch<0, 1>(L, s, nb, fused, keys_out, movers, packed);
+  if (ph == 1 && bc == 0)
+    return forces_dispatch<1, 0>(L, s, nb, fused, keys_out, movers, packed);
+  return forces_dispatch<1, 1>(L, s, nb, fused, keys_out, movers, packed);
 }
 
 hipError_t launch_integrate(const Launch& L, const DevState& s) {

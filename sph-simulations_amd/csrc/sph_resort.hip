@@ -660,7 +660,7 @@ __global__ __launch_bounds__(TW) void k_merge(
     float* __restrict__ ry, float* __restrict__ vx, float* __restrict__ vy,
     uint32_t* __restrict__ pid_dst, uint64_t* __restrict__ keys_dst, uint32_t* __restrict__ perm,
     int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
-    uint32_t* __restrict__ bar, uint32_t* __restrict__ lcount) {
+    uint32_t* __restrict__ bar, uint32_t* __restrict__ lcount, float4* __restrict__ xyuv) {
   __shared__ uint64_t s_tk[TW];  // the tile's previous keys (all ones past n)
   __shared__ uint64_t s_mk[WIN];
   __shared__ uint32_t s_mi[WIN];
@@ -741,6 +741,7 @@ __global__ __launch_bounds__(TW) void k_merge(
       ry[pos] = my;
       vx[pos] = mu;
       vy[pos] = mv;
+      xyuv[pos] = make_float4(mx, my, mu, mv);
       pid_dst[pos] = mid_;
       keys_dst[pos] = km;
       perm[pos] = src;
@@ -754,6 +755,7 @@ __global__ __launch_bounds__(TW) void k_merge(
     ry[pos] = y;
     vx[pos] = u;
     vy[pos] = v;
+    xyuv[pos] = make_float4(x, y, u, v);
     pid_dst[pos] = id;
     keys_dst[pos] = k;
     perm[pos] = (uint32_t)j;
@@ -790,7 +792,8 @@ hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers&
   k_merge<<<tiles, TW, 0, L.stream>>>(s.n, s.keys_sorted, r.flags, r.tile_base, r.tile_lo,
                                       r.skey, r.sidx, s.tx, s.ty, s.tvx, s.tvy, s.pid, s.rx, s.ry,
                                       s.vx, s.vy, s.pid_alt, keys_dst, s.perm, s.cells, s.blocks,
-                                      s.block_bits, (uint64_t)s.n_keys, r.bar, r.lcount);
+                                      s.block_bits, (uint64_t)s.n_keys, r.bar, r.lcount,
+                                      s.xyuv);
   return hipGetLastError();
 }
 
