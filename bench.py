@@ -45,11 +45,12 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
         # R r,v 16; W rho,pr 8, a 8, T r,v 16, key 8
         phys = {"block_step": 56.0}
     else:
-        # density: R r 8, W rho,pr 8, count 4, list 4k;
-        # force: R r,v 16, rho,pr 8, count 4, list 4k, W a 8, T 16, key 8
-        # (+ R previous key 8, W one flag bit and the movers' list entries
-        # 12 f when it reports the movers)
-        phys = {"approximate_density": 20.0 + 4.0 * k,
+        # density: R r 8, W rho,pr 8 (+ the packed (rho, pr) copy 8), count
+        # 4, list 4k; force: R r,v 16, rho,pr 8, count 4, list 4k, W a 8, T 16,
+        # key 8 (+ R previous key 8, W one flag bit and the movers' list
+        # entries 12 f when it reports the movers); neighbors come from the
+        # packed copies (cache hits)
+        phys = {"approximate_density": 28.0 + 4.0 * k,
                 "force_computation": 60.0 + 4.0 * k
                 + ((8.125 + 12.0 * movers_frac) if flags_in_force else 0.0)}
     if resort == "full":
@@ -58,8 +59,9 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
             "sort_histogram": 16.0 + 8.0 * occ,
             # first pass R key 8, W key+idx 12; later passes R 12, W 12 (mean per launch)
             "sort_pass": (20.0 + 24.0 * (passes - 1)) / passes,
-            # R perm 4, sorted key 8, T r,v 16, pid 4; W r,v 16, pid 4; ranges 8/occ. cell
-            "neighbor_search": 52.0 + 8.0 * occ,
+            # R perm 4, sorted key 8, T r,v 16, pid 4; W r,v 16 (+ packed copy
+            # 16), pid 4; ranges 8/occ. cell
+            "neighbor_search": 68.0 + 8.0 * occ,
         }
     else:
         f = movers_frac
@@ -70,9 +72,9 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
             # 12 f when no kernel before reported the movers)
             "movers": 16.0 / 256.0 + 24.0 * f
             + (0.0 if flags_in_force else 16.125 + 12.0 * f),
-            # R old key 8, T r,v 16, pid 4; W r,v 16, pid 4, key 8, perm 4;
-            # clear the old ranges 8 occ
-            "merge": 60.0 + 8.0 * occ,
+            # R old key 8, T r,v 16, pid 4; W r,v 16 (+ packed copy 16), pid 4,
+            # key 8, perm 4; clear the old ranges 8 occ
+            "merge": 76.0 + 8.0 * occ,
             # ranges: R key 8, W 8 occ
             "neighbor_search": 8.0 + 8.0 * occ,
         }
