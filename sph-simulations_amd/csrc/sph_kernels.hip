@@ -232,7 +232,15 @@ __global__ __launch_bounds__(BLK) void k_forces(
     const float2* __restrict__ rp) {
   const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
-  const float xi = rx[i], yi = ry[i], vxi = vx[i], vyi = vy[i], rhoi = rho[i], pri = pr[i];
+  float xi, yi, vxi, vyi, rhoi, pri;
+  if (NB == NB_ELL && xyuv) {
+    // the particle's own state from the packed copies: two loads instead of six
+    const float4 q = ld16(xyuv, i);
+    const float2 r = ld8(rp, i);
+    xi = q.x; yi = q.y; vxi = q.z; vyi = q.w; rhoi = r.x; pri = r.y;
+  } else {
+    xi = rx[i]; yi = ry[i]; vxi = vx[i]; vyi = vy[i]; rhoi = rho[i]; pri = pr[i];
+  }
   const uint64_t ko = (FUSED && movers.flags) ? kold[i] : 0ull;
   Acc a{0.f, 0.f, 0.f, 0.f};
   auto pair = [&](int j, float dx, float dy, float r2) {
