@@ -50,6 +50,9 @@ struct Ctx {
   std::vector<PendingTiming> pending;
   double stage_ms[SPH_T_COUNT] = {};
   int64_t launches[SPH_T_COUNT] = {};
+  // inside sph_step on the per-particle path: the re-sort writes only the
+  // packed copy of the state, the separate arrays are restored when it ends
+  bool soa_lazy = false;
   // graph
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
@@ -359,7 +362,7 @@ int incremental_resort(Ctx* c, bool flags_ready) {
   }
   {
     StageTimer t(c, SPH_T_MERGE);
-    HIPCHK(c, launch_merge(c->L, s, r, r.keys_alt));
+    HIPCHK(c, launch_merge(c->L, s, r, r.keys_alt, !c->soa_lazy));
   }
   std::swap(s.keys_sorted, r.keys_alt);
   c->sb.keys_out = s.keys_sorted;
@@ -1070,15 +1073,34 @@ int sph_step(void* ctx, int32_t n_steps) {
   // the packed copy of the sorted state the step kernels gather from (other
   // calls may have changed the state since the last re-sort wrote it)
   HIPCHK(c, launch_pack(c->L, c->s));
-  int done = 0;
-  if (c->use_graph && !c->timing) {
-    if (n_steps >= 2 && !graph_matches(c) && (rc = capture_graph(c))) return rc;
-    for (; done + 2 <= n_steps; done += 2) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+  if (n_steps <= 0) {
+    c->csr_total = -1;
+    return SPH_OK;
   }
-  for (; done < n_steps; ++done)
-    if ((rc = fused_iteration(c))) return rc;
+  // per-particle path, one scene: the steps keep only the packed copy current
+  // and the separate arrays are written back once at the end
+  const bool lazy = !c->n_scenes && !(c->s.block_bits && (c->step_kernel == SPH_STEP_FUSED_BLOCK ||
+                                                          c->step_kernel == SPH_STEP_SPLIT_BLOCK));
+  c->soa_lazy = lazy;
+  auto run = [&]() -> int {
+    int done = 0, rc2 = SPH_OK;
+    if (c->use_graph && !c->timing) {
+      if (n_steps >= 2 && !graph_matches(c) && (rc2 = capture_graph(c))) return rc2;
+      for (; done + 2 <= n_steps; done += 2) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    }
+    for (; done < n_steps; ++done)
+      if ((rc2 = fused_iteration(c))) return rc2;
+    return SPH_OK;
+  };
+  rc = run();
+  c->soa_lazy = false;
+  if (lazy) {
+    const hipError_t e = launch_unpack(c->L, c->s);
+    if (rc == SPH_OK && e != hipSuccess) rc = hip_fail(c, e, "unpack");
+  }
+  if (rc) return rc;
   c->csr_total = -1;
-  if (n_steps > 0) c->density_ell = false;
+  c->density_ell = false;
   return SPH_OK;
 }
 

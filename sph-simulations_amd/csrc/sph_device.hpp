@@ -218,12 +218,11 @@ __device__ __forceinline__ void integrate_one(const DevPhys& c, float& x, float&
 // Visit the neighbors of sorted particle i in stencil order (x offset outer,
 // y offset inner, ascending index in a cell), self excluded unless SELF. cells
 // holds one int2 (start, end) per key, -1 for empty cells.
-template <int BC, bool SELF = false, class F>
-__device__ __forceinline__ void for_each_neighbor(const DevGrid& g, const DevPhys& c,
-                                                  const float* __restrict__ rx,
-                                                  const float* __restrict__ ry,
-                                                  const int2* __restrict__ cells, int n, int i,
-                                                  float xi, float yi, F&& fn) {
+// pos(j) gives particle j's position (float2).
+template <int BC, bool SELF = false, class P, class F>
+__device__ __forceinline__ void for_each_neighbor_p(const DevGrid& g, const DevPhys& c, P&& pos,
+                                                    const int2* __restrict__ cells, int n, int i,
+                                                    float xi, float yi, F&& fn) {
   int cx = cell_coord(xi, g.lo_x, g.cs_x, g.nx);
   int cy = cell_coord(yi, g.lo_y, g.cs_y, g.ny);
 #pragma unroll
@@ -251,13 +250,24 @@ __device__ __forceinline__ void for_each_neighbor(const DevGrid& g, const DevPhy
           continue;
         }
         float dx, dy;
-        separation<BC>(c, xi, yi, rx[j], ry[j], dx, dy);
+        const float2 pj = pos(j);
+        separation<BC>(c, xi, yi, pj.x, pj.y, dx, dy);
         float r2 = dx * dx + dy * dy;
         if (r2 > c.h2) continue;
         fn(j, dx, dy, r2);
       }
     }
   }
+}
+
+template <int BC, bool SELF = false, class F>
+__device__ __forceinline__ void for_each_neighbor(const DevGrid& g, const DevPhys& c,
+                                                  const float* __restrict__ rx,
+                                                  const float* __restrict__ ry,
+                                                  const int2* __restrict__ cells, int n, int i,
+                                                  float xi, float yi, F&& fn) {
+  for_each_neighbor_p<BC, SELF>(
+      g, c, [&](int j) { return make_float2(rx[j], ry[j]); }, cells, n, i, xi, yi, fn);
 }
 
 constexpr int kEllStage = 8;   // list slots per thread staged in LDS before the store
@@ -447,7 +457,13 @@ __device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
                                           const float2* __restrict__ rp = nullptr) {
   Acc a{0.f, 0.f, 0.f, 0.f};
   auto pair = [&](int j, float dx, float dy, float r2) {
-    force_pair<PHYS>(c, dx, dy, r2, rhoi, rho[j], pri, pr[j], vxi, vyi, vx[j], vy[j], a);
+    if constexpr (PACKED) {
+      const float4 q = xyuv[j];
+      const float2 r = rp[j];
+      force_pair<PHYS>(c, dx, dy, r2, rhoi, r.x, pri, r.y, vxi, vyi, q.z, q.w, a);
+    } else {
+      force_pair<PHYS>(c, dx, dy, r2, rhoi, rho[j], pri, pr[j], vxi, vyi, vx[j], vy[j], a);
+    }
   };
   // Slot loads are unconditional (clamped to the list width, so always inside
   // the buffer; slots past the count are replaced by i before any use): the
@@ -508,6 +524,11 @@ __device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
         jl[u] = ell_idx[(int64_t)min(k0 + u, ell_w - 1) * n + i];
       batch(k0);
     }
+  } else if constexpr (PACKED) {
+    // the list overflowed: the stencil, state from the packed copies
+    for_each_neighbor_p<BC>(
+        g, c, [&](int j) { return make_float2(xyuv[j].x, xyuv[j].y); }, cells, n, i, xi, yi,
+        pair);
   } else {
     for_each_neighbor<BC>(g, c, rx, ry, cells, n, i, xi, yi, pair);
   }

@@ -175,8 +175,9 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
 // to s.rp as well
 hipError_t launch_density(const Launch& L, const DevState& s, int nb_mode, bool write_p,
                           bool packed = false);
-// s.xyuv from the sorted state
+// s.xyuv from the sorted state, and back
 hipError_t launch_pack(const Launch& L, const DevState& s);
+hipError_t launch_unpack(const Launch& L, const DevState& s);
 // fused: also integrate into T and write the next keys; movers (fused only,
 // flags may be null): the particles whose next key differs from s.keys_sorted
 // (the input of the incremental re-sort), as flag words and the mover list
@@ -197,8 +198,9 @@ int64_t resort_tiles(int n);
 // launch; a second one first when the producer of knew reported no movers)
 hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers& r,
                          const uint64_t* knew, int key_bits, bool flags_ready);
+// soa: also write the separate arrays rx, ry, vx, vy (else only s.xyuv)
 hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
-                        uint64_t* keys_dst);
+                        uint64_t* keys_dst, bool soa = true);
 // cell and block ranges of the sorted keys s.keys_sorted; clears the histogram
 hipError_t launch_ranges(const Launch& L, const DevState& s, int hist_words);
 hipError_t launch_update_velocities(const Launch& L, const DevState& s);

@@ -177,7 +177,15 @@ __global__ __launch_bounds__(BLK) void k_density(
     float2* __restrict__ rp_out) {
   const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
-  const float xi = rx[i], yi = ry[i];
+  float xi, yi;
+  if (NB == NB_ELL && xyuv) {
+    const float2 q = ld8_of16(xyuv, i);  // the packed copy is the current state
+    xi = q.x;
+    yi = q.y;
+  } else {
+    xi = rx[i];
+    yi = ry[i];
+  }
   float acc = density_term(c.h2, 0.0f);
   if (NB == NB_CSR) {
     for (int64_t k = csr_off[i]; k < csr_off[i + 1]; ++k) {
@@ -204,6 +212,19 @@ __global__ __launch_bounds__(BLK) void k_density(
   pr_out[i] = pr;
   if (p_out) p_out[i] = p;
   if (rp_out) rp_out[i] = make_float2(rho, pr);
+}
+
+// The separate arrays of the sorted state from its packed copy.
+__global__ __launch_bounds__(BLK) void k_unpack(int n, const float4* __restrict__ xyuv,
+                                                float* __restrict__ rx, float* __restrict__ ry,
+                                                float* __restrict__ vx, float* __restrict__ vy) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const float4 q = xyuv[i];
+  rx[i] = q.x;
+  ry[i] = q.y;
+  vx[i] = q.z;
+  vy[i] = q.w;
 }
 
 // The sorted state packed per particle (x, y, vx, vy) for the neighbor gathers.
@@ -635,6 +656,12 @@ hipError_t launch_density(const Launch& L, const DevState& s, int nb, bool write
   if (ph == 0 && bc == 1) return density_dispatch<0, 1>(L, s, nb, write_p, packed);
   if (ph == 1 && bc == 0) return density_dispatch<1, 0>(L, s, nb, write_p, packed);
   return density_dispatch<1, 1>(L, s, nb, write_p, packed);
+}
+
+hipError_t launch_unpack(const Launch& L, const DevState& s) {
+  if (s.n > 0)
+    k_unpack<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, s.xyuv, s.rx, s.ry, s.vx, s.vy);
+  return hipGetLastError();
 }
 
 hipError_t launch_pack(const Launch& L, const DevState& s) {

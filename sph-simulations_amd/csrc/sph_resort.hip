@@ -737,10 +737,12 @@ __global__ __launch_bounds__(TW) void k_merge(
       uint32_t mb = tb + s_wpre[w];
       if (w < TW / kWave) mb += (uint32_t)__popcll(s_fw[w] & ((1ull << l) - 1ull));
       const int pos = (c0 + r) + (j0 + a - (int)mb);
-      rx[pos] = mx;
-      ry[pos] = my;
-      vx[pos] = mu;
-      vy[pos] = mv;
+      if (rx) {
+        rx[pos] = mx;
+        ry[pos] = my;
+        vx[pos] = mu;
+        vy[pos] = mv;
+      }
       xyuv[pos] = make_float4(mx, my, mu, mv);
       pid_dst[pos] = mid_;
       keys_dst[pos] = km;
@@ -751,10 +753,12 @@ __global__ __launch_bounds__(TW) void k_merge(
   if (live && !mv) {
     const uint32_t mb = tb + s_wpre[wave] + (uint32_t)__popcll(fw & ((1ull << lane) - 1ull));
     const int pos = (j - (int)mb) + lo + before_movers;
-    rx[pos] = x;
-    ry[pos] = y;
-    vx[pos] = u;
-    vy[pos] = v;
+    if (rx) {
+      rx[pos] = x;
+      ry[pos] = y;
+      vx[pos] = u;
+      vy[pos] = v;
+    }
     xyuv[pos] = make_float4(x, y, u, v);
     pid_dst[pos] = id;
     keys_dst[pos] = k;
@@ -786,12 +790,13 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
 }
 
 hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
-                        uint64_t* keys_dst) {
+                        uint64_t* keys_dst, bool soa) {
   if (s.n <= 0) return hipSuccess;
   const int tiles = (int)resort_tiles(s.n);
   k_merge<<<tiles, TW, 0, L.stream>>>(s.n, s.keys_sorted, r.flags, r.tile_base, r.tile_lo,
-                                      r.skey, r.sidx, s.tx, s.ty, s.tvx, s.tvy, s.pid, s.rx, s.ry,
-                                      s.vx, s.vy, s.pid_alt, keys_dst, s.perm, s.cells, s.blocks,
+                                      r.skey, r.sidx, s.tx, s.ty, s.tvx, s.tvy, s.pid,
+                                      soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt,
+                                      keys_dst, s.perm, s.cells, s.blocks,
                                       s.block_bits, (uint64_t)s.n_keys, r.bar, r.lcount,
                                       s.xyuv);
   return hipGetLastError();
