@@ -41,7 +41,8 @@ __global__ __launch_bounds__(kMaxTile) void k_bdensity(
   const SceneDev& S = scenes[t.scene];
   const DevGrid g = S.g;
   const DevPhys c = S.c;
-  const float xi = rx[i], yi = ry[i];
+  const float2 qi = ld8_of16(xyuv, i);  // the packed copy is the current state
+  const float xi = qi.x, yi = qi.y;
   const float acc = density_ell<BC, true>(g, c, rx, ry, cells + S.base, n, i, xi, yi,
                                           s_list + threadIdx.x, (int)blockDim.x, ell_count,
                                           ell_idx, ell_w, xyuv);
@@ -70,7 +71,9 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
   const DevGrid g = S.g;
   const DevPhys c = S.c;
   const uint64_t base = S.base;
-  const float xi = rx[i], yi = ry[i], vxi = vx[i], vyi = vy[i], rhoi = rho[i], pri = pr[i];
+  const float4 qi = ld16(xyuv, i);  // the packed copies are the current state
+  const float2 ri = ld8(rp, i);
+  const float xi = qi.x, yi = qi.y, vxi = qi.z, vyi = qi.w, rhoi = ri.x, pri = ri.y;
   const Acc a = forces_ell<PHYS, BC, true>(g, c, rx, ry, vx, vy, rho, pr, cells + base,
                                            ell_count, ell_idx, ell_w, n, i, xi, yi, vxi, vyi,
                                            rhoi, pri, xyuv, rp);

@@ -50,7 +50,7 @@ struct Ctx {
   std::vector<PendingTiming> pending;
   double stage_ms[SPH_T_COUNT] = {};
   int64_t launches[SPH_T_COUNT] = {};
-  // inside sph_step on the per-particle path: the re-sort writes only the
+  // inside sph_step on the per-particle kernels: the re-sort writes only the
   // packed copy of the state, the separate arrays are restored when it ends
   bool soa_lazy = false;
   // graph
@@ -1077,10 +1077,10 @@ int sph_step(void* ctx, int32_t n_steps) {
     c->csr_total = -1;
     return SPH_OK;
   }
-  // per-particle path, one scene: the steps keep only the packed copy current
-  // and the separate arrays are written back once at the end
-  const bool lazy = !c->n_scenes && !(c->s.block_bits && (c->step_kernel == SPH_STEP_FUSED_BLOCK ||
-                                                          c->step_kernel == SPH_STEP_SPLIT_BLOCK));
+  // per-particle kernels (one scene or batched): the steps keep only the
+  // packed copy current and the separate arrays are written back once at the end
+  const bool lazy = c->n_scenes || !(c->s.block_bits && (c->step_kernel == SPH_STEP_FUSED_BLOCK ||
+                                                         c->step_kernel == SPH_STEP_SPLIT_BLOCK));
   c->soa_lazy = lazy;
   auto run = [&]() -> int {
     int done = 0, rc2 = SPH_OK;
