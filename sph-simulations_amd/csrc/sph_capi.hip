@@ -1077,10 +1077,14 @@ int sph_step(void* ctx, int32_t n_steps) {
     c->csr_total = -1;
     return SPH_OK;
   }
-  // per-particle kernels (one scene or batched): the steps keep only the
-  // packed copy current and the separate arrays are written back once at the end
-  const bool lazy = c->n_scenes || !(c->s.block_bits && (c->step_kernel == SPH_STEP_FUSED_BLOCK ||
-                                                         c->step_kernel == SPH_STEP_SPLIT_BLOCK));
+  // per-particle kernels (one scene or batched), two steps or more: the steps
+  // keep only the packed copy current and the separate arrays are written back
+  // once at the end (a single step has the merge write them, which costs less
+  // than the pass over the packed copy)
+  const bool lazy =
+      n_steps >= 2 &&
+      (c->n_scenes || !(c->s.block_bits && (c->step_kernel == SPH_STEP_FUSED_BLOCK ||
+                                            c->step_kernel == SPH_STEP_SPLIT_BLOCK)));
   c->soa_lazy = lazy;
   auto run = [&]() -> int {
     int done = 0, rc2 = SPH_OK;
