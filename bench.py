@@ -45,13 +45,14 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
         # R r,v 16; W rho,pr 8, a 8, T r,v 16, key 8
         phys = {"block_step": 56.0}
     else:
-        # density: R r 8, W rho,pr 8 (+ the packed (rho, pr) copy 8), count
-        # 4, list 4k; force: R r,v 16, rho,pr 8, count 4, list 4k, W a 8, T 16,
-        # key 8 (+ R previous key 8, W one flag bit and the movers' list
-        # entries 12 f when it reports the movers); neighbors come from the
-        # packed copies (cache hits)
-        phys = {"approximate_density": 28.0 + 4.0 * k,
-                "force_computation": 60.0 + 4.0 * k
+        # density: R r 8, W the packed (rho, pr) copy 8, count 4, list 4k;
+        # force: R r,v 16, rho,pr 8, count 4, list 4k, T 16, key 8 (+ R
+        # previous key 8, W one flag bit and the movers' list entries 12 f when
+        # it reports the movers); the separate rho, pr and a arrays (8 + 8) are
+        # written only in the last step of a sph_step call; neighbors come from
+        # the packed copies (cache hits)
+        phys = {"approximate_density": 20.0 + 4.0 * k,
+                "force_computation": 52.0 + 4.0 * k
                 + ((8.125 + 12.0 * movers_frac) if flags_in_force else 0.0)}
     if resort == "full":
         sort = {

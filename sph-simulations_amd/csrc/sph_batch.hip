@@ -48,8 +48,10 @@ __global__ __launch_bounds__(kMaxTile) void k_bdensity(
                                           ell_idx, ell_w, xyuv);
   float rho, p, pr;
   eos<PHYS>(c, acc, rho, p, pr);
-  rho_out[i] = rho;
-  pr_out[i] = pr;
+  if (rho_out) {  // (null: a step whose separate fields nobody reads)
+    rho_out[i] = rho;
+    pr_out[i] = pr;
+  }
   if (p_out) p_out[i] = p;
   rp_out[i] = make_float2(rho, pr);
 }
@@ -79,8 +81,10 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
                                            rhoi, pri, xyuv, rp);
   float axi, ayi;
   force_finish<PHYS>(c, a, axi, ayi);
-  ax_out[i] = axi;
-  ay_out[i] = ayi;
+  if (ax_out) {  // (null: a step whose accelerations nobody reads)
+    ax_out[i] = axi;
+    ay_out[i] = ayi;
+  }
   float x = xi, y = yi, u = vxi, v = vyi;
   integrate_one<BC>(c, x, y, u, v, axi, ayi);
   tx[i] = x;

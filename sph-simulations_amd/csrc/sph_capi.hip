@@ -53,6 +53,10 @@ struct Ctx {
   // inside sph_step on the per-particle kernels: the re-sort writes only the
   // packed copy of the state, the separate arrays are restored when it ends
   bool soa_lazy = false;
+  // the fused step writes the accelerations and the separate density and
+  // pressure arrays (off for the steps of a sph_step call but its last: only
+  // the last step's are observable; the steps read the packed copies)
+  bool fields_out = true;
   // graph
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
@@ -383,12 +387,14 @@ int fused_iteration(Ctx* c) {
   DevState& s = c->s;
   if (c->n_scenes) {
     // batched scenes: per-tile scene constants, fixed-width list
+    DevState fs = s;
+    if (!c->fields_out) fs.ax = fs.ay = fs.rho = fs.pr = nullptr;
     {
       StageTimer t(c, SPH_T_DENSITY);
-      HIPCHK(c, launch_batch_density(c->L, c->B, s));
+      HIPCHK(c, launch_batch_density(c->L, c->B, fs));
     }
     StageTimer t(c, SPH_T_FORCE);
-    HIPCHK(c, launch_batch_forces(c->L, c->B, s, c->sb.keys_in));
+    HIPCHK(c, launch_batch_forces(c->L, c->B, fs, c->sb.keys_in));
   } else if (s.block_bits && c->step_kernel == SPH_STEP_FUSED_BLOCK) {
     // Morton order, one kernel: density, forces, integration and keys
     StageTimer t(c, SPH_T_BLOCK_STEP);
@@ -407,15 +413,17 @@ int fused_iteration(Ctx* c) {
     // per-particle kernels: density with the fixed-width list, then forces from
     // it; neighbor data gathered from the packed copy s.xyuv (kept by the
     // re-sorts, made by sph_step on entry) and s.rp (written by the density)
+    DevState fs = s;
+    if (!c->fields_out) fs.ax = fs.ay = fs.rho = fs.pr = nullptr;
     {
       StageTimer t(c, SPH_T_DENSITY);
-      HIPCHK(c, launch_density(c->L, s, NB_ELL, false, true));
+      HIPCHK(c, launch_density(c->L, fs, NB_ELL, false, true));
     }
     // the force kernel also flags the movers for the incremental re-sort
     const bool inc = c->resort == SPH_RESORT_INCREMENTAL;
     {
       StageTimer t(c, SPH_T_FORCE);
-      HIPCHK(c, launch_forces(c->L, s, NB_ELL, true, c->sb.keys_in,
+      HIPCHK(c, launch_forces(c->L, fs, NB_ELL, true, c->sb.keys_in,
                               inc ? mover_sink(c->rb) : MoverSink{}, true));
     }
     return resort_integrated(c, inc);
@@ -455,7 +463,10 @@ int capture_graph(Ctx* c) {
   bool was_timing = c->timing;
   c->timing = false;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  // the first of the two iterations leaves the observable fields to the second
+  c->fields_out = false;
   int rc = fused_iteration(c);
+  c->fields_out = true;
   if (rc == SPH_OK) rc = fused_iteration(c);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -1092,11 +1103,14 @@ int sph_step(void* ctx, int32_t n_steps) {
       if (n_steps >= 2 && !graph_matches(c) && (rc2 = capture_graph(c))) return rc2;
       for (; done + 2 <= n_steps; done += 2) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     }
-    for (; done < n_steps; ++done)
+    for (; done < n_steps; ++done) {
+      c->fields_out = done + 1 == n_steps;
       if ((rc2 = fused_iteration(c))) return rc2;
+    }
     return SPH_OK;
   };
   rc = run();
+  c->fields_out = true;
   c->soa_lazy = false;
   if (lazy) {
     const hipError_t e = launch_unpack(c->L, c->s);

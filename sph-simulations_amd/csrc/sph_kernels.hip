@@ -208,8 +208,10 @@ __global__ __launch_bounds__(BLK) void k_density(
   }
   float rho, p, pr;
   eos<PHYS>(c, acc, rho, p, pr);
-  rho_out[i] = rho;
-  pr_out[i] = pr;
+  if (rho_out) {  // (null: a step whose separate fields nobody reads)
+    rho_out[i] = rho;
+    pr_out[i] = pr;
+  }
   if (p_out) p_out[i] = p;
   if (rp_out) rp_out[i] = make_float2(rho, pr);
 }
@@ -286,8 +288,10 @@ __global__ __launch_bounds__(BLK) void k_forces(
   }
   float axi, ayi;
   force_finish<PHYS>(c, a, axi, ayi);
-  ax_out[i] = axi;
-  ay_out[i] = ayi;
+  if (ax_out) {  // (null: a step whose accelerations nobody reads)
+    ax_out[i] = axi;
+    ay_out[i] = ayi;
+  }
   if (FUSED) {
     float x = xi, y = yi, u = vxi, v = vyi;
     integrate_one<BC>(c, x, y, u, v, axi, ayi);
