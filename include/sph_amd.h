@@ -193,7 +193,8 @@ int sph_snapshot_load(void* ctx, const char* path, int64_t* step);
  * sph_prepare: find neighbors + density with the fixed-width neighbor list.
  * sph_step: n iterations of [density + forces + integration + keys] -> radix
  *   sort -> [permutation + cell ranges]; density and forces are left for the
- *   state before the last integration (sph_density_pressure + sph_forces refresh).
+ *   state before the last integration (written by the last iteration only;
+ *   sph_density_pressure + sph_forces refresh).
  * Reference-equivalent run of S iterations (simulation_configuration.h:297-387):
  *   sph_prepare; sph_step(S-1); sph_density_pressure(SORTED); sph_forces(SORTED).
  * By default (SPH_STEP_PARTICLE, SPH_RESORT_INCREMENTAL) one iteration is: the
