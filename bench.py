@@ -102,6 +102,24 @@ def measured_traffic(kernel):
     return None
 
 
+def measured_issue(kernel):
+    """VALU and texture-address busy fractions of `kernel` at the 1M workload,
+    from the committed rocprofv3 counter summary, or None: the step kernels are
+    limited by instruction issue and the address path rather than by HBM."""
+    path = os.path.join(ROOT, "profiles", "issue_bound_dam_break_1m.md")
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for line in open(path):
+        for key in ("valu_busy", "ta_busy"):
+            if line.startswith(key + ":"):
+                for item in line.split()[1:]:
+                    name, _, val = item.partition("=")
+                    if name == kernel:
+                        out[key] = float(val)
+    return out or None
+
+
 def cpu_baseline(sph, scene, budget_s):
     from oracle import oracle as orc
     threads, t, _ = orc.baseline_run(scene.params, scene.rx, scene.ry, scene.vx, scene.vy, 2)
@@ -457,7 +475,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL.get(dom_name, dom_name),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(STAGE_KERNEL.get(dom_name, dom_name))},
+                     "traffic": measured_traffic(STAGE_KERNEL.get(dom_name, dom_name)),
+                     "issue": measured_issue(STAGE_KERNEL.get(dom_name, dom_name))},
         "stages": stages,
         "step": {"alg_bytes_per_particle": round(step_bytes, 1),
                  "step_frac_of_hbm": step_bytes * value / (HBM_PEAK_GBS * 1e9),
