@@ -12,9 +12,11 @@ namespace sphamd {
 
 constexpr int kMaxPasses = 8;  // 64-bit keys, 8-bit digits
 constexpr int kSmallSortCap = 4096;  // movers sorted in LDS by one workgroup
+constexpr int kListSegs = 8;         // mover list segments (one per XCD)
+// movers grouped over the whole mover grid (one per thread) when all are listed
+constexpr int kMidSortCap = kListSegs * kSmallSortCap;
 constexpr int kMoverGrid = 64;       // workgroups of the mover sort (all resident)
 constexpr int kResortTile = 256;     // particles per tile of the incremental re-sort
-constexpr int kListSegs = 8;         // mover list segments (one per XCD)
 constexpr int kListStride = 32;      // words between segment counters (one 128-byte line each)
 
 struct SortBuffers {
@@ -111,9 +113,9 @@ struct ResortBuffers {
 // Where the producer of the new keys reports the movers: the flag words and,
 // per segment, a counter and up to kSmallSortCap (previous index, new key)
 // entries. flags == nullptr: report nothing.
-// prev: the previous re-sort's mover count; above kSmallSortCap the producer
+// prev: the previous re-sort's mover count; above kMidSortCap the producer
 // lists nothing (most likely the total exceeds the cap again) and marks the
-// lists full instead (list_full), saving the atomics.
+// lists full instead (list_movers), saving the atomics.
 struct MoverSink {
   unsigned long long* flags = nullptr;
   uint32_t* count = nullptr;
@@ -129,7 +131,7 @@ inline MoverSink mover_sink(const ResortBuffers& r) {
 // Whether this launch lists the movers (uniform); if not, one thread marks the
 // lists full so that the consumer takes the movers from the flag words.
 __device__ __forceinline__ bool list_movers(const MoverSink& m) {
-  const bool listing = *m.prev <= kSmallSortCap;
+  const bool listing = *m.prev <= kMidSortCap;
   if (!listing && blockIdx.x == 0 && threadIdx.x == 0) m.count[0] = (uint32_t)kSmallSortCap + 1u;
   return listing;
 }
@@ -137,9 +139,9 @@ __device__ __forceinline__ bool list_movers(const MoverSink& m) {
 // One wave's part of the report, called by all its active lanes (the wave's
 // particles are the 64 indices from a multiple of 64, its first lane active):
 // the wave's flag word, then one atomic on the segment's counter reserves the
-// slots of its movers. Entries past kSmallSortCap are not written; the consumer
-// then has more than kSmallSortCap movers in total and takes them from the flag
-// words.
+// slots of its movers. Entries past kSmallSortCap in a segment are not
+// written; the consumer then sees that segment's count above it and takes the
+// movers from the flag words.
 __device__ __forceinline__ void record_movers(const MoverSink& m, int i, bool moved, uint64_t kn,
                                               int seg, bool listing) {
   const unsigned long long bal = __ballot(moved);

@@ -206,6 +206,14 @@ __device__ __forceinline__ int rank_in_sorted(const uint64_t* keys, const uint32
 // Merge tiles whose movers the LDS path counts in LDS (and whose first keys it
 // searches there directly; above, every S-th one).
 constexpr int kTileTab = 4096;
+// The grid path for up to kMidSortCap listed movers: one mover per thread, and
+// at most kMidGroupMax movers per merge window (each is ranked by counting over
+// its window; a larger window takes the radix path); kMidRankBatch window
+// entries are loaded together.
+static_assert(kMidSortCap <= kMoverGrid * MS_TW, "one listed mover per thread");
+static_assert(kMidSortCap < (1 << 16), "packed 16-bit tile counters");
+constexpr uint32_t kMidGroupMax = 256;
+constexpr int kMidRankBatch = 8;
 
 // Flag words per thread and per batch of the ordered compaction.
 constexpr int MS_WPT = 12;
@@ -545,6 +553,142 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
     return;
   }
 
+  // ---- more movers than workgroup 0 groups, all of them listed (no list
+  // past its capacity) and at most kMidSortCap: the same grouping over the
+  // grid, one mover per thread. Every workgroup finds its movers' owner tiles
+  // (the tile-first table of every S-th tile in its LDS) and counts owner and
+  // previous-index tiles with device atomics on the packed counters
+  // (tile_count, zero on entry); workgroup 0 scans them into the tile prefixes
+  // and window starts; each mover goes into its owner's window at the slot its
+  // atomic returned and is then ranked inside the window by counting. A
+  // window of more than kMidGroupMax movers sends every workgroup on to the
+  // radix path instead (decided by workgroup 0, read by all after a barrier).
+  bool mid = listed <= (uint32_t)kMidSortCap;
+#pragma unroll
+  for (int g = 0; g < kListSegs; ++g) mid = mid && cnt[g] <= (uint32_t)kSmallSortCap;
+  uint32_t gen = 0;
+  if (mid) {
+    const int m = (int)listed;  // exact: every list within its capacity
+    const int e = (int)blockIdx.x * MS_TW + tid;
+    const bool valid = e < m;
+    // listed entry e (clamped for threads without a mover)
+    const uint32_t ec = (uint32_t)min(e, m - 1);
+    int g = 0;
+#pragma unroll
+    for (int h = 1; h < kListSegs; ++h) g += ec >= seg_lo[h] ? 1 : 0;
+    const int at = g * kSmallSortCap + (int)(ec - seg_lo[g]);
+    const uint32_t mi = lidx[at];
+    const uint64_t mk = lkey[at];
+    uint64_t* s_tab = reinterpret_cast<uint64_t*>(s_wc);
+    for (int q = tid; q < ne; q += MS_TW) {
+      const uint64_t tk = tfirst[(size_t)q * S];
+      if constexpr (WIDE) s_tab[q] = tk;
+      else s_tab[q] = O::make(tk, (uint32_t)(q * S * TW)).w;
+    }
+    auto tab = [&](int q) {
+      if constexpr (WIDE) return O::make(s_tab[q], (uint32_t)(q * S * TW));
+      else return O{s_tab[q]};
+    };
+    __syncthreads();
+    const O mo = O::make(mk, mi);
+    int own = 0;
+    uint32_t slot = 0;
+    if (__any(valid)) {
+      int base = 0;
+      for (int len = ne; len > 1;) {
+        const int half = len >> 1;
+        if (tab(base + half - 1) < mo) base += half;
+        len -= half;
+      }
+      if (tab(base) < mo) ++base;
+      const int c = base - 1;  // last table entry before the mover, -1 if none
+      own = max(c, 0) * S;
+      if (c >= 0)
+        for (int k = 1; k < S; ++k) {
+          const int t = c * S + k;
+          if (t < tiles && O::make(tfirst[t], (uint32_t)t * TW) < mo) own = t;
+        }
+      if (valid) {
+        slot = atomicAdd(&tcount[own], 1u << 16) >> 16;
+        atomicAdd(&tcount[mi / (uint32_t)TW], 1u);
+      }
+    }
+    if (!grid_barrier(bar, gen, status, &s_ok)) return;
+    if (blockIdx.x == 0) {
+      // exclusive scan of the counter words, four tiles per thread per round
+      // (plain loads: the barrier's fence made the atomics visible, and this
+      // kernel has not read the counters before); both halves' prefixes stay
+      // below 2^16, so no carry between them
+      if (tid == 0) s_base[0] = 0;
+      uint32_t carry = 0, gmax = 0;
+      for (int t0 = 0; t0 < tiles; t0 += 4 * MS_TW) {
+        uint32_t cv[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int t = t0 + 4 * tid + k, tc = min(t, tiles - 1);
+          const uint32_t v = tcount[tc];
+          cv[k] = t < tiles ? v : 0u;
+          sum += cv[k];
+          gmax = max(gmax, cv[k] >> 16);
+        }
+        uint32_t rtot;
+        uint32_t run = carry + block_excl_scan(sum, s_w, &rtot);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int t = t0 + 4 * tid + k;
+          if (t < tiles) {
+            tcount[t] = run;
+            tile_base[t] = run & 0xffffu;
+            tile_lo[t] = run >> 16;
+          }
+          run += cv[k];
+        }
+        carry += rtot;
+      }
+      atomicMax(&s_base[0], gmax);
+      __syncthreads();
+      if (tid == 0) {
+        counts[0] = s_base[0] > kMidGroupMax ? 1u : 0u;
+        *n_movers = m;
+        tile_lo[tiles] = (uint32_t)m;
+      }
+    }
+    if (!grid_barrier(bar, gen, status, &s_ok)) return;
+    if (counts[0] == 0u) {  // (plain loads after the barrier's fence, as above)
+      const uint32_t w0 = tcount[own];
+      const uint32_t w1 = own + 1 < tiles ? tcount[own + 1] : (uint32_t)m << 16;
+      const uint32_t st = w0 >> 16, en = w1 >> 16;
+      if (valid) {
+        ktmp[st + slot] = mk;
+        vtmp[st + slot] = mi;
+      }
+      if (!grid_barrier(bar, gen, status, &s_ok)) return;
+      if (valid) {
+        // the window's movers kMidRankBatch at a time, their loads in flight
+        // together (indices clamped into the window, which holds this mover)
+        uint32_t rank = st;
+        for (uint32_t u0 = st; u0 < en; u0 += kMidRankBatch) {
+          uint64_t kk[kMidRankBatch];
+          uint32_t ii[kMidRankBatch];
+#pragma unroll
+          for (int q = 0; q < kMidRankBatch; ++q) {
+            const uint32_t u = min(u0 + (uint32_t)q, en - 1u);
+            kk[q] = ktmp[u];
+            ii[q] = vtmp[u];
+          }
+#pragma unroll
+          for (int q = 0; q < kMidRankBatch; ++q)
+            rank += (u0 + (uint32_t)q < en && O::make(kk[q], ii[q]) < mo) ? 1u : 0u;
+        }
+        skey[rank] = mk;
+        sidx[rank] = mi;
+      }
+      return;
+    }
+    // a window too large to rank by counting: the radix path below rewrites
+    // every output (the counters are cleared by the ranges pass as always)
+  }
+
   // ---- more movers than one workgroup groups: workgroup 0 writes the tile
   // prefixes and compacts the movers in previous-index order into midx
   static_assert(MS_BATCH * 4 <= (int)sizeof(s_k), "word prefixes exceed the key buffers");
@@ -555,7 +699,6 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
   // ---- more movers than one workgroup sorts: LSD radix sort over the grid
   // (the indices in midx from workgroup 0; the first pass reads the keys
   // through them)
-  uint32_t gen = 0;
   if (!grid_barrier(bar, gen, status, &s_ok)) return;
   // the exact number of movers, counted from the flag words by workgroup 0 (the
   // list counters may stop short of it once past kSmallSortCap)

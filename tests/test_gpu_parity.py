@@ -460,12 +460,58 @@ def test_incremental_resort_listed_movers_many_tiles(sph, orc):
     assert np.array_equal(perm, operm)
 
 
+@pytest.mark.parametrize("pattern", ["spread", "converge"])
+def test_incremental_resort_mid_movers(sph, orc, pattern):
+    """Between 4096 and 32768 movers, all listed: the mover kernel groups them
+    over the whole grid (owner tiles found against the sampled tile-first table,
+    5625 tiles of 256). "spread": ~20 K fast movers over many merge windows;
+    "converge": 6000 particles sent into one spot, so one window holds more
+    movers than are ranked by counting and the radix path takes over. Two steps
+    in a row; the permutation is the oracle's stable sort, the count exact and
+    the cell ranges the oracle's."""
+    s = sph.scenes.dam_break(1200, 1200, box=(4.0, 4.0), seed=5)
+    rng = np.random.default_rng(17)
+    if pattern == "spread":
+        pick = rng.choice(s.n, 18000, replace=False)
+        s.vx[pick] = rng.normal(0, 400.0, pick.size).astype(np.float32)
+        s.vy[pick] = rng.normal(0, 400.0, pick.size).astype(np.float32)
+    else:
+        pick = rng.choice(s.n, 6000, replace=False)
+        dt = s.params.dt
+        s.vx[pick] = ((2.0 - s.rx[pick]) / dt).astype(np.float32)
+        s.vy[pick] = ((1.0 - s.ry[pick]) / dt).astype(np.float32)
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.prepare()
+        for step in range(2):
+            k_before = ctx.sorted_keys()
+            ctx.step(1)
+            after = ctx.download()
+            perm = ctx.permutation()
+            m = ctx.movers()
+            sk = ctx.sorted_keys()
+            st, en = ctx.cell_ranges()
+            note(f"[gpu] mid movers ({pattern}), step {step}: {m} of {s.n}")
+            if step == 0:
+                assert 4096 < m <= 32768
+            x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
+            x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
+            k_new = orc.keys(s.params, x_prev, y_prev)
+            assert m == int(np.sum(k_new != k_before))
+            operm, osk = orc.stable_sort(k_new)
+            assert np.array_equal(perm, operm)
+            assert np.array_equal(sk, osk)
+            ost, oen = orc.cell_ranges(osk, int(ctx.grid().n_keys))
+            assert np.array_equal(st, ost) and np.array_equal(en, oen)
+
+
 @pytest.mark.parametrize("side,sigma", [(100, 40.0), (512, 400.0)])
 def test_incremental_resort_many_movers(sph, orc, side, sigma):
-    """More movers than one workgroup sorts in LDS: the grid-wide radix sort of
-    the mover-sort kernel takes over (one and several 1024-element sub-tiles per
-    workgroup), two steps in a row; the permutation is still the oracle's
-    stable sort and the mover count exact."""
+    """More movers than one workgroup sorts in LDS, two steps in a row: 10 K
+    particles (the grid grouping of listed movers) and 262 K particles (more
+    movers than are listed: the grid-wide radix sort of the mover kernel, one
+    and several 1024-element sub-tiles per workgroup); the permutation is
+    still the oracle's stable sort and the mover count exact."""
     s = sph.scenes.dam_break(side, side, seed=12)
     rng = np.random.default_rng(3)
     s.vx = rng.normal(0, sigma, s.n).astype(np.float32)  # ~0.5 to 5 cells per step
