@@ -17,9 +17,12 @@
 //             waits for another to decide the path). Up to kSmallSortCap movers
 //             workgroup 0 takes them from the lists, groups them by the merge
 //             tile they fall in and ranks them inside each group, counting the
-//             tile prefixes on the way; above that it compacts the movers from
-//             the flag words in previous-index order and the grid runs an LSD
-//             radix sort with bounded grid barriers. Either way it ends with
+//             tile prefixes on the way; up to kMidSortCap listed movers the
+//             whole grid does the same grouping, one mover per thread, with
+//             three grid barriers; above that (or with a window too large to
+//             rank by counting) workgroup 0 compacts the movers from the flag
+//             words in previous-index order and the grid runs an LSD radix sort
+//             with bounded grid barriers. Either way it ends with
 //             the movers sorted by (key, previous index), the 256-particle tile
 //             prefixes and, per merge tile, the number of movers before the
 //             tile's first particle.
