@@ -217,6 +217,7 @@ static_assert(kMidSortCap <= kMoverGrid * MS_TW, "one listed mover per thread");
 static_assert(kMidSortCap < (1 << 16), "packed 16-bit tile counters");
 constexpr uint32_t kMidGroupMax = 256;
 constexpr int kMidRankBatch = 8;
+constexpr int kMidRefine = 4;  // tile-first keys loaded together in the owner search
 
 // Flag words per thread and per batch of the ordered compaction.
 constexpr int MS_WPT = 12;
@@ -582,11 +583,19 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
     const int at = g * kSmallSortCap + (int)(ec - seg_lo[g]);
     const uint32_t mi = lidx[at];
     const uint64_t mk = lkey[at];
+    // the sampled tile-first table, its loads issued together (clamped)
     uint64_t* s_tab = reinterpret_cast<uint64_t*>(s_wc);
-    for (int q = tid; q < ne; q += MS_TW) {
-      const uint64_t tk = tfirst[(size_t)q * S];
-      if constexpr (WIDE) s_tab[q] = tk;
-      else s_tab[q] = O::make(tk, (uint32_t)(q * S * TW)).w;
+    uint64_t tk[kTileTab / MS_TW];
+#pragma unroll
+    for (int q = 0; q < kTileTab / MS_TW; ++q)
+      tk[q] = tfirst[(size_t)min(q * MS_TW + tid, ne - 1) * S];
+#pragma unroll
+    for (int q = 0; q < kTileTab / MS_TW; ++q) {
+      const int e2 = q * MS_TW + tid;
+      if (e2 < ne) {
+        if constexpr (WIDE) s_tab[e2] = tk[q];
+        else s_tab[e2] = O::make(tk[q], (uint32_t)(e2 * S * TW)).w;
+      }
     }
     auto tab = [&](int q) {
       if constexpr (WIDE) return O::make(s_tab[q], (uint32_t)(q * S * TW));
@@ -607,9 +616,18 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
       const int c = base - 1;  // last table entry before the mover, -1 if none
       own = max(c, 0) * S;
       if (c >= 0)
-        for (int k = 1; k < S; ++k) {
-          const int t = c * S + k;
-          if (t < tiles && O::make(tfirst[t], (uint32_t)t * TW) < mo) own = t;
+        // the tiles between two table entries, kMidRefine first keys loaded
+        // together (clamped) per round
+        for (int k0 = 1; k0 < S; k0 += kMidRefine) {
+          uint64_t fk[kMidRefine];
+#pragma unroll
+          for (int q = 0; q < kMidRefine; ++q)
+            fk[q] = tfirst[min(c * S + k0 + q, tiles - 1)];
+#pragma unroll
+          for (int q = 0; q < kMidRefine; ++q) {
+            const int t = c * S + k0 + q;
+            if (k0 + q < S && t < tiles && O::make(fk[q], (uint32_t)t * TW) < mo) own = t;
+          }
         }
       if (valid) {
         slot = atomicAdd(&tcount[own], 1u << 16) >> 16;
