@@ -282,13 +282,53 @@ constexpr int kCandBatch = SPH_CAND_BATCH;  // stencil candidates loaded togethe
 constexpr int kListBatch = SPH_LIST_BATCH;  // list slots loaded together (forces)
 
 // Cell ranges of the 3x3 stencil of position (x, y) in visiting order (x offset
-// outer, y offset inner); cells outside a walled box and empty cells give the
-// empty range (0, 0). Issuing the nine loads together keeps them in flight at once.
+// outer, y offset inner); cells outside a walled box give (0, 0) and empty
+// cells (-1, -1) or (0, 0): a length max(end - start, 0) of zero either way.
+// Issuing the nine loads together keeps them in flight at once.
 template <int BC>
 __device__ __forceinline__ void stencil_ranges(const DevGrid& g, const int2* __restrict__ cells,
                                                float x, float y, int2* rng) {
   const int cx = cell_coord(x, g.lo_x, g.cs_x, g.nx);
   const int cy = cell_coord(y, g.lo_y, g.cs_y, g.ny);
+  if (g.order == 0 && g.nx <= 16384 && g.ny <= 16384) {
+    // Morton keys below 2^28: the cell's coordinates are dilated once and the
+    // neighbors' parts stepped in dilated form (x on the even bits: x + 1 is
+    // ((d | odd bits) + 1) & even bits, x - 1 is (d - 1) & even bits; y the
+    // same on the odd bits), and the nine loads take 32-bit byte offsets from
+    // the table base. Empty cells keep (-1, -1), which the callers read as no
+    // candidates; cells outside a walled box are loaded at the cell's own key
+    // and masked to (0, 0).
+    constexpr uint32_t kEven = 0x55555555u, kOdd = 0xAAAAAAAAu;
+    const uint32_t dx = dilate16((uint32_t)cx), dy = dilate16((uint32_t)cy) << 1;
+    const uint32_t dxm = (dx - 1u) & kEven, dxp = ((dx | kOdd) + 1u) & kEven;
+    const uint32_t dym = (dy - 2u) & kOdd, dyp = ((dy | kEven) + 2u) & kOdd;
+    const bool xlo = cx > 0, xhi = cx + 1 < g.nx, ylo = cy > 0, yhi = cy + 1 < g.ny;
+    uint32_t kx[3], ky[3];
+    kx[1] = dx;
+    ky[1] = dy;
+    if (BC == 1) {
+      kx[0] = xlo ? dxm : dilate16((uint32_t)(g.nx - 1));
+      kx[2] = xhi ? dxp : 0u;
+      ky[0] = ylo ? dym : dilate16((uint32_t)(g.ny - 1)) << 1;
+      ky[2] = yhi ? dyp : 0u;
+    } else {
+      kx[0] = xlo ? dxm : dx;
+      kx[2] = xhi ? dxp : dx;
+      ky[0] = ylo ? dym : dy;
+      ky[2] = yhi ? dyp : dy;
+    }
+    const char* base = reinterpret_cast<const char*>(cells);
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+      rng[q] = *reinterpret_cast<const int2*>(base + ((kx[q / 3] + ky[q % 3]) << 3));
+    if (BC == 0) {
+      const bool okx[3] = {xlo, true, xhi}, oky[3] = {ylo, true, yhi};
+#pragma unroll
+      for (int q = 0; q < 9; ++q)
+        if (!(okx[q / 3] && oky[q % 3])) rng[q] = make_int2(0, 0);
+    }
+    return;
+  }
   // Key = (x part) + (y part) in both orders (the Morton parts have disjoint
   // bits). Cells outside a walled box are loaded at a clamped, valid key and
   // masked afterwards, so the nine loads carry no branch between them.
