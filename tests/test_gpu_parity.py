@@ -460,6 +460,32 @@ def test_incremental_resort_listed_movers_many_tiles(sph, orc):
     assert np.array_equal(perm, operm)
 
 
+@pytest.mark.parametrize("name", ["dam_break_16k", "periodic_9k", "random_clusters"])
+def test_step_call_splits_equivalent(sph, all_scenes, name):
+    """Six steps as one sph_step call or split over several calls give the same
+    state and the same fields: single-step calls have the merge write the
+    separate state arrays, longer calls keep only the packed copies until they
+    return, and the accelerations and separate density arrays come from a
+    call's last iteration only (the first of two graph-captured iterations
+    skips them)."""
+    s = all_scenes[name]
+    outs = []
+    for split in ([6], [1] * 6, [2, 3, 1], [1, 5], [3, 3]):
+        with sph.Context(s.params, s.n) as ctx:
+            ctx.upload(s.rx, s.ry, s.vx, s.vy)
+            ctx.prepare()
+            for k in split:
+                ctx.step(k)
+            outs.append((split, ctx.download(), ctx.fields()))
+    _, d0, f0 = outs[0]
+    for split, d, f in outs[1:]:
+        note(f"[gpu] step split {split} vs [6] on {name}")
+        for k in ("pid", "rx", "ry", "vx", "vy"):
+            assert np.array_equal(d[k], d0[k]), (split, k)
+        for k in ("ax", "ay", "rho"):
+            assert np.array_equal(f[k], f0[k]), (split, k)
+
+
 @pytest.mark.parametrize("pattern", ["spread", "converge"])
 def test_incremental_resort_mid_movers(sph, orc, pattern):
     """Between 4096 and 32768 movers, all listed: the mover kernel groups them
