@@ -18,8 +18,9 @@
 //             workgroup 0 takes them from the lists, groups them by the merge
 //             tile they fall in and ranks them inside each group, counting the
 //             tile prefixes on the way; up to kMidSortCap listed movers the
-//             whole grid does the same grouping, one mover per thread, with
-//             three grid barriers; above that (or with a window too large to
+//             whole grid does the same grouping, one mover per thread (the
+//             counters scanned in one chunk per workgroup), with three grid
+//             barriers; above that (or with a window too large to
 //             rank by counting) workgroup 0 compacts the movers from the flag
 //             words in previous-index order and the grid runs an LSD radix sort
 //             with bounded grid barriers. Either way it ends with
@@ -562,11 +563,13 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
   // grid, one mover per thread. Every workgroup finds its movers' owner tiles
   // (the tile-first table of every S-th tile in its LDS) and counts owner and
   // previous-index tiles with device atomics on the packed counters
-  // (tile_count, zero on entry); workgroup 0 scans them into the tile prefixes
-  // and window starts; each mover goes into its owner's window at the slot its
-  // atomic returned and is then ranked inside the window by counting. A
-  // window of more than kMidGroupMax movers sends every workgroup on to the
-  // radix path instead (decided by workgroup 0, read by all after a barrier).
+  // (tile_count, zero on entry); each workgroup scans one chunk of the
+  // counters, and after a barrier every workgroup combines the chunk totals
+  // into the tile prefixes and window starts; each mover goes into its
+  // owner's window at the slot its atomic returned and is then ranked inside
+  // the window by counting. A window of more than kMidGroupMax movers (the
+  // largest of the chunks' largest, seen by every workgroup) sends all of them
+  // on to the radix path instead.
   bool mid = listed <= (uint32_t)kMidSortCap;
 #pragma unroll
   for (int g = 0; g < kListSegs; ++g) mid = mid && cnt[g] <= (uint32_t)kSmallSortCap;
@@ -635,20 +638,25 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
       }
     }
     if (!grid_barrier(bar, gen, status, &s_ok)) return;
-    if (blockIdx.x == 0) {
-      // exclusive scan of the counter words, four tiles per thread per round
-      // (plain loads: the barrier's fence made the atomics visible, and this
-      // kernel has not read the counters before); both halves' prefixes stay
-      // below 2^16, so no carry between them
+    // Scan of the packed counters in chunks of consecutive tiles, one chunk
+    // per workgroup (plain loads: the barrier's fence made the atomics
+    // visible): each workgroup writes its chunk's local prefixes in place, the
+    // chunk's total and its largest window to counts[1 + wg] and
+    // counts[1 + nb + wg]. Both halves' prefixes stay below 2^16: no carry
+    // between them.
+    const int nbw = (int)gridDim.x;
+    const int chunk = (tiles + nbw - 1) / nbw;
+    const int c_lo = min((int)blockIdx.x * chunk, tiles), c_hi = min(c_lo + chunk, tiles);
+    {
       if (tid == 0) s_base[0] = 0;
       uint32_t carry = 0, gmax = 0;
-      for (int t0 = 0; t0 < tiles; t0 += 4 * MS_TW) {
+      for (int t0 = c_lo; t0 < c_hi; t0 += 4 * MS_TW) {
         uint32_t cv[4], sum = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const int t = t0 + 4 * tid + k, tc = min(t, tiles - 1);
-          const uint32_t v = tcount[tc];
-          cv[k] = t < tiles ? v : 0u;
+          const int t = t0 + 4 * tid + k, tc = min(t, c_hi - 1);
+          const uint32_t v = tcount[max(tc, 0)];
+          cv[k] = t < c_hi ? v : 0u;
           sum += cv[k];
           gmax = max(gmax, cv[k] >> 16);
         }
@@ -657,11 +665,7 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const int t = t0 + 4 * tid + k;
-          if (t < tiles) {
-            tcount[t] = run;
-            tile_base[t] = run & 0xffffu;
-            tile_lo[t] = run >> 16;
-          }
+          if (t < c_hi) tcount[t] = run;
           run += cv[k];
         }
         carry += rtot;
@@ -669,16 +673,40 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
       atomicMax(&s_base[0], gmax);
       __syncthreads();
       if (tid == 0) {
-        counts[0] = s_base[0] > kMidGroupMax ? 1u : 0u;
-        *n_movers = m;
-        tile_lo[tiles] = (uint32_t)m;
+        counts[1 + blockIdx.x] = carry;
+        counts[1 + nbw + blockIdx.x] = s_base[0];
       }
     }
     if (!grid_barrier(bar, gen, status, &s_ok)) return;
-    if (counts[0] == 0u) {  // (plain loads after the barrier's fence, as above)
-      const uint32_t w0 = tcount[own];
-      const uint32_t w1 = own + 1 < tiles ? tcount[own + 1] : (uint32_t)m << 16;
-      const uint32_t st = w0 >> 16, en = w1 >> 16;
+    // every workgroup: the chunk prefixes (one wave scans the nb totals) and
+    // the largest window of all; then its own chunk's tile prefixes and window
+    // starts
+    static_assert(kMoverGrid == kWave, "one wave scans the chunk totals");
+    if (tid < kWave) {
+      const uint32_t tot = counts[1 + tid], big = counts[1 + nbw + tid];
+      const uint32_t incl = wave_incl_scan(tot);
+      s_base[tid] = incl - tot;
+      uint32_t mx = big;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+      if (tid == 0) s_base[kWave] = mx;
+    }
+    __syncthreads();
+    if (s_base[kWave] <= kMidGroupMax) {
+      const uint32_t cpre = s_base[blockIdx.x];
+      for (int t = c_lo + tid; t < c_hi; t += MS_TW) {
+        const uint32_t v = tcount[t] + cpre;
+        tile_base[t] = v & 0xffffu;
+        tile_lo[t] = v >> 16;
+      }
+      if (blockIdx.x == 0 && tid == 0) {
+        *n_movers = m;
+        tile_lo[tiles] = (uint32_t)m;
+      }
+      // window of the mover's owner tile
+      auto prefix = [&](int t) { return (tcount[t] + s_base[t / chunk]) >> 16; };
+      const uint32_t st = prefix(own);
+      const uint32_t en = own + 1 < tiles ? prefix(own + 1) : (uint32_t)m;
       if (valid) {
         ktmp[st + slot] = mk;
         vtmp[st + slot] = mi;
