@@ -74,8 +74,9 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
             "movers": 16.0 / 256.0 + 24.0 * f
             + (0.0 if flags_in_force else 16.125 + 12.0 * f),
             # R old key 8, T r,v 16, pid 4; W r,v 16 (only the packed copy
-            # inside sph_step), pid 4, key 8, perm 4; clear the old ranges 8 occ
-            "merge": 60.0 + 8.0 * occ,
+            # inside sph_step), pid 4, key 8 (the permutation 4 only in the
+            # last step of a call); clear the old ranges 8 occ
+            "merge": 56.0 + 8.0 * occ,
             # ranges: R key 8, W 8 occ
             "neighbor_search": 8.0 + 8.0 * occ,
         }

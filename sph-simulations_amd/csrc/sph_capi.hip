@@ -53,9 +53,10 @@ struct Ctx {
   // inside sph_step on the per-particle kernels: the re-sort writes only the
   // packed copy of the state, the separate arrays are restored when it ends
   bool soa_lazy = false;
-  // the fused step writes the accelerations and the separate density and
-  // pressure arrays (off for the steps of a sph_step call but its last: only
-  // the last step's are observable; the steps read the packed copies)
+  // the fused step writes the accelerations, the separate density and
+  // pressure arrays and the incremental re-sort's permutation (off for the
+  // steps of a sph_step call but its last: only the last step's are
+  // observable; the steps read the packed copies)
   bool fields_out = true;
   // graph
   bool use_graph = false;
@@ -366,7 +367,7 @@ int incremental_resort(Ctx* c, bool flags_ready) {
   }
   {
     StageTimer t(c, SPH_T_MERGE);
-    HIPCHK(c, launch_merge(c->L, s, r, r.keys_alt, !c->soa_lazy));
+    HIPCHK(c, launch_merge(c->L, s, r, r.keys_alt, !c->soa_lazy, c->fields_out));
   }
   std::swap(s.keys_sorted, r.keys_alt);
   c->sb.keys_out = s.keys_sorted;

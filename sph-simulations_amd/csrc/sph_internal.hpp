@@ -200,9 +200,10 @@ int64_t resort_tiles(int n);
 // launch; a second one first when the producer of knew reported no movers)
 hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers& r,
                          const uint64_t* knew, int key_bits, bool flags_ready);
-// soa: also write the separate arrays rx, ry, vx, vy (else only s.xyuv)
+// soa: also write the separate arrays rx, ry, vx, vy (else only s.xyuv);
+// perm: also write the permutation s.perm (read only through the API)
 hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
-                        uint64_t* keys_dst, bool soa = true);
+                        uint64_t* keys_dst, bool soa = true, bool perm = true);
 // cell and block ranges of the sorted keys s.keys_sorted; clears the histogram
 hipError_t launch_ranges(const Launch& L, const DevState& s, int hist_words);
 hipError_t launch_update_velocities(const Launch& L, const DevState& s);

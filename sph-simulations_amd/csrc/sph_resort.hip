@@ -938,7 +938,7 @@ __global__ __launch_bounds__(TW) void k_merge(
       xyuv[pos] = make_float4(mx, my, mu, mv);
       pid_dst[pos] = mid_;
       keys_dst[pos] = km;
-      perm[pos] = src;
+      if (perm) perm[pos] = src;
     }
     __syncthreads();
   }
@@ -954,7 +954,7 @@ __global__ __launch_bounds__(TW) void k_merge(
     xyuv[pos] = make_float4(x, y, u, v);
     pid_dst[pos] = id;
     keys_dst[pos] = k;
-    perm[pos] = (uint32_t)j;
+    if (perm) perm[pos] = (uint32_t)j;
   }
 }
 
@@ -982,13 +982,13 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
 }
 
 hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
-                        uint64_t* keys_dst, bool soa) {
+                        uint64_t* keys_dst, bool soa, bool perm) {
   if (s.n <= 0) return hipSuccess;
   const int tiles = (int)resort_tiles(s.n);
   k_merge<<<tiles, TW, 0, L.stream>>>(s.n, s.keys_sorted, r.flags, r.tile_base, r.tile_lo,
                                       r.skey, r.sidx, s.tx, s.ty, s.tvx, s.tvy, s.pid,
                                       soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt,
-                                      keys_dst, s.perm, s.cells, s.blocks,
+                                      keys_dst, perm ? s.perm : nullptr, s.cells, s.blocks,
                                       s.block_bits, (uint64_t)s.n_keys, r.bar, r.lcount,
                                       s.xyuv);
   return hipGetLastError();
