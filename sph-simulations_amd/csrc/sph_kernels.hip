@@ -136,13 +136,25 @@ __global__ __launch_bounds__(BLK) void k_ranges(int n, const uint64_t* __restric
   const int i0 = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
+  // each slot's key, and one neighbor key per lane (the one before for lane
+  // 0, after for the others) loaded together; the neighbors inside the wave
+  // come from the adjacent lanes (slot q of the wave's lanes is 64
+  // consecutive indices)
+  const int lane = threadIdx.x & (kWave - 1);
   uint64_t kc[kRangesPer], kp[kRangesPer], kn[kRangesPer];
 #pragma unroll
   for (int q = 0; q < kRangesPer; ++q) {
     const int ic = min(i0 + q * stride, n - 1);
     kc[q] = keys[ic];
-    kp[q] = keys[max(ic - 1, 0)];
-    kn[q] = keys[min(ic + 1, n - 1)];
+    kn[q] = keys[lane == 0 ? max(ic - 1, 0) : min(ic + 1, n - 1)];
+  }
+#pragma unroll
+  for (int q = 0; q < kRangesPer; ++q) {
+    const uint64_t edge = kn[q];
+    kp[q] = __shfl_up(kc[q], 1);
+    kn[q] = __shfl_down(kc[q], 1);
+    if (lane == 0) kp[q] = edge;
+    if (lane == kWave - 1) kn[q] = edge;
   }
   const int sh = 2 * block_bits;
 #pragma unroll
