@@ -67,7 +67,7 @@ __device__ __forceinline__ unsigned long long flag_word(const unsigned long long
 // The movers' report for producers of the new keys that do not write it: the
 // flag words, and the lists with one atomic per workgroup of MF_TW threads (the
 // waves' counts summed in LDS), since such producers may have many movers
-// (list_movers: none at all after a step with more than kSmallSortCap).
+// (list_movers: none at all after a step with more than kMidSortCap).
 constexpr int MF_TW = 1024;
 __global__ __launch_bounds__(MF_TW) void k_mover_flags(int n, int words,
                                                        const uint64_t* __restrict__ knew,
@@ -355,7 +355,8 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
   const int words = 4 * tiles;
 
   // movers listed per segment by the producer (their sum is the number of set
-  // flag bits; the lists are whole when it is at most kSmallSortCap).
+  // flag bits; the lists are whole when no segment's count passes
+  // kSmallSortCap, and always when the sum is at most kSmallSortCap).
   // Workgroup 0 loads in the same round the tile-first keys of every S-th tile
   // (all of them when tiles <= kTileTab) and the first kSpec entries of every
   // list.
@@ -750,7 +751,7 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
   // through them)
   if (!grid_barrier(bar, gen, status, &s_ok)) return;
   // the exact number of movers, counted from the flag words by workgroup 0 (the
-  // list counters may stop short of it once past kSmallSortCap)
+  // list counters stop short of it when the producer listed nothing)
   const uint32_t total = ld_agent(reinterpret_cast<const uint32_t*>(n_movers));
   uint32_t* s_cnt = s_wc;    // [MS_WAVES][256] per-wave digit counts
   uint32_t* s_off = s_base;  // [256] running digit offsets
