@@ -5,6 +5,7 @@ cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 
@@ -52,11 +53,33 @@ _lib = None
 
 def lib():
     global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB):
-        raise RuntimeError(f"{LIB} missing: run `make -C oracle`")
-    L = C.CDLL(LIB)
+    if _lib is None:
+        _lib = _load(LIB)
+    return _lib
+
+
+def mutant_path(k: int) -> str:
+    """The checker built with planted mistake k (`make -C oracle mutants`)."""
+    return os.path.join(_HERE, "_build", f"liboracle_m{k}.so")
+
+
+@contextlib.contextmanager
+def variant(path: str):
+    """Route every call of this module through another build of the checker
+    (a mutant) inside the `with` block."""
+    global _lib
+    saved = lib()
+    _lib = _load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def _load(path: str):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing: run `make -C oracle`")
+    L = C.CDLL(path)
     vp, i32, i64, u32, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64
     P = C.POINTER
     sig = {
@@ -88,7 +111,6 @@ def lib():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = L
     return L
 
 

@@ -200,12 +200,30 @@ inline R ipow(R x, int e) {
   return r;
 }
 
+// Test hook: a nonzero ORC_MUTANT plants one known mistake in the restatement so
+// that tests/test_oracle.py can show its closed-form checks catch it (1: Monaghan
+// gate flipped, 2: mean density without the 1/2, 3: +alpha instead of -alpha,
+// 4: minimum image wraps one side only, 5: minimum image wraps the wrong way).
+// The checker and the CPU baseline are always built with 0.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 // Minimum-image separation for periodic boxes (no counterpart in the reference).
 template <class R>
 inline R min_image(R d, R len, R half) {
+#if ORC_MUTANT == 4
+  if (d > half) return d - len;
+  return d;
+#elif ORC_MUTANT == 5
+  if (d > half) return d + len;
+  if (d < -half) return d - len;
+  return d;
+#else
   if (d > half) return d - len;
   if (d < -half) return d + len;
   return d;
+#endif
 }
 
 template <class R>
@@ -360,10 +378,20 @@ inline void force_pair(const Consts<R>& c, R dx, R dy, R r2, R rho_i, R rho_j, R
       R dvx = vxi - vxj, dvy = vyi - vyj;
       R vr = dvx * dx + dvy * dy;
       R pij = (R)0;
+#if ORC_MUTANT == 1
+      if (vr > (R)0) {
+#else
       if (vr < (R)0) {
+#endif
         // Monaghan: -alpha c0 mu_ij / rho_ij with mu_ij = h vr / (r^2 + eps h^2),
         // rho_ij = (rho_i + rho_j) / 2, as one quotient
+#if ORC_MUTANT == 2
+        pij = (c.neg_alpha_c0 * (c.h * vr)) / ((r2 + c.eps_h2) * (rho_i + rho_j));
+#elif ORC_MUTANT == 3
+        pij = (-c.neg_alpha_c0 * (c.h * vr)) / ((r2 + c.eps_h2) * ((rho_i + rho_j) * (R)0.5));
+#else
         pij = (c.neg_alpha_c0 * (c.h * vr)) / ((r2 + c.eps_h2) * ((rho_i + rho_j) * (R)0.5));
+#endif
       }
       R coef = ((c.c_pres * (s * s)) / r) * ((pr_i + pr_j) + pij);
       a.fpx += coef * dx;
