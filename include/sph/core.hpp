@@ -519,8 +519,8 @@ class GenericNeighborLists {
 // ------------------------------------------------------- force computation
 namespace sph {
 // density + EOS, then forces, on the GPU; results mirrored into the container.
-// The list classes evaluate over the GPU cell stencil, which visits the same
-// neighbors in the same order as the (self-inclusive) lists they are given.
+// GenericForceComputation evaluates over the GPU cell stencil; the list classes
+// upload the lists they hold and evaluate those.
 inline void GpuDensity(SoAContainer& c, ScalarField& pressure) {
   void* ctx = c.DeviceContext();
   if (!ctx) throw Error("FindNeighbors must run first");
@@ -530,12 +530,44 @@ inline void GpuDensity(SoAContainer& c, ScalarField& pressure) {
   check(sph_download_fields(ctx, nullptr, nullptr, P.density.data(), pressure.data()), ctx,
         "sph_download_fields");
 }
-inline void GpuForces(SoAContainer& c) {
+inline void GpuForces(SoAContainer& c, int compute = SPH_COMPUTE_SORTED) {
   void* ctx = c.DeviceContext();
   if (!ctx) throw Error("FindNeighbors must run first");
-  check(sph_forces(ctx, SPH_COMPUTE_SORTED), ctx, "sph_forces");
+  check(sph_forces(ctx, compute), ctx, "sph_forces");
   auto& P = c.GetParticles();
   check(sph_download_fields(ctx, P.fx.data(), P.fy.data(), nullptr, nullptr), ctx,
+        "sph_download_fields");
+}
+// Hands the caller's self-inclusive lists (CSR: offsets[n + 1], indices) to the
+// GPU for SPH_COMPUTE_LISTS; they are evaluated as given, like the reference's
+// list loops (ListsForceComputation.cpp:3-145, MPListsForceComputation.cpp:3-147).
+template <class Off>
+inline void UploadLists(SoAContainer& c, const std::vector<Off>& offsets,
+                        const std::vector<int>& indices) {
+  void* ctx = c.DeviceContext();
+  if (!ctx) throw Error("FindNeighbors must run first");
+  const size_t n = c.Size();
+  if (offsets.size() != n + 1) throw Error("neighbor offsets must hold n + 1 entries");
+  std::vector<int64_t> off(offsets.begin(), offsets.end());
+  if ((size_t)off[n] > indices.size()) throw Error("neighbor offsets exceed the list");
+  check(sph_set_neighbor_lists(ctx, off.data(), indices.data(), 1), ctx,
+        "sph_set_neighbor_lists");
+}
+inline void UploadLists(SoAContainer& c, const std::vector<std::vector<int>>& lists) {
+  std::vector<int64_t> off(lists.size() + 1, 0);
+  std::vector<int> idx;
+  for (size_t i = 0; i < lists.size(); ++i) {
+    idx.insert(idx.end(), lists[i].begin(), lists[i].end());
+    off[i + 1] = (int64_t)idx.size();
+  }
+  UploadLists(c, off, idx);
+}
+inline void GpuListDensity(SoAContainer& c, ScalarField& pressure) {
+  void* ctx = c.DeviceContext();
+  check(sph_density_pressure(ctx, SPH_COMPUTE_LISTS), ctx, "sph_density_pressure");
+  auto& P = c.GetParticles();
+  pressure.resize(c.Size());
+  check(sph_download_fields(ctx, nullptr, nullptr, P.density.data(), pressure.data()), ctx,
         "sph_download_fields");
 }
 }  // namespace sph
@@ -561,6 +593,9 @@ class GenericForceComputation {
   GenericNeighborSearch<NeighborSearchType>& neighbor_search;
 };
 
+// Evaluates the per-particle lists it holds (self included, each particle
+// listing itself once), whatever their content: edited or filtered lists give
+// the list-based result, as in ListsForceComputation.cpp:3-145.
 class ListsForceComputation {
  public:
   ListsForceComputation(Real radius, std::vector<std::vector<int>>& nl) : h1(radius), neighbor_list(nl) {}
@@ -569,9 +604,15 @@ class ListsForceComputation {
     CalculatePressure(p);
     ComputeForces(p);
   }
-  void ApproximateDensity(SoAContainer& p) { sph::GpuDensity(p, pressure_field); }
+  void ApproximateDensity(SoAContainer& p) {
+    sph::UploadLists(p, neighbor_list);
+    sph::GpuListDensity(p, pressure_field);
+  }
   void CalculatePressure(SoAContainer&) {}
-  void ComputeForces(SoAContainer& p) { sph::GpuForces(p); }
+  void ComputeForces(SoAContainer& p) {
+    sph::UploadLists(p, neighbor_list);
+    sph::GpuForces(p, SPH_COMPUTE_LISTS);
+  }
   ScalarField& GetPressure() { return pressure_field; }
 
  protected:
@@ -580,6 +621,8 @@ class ListsForceComputation {
   std::vector<std::vector<int>>& neighbor_list;
 };
 
+// Evaluates the CSR lists it holds (offsets and indices, self included), as
+// MPListsForceComputation.cpp:3-147 does.
 class MPListsForceComputation {
  public:
   MPListsForceComputation(Real radius, std::vector<int>& nl, std::vector<int>& opp)
@@ -589,9 +632,15 @@ class MPListsForceComputation {
     CalculatePressure(p);
     ComputeForces(p);
   }
-  void ApproximateDensity(SoAContainer& p) { sph::GpuDensity(p, pressure_field); }
+  void ApproximateDensity(SoAContainer& p) {
+    sph::UploadLists(p, offset_per_particle, neighbor_list);
+    sph::GpuListDensity(p, pressure_field);
+  }
   void CalculatePressure(SoAContainer&) {}
-  void ComputeForces(SoAContainer& p) { sph::GpuForces(p); }
+  void ComputeForces(SoAContainer& p) {
+    sph::UploadLists(p, offset_per_particle, neighbor_list);
+    sph::GpuForces(p, SPH_COMPUTE_LISTS);
+  }
   ScalarField& GetPressure() { return pressure_field; }
 
  protected:

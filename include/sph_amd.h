@@ -146,6 +146,18 @@ int sph_find_neighbors(void* ctx);
  * (core/algorithmic/MultipassNeighborLists.h:14-18): CSR lists in stencil order.
  * include_self = 1 gives the CPU convention, 0 the GPU convention. */
 int sph_build_neighbor_lists(void* ctx, int32_t include_self, int64_t* total);
+/* Caller-built lists for SPH_COMPUTE_LISTS: the neighbor_list / offset_per_particle
+ * pair that ListsForceComputation and MPListsForceComputation are constructed with
+ * (core/compute/ListsForceComputation.h:5-66) and evaluate as given, without a
+ * distance test (ListsForceComputation.cpp:3-145, MPListsForceComputation.cpp:
+ * 3-147). Host CSR over the context's current sorted particles: offsets[n + 1]
+ * (offsets[0] = 0, non-decreasing) and indices in [0, n). include_self = 1: every
+ * row lists its own particle exactly once (the CPU convention; that entry is
+ * dropped and the kernels add the self term first); 0: no row lists itself.
+ * Violations return SPH_ERR_INVALID and leave the previous lists in place.
+ * Replaces the lists of sph_build_neighbor_lists until the next one. Synchronises. */
+int sph_set_neighbor_lists(void* ctx, const int64_t* offsets, const int32_t* indices,
+                           int32_t include_self);
 /* cuCallbackApproximateDensitySorted + cuInvertBuffer + cuKernelComputePressureSorted
  * (cuda_callbacks.cuh:40-46; cudaLaunchers.cuh:17-53), or the list variant
  * cuKernelApproximateDensityWithNeighborLists for SPH_COMPUTE_LISTS. */

@@ -149,6 +149,7 @@ def lib():
         "sph_size": (i64, [vp]),
         "sph_find_neighbors": (i32, [vp]),
         "sph_build_neighbor_lists": (i32, [vp, i32, P(i64)]),
+        "sph_set_neighbor_lists": (i32, [vp, vp, vp, i32]),
         "sph_density_pressure": (i32, [vp, i32]),
         "sph_forces": (i32, [vp, i32]),
         "sph_integrate": (i32, [vp]),
@@ -215,7 +216,8 @@ def lib():
 
 EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_get_constants",
             "sph_upload", "sph_download", "sph_download_fields", "sph_size",
-            "sph_find_neighbors", "sph_build_neighbor_lists", "sph_density_pressure",
+            "sph_find_neighbors", "sph_build_neighbor_lists", "sph_set_neighbor_lists",
+            "sph_density_pressure",
             "sph_forces", "sph_integrate", "sph_update_velocities", "sph_update_positions",
             "sph_set_dt", "sph_measure", "sph_prepare", "sph_step",
             "sph_set_graph", "sph_ell_overflow", "sph_set_step_kernel", "sph_set_resort", "sph_movers", "sph_block_stats", "sph_get_sorted_keys", "sph_get_permutation",
@@ -368,6 +370,17 @@ class Context:
         self._chk(self._L.sph_build_neighbor_lists(self._h, int(bool(include_self)),
                                                    C.byref(tot)), "sph_build_neighbor_lists")
         return int(tot.value)
+
+    def set_neighbor_lists(self, offsets, indices, include_self=False):
+        """Caller-built CSR lists (offsets int64 [n + 1], indices int32) over the
+        current sorted particles, evaluated by COMPUTE_LISTS as given."""
+        off = np.ascontiguousarray(offsets, np.int64)
+        idx = np.ascontiguousarray(indices, np.int32)
+        if idx.size == 0:
+            idx = np.zeros(1, np.int32)
+        self._chk(self._L.sph_set_neighbor_lists(self._h, _ptr(off), _ptr(idx),
+                                                 int(bool(include_self))),
+                  "sph_set_neighbor_lists")
 
     def density_pressure(self, compute=COMPUTE_SORTED):
         self._chk(self._L.sph_density_pressure(self._h, int(compute)), "sph_density_pressure")

@@ -58,6 +58,9 @@ struct Ctx {
   // steps of a sph_step call but its last: only the last step's are
   // observable; the steps read the packed copies)
   bool fields_out = true;
+  // ax, ay, rho, pr and p hold sph_step's last iteration in the order before its
+  // final re-sort; settle_fields moves them into the current order
+  bool fields_prev_order = false;
   // graph
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
@@ -322,10 +325,14 @@ int sort_keys(Ctx* c) {
 
 int hist_words(Ctx* c) { return sort_passes(c->grid.key_bits) * 256; }
 
+int settle_fields(Ctx* c);
+
 // Standalone find-neighbors: keys of S, sort, permute S (r, v, a, pid) into T,
 // ranges, then swap S and T.
 int find_neighbors(Ctx* c) {
   DevState& s = c->s;
+  int rc0 = settle_fields(c);  // the gather permutes a and uses the T arrays
+  if (rc0) return rc0;
   {
     StageTimer t(c, SPH_T_NEIGHBOR_SEARCH);
     if (c->n_scenes)
@@ -493,7 +500,31 @@ int sync_and_check(Ctx* c) {
   uint32_t st = 0;
   HIPCHK(c, hipMemcpyAsync(&st, c->sb.status, sizeof(st), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (st != 0) return fail(c, SPH_ERR_HIP, "radix sort look-back gave up waiting (device status)");
+  if (st & 4u)
+    return fail(c, SPH_ERR_HIP,
+                "incremental re-sort: a grid barrier of the mover kernel gave up waiting "
+                "(device status 4; its workgroups were not all resident) and the merge was "
+                "skipped: the state is invalid");
+  if (st != 0)
+    return fail(c, SPH_ERR_HIP, "radix sort look-back gave up waiting (device status " +
+                                    std::to_string(st) + ")");
+  return SPH_OK;
+}
+
+// The first call after sph_step that reads or changes the fields (or the T
+// arrays, which are free between calls) puts them into the current order,
+// so that index i of every array is the same particle. Costs one gather of
+// five arrays, once; the steps themselves never pay for it.
+int settle_fields(Ctx* c) {
+  if (!c->fields_prev_order) return SPH_OK;
+  c->fields_prev_order = false;
+  DevState& s = c->s;
+  float* tmp[5] = {s.tx, s.ty, s.tvx, s.tvy, s.tax};
+  HIPCHK(c, launch_settle_fields(c->L, s, c->n_scenes != 0, tmp));
+  float* dst[5] = {s.ax, s.ay, s.rho, s.pr, s.pres};
+  const size_t fb = (size_t)s.n * sizeof(float);
+  for (int k = 0; k < 5; ++k)
+    HIPCHK(c, hipMemcpyAsync(dst[k], tmp[k], fb, hipMemcpyDeviceToDevice, c->stream));
   return SPH_OK;
 }
 
@@ -697,6 +728,7 @@ int batch_configure(Ctx* c, const sph_batch_tuning& t) {
   c->s.n_prev = 0;  // the whole table was just cleared
   c->sorted = false;
   c->density_ell = false;
+  c->fields_prev_order = false;
   c->csr_total = -1;
   return SPH_OK;
 }
@@ -857,6 +889,7 @@ int sph_upload(void* ctx, int64_t n, const float* rx, const float* ry, const flo
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->sorted = false;
   c->density_ell = false;
+  c->fields_prev_order = false;
   c->csr_total = -1;
   if (c->graph_n != s.n) destroy_graph(c);
   return SPH_OK;
@@ -878,6 +911,8 @@ int sph_download(void* ctx, float* rx, float* ry, float* vx, float* vy, uint32_t
 int sph_download_fields(void* ctx, float* ax, float* ay, float* rho, float* pres) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
+  int rc = settle_fields(c);
+  if (rc) return rc;
   DevState& s = c->s;
   size_t fb = (size_t)s.n * sizeof(float);
   if (ax) HIPCHK(c, hipMemcpyAsync(ax, s.ax, fb, hipMemcpyDeviceToHost, c->stream));
@@ -934,11 +969,78 @@ int sph_build_neighbor_lists(void* ctx, int32_t include_self, int64_t* total) {
   return SPH_OK;
 }
 
+int sph_set_neighbor_lists(void* ctx, const int64_t* offsets, const int32_t* indices,
+                           int32_t include_self) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  if (c->n_scenes) return single_only(c);
+  int rc = check_n(c);
+  if (rc) return rc;
+  if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_find_neighbors first");
+  if (!offsets) return fail(c, SPH_ERR_INVALID, "offsets is null");
+  const int64_t n = c->s.n;
+  if (offsets[0] != 0) return fail(c, SPH_ERR_INVALID, "offsets[0] must be 0");
+  for (int64_t i = 0; i < n; ++i)
+    if (offsets[i + 1] < offsets[i])
+      return fail(c, SPH_ERR_INVALID, "offsets decrease at particle " + std::to_string(i));
+  const int64_t total = offsets[n];
+  if (total > 0 && !indices) return fail(c, SPH_ERR_INVALID, "indices is null");
+  if (total > (int64_t)INT32_MAX) return fail(c, SPH_ERR_INVALID, "more than 2^31-1 entries");
+  // every index is checked here: the kernels gather through them unchecked
+  std::vector<int64_t> off(n + 1);
+  std::vector<int32_t> idx;
+  idx.reserve((size_t)total);
+  off[0] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int self = 0;
+    for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+      const int32_t j = indices[k];
+      if (j < 0 || (int64_t)j >= n)
+        return fail(c, SPH_ERR_INVALID, "index " + std::to_string(j) + " of particle " +
+                                            std::to_string(i) + " outside [0, n)");
+      if (j == i) {
+        ++self;
+        if (include_self) continue;
+        return fail(c, SPH_ERR_INVALID, "particle " + std::to_string(i) +
+                                            " lists itself (pass include_self = 1)");
+      }
+      idx.push_back(j);
+    }
+    if (include_self && self != 1)
+      return fail(c, SPH_ERR_INVALID, "particle " + std::to_string(i) + " lists itself " +
+                                          std::to_string(self) + " times (expected once)");
+    off[i + 1] = (int64_t)idx.size();
+  }
+  DevState& s = c->s;
+  const int64_t tot = (int64_t)idx.size();
+  if (tot > c->csr_cap) {
+    int64_t cap = std::max<int64_t>(tot + tot / 4, 1024);
+    if (s.csr_idx) {
+      auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)s.csr_idx);
+      if (it != c->allocs.end()) c->allocs.erase(it);
+      HIPCHK(c, hipFree(s.csr_idx));
+      s.csr_idx = nullptr;
+    }
+    if ((rc = dev_alloc(c, &s.csr_idx, cap))) return rc;
+    c->csr_cap = cap;
+  }
+  HIPCHK(c, hipMemcpyAsync(s.csr_off, off.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice,
+                           c->stream));
+  if (tot > 0)
+    HIPCHK(c, hipMemcpyAsync(s.csr_idx, idx.data(), (size_t)tot * 4, hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // the host vectors go out of scope
+  c->csr_total = tot;
+  c->csr_self = 0;
+  c->density_ell = false;  // SPH_COMPUTE_LISTS now means these lists
+  return SPH_OK;
+}
+
 int sph_density_pressure(void* ctx, int32_t compute) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
   int rc = check_n(c);
-  if (rc) return rc;
+  if (rc || (rc = settle_fields(c))) return rc;
   if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_find_neighbors first");
   if (c->n_scenes) {
     if (compute != SPH_COMPUTE_SORTED && compute != SPH_COMPUTE_LISTS)
@@ -967,7 +1069,7 @@ int sph_forces(void* ctx, int32_t compute) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
   int rc = check_n(c);
-  if (rc) return rc;
+  if (rc || (rc = settle_fields(c))) return rc;
   if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_find_neighbors first");
   if (c->n_scenes) {
     if (compute != SPH_COMPUTE_SORTED && compute != SPH_COMPUTE_LISTS)
@@ -1001,7 +1103,7 @@ int sph_integrate(void* ctx) {
   if (!c) return SPH_ERR_INVALID;
   if (c->n_scenes) return single_only(c);
   int rc = check_n(c);
-  if (rc) return rc;
+  if (rc || (rc = settle_fields(c))) return rc;
   StageTimer t(c, SPH_T_INTEGRATION);
   HIPCHK(c, launch_integrate(c->L, c->s));
   c->density_ell = false;  // positions moved: neighbor data is stale
@@ -1025,7 +1127,7 @@ int sph_update_velocities(void* ctx) {
   if (!c) return SPH_ERR_INVALID;
   if (c->n_scenes) return single_only(c);
   int rc = check_n(c);
-  if (rc) return rc;
+  if (rc || (rc = settle_fields(c))) return rc;
   StageTimer t(c, SPH_T_INTEGRATION);
   HIPCHK(c, launch_update_velocities(c->L, c->s));
   c->density_ell = false;
@@ -1037,7 +1139,7 @@ int sph_update_positions(void* ctx) {
   if (!c) return SPH_ERR_INVALID;
   if (c->n_scenes) return single_only(c);
   int rc = check_n(c);
-  if (rc) return rc;
+  if (rc || (rc = settle_fields(c))) return rc;
   StageTimer t(c, SPH_T_INTEGRATION);
   HIPCHK(c, launch_update_positions(c->L, c->s));
   c->density_ell = false;
@@ -1110,6 +1212,7 @@ int sph_step(void* ctx, int32_t n_steps) {
     }
     return SPH_OK;
   };
+  c->fields_prev_order = false;  // every field is rewritten by the last iteration
   rc = run();
   c->fields_out = true;
   c->soa_lazy = false;
@@ -1120,6 +1223,7 @@ int sph_step(void* ctx, int32_t n_steps) {
   if (rc) return rc;
   c->csr_total = -1;
   c->density_ell = false;
+  c->fields_prev_order = true;  // computed before the final re-sort
   return SPH_OK;
 }
 
@@ -1264,6 +1368,8 @@ int sph_get_ell(void* ctx, int32_t* counts, int32_t* slots) {
 int sph_get_device_view(void* ctx, sph_device_view* out) {
   Ctx* c = as_ctx(ctx);
   if (!c || !out) return SPH_ERR_INVALID;
+  int rc = settle_fields(c);  // the view's fields follow the current order
+  if (rc) return rc;
   DevState& s = c->s;
   out->n = s.n;
   out->rx = s.rx; out->ry = s.ry; out->vx = s.vx; out->vy = s.vy;
@@ -1586,6 +1692,7 @@ int sph_upload_device(void* ctx, int64_t n, const float* d_rx, const float* d_ry
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->sorted = false;
   c->density_ell = false;
+  c->fields_prev_order = false;
   c->csr_total = -1;
   if (c->graph_n != s.n) destroy_graph(c);
   return SPH_OK;
@@ -1779,7 +1886,8 @@ int sph_surrogate_run(void* ctx, void* nn, int32_t n_steps, int32_t check_every,
   if (c->s.n > h->net.n_cap) return fail(c, SPH_ERR_CAPACITY, "more particles than max_nodes");
   DevState& s = c->s;
   const int64_t N = c->n_max;
-  int rc = SPH_OK;
+  int rc = settle_fields(c);
+  if (rc) return rc;
   if (mode == 1 && !c->backup) {
     if ((rc = dev_alloc(c, &c->backup, 6 * N)) || (rc = dev_alloc(c, &c->backup_pid, N)))
       return rc;
@@ -1872,7 +1980,7 @@ int sph_cfl(void* ctx, double cfl, double* out3) {
   if (!(cfl > 0) || !std::isfinite(cfl)) return fail(c, SPH_ERR_INVALID, "cfl must be positive");
   if (c->n_scenes) return single_only(c);
   int rc = check_n(c);
-  if (rc) return rc;
+  if (rc || (rc = settle_fields(c))) return rc;
   double m[2] = {0, 0};
   HIPCHK(c, launch_max_va(c->L, c->s, c->meas_part, c->meas_out));
   HIPCHK(c, hipMemcpyAsync(m, c->meas_out, sizeof(m), hipMemcpyDeviceToHost, c->stream));

@@ -134,17 +134,20 @@ __device__ __forceinline__ float ipowf(float x, int e) {
   return r;
 }
 
+// p from the EOS: linear (reference) or Tait.
+template <int PHYS>
+__device__ __forceinline__ float eos_pressure(const DevPhys& c, float rho) {
+  if (PHYS == 0) return c.eos_k * (rho - c.rho0);
+  const float x = rho * c.inv_rho0;
+  return c.tait_b * (ipowf(x, c.gamma) - 1.0f);
+}
+
 // rho from the kernel sum; p from the EOS; pr = p / rho^2.
 template <int PHYS>
 __device__ __forceinline__ void eos(const DevPhys& c, float acc, float& rho, float& p,
                                    float& pr) {
   rho = c.mc_rho * acc;
-  if (PHYS == 0) {
-    p = c.eos_k * (rho - c.rho0);
-  } else {
-    float x = rho * c.inv_rho0;
-    p = c.tait_b * (ipowf(x, c.gamma) - 1.0f);
-  }
+  p = eos_pressure<PHYS>(c, rho);
   pr = p / (rho * rho);
 }
 

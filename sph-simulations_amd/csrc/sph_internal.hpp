@@ -177,6 +177,10 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
 // to s.rp as well
 hipError_t launch_density(const Launch& L, const DevState& s, int nb_mode, bool write_p,
                           bool packed = false);
+// ax, ay, rho, pr, p of the previous order into out5[0..4] through s.perm (p
+// gathered from s.pres when gather_p, else recomputed from rho)
+hipError_t launch_settle_fields(const Launch& L, const DevState& s, bool gather_p,
+                                float* const* out5);
 // s.xyuv from the sorted state, and back
 hipError_t launch_pack(const Launch& L, const DevState& s);
 hipError_t launch_unpack(const Launch& L, const DevState& s);

@@ -674,6 +674,43 @@ hipError_t launch_density(const Launch& L, const DevState& s, int nb, bool write
   return density_dispatch<1, 1>(L, s, nb, write_p, packed);
 }
 
+// The fields of the last iteration of sph_step (computed before its final
+// re-sort) moved into the current order: out[i] = field[perm[i]]. p is
+// recomputed from rho with the density kernel's EOS (same function, same
+// input: the same bits) unless the step wrote it (batched scenes, whose
+// constants differ per scene), in which case it is gathered.
+template <int PHYS>
+__global__ __launch_bounds__(BLK) void k_settle_fields(
+    int n, DevPhys c, const uint32_t* __restrict__ perm, const float* __restrict__ ax,
+    const float* __restrict__ ay, const float* __restrict__ rho, const float* __restrict__ pr,
+    const float* __restrict__ pres, float* __restrict__ oax, float* __restrict__ oay,
+    float* __restrict__ orho, float* __restrict__ opr, float* __restrict__ opres) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = min(perm[i], (uint32_t)(n - 1));  // in bounds even if perm were bad
+  const float r = rho[j];
+  oax[i] = ax[j];
+  oay[i] = ay[j];
+  orho[i] = r;
+  opr[i] = pr[j];
+  opres[i] = pres ? pres[j] : eos_pressure<PHYS>(c, r);
+}
+
+hipError_t launch_settle_fields(const Launch& L, const DevState& s, bool gather_p,
+                                float* const* out5) {
+  if (s.n <= 0) return hipSuccess;
+  const float* pres = gather_p ? s.pres : nullptr;
+  if (L.phys.physics == 0)
+    k_settle_fields<0><<<blocks_for(s.n), BLK, 0, L.stream>>>(
+        s.n, L.phys, s.perm, s.ax, s.ay, s.rho, s.pr, pres, out5[0], out5[1], out5[2], out5[3],
+        out5[4]);
+  else
+    k_settle_fields<1><<<blocks_for(s.n), BLK, 0, L.stream>>>(
+        s.n, L.phys, s.perm, s.ax, s.ay, s.rho, s.pr, pres, out5[0], out5[1], out5[2], out5[3],
+        out5[4]);
+  return hipGetLastError();
+}
+
 hipError_t launch_unpack(const Launch& L, const DevState& s) {
   if (s.n > 0)
     k_unpack<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, s.xyuv, s.rx, s.ry, s.vx, s.vy);

@@ -853,7 +853,11 @@ __global__ __launch_bounds__(TW) void k_merge(
     float* __restrict__ ry, float* __restrict__ vx, float* __restrict__ vy,
     uint32_t* __restrict__ pid_dst, uint64_t* __restrict__ keys_dst, uint32_t* __restrict__ perm,
     int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
-    uint32_t* __restrict__ bar, uint32_t* __restrict__ lcount, float4* __restrict__ xyuv) {
+    uint32_t* __restrict__ bar, uint32_t* __restrict__ lcount, float4* __restrict__ xyuv,
+    const uint32_t* __restrict__ status) {
+  // A mover kernel whose grid barrier gave up left its windows partly written:
+  // merge nothing (the host reports the status word at the next check).
+  if (*status != 0u) return;
   __shared__ uint64_t s_tk[TW];  // the tile's previous keys (all ones past n)
   __shared__ uint64_t s_mk[WIN];
   __shared__ uint32_t s_mi[WIN];
@@ -866,7 +870,8 @@ __global__ __launch_bounds__(TW) void k_merge(
   // every load that depends only on j, issued together
   const unsigned long long fw = flag_word(flags, j0 / kWave + wave, n);
   const uint32_t tb = tile_base[tile];
-  const int lo = (int)tile_lo[tile], hi = (int)tile_lo[tile + 1];
+  // windows lie in [0, n) (clamped, so that a bad table cannot read past skey)
+  const int lo = min((int)tile_lo[tile], n), hi = max(lo, min((int)tile_lo[tile + 1], n));
   const int jc = min(j, n - 1);
   const uint64_t kc = kold[jc], kpc = kold[max(jc - 1, 0)];
   const uint64_t k = live ? kc : ~0ull;
@@ -930,6 +935,7 @@ __global__ __launch_bounds__(TW) void k_merge(
       uint32_t mb = tb + s_wpre[w];
       if (w < TW / kWave) mb += (uint32_t)__popcll(s_fw[w] & ((1ull << l) - 1ull));
       const int pos = (c0 + r) + (j0 + a - (int)mb);
+      if ((unsigned)pos >= (unsigned)n) continue;  // only a corrupt table gets here
       if (rx) {
         rx[pos] = mx;
         ry[pos] = my;
@@ -946,6 +952,7 @@ __global__ __launch_bounds__(TW) void k_merge(
   if (live && !mv) {
     const uint32_t mb = tb + s_wpre[wave] + (uint32_t)__popcll(fw & ((1ull << lane) - 1ull));
     const int pos = (j - (int)mb) + lo + before_movers;
+    if ((unsigned)pos >= (unsigned)n) return;  // only a corrupt table gets here
     if (rx) {
       rx[pos] = x;
       ry[pos] = y;
@@ -991,7 +998,7 @@ hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers&
                                       soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt,
                                       keys_dst, perm ? s.perm : nullptr, s.cells, s.blocks,
                                       s.block_bits, (uint64_t)s.n_keys, r.bar, r.lcount,
-                                      s.xyuv);
+                                      s.xyuv, r.status);
   return hipGetLastError();
 }
 
