@@ -178,6 +178,20 @@ int sph_set_dt(void* ctx, double dt);
  * domain, out[3] potential energy, out[4] total mass. Synchronises. */
 int sph_measure(void* ctx, double* out5);
 
+/* Allocation-free sph_measure over caller device arrays. It has no
+ * counterpart in the reference; it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). No context: rx, ry, vx, vy are n device floats, out5 is 5
+ * device doubles (the values of sph_measure, bit for bit), params supplies
+ * mass, gravity_y and the domain. Runs on the calling thread's current
+ * device on `stream` (a hipStream_t; NULL = the null stream), asynchronously,
+ * and calls no hipMalloc/hipFree: the partial sums live in a static 1 MiB
+ * device array per GPU, so n <= 8,388,608 (larger n: SPH_ERR_CAPACITY).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * race. Errors: sph_last_error(NULL). */
+int sph_measure_noalloc(const sph_params* params, int64_t n, const float* rx, const float* ry,
+                        const float* vx, const float* vy, double* out5, void* stream);
+
 /* Time step from the CFL and force criteria with FP64 maxima over the current
  * state (velocities and the accelerations of the last force evaluation):
  *   dt = min(cfl * h / (c0 + max|v|), 0.25 * sqrt(h / max|a|))

@@ -205,6 +205,7 @@ def lib():
         "sph_nn_adam": (i32, [vp, vp, d, d, d, d]),
         "sph_nn_train_step": (i32, [vp, vp, d, P(d)]),
         "sph_surrogate_run": (i32, [vp, vp, i32, i32, d, i32, P(SurrogateStats)]),
+        "sph_measure_noalloc": (i32, [P(SphParams), i64, vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -231,7 +232,7 @@ EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_
             "sph_upload_device", "sph_nn_create", "sph_nn_destroy", "sph_nn_last_error",
             "sph_nn_num_params", "sph_nn_set_params", "sph_nn_get_params", "sph_nn_get_grad",
             "sph_nn_reset_optimizer", "sph_nn_set_clip", "sph_nn_forward", "sph_nn_loss_grad", "sph_nn_adam",
-            "sph_nn_train_step", "sph_surrogate_run")
+            "sph_nn_train_step", "sph_surrogate_run", "sph_measure_noalloc")
 
 
 class DeviceBuffer:

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,8 @@ using namespace sphamd;
 namespace {
 
 thread_local std::string g_create_error;
+// message of the last failed context-free call that must not allocate (static text)
+thread_local const char* g_static_error = nullptr;
 
 struct PendingTiming {
   int stage;
@@ -772,6 +775,7 @@ extern "C" {
 
 int sph_create(const sph_params* params, int64_t n_max, int device, void** out_ctx) {
   g_create_error.clear();
+  g_static_error = nullptr;
   if (!params || !out_ctx || n_max <= 0 || n_max >= ((int64_t)1 << 30)) {
     g_create_error = "sph_create: invalid arguments (n_max must be in [1, 2^30))";
     return SPH_ERR_INVALID;
@@ -833,7 +837,7 @@ int sph_destroy(void* ctx) {
 
 const char* sph_last_error(const void* ctx) {
   const Ctx* c = static_cast<const Ctx*>(ctx);
-  if (!c) return g_create_error.c_str();
+  if (!c) return g_static_error ? g_static_error : g_create_error.c_str();
   return c->err.c_str();
 }
 
@@ -1415,6 +1419,7 @@ int sph_synchronize(void* ctx) {
 int sph_batch_create(const sph_params* scenes, int32_t n_scenes, const int64_t* counts,
                      int device, void** out_ctx) {
   g_create_error.clear();
+  g_static_error = nullptr;
   if (!scenes || !counts || !out_ctx || n_scenes < 1 || n_scenes > (1 << 20)) {
     g_create_error = "sph_batch_create: invalid arguments";
     return SPH_ERR_INVALID;
@@ -2061,6 +2066,7 @@ extern "C" {
 
 int sph_snapshot_info(const char* path, sph_params* params, int64_t* n, int64_t* step) {
   g_create_error.clear();
+  g_static_error = nullptr;
   if (!path) return SPH_ERR_INVALID;
   FILE* f = std::fopen(path, "rb");
   if (!f) {
@@ -2158,3 +2164,82 @@ int sph_snapshot_load(void* ctx, const char* path, int64_t* step) {
 }
 
 }  // extern "C"
+
+// ---- the allocation-free measurement (sph_measure_noalloc) ----
+namespace {
+
+// Partial sums of the FP64 measurement kernels for callers that cannot
+// allocate: 4 doubles per 256-particle block, 1 MiB per GPU (held until the
+// process ends), so at most kNoAllocMaxN particles per call. Host code never
+// passes `scratch` itself to a kernel (its host-side name is no device
+// address): it uses the address hipGetSymbolAddress returns for the current
+// device, looked up once per device.
+constexpr int64_t kNoAllocWords = 131072;
+constexpr int64_t kNoAllocMaxN = kNoAllocWords / 4 * 256;  // 8,388,608
+constexpr int kNoAllocDevices = 64;
+__device__ __align__(256) double scratch[kNoAllocWords];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoAllocDevices] = {};
+
+// Address of this device's copy of `scratch` (set up on the first call on a
+// device, with that device current; no allocation).
+hipError_t scratch_for_current_device(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kNoAllocDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+int noalloc_fail(int code, const char* msg) {
+  g_static_error = msg;
+  return code;
+}
+
+}  // namespace
+
+extern "C" int sph_measure_noalloc(const sph_params* params, int64_t n, const float* rx,
+                                   const float* ry, const float* vx, const float* vy,
+                                   double* out5, void* stream) {
+  if (!params || !out5) return noalloc_fail(SPH_ERR_INVALID, "sph_measure_noalloc: null argument");
+  if (n < 0 || n > kNoAllocMaxN)
+    return noalloc_fail(SPH_ERR_CAPACITY,
+                        "sph_measure_noalloc: n must be in [0, 8388608] (the static scratch "
+                        "holds the partial sums of 8,388,608 particles)");
+  if (n > 0 && (!rx || !ry || !vx || !vy))
+    return noalloc_fail(SPH_ERR_INVALID, "sph_measure_noalloc: null particle array");
+  double* part = nullptr;
+  if (scratch_for_current_device(&part) != hipSuccess)
+    return noalloc_fail(SPH_ERR_HIP, "sph_measure_noalloc: hipGetSymbolAddress of the scratch "
+                                     "array failed on the current device");
+  if (measure_partial_words((int)n) > kNoAllocWords)
+    return noalloc_fail(SPH_ERR_CAPACITY, "sph_measure_noalloc: scratch too small");
+  // the same kernels and partial layout as sph_measure (bit-identical results);
+  // of the physics constants only the domain bounds are read
+  Launch L{};
+  L.stream = static_cast<hipStream_t>(stream);
+  L.phys.lo_x = (float)params->lo[0];
+  L.phys.lo_y = (float)params->lo[1];
+  L.phys.hi_x = (float)params->hi[0];
+  L.phys.hi_y = (float)params->hi[1];
+  DevState s{};
+  s.n = (int)n;
+  s.rx = const_cast<float*>(rx);
+  s.ry = const_cast<float*>(ry);
+  s.vx = const_cast<float*>(vx);
+  s.vy = const_cast<float*>(vy);
+  if (launch_measure(L, s, params->mass, params->gravity_y, part, out5) != hipSuccess)
+    return noalloc_fail(SPH_ERR_HIP, "sph_measure_noalloc: kernel launch failed");
+  g_static_error = nullptr;
+  return SPH_OK;
+}
+
