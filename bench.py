@@ -121,24 +121,257 @@ def measured_issue(kernel):
     return out or None
 
 
-def cpu_baseline(sph, scene, budget_s):
-    from oracle import oracle as orc
-    threads, t, _ = orc.baseline_run(scene.params, scene.rx, scene.ry, scene.vx, scene.vy, 2)
-    per_step = t["main_loop"] / 2.0
-    steps = int(max(2, min(60, budget_s / max(per_step, 1e-6))))
-    log(f"cpu baseline: {per_step:.3f} s/step on {threads} threads -> sampling {steps} steps")
-    threads, t, _ = orc.baseline_run(scene.params, scene.rx, scene.ry, scene.vx, scene.vy, steps)
-    value = scene.n * steps / t["main_loop"]
+def cpu_info():
+    """CPU model and the physical cores this process may run on (affinity)."""
+    model, cores = "unknown", set()
+    try:
+        allowed = os.sched_getaffinity(0)
+        cur = {}
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                if "processor" in cur and int(cur["processor"]) in allowed:
+                    cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                cur = {}
+                continue
+            k, _, v = line.partition(":")
+            cur[k.strip()] = v.strip()
+            if k.strip() == "model name" and model == "unknown":
+                model = v.strip()
+        if "processor" in cur and int(cur["processor"]) in allowed:
+            cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+    except OSError:
+        pass
+    return model, len(cores) or (os.cpu_count() or 1)
+
+
+def cpu_baseline(scene, budget_s):
+    """The reference-style OpenMP CPU step (oracle/cpu_baseline.cpp: compute_type
+    mp-lists, and sorted) on a bounded sample of the same workload, in a fresh
+    child process started with the reference's OpenMP recipe
+    (blog/blog.md:126-127): OMP_SCHEDULE=static, OMP_PROC_BIND=true,
+    OMP_NUM_THREADS = the physical cores this process may use (capped by an
+    OMP_NUM_THREADS already set, the host's share for one GPU)."""
+    import dataclasses
+    import subprocess
+    import tempfile
+    model, phys = cpu_info()
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(phys, cap) if cap > 0 else phys
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_SCHEDULE="static",
+               OMP_PROC_BIND="true")
+    with tempfile.TemporaryDirectory() as tmp:
+        npz = os.path.join(tmp, "state.npz")
+        np.savez(npz, rx=scene.rx, ry=scene.ry, vx=scene.vx, vy=scene.vy)
+        prm = os.path.join(tmp, "params.json")
+        json.dump(dataclasses.asdict(scene.params), open(prm, "w"))
+        log(f"cpu baseline: {threads} threads ({phys} physical cores visible, {model})")
+        r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline_run", npz, prm,
+                            str(budget_s)], cwd=ROOT, env=env, capture_output=True, text=True,
+                           timeout=max(120.0, 6 * budget_s))
+    sys.stderr.write(r.stderr)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline child failed ({r.returncode})")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    main = res["mp-lists"]
     return {
-        "value": value,
+        "value": main["value"],
         "unit": "particle-steps/s",
-        "cores": threads,
+        "cores": main["threads"],
         "kind": "port",
-        "sample": f"{scene.name}: {scene.n} particles x {steps} steps of the reference-style "
-                  f"OpenMP mp-lists/Morton CPU step (serial sort/gather/ranges/scan/EOS/"
-                  f"integrator), -O3 -ffast-math",
-        "stage_seconds": {k: round(v, 4) for k, v in t.items()},
+        "sample": f"{scene.name}: {scene.n} particles x {main['steps']} steps of the "
+                  f"reference-style OpenMP mp-lists/Morton CPU step (serial sort/gather/"
+                  f"ranges/scan/EOS/integrator), -O3 -ffast-math; the sorted (cell-stencil) "
+                  f"variant under 'variants'",
+        "cpu_model": model,
+        "physical_cores_visible": phys,
+        "omp_env": {"OMP_NUM_THREADS": threads, "OMP_SCHEDULE": "static",
+                    "OMP_PROC_BIND": "true"},
+        "variants": res,
+        "stage_seconds": main["stage_seconds"],
     }
+
+
+def mover_path(m):
+    """Which path of the incremental re-sort's mover kernel a step with m movers
+    takes: one workgroup (up to 4,096 listed movers), the grid-wide grouping (up
+    to 32,768) or the grid radix sort."""
+    return "workgroup" if m <= 4096 else ("grid_grouping" if m <= 32768 else "radix")
+
+
+def stage_table(stage, steps, n, k, occ, passes, variant, resort, mfrac):
+    """Per-stage launches, time and achieved algorithmic GB/s of an events pass
+    of `steps` steps; returns (stages, algorithmic bytes per particle-step, kernel ms)."""
+    flags_in_force = variant == "particle" and resort == "incremental"
+    stages, step_bytes, kern_ms = {}, 0.0, 0.0
+    for name, (ms, launches) in stage.items():
+        if launches == 0:
+            continue
+        per = ms / launches
+        entry = {"launches": launches, "ms_total": round(ms, 4),
+                 "us_per_launch": round(per * 1e3, 3)}
+        bpp = alg_bytes_per_particle(name, k, occ, passes, variant, resort, mfrac,
+                                     flags_in_force)
+        if bpp is not None:
+            entry["alg_bytes_per_launch"] = int(bpp * n)
+            entry["achieved_GBs"] = round(bpp * n / (per * 1e-3) / 1e9, 1)
+            step_bytes += bpp * launches / steps
+        stages[name] = entry
+        kern_ms += ms
+    return stages, step_bytes, kern_ms / steps
+
+
+def full_run_profile(sph, scene, total, chunk=100):
+    """The workload's whole run (`total` steps from rest, graph-replayed), timed
+    in chunks of `chunk` steps; after each chunk the movers of its last step and
+    the mover path they take. Returns the run's mean step time and the profile."""
+    n = scene.n
+    ctx = sph.Context(scene.params, n)
+    ctx.upload(scene.rx, scene.ry, scene.vx, scene.vy)
+    ctx.prepare()
+    ctx.set_graph(True)
+    rows, t_sum, done = [], 0.0, 0
+    while done < total:
+        kk = min(chunk, total - done)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.step(kk)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        t_sum += dt
+        done += kk
+        rows.append([done, round(dt * 1e3 / kk, 4), ctx.movers()])
+    m = ctx.measure()
+    ctx.close()
+    paths = {}
+    for r in rows:
+        paths[mover_path(r[2])] = paths.get(mover_path(r[2]), 0) + 1
+    q = max(1, len(rows) // 10)
+    out = {"steps": total, "value": n * total / t_sum, "ms_per_step": t_sum * 1e3 / total,
+           "chunk_steps": chunk,
+           "ms_per_step_first_tenth": float(np.mean([r[1] for r in rows[:q]])),
+           "ms_per_step_last_tenth": float(np.mean([r[1] for r in rows[-q:]])),
+           "movers_last_step_of_chunk": {"min": int(min(r[2] for r in rows)),
+                                         "max": int(max(r[2] for r in rows)),
+                                         "mean": float(np.mean([r[2] for r in rows]))},
+           "mover_path_chunks": paths,
+           "chunks": rows,
+           "final": {"kinetic_energy": m[0], "max_v2": m[1], "inside": int(m[2])}}
+    log(f"full run {total} steps: {out['ms_per_step']:.4f} ms/step "
+        f"(first tenth {out['ms_per_step_first_tenth']:.4f}, last "
+        f"{out['ms_per_step_last_tenth']:.4f}), movers up to {out['movers_last_step_of_chunk']['max']}")
+    return out
+
+
+def bench_periodic(sph, steps, ref_steps=20):
+    """The 4,194,304-particle periodic square: `steps` graph-replayed fused steps
+    timed (host clock), again with per-launch hipEvents (the fused pipeline's
+    kernels), and `ref_steps` iterations of the reference's stage order through
+    the stage API (integrate, find neighbors = keys + radix sort + permutation +
+    cell ranges, CSR lists, density + EOS over the lists, forces over the
+    lists) timed per stage under the reference's timer names
+    (app/simulation_configuration.h:72-91)."""
+    sc = sph.scenes.periodic_shear()
+    n = sc.n
+    log(f"periodic: n={n}")
+    ctx = sph.Context(sc.params, n)
+    ctx.upload(sc.rx, sc.ry, sc.vx, sc.vy)
+    ctx.prepare()
+    ctx.set_graph(True)
+    ctx.step(10)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    value = n * steps / wall
+    ev = min(steps, 100)
+    ctx.set_timing(True)
+    ctx.step(ev)
+    ctx.synchronize()
+    stage = ctx.stage_times()
+    ctx.set_timing(False)
+    mfrac = ctx.movers() / n
+    k = ctx.build_neighbor_lists(False) / n
+    st, _ = ctx.cell_ranges()
+    occ = float((st >= 0).sum()) / n
+    passes = (int(ctx.grid().key_bits) + 7) // 8
+    stages, step_bytes, kern_ms = stage_table(stage, ev, n, k, occ, passes, "particle",
+                                              "incremental", mfrac)
+    # the reference's stage order, each stage its own launches
+    ctx.find_neighbors()
+    ctx.build_neighbor_lists(False)
+    ctx.density_pressure(sph.COMPUTE_LISTS)
+    ctx.forces(sph.COMPUTE_LISTS)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(ref_steps):
+        ctx.integrate()
+        ctx.find_neighbors()
+        ctx.build_neighbor_lists(False)
+        ctx.density_pressure(sph.COMPUTE_LISTS)
+        ctx.forces(sph.COMPUTE_LISTS)
+    ctx.synchronize()
+    ref_wall = time.perf_counter() - t0
+    rs = ctx.stage_times()
+    ctx.set_timing(False)
+    ctx.close()
+    per = lambda name: rs[name][0] / ref_steps
+    reference_stages_ms = {
+        "integration": per("integration"),
+        "neighbor_search": per("neighbor_search") + per("sort_histogram") + per("sort_pass"),
+        "morton_sort (part of neighbor_search)": per("sort_histogram") + per("sort_pass"),
+        "cell_build (part of neighbor_search: keys, permutation, ranges)": per("neighbor_search"),
+        "neighbor_list": per("neighbor_list"),
+        "approximate_density (with calculate_pressure)": per("approximate_density"),
+        "force_computation": per("force_computation"),
+    }
+    b_canon = 296.0
+    return {
+        "workload": sc.name, "particles": n, "steps": steps, "value": value,
+        "unit": "particle-steps/s", "ms_per_step": wall * 1e3 / steps,
+        "target_60pct_of_roofline": 1.62e10, "vs_target": value / 1.62e10,
+        "roofline_b_canon": {"bytes_per_particle_step": b_canon,
+                             "frac": value * b_canon / (HBM_PEAK_GBS * 1e9)},
+        "step_alg_bytes_per_particle": round(step_bytes, 1),
+        "step_frac_of_hbm": step_bytes * value / (HBM_PEAK_GBS * 1e9),
+        "mean_neighbors": round(k, 3), "occupied_cells_per_particle": round(occ, 4),
+        "movers_last_step": int(round(mfrac * n)),
+        "fused_stages": stages, "fused_kernel_ms_per_step": kern_ms,
+        "reference_order": {"steps": ref_steps, "ms_per_step": ref_wall * 1e3 / ref_steps,
+                            "stage_ms_per_step": {kk: round(v, 4) for kk, v in
+                                                  reference_stages_ms.items()}},
+    }
+
+
+def bench_small(sph, steps):
+    """The 16,384-particle dam break (1,000 steps of the reference's CPU-sized
+    case): one graph-replayed sph_step call of `steps` steps, then 100 steps
+    with per-launch hipEvents. Launch-latency bound: reported as latency."""
+    sc = sph.scenes.dam_break()
+    n = sc.n
+    ctx = sph.Context(sc.params, n)
+    ctx.upload(sc.rx, sc.ry, sc.vx, sc.vy)
+    ctx.prepare()
+    ctx.set_graph(True)
+    ctx.step(20)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.set_timing(True)
+    ctx.step(100)
+    ctx.synchronize()
+    stage = ctx.stage_times()
+    ctx.set_timing(False)
+    ctx.close()
+    kern = {kk: round(v[0] / v[1] * 1e3, 3) for kk, v in stage.items() if v[1]}
+    log(f"small: {wall * 1e3 / steps:.4f} ms/step")
+    return {"workload": sc.name, "particles": n, "steps": steps,
+            "value": n * steps / wall, "unit": "particle-steps/s",
+            "us_per_step": wall * 1e6 / steps, "us_per_launch_events": kern,
+            "bound": "launch latency (five dependent kernels per step, graph-replayed)"}
 
 
 def bench_batched(sph, steps, trial_steps=10):
@@ -316,7 +549,16 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="dam_break_1m")
-    ap.add_argument("--batched-steps", type=int, default=200,
+    ap.add_argument("--window-start", type=int, default=2500,
+                    help="steps of the run from rest before the timed window (the collapse "
+                         "is under way; movers per step grow with it)")
+    ap.add_argument("--full-steps", type=int, default=5000,
+                    help="length of the whole-run profile from rest (0 disables)")
+    ap.add_argument("--periodic-steps", type=int, default=200,
+                    help="steps of the 4,194,304-particle periodic workload (0 disables)")
+    ap.add_argument("--small-steps", type=int, default=1000,
+                    help="steps of the 16,384-particle dam break (0 disables)")
+    ap.add_argument("--batched-steps", type=int, default=500,
                     help="steps of the 256-scene batched workload (0 disables)")
     ap.add_argument("--surrogate-steps", type=int, default=200,
                     help="steps of the surrogate-in-the-loop comparison (0 disables)")
@@ -332,8 +574,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied device-to-device before the workload (a copy kernel of "
                          "known size for scaling HBM counters)")
-    ap.add_argument("--cpu-budget", type=float, default=15.0,
-                    help="seconds of CPU work for the cpu_baseline sample (0 disables)")
+    ap.add_argument("--cpu-budget", type=float, default=10.0,
+                    help="seconds of CPU work per compute type for the cpu_baseline sample "
+                         "(0 disables)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -356,13 +599,16 @@ def main():
     scene = sph.scenes.by_name(args.workload)
     n = scene.n
     log(f"workload {scene.name}: n={n}")
+    full = full_run_profile(sph, scene, args.full_steps) if args.full_steps > 0 else None
+
     ctx = sph.Context(scene.params, n)
     ctx.upload(scene.rx, scene.ry, scene.vx, scene.vy)
     ctx.prepare()
     ctx.set_graph(True)
-    ctx.step(args.warmup)
-    ctx.synchronize()
-    log(f"warmup {args.warmup} steps done")
+    if args.window_start > 0:
+        ctx.step(args.window_start)
+        ctx.synchronize()
+        log(f"advanced {args.window_start} steps to the timed window")
 
     # step-kernel and re-sort choice: time each combination on graph replay (all
     # give identical results)
@@ -388,14 +634,18 @@ def main():
     ctx.set_step_kernel(kernels[best])
     ctx.set_resort(resorts[best_resort])
     bstats = ctx.block_stats()
-    log(f"block stats {bstats}")
+    ctx.step(args.warmup)
+    ctx.synchronize()
+    log(f"warmup {args.warmup} steps done")
     # movers per step (particles whose cell changed) over a short sample
     mv = []
     for _ in range(20):
         ctx.step(1)
         mv.append(ctx.movers())
-    mover_stats = {"min": int(min(mv)), "mean": float(np.mean(mv)), "max": int(max(mv))}
+    mover_stats = {"min": int(min(mv)), "mean": float(np.mean(mv)), "max": int(max(mv)),
+                   "path": mover_path(int(np.mean(mv)))}
     log(f"movers per step {mover_stats}")
+    window_first = args.window_start + 24 * len(combos) + args.warmup + 20
 
     # timed region: exactly `steps` graph-replayed steps, synchronised on both sides
     ctx.synchronize()
@@ -426,31 +676,13 @@ def main():
     occ = float((st >= 0).sum()) / n
     passes = (int(ctx.grid().key_bits) + 7) // 8
     m = ctx.measure()
-
-    stages = {}
-    kern_total = 0.0
-    for name, (ms, launches) in stage.items():
-        if launches == 0:
-            continue
-        per_launch_ms = ms / launches
-        entry = {"launches": launches, "ms_total": round(ms, 4),
-                 "us_per_launch": round(per_launch_ms * 1e3, 3)}
-        bpp = alg_bytes_per_particle(name, k, occ, passes, best, best_resort, mfrac,
-                                     best == "particle" and best_resort == "incremental")
-        if bpp is not None:
-            b = bpp * n
-            entry["alg_bytes_per_launch"] = int(b)
-            entry["achieved_GBs"] = round(b / (per_launch_ms * 1e-3) / 1e9, 1)
-        stages[name] = entry
-        kern_total += ms
-    dom_name = max((s for s in stages if "alg_bytes_per_launch" in stages[s]),
-                   key=lambda s: stages[s]["ms_total"])
+    stages, step_bytes, kern_ms = stage_table(stage, args.steps, n, k, occ, passes, best,
+                                              best_resort, mfrac)
+    dom_name = max((s_ for s_ in stages if "alg_bytes_per_launch" in stages[s_]),
+                   key=lambda s_: stages[s_]["ms_total"])
     dom = stages[dom_name]
     achieved = dom["achieved_GBs"]
-    step_bytes = sum(alg_bytes_per_particle(s, k, occ, passes, best, best_resort, mfrac,
-                                            best == "particle" and best_resort == "incremental")
-                     * stages[s]["launches"] / args.steps
-                     for s in stages if "alg_bytes_per_launch" in stages[s])
+    kernel = STAGE_KERNEL.get(dom_name, dom_name)
     out = {
         "metric": "particle-steps/s at 1M particles: surrogate vs plain SPH, \u22642% accel error",
         "value_path": "plain GPU SPH (the faster of the two paths within the 2% bound; the "
@@ -468,31 +700,42 @@ def main():
         "data": "synthetic (seeded dam-break lattice with jitter)",
         "config": {"workload": scene.name, "particles": n, "box": list(scene.params.hi),
                    "h": scene.params.h, "dt": scene.params.dt, "physics": "wcsph",
+                   "window": {"first_step": window_first, "steps": args.steps,
+                              "note": "timed late in the collapse, after the run from rest"},
                    "mean_neighbors": round(k, 3), "occupied_cells_per_particle": round(occ, 4),
                    "radix_passes": passes, "step_kernel": best, "resort": best_resort,
                    "movers_last_step": ctx.movers(), "movers_per_step": mover_stats,
                    "step_kernel_ms": {k2: round(v, 4) for k2, v in tune.items()},
                    "block_stats": bstats},
-        "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL.get(dom_name, dom_name),
+        "roofline": {"bound": "hbm", "kernel": kernel,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(STAGE_KERNEL.get(dom_name, dom_name)),
-                     "issue": measured_issue(STAGE_KERNEL.get(dom_name, dom_name))},
+                     "traffic": measured_traffic(kernel),
+                     "traffic_source": "profiles/hbm_traffic_dam_break_1m.md (rocprofv3 "
+                                       "FETCH_SIZE / WRITE_SIZE per dispatch, calibrated "
+                                       "against a device copy; not measured in this run)",
+                     "issue": measured_issue(kernel)},
         "stages": stages,
         "step": {"alg_bytes_per_particle": round(step_bytes, 1),
                  "step_frac_of_hbm": step_bytes * value / (HBM_PEAK_GBS * 1e9),
-                 "kernel_ms_per_step": kern_total / args.steps,
+                 "b_canon_frac_of_hbm": 304.0 * value / (HBM_PEAK_GBS * 1e9),
+                 "kernel_ms_per_step": kern_ms,
                  "events_pass_ms_per_step": ev_ms},
         "diagnostics": {"kinetic_energy": m[0], "max_v2": m[1], "inside": int(m[2])},
+        "full_run": full,
     }
     ctx.close()
+    if args.periodic_steps > 0:
+        out["periodic"] = bench_periodic(sph, args.periodic_steps)
+    if args.small_steps > 0:
+        out["small"] = bench_small(sph, args.small_steps)
     if args.batched_steps > 0:
         out["batched"] = bench_batched(sph, args.batched_steps)
     if args.surrogate_steps > 0:
         out["surrogate"] = bench_surrogate(sph, scene, args.surrogate_steps, args.train_iters,
                                            args.train_scenes, args.train_steps)
     if args.cpu_budget > 0:
-        out["cpu_baseline"] = cpu_baseline(sph, scene, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

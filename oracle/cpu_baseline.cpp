@@ -6,8 +6,10 @@
 // SoA gather of every particle array, serial cell ranges over the whole table,
 // multipass CSR lists (OpenMP count, serial scan, OpenMP fill; self included as
 // on the CPU), OpenMP density, serial EOS, OpenMP forces, serial integrator.
-// Built with the reference's -O3 -ffast-math flags and -fopenmp. Only bench.py's
-// cpu_baseline leg runs it.
+// compute_type=sorted (app/simulation_configuration.h:497-533) is timed too:
+// the same search, no lists, density and forces over the 3x3 cell stencil
+// (GenericForceComputation.tpp:34-253). Built with the reference's -O3
+// -ffast-math flags and -fopenmp. Only bench.py's cpu_baseline leg runs it.
 #include <omp.h>
 
 #include <cstring>
@@ -37,9 +39,12 @@ enum { T_INTEGRATION, T_SEARCH, T_LIST, T_DENSITY, T_PRESSURE, T_FORCE, T_MAIN }
 
 }  // namespace
 
-extern "C" int orc_baseline_run(const orc_params* p, int64_t n, const float* rx0,
-                                const float* ry0, const float* vx0, const float* vy0,
-                                int32_t steps, double* stage_seconds, double* checksum) {
+// compute_type 0: mp-lists (CSR lists, then density and forces over them);
+// 1: sorted (density and forces straight over the cell stencil).
+extern "C" int orc_baseline_run2(const orc_params* p, int64_t n, const float* rx0,
+                                 const float* ry0, const float* vx0, const float* vy0,
+                                 int32_t steps, int32_t compute_type, double* stage_seconds,
+                                 double* checksum) {
   orc_grid g;
   if (make_grid(p, &g) != 0) return -1;
   Consts<float> c = make_consts<float>(p, g);
@@ -114,6 +119,43 @@ extern "C" int orc_baseline_run(const orc_params* p, int64_t n, const float* rx0
     tm.t[T_LIST] += omp_get_wtime() - t0;
   };
 
+  // GenericForceComputation<MortonOrder> (GenericForceComputation.tpp:34-253):
+  // the stencil itself, self included in the density, excluded in the forces
+  auto compute_sorted = [&]() {
+    double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(runtime)
+    for (int64_t i = 0; i < n; ++i) {
+      float acc = 0.f;
+      for_each_neighbor<float>(g, c, s.rx.data(), s.ry.data(), cstart.data(), cend.data(), i,
+                               true, [&](int32_t, float, float, float r2) {
+                                 acc += density_term<float>(c, r2);
+                               });
+      s.density[i] = c.mc_rho * acc;
+    }
+    double t1 = omp_get_wtime();
+    tm.t[T_DENSITY] += t1 - t0;
+    for (int64_t i = 0; i < n; ++i) {
+      float rho, p_, pr_;
+      density_one<float>(c, s.density[i] / c.mc_rho, rho, p_, pr_);
+      pres[i] = p_;
+      pr[i] = pr_;
+    }
+    double t2 = omp_get_wtime();
+    tm.t[T_PRESSURE] += t2 - t1;
+#pragma omp parallel for schedule(runtime)
+    for (int64_t i = 0; i < n; ++i) {
+      Accum<float> a;
+      for_each_neighbor<float>(g, c, s.rx.data(), s.ry.data(), cstart.data(), cend.data(), i,
+                               false, [&](int32_t j, float dx, float dy, float r2) {
+                                 force_pair<float>(c, dx, dy, r2, s.density[i], s.density[j],
+                                                   pr[i], pr[j], s.vx[i], s.vy[i], s.vx[j],
+                                                   s.vy[j], a);
+                               });
+      force_finish<float>(c, a, s.fx[i], s.fy[i]);
+    }
+    tm.t[T_FORCE] += omp_get_wtime() - t2;
+  };
+
   auto compute = [&]() {  // MPListsForceComputation.cpp:3-148
     double t0 = omp_get_wtime();
 #pragma omp parallel for schedule(runtime)
@@ -153,18 +195,24 @@ extern "C" int orc_baseline_run(const orc_params* p, int64_t n, const float* rx0
     tm.t[T_FORCE] += omp_get_wtime() - t2;
   };
 
+  auto evaluate = [&]() {
+    if (compute_type == 1) {
+      compute_sorted();
+    } else {
+      build_lists();
+      compute();
+    }
+  };
   double t_start = omp_get_wtime();
   find_neighbors();
-  build_lists();
-  compute();
+  evaluate();
   for (int it = 1; it < steps; ++it) {
     double t0 = omp_get_wtime();
     for (int64_t i = 0; i < n; ++i)  // Integrator::UpdateVelocities/UpdatePositions (serial)
       integrate_one<float>(c, s.rx[i], s.ry[i], s.vx[i], s.vy[i], s.fx[i], s.fy[i]);
     tm.t[T_INTEGRATION] += omp_get_wtime() - t0;
     find_neighbors();
-    build_lists();
-    compute();
+    evaluate();
   }
   tm.t[T_MAIN] = omp_get_wtime() - t_start;
   for (int k = 0; k < 7; ++k) stage_seconds[k] = tm.t[k];
@@ -174,4 +222,10 @@ extern "C" int orc_baseline_run(const orc_params* p, int64_t n, const float* rx0
     *checksum = cs;
   }
   return omp_get_max_threads();
+}
+
+extern "C" int orc_baseline_run(const orc_params* p, int64_t n, const float* rx0,
+                                const float* ry0, const float* vx0, const float* vy0,
+                                int32_t steps, double* stage_seconds, double* checksum) {
+  return orc_baseline_run2(p, n, rx0, ry0, vx0, vy0, steps, 0, stage_seconds, checksum);
 }

@@ -286,18 +286,20 @@ def delta_decode(d):
     return out
 
 
-def baseline_run(p, rx, ry, vx, vy, steps):
-    """Timed CPU baseline (reference-style OpenMP mp-lists step)."""
+def baseline_run(p, rx, ry, vx, vy, steps, compute_type="mp-lists"):
+    """Timed CPU baseline (reference-style OpenMP step; compute_type "mp-lists"
+    or "sorted", the reference's names)."""
     L = C.CDLL(BASELINE_LIB)
-    f = L.orc_baseline_run
+    f = L.orc_baseline_run2
     f.restype = C.c_int32
-    f.argtypes = [C.POINTER(OrcParams), C.c_int64] + [C.c_void_p] * 4 + [C.c_int32, C.c_void_p,
-                                                                          C.c_void_p]
+    f.argtypes = [C.POINTER(OrcParams), C.c_int64] + [C.c_void_p] * 4 + [
+        C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
+    ct = {"mp-lists": 0, "sorted": 1}[compute_type]
     arrs = [np.ascontiguousarray(a, np.float32) for a in (rx, ry, vx, vy)]
     secs = np.zeros(7, np.float64)
     cs = np.zeros(1, np.float64)
-    threads = f(C.byref(params(p)), arrs[0].size, *[_p(a) for a in arrs], int(steps), _p(secs),
-                _p(cs))
+    threads = f(C.byref(params(p)), arrs[0].size, *[_p(a) for a in arrs], int(steps), ct,
+                _p(secs), _p(cs))
     names = ("integration", "neighbor_search", "neighbor_list", "approximate_density",
              "calculate_pressure", "force_computation", "main_loop")
     return threads, dict(zip(names, secs.tolist())), float(cs[0])

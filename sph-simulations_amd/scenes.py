@@ -177,9 +177,9 @@ def batched_dam_breaks(n_scenes: int = 256, seed: int = 0x5EED0002, n_lo: int = 
     [n_lo, n_hi], a block aspect ratio A = height/width uniform in [0.5, 2] and a
     block width W uniform in [0.2, 0.5]. Lattice nx = round(sqrt(n / A)),
     ny = round(n / nx) (at most 1/dx), dx = W / nx from the origin with +-0.01 dx
-    jitter (seeded seed + 1 + s); h = 1.3 dx, mass 1000 dx^2, Tait gamma 7 with
-    c0 = 10 sqrt(2 g H) for the block height H = ny dx, dt = 0.25 h / c0,
-    reflective walls, gravity -9.81."""
+    jitter (seeded seed + 1 + s); otherwise the 16k dam break's physics: h =
+    1.3 dx, mass 1000 dx^2, Tait gamma 7 with the same c0 = 10 sqrt(2 g 0.25)
+    for every scene, dt = 0.25 h / c0, reflective walls, gravity -9.81."""
     u = uniform(seed, 3 * n_scenes)
     params, xs, ys = [], [], []
     counts = np.zeros(n_scenes, np.int64)
@@ -193,9 +193,6 @@ def batched_dam_breaks(n_scenes: int = 256, seed: int = 0x5EED0002, n_lo: int = 
         X, Y = lattice(nx, ny, dx, dx)
         jx, jy = _jitter(seed + 1 + s, X.size, 0.01 * dx)
         p = wcsph_params(dx, (0.0, 0.0), (1.0, 1.0), order=order)
-        H = ny * dx
-        p.c0 = 10.0 * math.sqrt(2.0 * 9.81 * H)
-        p.dt = 0.25 * p.h / p.c0
         params.append(p)
         xs.append((X + jx).astype(np.float32))
         ys.append((Y + jy).astype(np.float32))

@@ -18,7 +18,7 @@ def note(msg):
 
 @pytest.fixture(scope="module")
 def small_batch(sph):
-    # uneven counts (a few hundred to a few thousand), own dx / h / dt / c0 each
+    # uneven counts (a few hundred to a few thousand), own dx / h / dt each
     return sph.scenes.batched_dam_breaks(n_scenes=6, seed=0x5EED0102, n_lo=300, n_hi=3000,
                                          steps=20)
 

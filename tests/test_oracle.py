@@ -380,7 +380,8 @@ def test_fp32_restatement_tracks_fp64(sph, orc):
 
 def test_batched_scene_generator(sph):
     """Seeded, deterministic, counts within the drawn range, blocks inside the
-    unit box, and per-scene physics from each scene's own lattice spacing."""
+    unit box, per-scene h, mass and dt from each scene's own lattice spacing and
+    the 16k dam break's sound speed."""
     b1 = sph.scenes.batched_dam_breaks(n_scenes=16)
     b2 = sph.scenes.batched_dam_breaks(n_scenes=16)
     assert np.array_equal(b1.rx, b2.rx) and np.array_equal(b1.counts, b2.counts)
@@ -392,6 +393,7 @@ def test_batched_scene_generator(sph):
         dx = p.h / 1.3
         assert abs(p.mass - 1000.0 * dx * dx) < 1e-12
         assert abs(p.dt - 0.25 * p.h / p.c0) < 1e-15
+        assert p.c0 == 10.0 * math.sqrt(2.0 * 9.81 * 0.25)  # the 16k dam break's sound speed
         assert sc.ry.max() < 1.0 and sc.rx.max() < 0.51
     assert len({p.h for p in b1.params}) == 16
 
