@@ -605,7 +605,9 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
         (rc = dev_alloc(c, &s.tile_first, resort_tiles((int)N))) ||
         (rc = dev_alloc(c, &s.tile_count, resort_tiles((int)N))) ||
         (rc = dev_alloc(c, &r.vtmp, N)) || (rc = dev_alloc(c, &r.counts, kMoverGrid * 256)) ||
-        (rc = dev_alloc(c, &r.bar, 1)) || (rc = dev_alloc(c, &r.n_movers, 1)))
+        (rc = dev_alloc(c, &r.bar, 1)) || (rc = dev_alloc(c, &r.n_movers, 1)) ||
+        (rc = dev_alloc(c, &r.bucket, resort_tiles((int)N) * kBucketCap)) ||
+        (rc = dev_alloc(c, &r.bovf, 1)) || (rc = dev_alloc(c, &r.mode, 1)))
       return bail(rc);
   }
   b.blocks = s.blocks;
@@ -628,6 +630,8 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
   c->rb.status = b.status;
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.n_movers, 0, sizeof(int), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.bar, 0, sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->rb.bovf, 0, sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->rb.mode, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess)
     e = hipMemsetAsync(c->rb.lcount, 0, kListSegs * kListStride * sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, n_keys);

@@ -18,6 +18,7 @@ constexpr int kMidSortCap = kListSegs * kSmallSortCap;
 constexpr int kMoverGrid = 64;       // workgroups of the mover sort (all resident)
 constexpr int kResortTile = 256;     // particles per tile of the incremental re-sort
 constexpr int kListStride = 32;      // words between segment counters (one 128-byte line each)
+constexpr int kBucketCap = 32;       // listed movers a merge tile's bucket holds
 
 struct SortBuffers {
   // n passed to the launchers sizes the grids; when n_dev is set the kernels
@@ -108,6 +109,9 @@ struct ResortBuffers {
   int* n_movers;
   uint32_t* status;
   uint64_t* keys_alt;              // second sorted-key buffer (ping-pong)
+  uint4* bucket;                   // [tiles][kBucketCap] (key lo, key hi, previous index, 0)
+  uint32_t* bovf;                  // nonzero: a bucket overflowed this step
+  uint32_t* mode;                  // 1: the merge windows come from the buckets, 0: sorted
 };
 
 // Where the producer of the new keys reports the movers: the flag words and,

@@ -207,18 +207,10 @@ __device__ __forceinline__ int rank_in_sorted(const uint64_t* keys, const uint32
   return lo;
 }
 
-// Merge tiles whose movers the LDS path counts in LDS (and whose first keys it
-// searches there directly; above, every S-th one).
+// Tile-first keys k_bucket stages in LDS for its owner searches (every S-th
+// tile's when there are more tiles).
 constexpr int kTileTab = 4096;
-// The grid path for up to kMidSortCap listed movers: one mover per thread, and
-// at most kMidGroupMax movers per merge window (each is ranked by counting over
-// its window; a larger window takes the radix path); kMidRankBatch window
-// entries are loaded together.
-static_assert(kMidSortCap <= kMoverGrid * MS_TW, "one listed mover per thread");
 static_assert(kMidSortCap < (1 << 16), "packed 16-bit tile counters");
-constexpr uint32_t kMidGroupMax = 256;
-constexpr int kMidRankBatch = 8;
-constexpr int kMidRefine = 4;  // tile-first keys loaded together in the owner search
 
 // Flag words per thread and per batch of the ordered compaction.
 constexpr int MS_WPT = 12;
@@ -305,11 +297,6 @@ __device__ __forceinline__ void compact_movers(int n, int tiles,
   }
 }
 
-// List entries per segment that workgroup 0 reads before the counts are known
-// (group q of thread t: segment 2 q + t / kSpec, entry t % kSpec).
-constexpr int kSpec = kSmallSortCap / kListSegs;
-static_assert(kSpec * kListSegs == 4 * MS_TW && MS_TW == 2 * kSpec, "four list groups per thread");
-
 // (key, previous index) order of the listed movers: one word key << 32 | index
 // when the keys have at most 32 bits (one compare), else the pair.
 template <bool WIDE>
@@ -328,169 +315,160 @@ struct Ord<true> {
   __device__ bool operator<(const Ord& o) const { return pair_less(k, i, o.k, o.i); }
 };
 
-template <bool WIDE>
-__global__ __launch_bounds__(MS_TW) void k_movers(
-    int n, int tiles, int passes, const uint64_t* __restrict__ knew,
-    const uint64_t* __restrict__ tfirst, uint32_t* __restrict__ tcount,
-    const unsigned long long* __restrict__ flags, const uint32_t* __restrict__ lcount,
-    const uint64_t* __restrict__ lkey, const uint32_t* __restrict__ lidx,
-    uint32_t* __restrict__ tile_base, uint32_t* __restrict__ tile_lo, int* __restrict__ n_movers,
-    uint64_t* __restrict__ mkey, uint32_t* __restrict__ midx, uint64_t* __restrict__ ktmp,
-    uint32_t* __restrict__ vtmp, uint64_t* __restrict__ skey, uint32_t* __restrict__ sidx,
-    uint32_t* __restrict__ counts, uint32_t* __restrict__ bar, uint32_t* __restrict__ status) {
-  using O = Ord<WIDE>;
-  // key buffers: the word prefixes of the compaction and the digit counts of
-  // the grid sort; the second halves hold the movers grouped by merge tile
-  __shared__ uint64_t s_k[2 * kSmallSortCap];
-  __shared__ uint32_t s_i[kSmallSortCap];
-  // digit counts: u32 [MS_WAVES][256] per wave in the grid sort; the tile-first
-  // table of the grouping
-  __shared__ uint32_t s_wc[2 * MS_WAVES * 256];
-  __shared__ uint32_t s_base[256];
-  __shared__ uint32_t s_tot[256];
-  __shared__ uint32_t s_w[MS_WAVES];
-  __shared__ uint32_t s_bk[kTileTab];  // packed counters per merge tile (up to kTileTab tiles)
-  __shared__ int s_ok;
-  const int tid = threadIdx.x, wave = tid / kWave;
-  const int words = 4 * tiles;
+// ---- bucket path (all listed movers, at most kMidSortCap): every listed
+// mover goes into the bucket of the merge tile that owns it -- the tile of the
+// last previous-order particle ordered before it by (key, previous index).
+// When its new cell k held particles [s, e) in the previous order, that
+// particle is s - 1 if its previous index i < s (its old key was smaller)
+// and e - 1 otherwise (i lies outside [s, e): its old key differs), one load
+// of the cell table; movers into a cell that was empty search the tile-first
+// table in LDS. The owner's packed counter (tile_count, high half) gives the
+// slot in the bucket, the previous-index tile's counter (low half) counts the
+// movers before each tile. A bucket past kBucketCap raises the overflow word
+// and the step takes the radix path.
+constexpr int KB_TW = 256;
+constexpr int kBucketGrid = kMidSortCap / KB_TW;  // one listed mover per thread
+static_assert(kTileTab % KB_TW == 0, "tile-first table loads");
 
-  // movers listed per segment by the producer (their sum is the number of set
-  // flag bits; the lists are whole when no segment's count passes
-  // kSmallSortCap, and always when the sum is at most kSmallSortCap).
-  // Workgroup 0 loads in the same round the tile-first keys of every S-th tile
-  // (all of them when tiles <= kTileTab) and the first kSpec entries of every
-  // list.
+template <bool WIDE>
+__global__ __launch_bounds__(KB_TW) void k_bucket(
+    int tiles, const uint64_t* __restrict__ tfirst, uint32_t* __restrict__ tcount,
+    const uint32_t* __restrict__ lcount, const uint64_t* __restrict__ lkey,
+    const uint32_t* __restrict__ lidx, const int2* __restrict__ cells, uint64_t n_keys,
+    uint4* __restrict__ bucket, uint32_t* __restrict__ bovf) {
+  using O = Ord<WIDE>;
+  __shared__ uint64_t s_tab[kTileTab];
+  const int tid = threadIdx.x;
   uint32_t cnt[kListSegs];
 #pragma unroll
   for (int g = 0; g < kListSegs; ++g) cnt[g] = lcount[g * kListStride];
-  const int S = (tiles + kTileTab - 1) / kTileTab;
-  const int ne = (tiles + S - 1) / S;
-  uint64_t tv[kTileTab / MS_TW];
-  uint32_t gi[4];
-  uint64_t gk[4];
-  if (blockIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < kTileTab / MS_TW; ++q)
-      tv[q] = tfirst[(size_t)min(q * MS_TW + tid, ne - 1) * S];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int at = (2 * q + tid / kSpec) * kSmallSortCap + tid % kSpec;
-      gi[q] = lidx[at];
-      gk[q] = lkey[at];
-    }
-  }
   uint32_t seg_lo[kListSegs + 1];
   seg_lo[0] = 0;
 #pragma unroll
   for (int g = 0; g < kListSegs; ++g) seg_lo[g + 1] = seg_lo[g] + cnt[g];
   const uint32_t listed = seg_lo[kListSegs];
-  const bool small = listed <= (uint32_t)kSmallSortCap;
-  if (small && blockIdx.x != 0) return;
-
-  if (small) {
-    // Workgroup 0. A mover belongs to the merge tile t whose first particle
-    // (tile_first[t], 256 t) is the last one ordered before it (tile 0 if
-    // none). One counter word per tile counts, in its low half, the movers
-    // whose previous index lies in the tile and, in its high half, the movers
-    // the tile owns; the exclusive scan of the words gives the tile prefixes
-    // and the window starts at once (both at most kSmallSortCap < 2^16, so the
-    // halves never carry into each other). The movers are grouped by owner
-    // tile (the high-half counter gives the slot) and ranked by counting
-    // inside a group: the sorted order by (key, previous index) without a
-    // comparison sort, whatever the order of the lists.
-    const int m = (int)listed;  // exact: no counter passed kSmallSortCap
-    const bool lds_count = tiles <= kTileTab;  // counters in LDS, else tcount
-    bool spec = true;
+  bool whole = listed <= (uint32_t)kMidSortCap;
 #pragma unroll
-    for (int g = 0; g < kListSegs; ++g) spec = spec && cnt[g] <= (uint32_t)kSpec;
-    bool valid[4];
-    if (spec) {
+  for (int g = 0; g < kListSegs; ++g) whole = whole && cnt[g] <= (uint32_t)kSmallSortCap;
+  // lists past their capacity: the radix path counts the movers from the flags
+  if (!whole || (uint32_t)(blockIdx.x * KB_TW) >= listed) return;
+  const int S = (tiles + kTileTab - 1) / kTileTab;
+  const int ne = (tiles + S - 1) / S;
+  // the table, this thread's list entry and its new cell's range, loads in
+  // flight together (indices clamped)
+  uint64_t tv[kTileTab / KB_TW];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) valid[q] = (uint32_t)(tid % kSpec) < cnt[2 * q + tid / kSpec];
-    } else {
-      // a list longer than kSpec: entry e of the concatenated lists is entry
-      // e - seg_lo[g] of the segment g holding it (clamped when e >= m)
+  for (int q = 0; q < kTileTab / KB_TW; ++q)
+    tv[q] = tfirst[(size_t)min(q * KB_TW + tid, ne - 1) * S];
+  const int e = (int)blockIdx.x * KB_TW + tid;
+  const uint32_t ec = min((uint32_t)e, listed - 1u);
+  int g = 0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = q * MS_TW + tid;
-        const uint32_t ec = (uint32_t)min(e, max(m - 1, 0));
-        int g = 0;
+  for (int h = 1; h < kListSegs; ++h) g += ec >= seg_lo[h] ? 1 : 0;
+  const int at = g * kSmallSortCap + (int)(ec - seg_lo[g]);
+  const uint32_t mi = lidx[at];
+  const uint64_t mk = lkey[at];
+  const int2 c = cells[min(mk, n_keys - 1)];
 #pragma unroll
-        for (int h = 1; h < kListSegs; ++h) g += ec >= seg_lo[h] ? 1 : 0;
-        const int at = g * kSmallSortCap + (int)(ec - seg_lo[g]);
-        gi[q] = lidx[at];
-        gk[q] = lkey[at];
-        valid[q] = e < m;
-      }
+  for (int q = 0; q < kTileTab / KB_TW; ++q) {
+    const int e2 = q * KB_TW + tid;
+    if (e2 < ne) {
+      if constexpr (WIDE) s_tab[e2] = tv[q];
+      else s_tab[e2] = O::make(tv[q], (uint32_t)(e2 * S * TW)).w;
     }
-    uint64_t* s_tab = reinterpret_cast<uint64_t*>(s_wc);
-    static_assert(kTileTab * 8 <= (int)sizeof(s_wc), "tile table exceeds the count buffer");
-    if (lds_count)
-      for (int t = tid; t < tiles; t += MS_TW) s_bk[t] = 0;
-#pragma unroll
-    for (int q = 0; q < kTileTab / MS_TW; ++q) {
-      const int e = q * MS_TW + tid;
-      if (e < ne) {
-        if constexpr (WIDE) s_tab[e] = tv[q];
-        else s_tab[e] = O::make(tv[q], (uint32_t)(e * S * TW)).w;
-      }
-    }
-    auto tab = [&](int e) {
-      if constexpr (WIDE) return O::make(s_tab[e], (uint32_t)(e * S * TW));
-      else return O{s_tab[e]};
+  }
+  __syncthreads();
+  if (e >= (int)listed) return;
+  const O mo = O::make(mk, mi);
+  uint32_t own;
+  if (c.x >= 0) {
+    const int p = (int)mi < c.x ? c.x - 1 : c.y - 1;
+    own = p < 0 ? 0u : (uint32_t)p / (uint32_t)TW;
+  } else {
+    auto tab = [&](int e2) {
+      if constexpr (WIDE) return O::make(s_tab[e2], (uint32_t)(e2 * S * TW));
+      else return O{s_tab[e2]};
     };
-    __syncthreads();
-    // owner tile and slot: table entries ordered before the mover (lower bound
-    // with a trip count set by ne alone, four movers in lockstep; waves
-    // without a mover skip it), then the tiles between two table entries
-    O mo[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) mo[q] = O::make(gk[q], gi[q]);
-    int own[4] = {0, 0, 0, 0};
-    uint32_t slot[4] = {0, 0, 0, 0};
-    if (__any(valid[0] || valid[1] || valid[2] || valid[3])) {
-      int base[4] = {0, 0, 0, 0};
-      for (int len = ne; len > 1;) {
-        const int half = len >> 1;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (tab(base[q] + half - 1) < mo[q]) base[q] += half;
-        len -= half;
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (tab(base[q]) < mo[q]) ++base[q];
-        const int c = base[q] - 1;  // last table entry before the mover, -1 if none
-        int o = max(c, 0) * S;
-        if (c >= 0)
-          for (int k = 1; k < S; ++k) {
-            const int t = c * S + k;
-            if (t < tiles && O::make(tfirst[t], (uint32_t)t * TW) < mo[q]) o = t;
-          }
-        own[q] = o;
-        if (valid[q]) {
-          const int home = (int)(gi[q] / (uint32_t)TW);
-          if (lds_count) {
-            slot[q] = atomicAdd(&s_bk[o], 1u << 16) >> 16;
-            atomicAdd(&s_bk[home], 1u);
-          } else {
-            slot[q] = atomicAdd(&tcount[o], 1u << 16) >> 16;
-            atomicAdd(&tcount[home], 1u);
-          }
-        }
-      }
+    // the last sampled entry before the mover, then the tiles between two samples
+    int base = 0;
+    for (int len = ne; len > 1;) {
+      const int half = len >> 1;
+      if (tab(base + half - 1) < mo) base += half;
+      len -= half;
     }
-    __syncthreads();
-    // exclusive scan of the counter words (in place), four tiles per thread
-    // per round: tile prefixes from the low halves, window starts from the
-    // high halves (written to global memory at the end when in LDS)
+    if (tab(base) < mo) ++base;
+    const int cc = base - 1;
+    int o = max(cc, 0) * S;
+    if (cc >= 0)
+      for (int k = 1; k < S; ++k) {
+        const int t = cc * S + k;
+        if (t < tiles && O::make(tfirst[t], (uint32_t)t * TW) < mo) o = t;
+      }
+    own = (uint32_t)o;
+  }
+  const uint32_t slot = atomicAdd(&tcount[own], 1u << 16) >> 16;
+  atomicAdd(&tcount[mi / (uint32_t)TW], 1u);
+  if (slot < (uint32_t)kBucketCap)
+    bucket[(size_t)own * kBucketCap + slot] =
+        make_uint4((uint32_t)mk, (uint32_t)(mk >> 32), mi, 0u);
+  else
+    *bovf = 1u;
+}
+
+// ---- tile prefixes and, when the buckets cannot be used, the movers sorted.
+// Bucket path (listed movers in their buckets, none past its capacity):
+// workgroup 0 scans the packed tile counters into the tile prefixes (movers
+// before each tile's first particle by previous index) and the window starts
+// (movers ordered before each tile's first particle); the merge sorts each
+// window from its bucket. Otherwise (lists past their capacity or a bucket
+// overflowed) workgroup 0 compacts the movers from the flag words in
+// previous-index order and the grid runs an LSD radix sort of all of them,
+// and the merge reads its window from the sorted movers. *mode tells the
+// merge which: 1 buckets, 0 sorted.
+template <bool WIDE>
+__global__ __launch_bounds__(MS_TW) void k_movers(
+    int n, int tiles, int passes, const uint64_t* __restrict__ knew,
+    const uint64_t* __restrict__ tfirst, const uint32_t* __restrict__ tcount,
+    const unsigned long long* __restrict__ flags, const uint32_t* __restrict__ lcount,
+    const uint32_t* __restrict__ bovf, uint32_t* __restrict__ mode,
+    uint32_t* __restrict__ tile_base, uint32_t* __restrict__ tile_lo, int* __restrict__ n_movers,
+    uint64_t* __restrict__ mkey, uint32_t* __restrict__ midx, uint64_t* __restrict__ ktmp,
+    uint32_t* __restrict__ vtmp, uint64_t* __restrict__ skey, uint32_t* __restrict__ sidx,
+    uint32_t* __restrict__ counts, uint32_t* __restrict__ bar, uint32_t* __restrict__ status) {
+  // key buffers: the word prefixes of the compaction and the digit counts of
+  // the grid sort
+  __shared__ uint64_t s_k[2 * kSmallSortCap];
+  // digit counts: u32 [MS_WAVES][256] per wave in the grid sort
+  __shared__ uint32_t s_wc[2 * MS_WAVES * 256];
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_tot[256];
+  __shared__ uint32_t s_w[MS_WAVES];
+  __shared__ int s_ok;
+  const int tid = threadIdx.x, wave = tid / kWave;
+  uint32_t cnt[kListSegs];
+#pragma unroll
+  for (int g = 0; g < kListSegs; ++g) cnt[g] = lcount[g * kListStride];
+  const uint32_t overflow = *bovf;
+  uint32_t listed = 0;
+  bool bucketed = overflow == 0u;
+#pragma unroll
+  for (int g = 0; g < kListSegs; ++g) {
+    listed += cnt[g];
+    bucketed = bucketed && cnt[g] <= (uint32_t)kSmallSortCap;
+  }
+  bucketed = bucketed && listed <= (uint32_t)kMidSortCap;
+  if (bucketed) {
+    if (blockIdx.x != 0) return;
+    // exclusive scan of the packed counters (previous-index tile low half,
+    // owner tile high half; both prefixes stay below 2^16), four tiles per
+    // thread per round
     uint32_t carry = 0;
     for (int t0 = 0; t0 < tiles; t0 += 4 * MS_TW) {
       uint32_t c[4], sum = 0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const int t = t0 + 4 * tid + k, tc = min(t, tiles - 1);
-        const uint32_t v = lds_count ? s_bk[tc] : ld_agent(&tcount[tc]);
+        const int t = t0 + 4 * tid + k;
+        const uint32_t v = tcount[min(t, tiles - 1)];
         c[k] = t < tiles ? v : 0u;
         sum += c[k];
       }
@@ -500,248 +478,26 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
       for (int k = 0; k < 4; ++k) {
         const int t = t0 + 4 * tid + k;
         if (t < tiles) {
-          if (lds_count) {
-            s_bk[t] = run;
-          } else {
-            tcount[t] = run;
-            tile_base[t] = run & 0xffffu;
-            tile_lo[t] = run >> 16;
-          }
+          tile_base[t] = run & 0xffffu;
+          tile_lo[t] = run >> 16;
         }
         run += c[k];
       }
       carry += rtot;
     }
-    __syncthreads();
-    // group placement, then the rank inside the group
-    uint64_t* s_gk = s_k + kSmallSortCap;
-    uint32_t* s_gi = s_i;
-    uint32_t st[4], en[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int o = own[q];
-      const uint32_t w0 = lds_count ? s_bk[o] : ld_agent(&tcount[o]);
-      const uint32_t w1 =
-          o + 1 < tiles ? (lds_count ? s_bk[o + 1] : ld_agent(&tcount[o + 1])) : (uint32_t)m << 16;
-      st[q] = w0 >> 16;
-      en[q] = w1 >> 16;
-      if (valid[q]) {
-        if constexpr (WIDE) {
-          s_gk[st[q] + slot[q]] = gk[q];
-          s_gi[st[q] + slot[q]] = gi[q];
-        } else {
-          s_gk[st[q] + slot[q]] = mo[q].w;
-        }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (!valid[q]) continue;
-      uint32_t rank = st[q];
-      for (uint32_t u = st[q]; u < en[q]; ++u) {
-        if constexpr (WIDE) rank += O::make(s_gk[u], s_gi[u]) < mo[q];
-        else rank += O{s_gk[u]} < mo[q];
-      }
-      skey[rank] = gk[q];
-      sidx[rank] = gi[q];
-    }
-    if (lds_count)
-      for (int t = tid; t < tiles; t += MS_TW) {
-        const uint32_t w = s_bk[t];
-        tile_base[t] = w & 0xffffu;
-        tile_lo[t] = w >> 16;
-      }
     if (tid == 0) {
-      *n_movers = m;
-      tile_lo[tiles] = (uint32_t)m;
+      *n_movers = (int)listed;
+      tile_lo[tiles] = listed;
+      *mode = 1u;
     }
     return;
   }
-
-  // ---- more movers than workgroup 0 groups, all of them listed (no list
-  // past its capacity) and at most kMidSortCap: the same grouping over the
-  // grid, one mover per thread. Every workgroup finds its movers' owner tiles
-  // (the tile-first table of every S-th tile in its LDS) and counts owner and
-  // previous-index tiles with device atomics on the packed counters
-  // (tile_count, zero on entry); each workgroup scans one chunk of the
-  // counters, and after a barrier every workgroup combines the chunk totals
-  // into the tile prefixes and window starts; each mover goes into its
-  // owner's window at the slot its atomic returned and is then ranked inside
-  // the window by counting. A window of more than kMidGroupMax movers (the
-  // largest of the chunks' largest, seen by every workgroup) sends all of them
-  // on to the radix path instead.
-  bool mid = listed <= (uint32_t)kMidSortCap;
-#pragma unroll
-  for (int g = 0; g < kListSegs; ++g) mid = mid && cnt[g] <= (uint32_t)kSmallSortCap;
+  if (blockIdx.x == 0 && tid == 0) *mode = 0u;
   uint32_t gen = 0;
-  if (mid) {
-    const int m = (int)listed;  // exact: every list within its capacity
-    const int e = (int)blockIdx.x * MS_TW + tid;
-    const bool valid = e < m;
-    // listed entry e (clamped for threads without a mover)
-    const uint32_t ec = (uint32_t)min(e, m - 1);
-    int g = 0;
-#pragma unroll
-    for (int h = 1; h < kListSegs; ++h) g += ec >= seg_lo[h] ? 1 : 0;
-    const int at = g * kSmallSortCap + (int)(ec - seg_lo[g]);
-    const uint32_t mi = lidx[at];
-    const uint64_t mk = lkey[at];
-    // the sampled tile-first table, its loads issued together (clamped)
-    uint64_t* s_tab = reinterpret_cast<uint64_t*>(s_wc);
-    uint64_t tk[kTileTab / MS_TW];
-#pragma unroll
-    for (int q = 0; q < kTileTab / MS_TW; ++q)
-      tk[q] = tfirst[(size_t)min(q * MS_TW + tid, ne - 1) * S];
-#pragma unroll
-    for (int q = 0; q < kTileTab / MS_TW; ++q) {
-      const int e2 = q * MS_TW + tid;
-      if (e2 < ne) {
-        if constexpr (WIDE) s_tab[e2] = tk[q];
-        else s_tab[e2] = O::make(tk[q], (uint32_t)(e2 * S * TW)).w;
-      }
-    }
-    auto tab = [&](int q) {
-      if constexpr (WIDE) return O::make(s_tab[q], (uint32_t)(q * S * TW));
-      else return O{s_tab[q]};
-    };
-    __syncthreads();
-    const O mo = O::make(mk, mi);
-    int own = 0;
-    uint32_t slot = 0;
-    if (__any(valid)) {
-      int base = 0;
-      for (int len = ne; len > 1;) {
-        const int half = len >> 1;
-        if (tab(base + half - 1) < mo) base += half;
-        len -= half;
-      }
-      if (tab(base) < mo) ++base;
-      const int c = base - 1;  // last table entry before the mover, -1 if none
-      own = max(c, 0) * S;
-      if (c >= 0)
-        // the tiles between two table entries, kMidRefine first keys loaded
-        // together (clamped) per round
-        for (int k0 = 1; k0 < S; k0 += kMidRefine) {
-          uint64_t fk[kMidRefine];
-#pragma unroll
-          for (int q = 0; q < kMidRefine; ++q)
-            fk[q] = tfirst[min(c * S + k0 + q, tiles - 1)];
-#pragma unroll
-          for (int q = 0; q < kMidRefine; ++q) {
-            const int t = c * S + k0 + q;
-            if (k0 + q < S && t < tiles && O::make(fk[q], (uint32_t)t * TW) < mo) own = t;
-          }
-        }
-      if (valid) {
-        slot = atomicAdd(&tcount[own], 1u << 16) >> 16;
-        atomicAdd(&tcount[mi / (uint32_t)TW], 1u);
-      }
-    }
-    if (!grid_barrier(bar, gen, status, &s_ok)) return;
-    // Scan of the packed counters in chunks of consecutive tiles, one chunk
-    // per workgroup (plain loads: the barrier's fence made the atomics
-    // visible): each workgroup writes its chunk's local prefixes in place, the
-    // chunk's total and its largest window to counts[1 + wg] and
-    // counts[1 + nb + wg]. Both halves' prefixes stay below 2^16: no carry
-    // between them.
-    const int nbw = (int)gridDim.x;
-    const int chunk = (tiles + nbw - 1) / nbw;
-    const int c_lo = min((int)blockIdx.x * chunk, tiles), c_hi = min(c_lo + chunk, tiles);
-    {
-      if (tid == 0) s_base[0] = 0;
-      uint32_t carry = 0, gmax = 0;
-      for (int t0 = c_lo; t0 < c_hi; t0 += 4 * MS_TW) {
-        uint32_t cv[4], sum = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int t = t0 + 4 * tid + k, tc = min(t, c_hi - 1);
-          const uint32_t v = tcount[max(tc, 0)];
-          cv[k] = t < c_hi ? v : 0u;
-          sum += cv[k];
-          gmax = max(gmax, cv[k] >> 16);
-        }
-        uint32_t rtot;
-        uint32_t run = carry + block_excl_scan(sum, s_w, &rtot);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int t = t0 + 4 * tid + k;
-          if (t < c_hi) tcount[t] = run;
-          run += cv[k];
-        }
-        carry += rtot;
-      }
-      atomicMax(&s_base[0], gmax);
-      __syncthreads();
-      if (tid == 0) {
-        counts[1 + blockIdx.x] = carry;
-        counts[1 + nbw + blockIdx.x] = s_base[0];
-      }
-    }
-    if (!grid_barrier(bar, gen, status, &s_ok)) return;
-    // every workgroup: the chunk prefixes (one wave scans the nb totals) and
-    // the largest window of all; then its own chunk's tile prefixes and window
-    // starts
-    static_assert(kMoverGrid == kWave, "one wave scans the chunk totals");
-    if (tid < kWave) {
-      const uint32_t tot = counts[1 + tid], big = counts[1 + nbw + tid];
-      const uint32_t incl = wave_incl_scan(tot);
-      s_base[tid] = incl - tot;
-      uint32_t mx = big;
-#pragma unroll
-      for (int o = 1; o < kWave; o <<= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
-      if (tid == 0) s_base[kWave] = mx;
-    }
-    __syncthreads();
-    if (s_base[kWave] <= kMidGroupMax) {
-      const uint32_t cpre = s_base[blockIdx.x];
-      for (int t = c_lo + tid; t < c_hi; t += MS_TW) {
-        const uint32_t v = tcount[t] + cpre;
-        tile_base[t] = v & 0xffffu;
-        tile_lo[t] = v >> 16;
-      }
-      if (blockIdx.x == 0 && tid == 0) {
-        *n_movers = m;
-        tile_lo[tiles] = (uint32_t)m;
-      }
-      // window of the mover's owner tile
-      auto prefix = [&](int t) { return (tcount[t] + s_base[t / chunk]) >> 16; };
-      const uint32_t st = prefix(own);
-      const uint32_t en = own + 1 < tiles ? prefix(own + 1) : (uint32_t)m;
-      if (valid) {
-        ktmp[st + slot] = mk;
-        vtmp[st + slot] = mi;
-      }
-      if (!grid_barrier(bar, gen, status, &s_ok)) return;
-      if (valid) {
-        // the window's movers kMidRankBatch at a time, their loads in flight
-        // together (indices clamped into the window, which holds this mover)
-        uint32_t rank = st;
-        for (uint32_t u0 = st; u0 < en; u0 += kMidRankBatch) {
-          uint64_t kk[kMidRankBatch];
-          uint32_t ii[kMidRankBatch];
-#pragma unroll
-          for (int q = 0; q < kMidRankBatch; ++q) {
-            const uint32_t u = min(u0 + (uint32_t)q, en - 1u);
-            kk[q] = ktmp[u];
-            ii[q] = vtmp[u];
-          }
-#pragma unroll
-          for (int q = 0; q < kMidRankBatch; ++q)
-            rank += (u0 + (uint32_t)q < en && O::make(kk[q], ii[q]) < mo) ? 1u : 0u;
-        }
-        skey[rank] = mk;
-        sidx[rank] = mi;
-      }
-      return;
-    }
-    // a window too large to rank by counting: the radix path below rewrites
-    // every output (the counters are cleared by the ranges pass as always)
-  }
 
   // ---- more movers than one workgroup groups: workgroup 0 writes the tile
   // prefixes and compacts the movers in previous-index order into midx
-  static_assert(MS_BATCH * 4 <= (int)sizeof(s_k), "word prefixes exceed the key buffers");
+  static_assert(MS_BATCH * 4 <= 2 * kSmallSortCap * 8, "word prefixes exceed the key buffers");
   if (blockIdx.x == 0)
     compact_movers(n, tiles, flags, tile_base, tile_lo, n_movers, midx,
                    (lds_u32*)(s_k), (lds_u32*)(s_w));
@@ -854,7 +610,8 @@ __global__ __launch_bounds__(TW) void k_merge(
     uint32_t* __restrict__ pid_dst, uint64_t* __restrict__ keys_dst, uint32_t* __restrict__ perm,
     int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
     uint32_t* __restrict__ bar, uint32_t* __restrict__ lcount, float4* __restrict__ xyuv,
-    const uint32_t* __restrict__ status) {
+    const uint32_t* __restrict__ status, const uint32_t* __restrict__ mode,
+    const uint4* __restrict__ bucket, uint32_t* __restrict__ bovf) {
   // A mover kernel whose grid barrier gave up left its windows partly written:
   // merge nothing (the host reports the status word at the next check).
   if (*status != 0u) return;
@@ -863,6 +620,8 @@ __global__ __launch_bounds__(TW) void k_merge(
   __shared__ uint32_t s_mi[WIN];
   __shared__ uint32_t s_wpre[TW / kWave + 1];  // movers of the tile before each wave
   __shared__ unsigned long long s_fw[TW / kWave];  // the tile's flag words
+  __shared__ uint64_t s_bk_k[kBucketCap];          // the tile's bucket, arrival order
+  __shared__ uint32_t s_bk_i[kBucketCap];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int tile = blockIdx.x;
   const int j0 = tile * TW, j = j0 + tid;
@@ -879,8 +638,11 @@ __global__ __launch_bounds__(TW) void k_merge(
   const float x = tx[jc], y = ty[jc], u = tvx[jc], v = tvy[jc];
   const uint32_t id = pid_src[jc];
   if (tile == 0 && tid == 0) *bar = 0;  // the next k_movers' grid barrier
-  // the next step's mover lists (k_movers, which read the counters, has ended)
+  // the next step's mover lists and bucket overflow word (k_bucket and
+  // k_movers, which read them, have ended)
   if (tile == 0 && tid < kListSegs) lcount[tid * kListStride] = 0;
+  if (tile == 0 && tid == kListSegs) *bovf = 0u;
+  const bool from_bucket = *mode != 0u;
   // the previous order's ranges (k_ranges writes the new ones)
   if (live && k != kp && k < n_keys) {
     cells[k] = make_int2(-1, -1);
@@ -902,9 +664,29 @@ __global__ __launch_bounds__(TW) void k_merge(
   int before_movers = 0;  // window movers ordered before stayer j
   for (int c0 = lo; c0 < hi; c0 += WIN) {
     const int len = min(WIN, hi - c0);
-    for (int r = tid; r < len; r += TW) {
-      s_mk[r] = skey[c0 + r];
-      s_mi[r] = sidx[c0 + r];
+    if (from_bucket) {
+      // the window is the tile's bucket (at most kBucketCap movers, one
+      // chunk): sorted here by (key, previous index), rank by counting
+      const int blen = min(len, kBucketCap);
+      for (int r = tid; r < blen; r += TW) {
+        const uint4 b = bucket[(size_t)tile * kBucketCap + r];
+        s_bk_k[r] = (uint64_t)b.x | ((uint64_t)b.y << 32);
+        s_bk_i[r] = b.z;
+      }
+      __syncthreads();
+      for (int r = tid; r < blen; r += TW) {
+        const uint64_t km = s_bk_k[r];
+        const uint32_t im = s_bk_i[r];
+        int rank = 0;
+        for (int u = 0; u < blen; ++u) rank += pair_less(s_bk_k[u], s_bk_i[u], km, im) ? 1 : 0;
+        s_mk[rank] = km;
+        s_mi[rank] = im;
+      }
+    } else {
+      for (int r = tid; r < len; r += TW) {
+        s_mk[r] = skey[c0 + r];
+        s_mi[r] = sidx[c0 + r];
+      }
     }
     __syncthreads();
     if (live && !mv) {
@@ -981,9 +763,13 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
     k_mover_flags<<<(s.n + MF_TW - 1) / MF_TW, MF_TW, 0, L.stream>>>(
         s.n, (int)resort_flag_words(s.n), knew, s.keys_sorted, mover_sink(r));
   // keys of at most 32 bits: (key, index) in one word
+  auto bk = passes <= 4 ? k_bucket<false> : k_bucket<true>;
+  bk<<<kBucketGrid, KB_TW, 0, L.stream>>>(tiles, s.tile_first, s.tile_count, r.lcount, r.lkey,
+                                          r.lidx, s.cells, (uint64_t)s.n_keys, r.bucket,
+                                          r.bovf);
   auto go = passes <= 4 ? k_movers<false> : k_movers<true>;
   go<<<kMoverGrid, MS_TW, 0, L.stream>>>(s.n, tiles, passes, knew, s.tile_first, s.tile_count,
-                                         r.flags, r.lcount, r.lkey, r.lidx, r.tile_base,
+                                         r.flags, r.lcount, r.bovf, r.mode, r.tile_base,
                                          r.tile_lo, r.n_movers, r.mkey, r.midx, r.ktmp, r.vtmp,
                                          r.skey, r.sidx, r.counts, r.bar, r.status);
   return hipGetLastError();
@@ -998,7 +784,7 @@ hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers&
                                       soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt,
                                       keys_dst, perm ? s.perm : nullptr, s.cells, s.blocks,
                                       s.block_bits, (uint64_t)s.n_keys, r.bar, r.lcount,
-                                      s.xyuv, r.status);
+                                      s.xyuv, r.status, r.mode, r.bucket, r.bovf);
   return hipGetLastError();
 }
 

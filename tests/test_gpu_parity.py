@@ -486,26 +486,44 @@ def test_step_call_splits_equivalent(sph, all_scenes, name):
             assert np.array_equal(f[k], f0[k]), (split, k)
 
 
-@pytest.mark.parametrize("pattern", ["spread", "converge"])
+MOVER_PATTERNS = {
+    # name: (particles sent, kind, listed-mover bounds of the first step)
+    "spread": (18000, "spread", (16384, 32768)),        # grid grouping
+    "spread_cells": (9000, "spread", (4096, 16384)),    # workgroup 0, cell-table owners
+    # (a sent particle's neighbors feel its approach through the artificial
+    # viscosity and move too: about three movers per particle sent)
+    "converge": (2500, "converge", (4096, 16384)),      # one window of thousands of movers
+    "converge_grid": (6000, "converge", (16384, 32768)),  # grid path -> radix
+    "empty_cells": (4500, "empty", (4096, 16384)),      # owner searches past the queue
+}
+
+
+@pytest.mark.parametrize("pattern", list(MOVER_PATTERNS))
 def test_incremental_resort_mid_movers(sph, orc, pattern):
-    """Between 4096 and 32768 movers, all listed: the mover kernel groups them
-    over the whole grid (owner tiles found against the sampled tile-first table,
-    5625 tiles of 256). "spread": ~20 K fast movers over many merge windows;
-    "converge": 6000 particles sent into one spot, so one window holds more
-    movers than are ranked by counting and the radix path takes over. Two steps
-    in a row; the permutation is the oracle's stable sort, the count exact and
-    the cell ranges the oracle's."""
+    """Between 4096 and 32768 movers, all listed. Up to 16384 workgroup 0 takes
+    each mover's owner tile from the previous order's range of its new cell
+    (movers into a previously empty cell search the tile-first table: queued,
+    and past the queue's 4096 entries each by itself), groups and ranks them;
+    above, the whole grid groups them against the sampled tile-first table
+    (5625 tiles of 256). "spread": fast movers over many merge windows;
+    "converge": particles sent into one spot, one window of thousands of
+    movers (ranked by counting on the workgroup path; on the grid path the
+    radix sort takes over); "empty_cells": particles sent into empty space.
+    Two steps in a row; the permutation is the oracle's stable sort, the
+    count exact and the cell ranges the oracle's."""
+    count, kind, (lo, hi) = MOVER_PATTERNS[pattern]
     s = sph.scenes.dam_break(1200, 1200, box=(4.0, 4.0), seed=5)
     rng = np.random.default_rng(17)
-    if pattern == "spread":
-        pick = rng.choice(s.n, 18000, replace=False)
+    pick = rng.choice(s.n, count, replace=False)
+    dt = s.params.dt
+    if kind == "spread":
         s.vx[pick] = rng.normal(0, 400.0, pick.size).astype(np.float32)
         s.vy[pick] = rng.normal(0, 400.0, pick.size).astype(np.float32)
-    else:
-        pick = rng.choice(s.n, 6000, replace=False)
-        dt = s.params.dt
+    elif kind == "converge":
         s.vx[pick] = ((2.0 - s.rx[pick]) / dt).astype(np.float32)
         s.vy[pick] = ((1.0 - s.ry[pick]) / dt).astype(np.float32)
+    else:
+        s.vx[pick] = ((3.0 + 0.5 * rng.random(pick.size) - s.rx[pick]) / dt).astype(np.float32)
     with sph.Context(s.params, s.n) as ctx:
         ctx.upload(s.rx, s.ry, s.vx, s.vy)
         ctx.prepare()
@@ -519,7 +537,7 @@ def test_incremental_resort_mid_movers(sph, orc, pattern):
             st, en = ctx.cell_ranges()
             note(f"[gpu] mid movers ({pattern}), step {step}: {m} of {s.n}")
             if step == 0:
-                assert 4096 < m <= 32768
+                assert lo < m <= hi
             x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
             x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
             k_new = orc.keys(s.params, x_prev, y_prev)
@@ -533,8 +551,8 @@ def test_incremental_resort_mid_movers(sph, orc, pattern):
 
 @pytest.mark.parametrize("side,sigma", [(100, 40.0), (512, 400.0)])
 def test_incremental_resort_many_movers(sph, orc, side, sigma):
-    """More movers than one workgroup sorts in LDS, two steps in a row: 10 K
-    particles (the grid grouping of listed movers) and 262 K particles (more
+    """More movers than the small workgroup path takes, two steps in a row: 10 K
+    particles (workgroup 0's cell-table path) and 262 K particles (more
     movers than are listed: the grid-wide radix sort of the mover kernel, one
     and several 1024-element sub-tiles per workgroup); the permutation is
     still the oracle's stable sort and the mover count exact."""
@@ -545,8 +563,8 @@ def test_incremental_resort_many_movers(sph, orc, side, sigma):
     with sph.Context(s.params, s.n) as ctx:
         ctx.upload(s.rx, s.ry, s.vx, s.vy)
         ctx.prepare()
-        # the second step follows one with more than 4096 movers: the force
-        # kernel then lists none and marks the lists full
+        # at 262 K particles the second step follows one with more than 32768
+        # movers: the force kernel then lists none and marks the lists full
         for step in range(2):
             k_before = ctx.sorted_keys()
             ctx.step(1)
