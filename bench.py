@@ -604,11 +604,18 @@ def main():
     ctx = sph.Context(scene.params, n)
     ctx.upload(scene.rx, scene.ry, scene.vx, scene.vy)
     ctx.prepare()
-    ctx.set_graph(True)
     if args.window_start > 0:
-        ctx.step(args.window_start)
-        ctx.synchronize()
+        # eager launches (no graph) for the untimed run up to the window: the
+        # profiler's kernel tracing has crashed here on long graph replays
+        # right after the set-up
+        done = 0
+        while done < args.window_start:
+            kk = min(100, args.window_start - done)
+            ctx.step(kk)
+            ctx.synchronize()
+            done += kk
         log(f"advanced {args.window_start} steps to the timed window")
+    ctx.set_graph(True)
 
     # step-kernel and re-sort choice: time each combination on graph replay (all
     # give identical results)

@@ -597,9 +597,6 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
         (rc = dev_alloc(c, &r.skey, N)) || (rc = dev_alloc(c, &r.sidx, N)) ||
         (rc = dev_alloc(c, &r.ktmp, N)) || (rc = dev_alloc(c, &r.keys_alt, N)) ||
         (rc = dev_alloc(c, &r.flags, resort_flag_words((int)N))) ||
-        (rc = dev_alloc(c, &r.lcount, kListSegs * kListStride)) ||
-        (rc = dev_alloc(c, &r.lkey, kListSegs * kSmallSortCap)) ||
-        (rc = dev_alloc(c, &r.lidx, kListSegs * kSmallSortCap)) ||
         (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
         (rc = dev_alloc(c, &r.tile_lo, resort_tiles((int)N) + 1)) ||
         (rc = dev_alloc(c, &s.tile_first, resort_tiles((int)N))) ||
@@ -632,8 +629,6 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.bar, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.bovf, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.mode, 0, sizeof(uint32_t), c->stream);
-  if (e == hipSuccess)
-    e = hipMemsetAsync(c->rb.lcount, 0, kListSegs * kListStride * sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, n_keys);
   if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

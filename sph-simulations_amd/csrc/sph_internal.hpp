@@ -11,14 +11,11 @@
 namespace sphamd {
 
 constexpr int kMaxPasses = 8;  // 64-bit keys, 8-bit digits
-constexpr int kSmallSortCap = 4096;  // movers sorted in LDS by one workgroup
-constexpr int kListSegs = 8;         // mover list segments (one per XCD)
-// movers grouped over the whole mover grid (one per thread) when all are listed
-constexpr int kMidSortCap = kListSegs * kSmallSortCap;
+constexpr int kMoverLds = 4096;      // LDS key-buffer entries of the mover radix path
 constexpr int kMoverGrid = 64;       // workgroups of the mover sort (all resident)
 constexpr int kResortTile = 256;     // particles per tile of the incremental re-sort
-constexpr int kListStride = 32;      // words between segment counters (one 128-byte line each)
-constexpr int kBucketCap = 32;       // listed movers a merge tile's bucket holds
+constexpr int kBucketCap = 32;       // movers a merge tile's bucket holds
+constexpr uint32_t kBucketOverflow = 0x80000000u;  // in the bucket count word
 
 struct SortBuffers {
   // n passed to the launchers sizes the grids; when n_dev is set the kernels
@@ -101,65 +98,29 @@ struct ResortBuffers {
   uint32_t* counts;                // [kMoverGrid][256] digit counts
   uint32_t* bar;                   // grid barrier counter
   unsigned long long* flags;       // mover bit per particle, one word per wave
-  uint32_t* lcount;                // [kListSegs * kListStride] movers listed per segment
-  uint64_t* lkey;                  // [kListSegs][kSmallSortCap] listed movers' new keys
-  uint32_t* lidx;                  // and previous indices (any order)
   uint32_t* tile_base;             // movers before each 256-particle tile
   uint32_t* tile_lo;               // [tiles + 1] sorted movers before each tile's first particle
   int* n_movers;
   uint32_t* status;
   uint64_t* keys_alt;              // second sorted-key buffer (ping-pong)
   uint4* bucket;                   // [tiles][kBucketCap] (key lo, key hi, previous index, 0)
-  uint32_t* bovf;                  // nonzero: a bucket overflowed this step
+  uint32_t* bovf;                  // movers bucketed this step; kBucketOverflow: the radix path
   uint32_t* mode;                  // 1: the merge windows come from the buckets, 0: sorted
 };
 
-// Where the producer of the new keys reports the movers: the flag words and,
-// per segment, a counter and up to kSmallSortCap (previous index, new key)
-// entries. flags == nullptr: report nothing.
-// prev: the previous re-sort's mover count; above kMidSortCap the producer
-// lists nothing (most likely the total exceeds the cap again) and marks the
-// lists full instead (list_movers), saving the atomics.
+// Where the producer of the new keys reports the movers: one ballot word per
+// 64 particles of the previous order (flags == nullptr: report nothing).
 struct MoverSink {
   unsigned long long* flags = nullptr;
-  uint32_t* count = nullptr;
-  uint64_t* key = nullptr;
-  uint32_t* idx = nullptr;
-  const int* prev = nullptr;
 };
 
-inline MoverSink mover_sink(const ResortBuffers& r) {
-  return MoverSink{r.flags, r.lcount, r.lkey, r.lidx, r.n_movers};
-}
-
-// Whether this launch lists the movers (uniform); if not, one thread marks the
-// lists full so that the consumer takes the movers from the flag words.
-__device__ __forceinline__ bool list_movers(const MoverSink& m) {
-  const bool listing = *m.prev <= kMidSortCap;
-  if (!listing && blockIdx.x == 0 && threadIdx.x == 0) m.count[0] = (uint32_t)kSmallSortCap + 1u;
-  return listing;
-}
+inline MoverSink mover_sink(const ResortBuffers& r) { return MoverSink{r.flags}; }
 
 // One wave's part of the report, called by all its active lanes (the wave's
-// particles are the 64 indices from a multiple of 64, its first lane active):
-// the wave's flag word, then one atomic on the segment's counter reserves the
-// slots of its movers. Entries past kSmallSortCap in a segment are not
-// written; the consumer then sees that segment's count above it and takes the
-// movers from the flag words.
-__device__ __forceinline__ void record_movers(const MoverSink& m, int i, bool moved, uint64_t kn,
-                                              int seg, bool listing) {
+// particles are the 64 indices from a multiple of 64, its first lane active).
+__device__ __forceinline__ void record_movers(const MoverSink& m, int i, bool moved) {
   const unsigned long long bal = __ballot(moved);
-  const int lane = (int)(threadIdx.x & (kWave - 1));
-  if (lane == 0) m.flags[i / kWave] = bal;
-  if (bal == 0ull || !listing) return;
-  uint32_t base = 0;
-  if (lane == 0) base = atomicAdd(&m.count[seg * kListStride], (uint32_t)__popcll(bal));
-  base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-  const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-  if (moved && slot < (uint32_t)kSmallSortCap) {
-    m.key[seg * kSmallSortCap + slot] = kn;
-    m.idx[seg * kSmallSortCap + slot] = (uint32_t)i;
-  }
+  if ((threadIdx.x & (kWave - 1)) == 0) m.flags[i / kWave] = bal;
 }
 
 struct Launch {

@@ -315,10 +315,8 @@ __global__ __launch_bounds__(BLK) void k_forces(
                                  cell_coord(y, g.lo_y, g.cs_y, g.ny));
     keys_out[i] = kn;
     // particles whose key changes (tiles are BLK-aligned, so wave w of the
-    // tile covers flag word i / 64; waves past n write none); list segment by
-    // the XCD the hardware places the workgroup on
-    if (movers.flags)
-      record_movers(movers, i, kn != ko, kn, blockIdx.x & (kListSegs - 1), list_movers(movers));
+    // tile covers flag word i / 64; waves past n write none)
+    if (movers.flags) record_movers(movers, i, kn != ko);
   }
 }
 

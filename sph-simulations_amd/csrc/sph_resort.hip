@@ -11,27 +11,28 @@
 // previous order, and lists of the movers' (previous index, new key) in eight
 // segments, each filled in any order through one atomic counter per wave with
 // movers; written by the force kernel of the step (or by k_mover_flags for other
-// producers) and read by k_movers; k_merge zeroes the counters.
+// producers); k_merge zeroes the counters.
 //
-//   k_movers  one launch: every workgroup reads the list counters (so none
-//             waits for another to decide the path). Up to kSmallSortCap movers
-//             workgroup 0 takes them from the lists, groups them by the merge
-//             tile they fall in and ranks them inside each group, counting the
-//             tile prefixes on the way; up to kMidSortCap listed movers the
-//             whole grid does the same grouping, one mover per thread (the
-//             counters scanned in one chunk per workgroup), with three grid
-//             barriers; above that (or with a window too large to
-//             rank by counting) workgroup 0 compacts the movers from the flag
-//             words in previous-index order and the grid runs an LSD radix sort
-//             with bounded grid barriers. Either way it ends with
-//             the movers sorted by (key, previous index), the 256-particle tile
-//             prefixes and, per merge tile, the number of movers before the
-//             tile's first particle.
+//   k_bucket  every listed mover (all of them listed, at most kMidSortCap) into
+//             the bucket of its owner tile -- the merge tile of the last
+//             previous-order particle ordered before it, found from the
+//             previous cell range of its new cell (a search of the tile-first
+//             table when that cell was empty); packed per-tile counters count
+//             the owners (slot in the bucket) and the previous-index tiles
+//   k_movers  bucket path: workgroup 0 scans the packed counters into the
+//             256-particle tile prefixes and the window starts. Radix path
+//             (lists past capacity, or a bucket past kBucketCap): workgroup 0
+//             compacts the movers from the flag words in previous-index order
+//             and the grid runs an LSD radix sort with bounded grid barriers,
+//             then the window starts by binary search. Either way the merge
+//             gets the tile prefixes and, per tile, the movers ordered before
+//             its first particle.
 //   k_merge   one workgroup per tile of the previous order: clears the tile's
-//             old cell / block ranges, places its stayers (rank among the
-//             movers of the tile's window, searched in LDS) and the movers of
-//             its window (rank among the tile's keys, searched in LDS); scatters
-//             r, v, pid, key and the permutation
+//             old cell / block ranges, takes its window (its bucket, sorted in
+//             LDS by counting, or a range of the sorted movers), places its
+//             stayers (rank among the window's movers, searched in LDS) and the
+//             window's movers (rank among the tile's keys, searched in LDS);
+//             scatters r, v, pid, key and the permutation
 //   k_ranges  cell start/end and Morton block ranges of the new order
 #include <hip/hip_runtime.h>
 
@@ -64,51 +65,26 @@ __device__ __forceinline__ unsigned long long flag_word(const unsigned long long
   return (int64_t)w * kWave < n ? v : 0ull;
 }
 
-// The movers' report for producers of the new keys that do not write it: the
-// flag words, and the lists with one atomic per workgroup of MF_TW threads (the
-// waves' counts summed in LDS), since such producers may have many movers
-// (list_movers: none at all after a step with more than kMidSortCap).
+// The movers' flag words for producers of the new keys that do not write them.
 constexpr int MF_TW = 1024;
 __global__ __launch_bounds__(MF_TW) void k_mover_flags(int n, int words,
                                                        const uint64_t* __restrict__ knew,
                                                        const uint64_t* __restrict__ kold,
                                                        MoverSink m) {
-  __shared__ uint32_t s_cnt[MF_TW / kWave];
-  __shared__ uint32_t s_base;
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
   const int i = blockIdx.x * MF_TW + tid;
   const int ic = min(i, n - 1);
   const uint64_t kn = knew[ic], ko = kold[ic];
-  const bool moved = i < n && kn != ko;
-  const unsigned long long bal = __ballot(moved);
+  const unsigned long long bal = __ballot(i < n && kn != ko);
   if (lane == 0 && i / kWave < words) m.flags[i / kWave] = bal;
-  if (!list_movers(m)) return;
-  if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(bal);
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t tot = 0;
-    for (int w = 0; w < MF_TW / kWave; ++w) tot += s_cnt[w];
-    s_base = tot ? atomicAdd(&m.count[(blockIdx.x & (kListSegs - 1)) * kListStride], tot)
-                 : (uint32_t)kSmallSortCap;
-  }
-  __syncthreads();
-  uint32_t base = s_base;
-  for (int w = 0; w < wave; ++w) base += s_cnt[w];
-  const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-  const int seg = blockIdx.x & (kListSegs - 1);
-  if (moved && slot < (uint32_t)kSmallSortCap) {
-    m.key[seg * kSmallSortCap + slot] = kn;
-    m.idx[seg * kSmallSortCap + slot] = (uint32_t)i;
-  }
 }
 
 // ------------------------------------------------------------------ movers
 constexpr int MS_TW = 1024;  // threads per k_movers workgroup
 constexpr int MS_WAVES = MS_TW / kWave;
-// the grid radix path reads every workgroup's 256 digit counts into the LDS of
-// the small-sort key buffers; the LDS sort gives each thread 4 elements
-static_assert(kMoverGrid * 256 <= 4 * kSmallSortCap, "digit counts exceed the LDS arrays");
-static_assert(4 * MS_TW == kSmallSortCap, "four movers per thread in the LDS path");
+// the grid radix path reads every workgroup's 256 digit counts into its LDS key
+// buffers
+static_assert(kMoverGrid * 256 <= 4 * kMoverLds, "digit counts exceed the LDS arrays");
 constexpr uint32_t MS_POLLS = 1u << 20;
 
 // Grid-wide barrier on a monotonic counter (zero at kernel start: k_merge
@@ -207,10 +183,6 @@ __device__ __forceinline__ int rank_in_sorted(const uint64_t* keys, const uint32
   return lo;
 }
 
-// Tile-first keys k_bucket stages in LDS for its owner searches (every S-th
-// tile's when there are more tiles).
-constexpr int kTileTab = 4096;
-static_assert(kMidSortCap < (1 << 16), "packed 16-bit tile counters");
 
 // Flag words per thread and per batch of the ordered compaction.
 constexpr int MS_WPT = 12;
@@ -315,62 +287,74 @@ struct Ord<true> {
   __device__ bool operator<(const Ord& o) const { return pair_less(k, i, o.k, o.i); }
 };
 
-// ---- bucket path (all listed movers, at most kMidSortCap): every listed
-// mover goes into the bucket of the merge tile that owns it -- the tile of the
-// last previous-order particle ordered before it by (key, previous index).
-// When its new cell k held particles [s, e) in the previous order, that
-// particle is s - 1 if its previous index i < s (its old key was smaller)
-// and e - 1 otherwise (i lies outside [s, e): its old key differs), one load
-// of the cell table; movers into a cell that was empty search the tile-first
-// table in LDS. The owner's packed counter (tile_count, high half) gives the
-// slot in the bucket, the previous-index tile's counter (low half) counts the
-// movers before each tile. A bucket past kBucketCap raises the overflow word
-// and the step takes the radix path.
+// ---- bucket path: every mover goes into the bucket of the merge tile that
+// owns it -- the tile of the last previous-order particle ordered before it
+// by (key, previous index). When its new cell k held particles [s, e) in the
+// previous order, that particle is s - 1 if its previous index i < s (its
+// old key was smaller) and e - 1 otherwise (i lies outside [s, e): its old
+// key differs), one load of the cell table; movers into a cell that was
+// empty search the tile-first table in LDS. Each workgroup takes KB_TW flag
+// words (one per thread) and lists their set bits in LDS; the owner's packed
+// counter (tile_count, high half) gives the slot in the bucket, the
+// previous-index tile's counter (low half) counts the movers before each
+// tile. The count word (bovf) sums the movers; a bucket past kBucketCap, a
+// workgroup past kBucketList movers or 2^16 movers in all send the step to
+// the radix path.
 constexpr int KB_TW = 256;
-constexpr int kBucketGrid = kMidSortCap / KB_TW;  // one listed mover per thread
-static_assert(kTileTab % KB_TW == 0, "tile-first table loads");
+constexpr int kBucketList = 2048;
+constexpr int kBucketTab = 1024;  // sampled tile-first keys in LDS (every S-th tile)
+static_assert(kBucketTab % KB_TW == 0, "tile-first table loads");
+static_assert(KB_TW % 4 == 0, "four flag words per merge tile");
 
 template <bool WIDE>
 __global__ __launch_bounds__(KB_TW) void k_bucket(
-    int tiles, const uint64_t* __restrict__ tfirst, uint32_t* __restrict__ tcount,
-    const uint32_t* __restrict__ lcount, const uint64_t* __restrict__ lkey,
-    const uint32_t* __restrict__ lidx, const int2* __restrict__ cells, uint64_t n_keys,
+    int n, int tiles, const uint64_t* __restrict__ knew,
+    const unsigned long long* __restrict__ flags, const uint64_t* __restrict__ tfirst,
+    uint32_t* __restrict__ tcount, const int2* __restrict__ cells, uint64_t n_keys,
     uint4* __restrict__ bucket, uint32_t* __restrict__ bovf) {
   using O = Ord<WIDE>;
-  __shared__ uint64_t s_tab[kTileTab];
-  const int tid = threadIdx.x;
-  uint32_t cnt[kListSegs];
-#pragma unroll
-  for (int g = 0; g < kListSegs; ++g) cnt[g] = lcount[g * kListStride];
-  uint32_t seg_lo[kListSegs + 1];
-  seg_lo[0] = 0;
-#pragma unroll
-  for (int g = 0; g < kListSegs; ++g) seg_lo[g + 1] = seg_lo[g] + cnt[g];
-  const uint32_t listed = seg_lo[kListSegs];
-  bool whole = listed <= (uint32_t)kMidSortCap;
-#pragma unroll
-  for (int g = 0; g < kListSegs; ++g) whole = whole && cnt[g] <= (uint32_t)kSmallSortCap;
-  // lists past their capacity: the radix path counts the movers from the flags
-  if (!whole || (uint32_t)(blockIdx.x * KB_TW) >= listed) return;
-  const int S = (tiles + kTileTab - 1) / kTileTab;
+  __shared__ uint32_t s_list[kBucketList];
+  __shared__ uint64_t s_tab[kBucketTab];
+  __shared__ uint32_t s_w[KB_TW / kWave];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int words = (n + kWave - 1) / kWave;
+  const int w = (int)blockIdx.x * KB_TW + tid;
+  const unsigned long long fw = flags[min(w, words - 1)];
+  // the sampled tile-first table for owner searches, loaded with the flags
+  const int S = (tiles + kBucketTab - 1) / kBucketTab;
   const int ne = (tiles + S - 1) / S;
-  // the table, this thread's list entry and its new cell's range, loads in
-  // flight together (indices clamped)
-  uint64_t tv[kTileTab / KB_TW];
+  uint64_t tv[kBucketTab / KB_TW];
 #pragma unroll
-  for (int q = 0; q < kTileTab / KB_TW; ++q)
+  for (int q = 0; q < kBucketTab / KB_TW; ++q)
     tv[q] = tfirst[(size_t)min(q * KB_TW + tid, ne - 1) * S];
-  const int e = (int)blockIdx.x * KB_TW + tid;
-  const uint32_t ec = min((uint32_t)e, listed - 1u);
-  int g = 0;
+  const unsigned long long bits = w < words ? fw : 0ull;
+  const uint32_t c = (uint32_t)__popcll(bits);
+  // movers whose previous index lies in each merge tile (four words per tile,
+  // four adjacent lanes)
+  uint32_t hc = c + (uint32_t)__shfl_xor((int)c, 1);
+  hc += (uint32_t)__shfl_xor((int)hc, 2);
+  if ((lane & 3) == 0 && hc != 0u) atomicAdd(&tcount[w >> 2], hc);
+  // the workgroup's movers listed in LDS (exclusive scan of the word counts)
+  const uint32_t x = wave_incl_scan(c);
+  if (lane == kWave - 1) s_w[wave] = x;
+  __syncthreads();
+  uint32_t off = x - c, tot = 0;
 #pragma unroll
-  for (int h = 1; h < kListSegs; ++h) g += ec >= seg_lo[h] ? 1 : 0;
-  const int at = g * kSmallSortCap + (int)(ec - seg_lo[g]);
-  const uint32_t mi = lidx[at];
-  const uint64_t mk = lkey[at];
-  const int2 c = cells[min(mk, n_keys - 1)];
+  for (int v = 0; v < KB_TW / kWave; ++v) {
+    const uint32_t t = s_w[v];
+    off += v < wave ? t : 0u;
+    tot += t;
+  }
+  if (tot == 0u) return;
+  if (tot > (uint32_t)kBucketList) {
+    if (tid == 0) atomicOr(bovf, kBucketOverflow);
+    return;
+  }
+  if (tid == 0) atomicAdd(bovf, tot);
+  for (unsigned long long b = bits; b != 0ull; b &= b - 1ull)
+    s_list[off++] = (uint32_t)w * kWave + (uint32_t)(__ffsll((long long)b) - 1);
 #pragma unroll
-  for (int q = 0; q < kTileTab / KB_TW; ++q) {
+  for (int q = 0; q < kBucketTab / KB_TW; ++q) {
     const int e2 = q * KB_TW + tid;
     if (e2 < ne) {
       if constexpr (WIDE) s_tab[e2] = tv[q];
@@ -378,41 +362,47 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
     }
   }
   __syncthreads();
-  if (e >= (int)listed) return;
-  const O mo = O::make(mk, mi);
-  uint32_t own;
-  if (c.x >= 0) {
-    const int p = (int)mi < c.x ? c.x - 1 : c.y - 1;
-    own = p < 0 ? 0u : (uint32_t)p / (uint32_t)TW;
-  } else {
-    auto tab = [&](int e2) {
-      if constexpr (WIDE) return O::make(s_tab[e2], (uint32_t)(e2 * S * TW));
-      else return O{s_tab[e2]};
-    };
-    // the last sampled entry before the mover, then the tiles between two samples
-    int base = 0;
-    for (int len = ne; len > 1;) {
-      const int half = len >> 1;
-      if (tab(base + half - 1) < mo) base += half;
-      len -= half;
-    }
-    if (tab(base) < mo) ++base;
-    const int cc = base - 1;
-    int o = max(cc, 0) * S;
-    if (cc >= 0)
-      for (int k = 1; k < S; ++k) {
-        const int t = cc * S + k;
-        if (t < tiles && O::make(tfirst[t], (uint32_t)t * TW) < mo) o = t;
+  for (uint32_t r0 = 0; r0 < tot; r0 += KB_TW) {
+    const uint32_t r = r0 + (uint32_t)tid;
+    if (r >= tot) break;
+    const uint32_t mi = s_list[r];
+    const uint64_t mk = knew[mi];
+    const int2 cr = cells[min(mk, n_keys - 1)];
+    const bool need = cr.x < 0;
+    const O mo = O::make(mk, mi);
+    uint32_t own;
+    if (!need) {
+      const int p = (int)mi < cr.x ? cr.x - 1 : cr.y - 1;
+      own = p < 0 ? 0u : (uint32_t)p / (uint32_t)TW;
+    } else {
+      auto tab = [&](int e2) {
+        if constexpr (WIDE) return O::make(s_tab[e2], (uint32_t)(e2 * S * TW));
+        else return O{s_tab[e2]};
+      };
+      // the last sampled entry before the mover, then the tiles between two samples
+      int base = 0;
+      for (int len = ne; len > 1;) {
+        const int half = len >> 1;
+        if (tab(base + half - 1) < mo) base += half;
+        len -= half;
       }
-    own = (uint32_t)o;
+      if (tab(base) < mo) ++base;
+      const int cc = base - 1;
+      int o = max(cc, 0) * S;
+      if (cc >= 0)
+        for (int k = 1; k < S; ++k) {
+          const int t = cc * S + k;
+          if (t < tiles && O::make(tfirst[t], (uint32_t)t * TW) < mo) o = t;
+        }
+      own = (uint32_t)o;
+    }
+    const uint32_t slot = atomicAdd(&tcount[own], 1u << 16) >> 16;
+    if (slot < (uint32_t)kBucketCap)
+      bucket[(size_t)own * kBucketCap + slot] =
+          make_uint4((uint32_t)mk, (uint32_t)(mk >> 32), mi, 0u);
+    else
+      atomicOr(bovf, kBucketOverflow);
   }
-  const uint32_t slot = atomicAdd(&tcount[own], 1u << 16) >> 16;
-  atomicAdd(&tcount[mi / (uint32_t)TW], 1u);
-  if (slot < (uint32_t)kBucketCap)
-    bucket[(size_t)own * kBucketCap + slot] =
-        make_uint4((uint32_t)mk, (uint32_t)(mk >> 32), mi, 0u);
-  else
-    *bovf = 1u;
 }
 
 // ---- tile prefixes and, when the buckets cannot be used, the movers sorted.
@@ -429,15 +419,15 @@ template <bool WIDE>
 __global__ __launch_bounds__(MS_TW) void k_movers(
     int n, int tiles, int passes, const uint64_t* __restrict__ knew,
     const uint64_t* __restrict__ tfirst, const uint32_t* __restrict__ tcount,
-    const unsigned long long* __restrict__ flags, const uint32_t* __restrict__ lcount,
-    const uint32_t* __restrict__ bovf, uint32_t* __restrict__ mode,
+    const unsigned long long* __restrict__ flags, const uint32_t* __restrict__ bovf,
+    uint32_t* __restrict__ mode,
     uint32_t* __restrict__ tile_base, uint32_t* __restrict__ tile_lo, int* __restrict__ n_movers,
     uint64_t* __restrict__ mkey, uint32_t* __restrict__ midx, uint64_t* __restrict__ ktmp,
     uint32_t* __restrict__ vtmp, uint64_t* __restrict__ skey, uint32_t* __restrict__ sidx,
     uint32_t* __restrict__ counts, uint32_t* __restrict__ bar, uint32_t* __restrict__ status) {
   // key buffers: the word prefixes of the compaction and the digit counts of
   // the grid sort
-  __shared__ uint64_t s_k[2 * kSmallSortCap];
+  __shared__ uint64_t s_k[2 * kMoverLds];
   // digit counts: u32 [MS_WAVES][256] per wave in the grid sort
   __shared__ uint32_t s_wc[2 * MS_WAVES * 256];
   __shared__ uint32_t s_base[256];
@@ -445,18 +435,15 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
   __shared__ uint32_t s_w[MS_WAVES];
   __shared__ int s_ok;
   const int tid = threadIdx.x, wave = tid / kWave;
-  uint32_t cnt[kListSegs];
+  const uint32_t bucketed_movers = *bovf;
+  // workgroup 0 loads the first tile counters with them (used on the bucket path)
+  uint32_t c0v[4] = {0u, 0u, 0u, 0u};
+  if (blockIdx.x == 0) {
 #pragma unroll
-  for (int g = 0; g < kListSegs; ++g) cnt[g] = lcount[g * kListStride];
-  const uint32_t overflow = *bovf;
-  uint32_t listed = 0;
-  bool bucketed = overflow == 0u;
-#pragma unroll
-  for (int g = 0; g < kListSegs; ++g) {
-    listed += cnt[g];
-    bucketed = bucketed && cnt[g] <= (uint32_t)kSmallSortCap;
+    for (int k = 0; k < 4; ++k) c0v[k] = tcount[min(4 * tid + k, tiles - 1)];
   }
-  bucketed = bucketed && listed <= (uint32_t)kMidSortCap;
+  // no overflow and fewer than 2^16 movers (the packed counters' prefixes fit)
+  const bool bucketed = bucketed_movers < (1u << 16);
   if (bucketed) {
     if (blockIdx.x != 0) return;
     // exclusive scan of the packed counters (previous-index tile low half,
@@ -468,7 +455,7 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int t = t0 + 4 * tid + k;
-        const uint32_t v = tcount[min(t, tiles - 1)];
+        const uint32_t v = t0 == 0 ? c0v[k] : tcount[min(t, tiles - 1)];
         c[k] = t < tiles ? v : 0u;
         sum += c[k];
       }
@@ -486,8 +473,8 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
       carry += rtot;
     }
     if (tid == 0) {
-      *n_movers = (int)listed;
-      tile_lo[tiles] = listed;
+      *n_movers = (int)bucketed_movers;
+      tile_lo[tiles] = bucketed_movers;
       *mode = 1u;
     }
     return;
@@ -497,7 +484,7 @@ __global__ __launch_bounds__(MS_TW) void k_movers(
 
   // ---- more movers than one workgroup groups: workgroup 0 writes the tile
   // prefixes and compacts the movers in previous-index order into midx
-  static_assert(MS_BATCH * 4 <= 2 * kSmallSortCap * 8, "word prefixes exceed the key buffers");
+  static_assert(MS_BATCH * 4 <= 2 * kMoverLds * 8, "word prefixes exceed the key buffers");
   if (blockIdx.x == 0)
     compact_movers(n, tiles, flags, tile_base, tile_lo, n_movers, midx,
                    (lds_u32*)(s_k), (lds_u32*)(s_w));
@@ -609,7 +596,7 @@ __global__ __launch_bounds__(TW) void k_merge(
     float* __restrict__ ry, float* __restrict__ vx, float* __restrict__ vy,
     uint32_t* __restrict__ pid_dst, uint64_t* __restrict__ keys_dst, uint32_t* __restrict__ perm,
     int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
-    uint32_t* __restrict__ bar, uint32_t* __restrict__ lcount, float4* __restrict__ xyuv,
+    uint32_t* __restrict__ bar, float4* __restrict__ xyuv,
     const uint32_t* __restrict__ status, const uint32_t* __restrict__ mode,
     const uint4* __restrict__ bucket, uint32_t* __restrict__ bovf) {
   // A mover kernel whose grid barrier gave up left its windows partly written:
@@ -638,10 +625,8 @@ __global__ __launch_bounds__(TW) void k_merge(
   const float x = tx[jc], y = ty[jc], u = tvx[jc], v = tvy[jc];
   const uint32_t id = pid_src[jc];
   if (tile == 0 && tid == 0) *bar = 0;  // the next k_movers' grid barrier
-  // the next step's mover lists and bucket overflow word (k_bucket and
-  // k_movers, which read them, have ended)
-  if (tile == 0 && tid < kListSegs) lcount[tid * kListStride] = 0;
-  if (tile == 0 && tid == kListSegs) *bovf = 0u;
+  // the next step's bucket count word (k_bucket and k_movers have ended)
+  if (tile == 0 && tid == 0) *bovf = 0u;
   const bool from_bucket = *mode != 0u;
   // the previous order's ranges (k_ranges writes the new ones)
   if (live && k != kp && k < n_keys) {
@@ -764,12 +749,13 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
         s.n, (int)resort_flag_words(s.n), knew, s.keys_sorted, mover_sink(r));
   // keys of at most 32 bits: (key, index) in one word
   auto bk = passes <= 4 ? k_bucket<false> : k_bucket<true>;
-  bk<<<kBucketGrid, KB_TW, 0, L.stream>>>(tiles, s.tile_first, s.tile_count, r.lcount, r.lkey,
-                                          r.lidx, s.cells, (uint64_t)s.n_keys, r.bucket,
-                                          r.bovf);
+  const int words = (int)((s.n + kWave - 1) / kWave);
+  bk<<<(words + KB_TW - 1) / KB_TW, KB_TW, 0, L.stream>>>(
+      s.n, tiles, knew, r.flags, s.tile_first, s.tile_count, s.cells, (uint64_t)s.n_keys,
+      r.bucket, r.bovf);
   auto go = passes <= 4 ? k_movers<false> : k_movers<true>;
   go<<<kMoverGrid, MS_TW, 0, L.stream>>>(s.n, tiles, passes, knew, s.tile_first, s.tile_count,
-                                         r.flags, r.lcount, r.bovf, r.mode, r.tile_base,
+                                         r.flags, r.bovf, r.mode, r.tile_base,
                                          r.tile_lo, r.n_movers, r.mkey, r.midx, r.ktmp, r.vtmp,
                                          r.skey, r.sidx, r.counts, r.bar, r.status);
   return hipGetLastError();
@@ -783,8 +769,8 @@ hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers&
                                       r.skey, r.sidx, s.tx, s.ty, s.tvx, s.tvy, s.pid,
                                       soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt,
                                       keys_dst, perm ? s.perm : nullptr, s.cells, s.blocks,
-                                      s.block_bits, (uint64_t)s.n_keys, r.bar, r.lcount,
-                                      s.xyuv, r.status, r.mode, r.bucket, r.bovf);
+                                      s.block_bits, (uint64_t)s.n_keys, r.bar, s.xyuv,
+                                      r.status, r.mode, r.bucket, r.bovf);
   return hipGetLastError();
 }
 
