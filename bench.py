@@ -537,6 +537,27 @@ def bench_surrogate(sph, scene, steps, train_iters, train_scenes=16, train_steps
                          "vs_plain": t_p / t_f, "mean_error": st_fast["mean_error"],
                          "max_error": st_fast["max_error"], "drift_vs_plain": drift(f_)},
     })
+    # the network alone against the BF16 MFMA peak: FLOPs of its node GEMMs over
+    # the time of whole forward passes (edge passes included)
+    fwd = sph.Context(scene.params, n)
+    fwd.upload(scene.rx, scene.ry, scene.vx, scene.vy)
+    fwd.prepare()
+    L = sph.lib()
+    if L.sph_nn_forward(net.handle, fwd.handle, None, None) != 0:
+        raise sph.SphError("sph_nn_forward failed")
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.sph_nn_forward(net.handle, fwd.handle, None, None)
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / reps
+    fwd.close()
+    W, K0 = 128, 32
+    flops = n * (2 * K0 * W + 2 * W * W + 3 * (4 * 2 * W * W + 2 * 2 * W * W) + 2 * 2 * W * W)
+    tflops = flops / (fwd_ms * 1e-3) / 1e12
+    out["roofline"] = {"bound": "mfma", "kernel": "network forward (node GEMMs, BF16 MFMA)",
+                       "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                       "frac": tflops / 2500.0, "forward_ms": fwd_ms,
+                       "gemm_flops_per_forward": flops}
     for c in (plain, surr, plain2, fast):
         c.close()
     net.close()

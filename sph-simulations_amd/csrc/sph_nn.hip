@@ -1121,9 +1121,11 @@ hipError_t nn_forward(Net* net, const DevState& s, hipStream_t st, float* ax, fl
          P_(net, lt(net, l, L_d)), nullptr, nullptr, b.Mb, 0);
     gemm(st, n, b.hb, W, b.Mb, W, WB(net, lt(net, l, L_E)), W, P_(net, lt(net, l, L_e)), nullptr,
          nullptr, nullptr, net->train ? b.preU : nullptr, b.Ub, 1);
-    // h_{l+1} = h_l + F U + f (in place when activations are not kept)
+    // h_{l+1} = h_l + F U + f (in place when activations are not kept; the
+    // readout reads only the bf16 copy, so inference skips the last FP32 h)
+    const bool keep_h = net->train || l + 1 < net->L;
     gemm(st, n, b.Ub, W, nullptr, 0, WB(net, lt(net, l, L_F)), W, P_(net, lt(net, l, L_f)),
-         nullptr, nullptr, b.h, nb.h, nb.hb, 0);
+         nullptr, nullptr, b.h, keep_h ? nb.h : nullptr, nb.hb, 0);
   }
   NnLayerBufs& f = net->lb[net->train ? net->L : 0];
   gemm(st, n, f.hb, W, nullptr, 0, WB(net, rt(net, R_A)), W, nullptr, nullptr, nullptr, nullptr,
