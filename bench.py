@@ -365,13 +365,36 @@ def bench_small(sph, steps):
     ctx.synchronize()
     stage = ctx.stage_times()
     ctx.set_timing(False)
-    ctx.close()
     kern = {kk: round(v[0] / v[1] * 1e3, 3) for kk, v in stage.items() if v[1]}
+    ctx.close()
     log(f"small: {wall * 1e3 / steps:.4f} ms/step")
+    # the cell-block mappings (a workgroup per 8 x 8-cell Morton block, halo in
+    # LDS) against one thread per particle at the batched scenes' sizes
+    mappings = {}
+    for side in (128, 256):
+        sc2 = sph.scenes.dam_break(side, side)
+        c2 = sph.Context(sc2.params, sc2.n)
+        c2.upload(sc2.rx, sc2.ry, sc2.vx, sc2.vy)
+        c2.prepare()
+        c2.set_graph(True)
+        row = {}
+        for kname, kv in (("particle", sph.STEP_PARTICLE), ("split_block", sph.STEP_SPLIT_BLOCK),
+                          ("fused_block", sph.STEP_FUSED_BLOCK)):
+            c2.set_step_kernel(kv)
+            c2.step(10)
+            c2.synchronize()
+            t0 = time.perf_counter()
+            c2.step(200)
+            c2.synchronize()
+            row[kname] = round((time.perf_counter() - t0) * 1e6 / 200, 2)
+        c2.close()
+        mappings[f"{sc2.n}_particles_us_per_step"] = row
+    log(f"mappings {mappings}")
     return {"workload": sc.name, "particles": n, "steps": steps,
             "value": n * steps / wall, "unit": "particle-steps/s",
             "us_per_step": wall * 1e6 / steps, "us_per_launch_events": kern,
-            "bound": "launch latency (five dependent kernels per step, graph-replayed)"}
+            "bound": "launch latency (six dependent kernels per step, graph-replayed)",
+            "cell_block_mappings": mappings}
 
 
 def bench_batched(sph, steps, trial_steps=10):
