@@ -191,13 +191,6 @@ def cpu_baseline(scene, budget_s):
     }
 
 
-def mover_path(m):
-    """Which path of the incremental re-sort's mover kernel a step with m movers
-    takes: one workgroup (up to 4,096 listed movers), the grid-wide grouping (up
-    to 32,768) or the grid radix sort."""
-    return "workgroup" if m <= 4096 else ("grid_grouping" if m <= 32768 else "radix")
-
-
 def stage_table(stage, steps, n, k, occ, passes, variant, resort, mfrac):
     """Per-stage launches, time and achieved algorithmic GB/s of an events pass
     of `steps` steps; returns (stages, algorithmic bytes per particle-step, kernel ms)."""
@@ -223,7 +216,8 @@ def stage_table(stage, steps, n, k, occ, passes, variant, resort, mfrac):
 def full_run_profile(sph, scene, total, chunk=100):
     """The workload's whole run (`total` steps from rest, graph-replayed), timed
     in chunks of `chunk` steps; after each chunk the movers of its last step and
-    the mover path they take. Returns the run's mean step time and the profile."""
+    the way the re-sort ordered them ("bucket" or "sorted"). Returns the run's
+    mean step time and the profile."""
     n = scene.n
     ctx = sph.Context(scene.params, n)
     ctx.upload(scene.rx, scene.ry, scene.vx, scene.vy)
@@ -239,12 +233,12 @@ def full_run_profile(sph, scene, total, chunk=100):
         dt = time.perf_counter() - t0
         t_sum += dt
         done += kk
-        rows.append([done, round(dt * 1e3 / kk, 4), ctx.movers()])
+        rows.append([done, round(dt * 1e3 / kk, 4), ctx.movers(), ctx.resort_path()])
     m = ctx.measure()
     ctx.close()
     paths = {}
     for r in rows:
-        paths[mover_path(r[2])] = paths.get(mover_path(r[2]), 0) + 1
+        paths[r[3]] = paths.get(r[3], 0) + 1
     q = max(1, len(rows) // 10)
     out = {"steps": total, "value": n * total / t_sum, "ms_per_step": t_sum * 1e3 / total,
            "chunk_steps": chunk,
@@ -689,12 +683,14 @@ def main():
     ctx.synchronize()
     log(f"warmup {args.warmup} steps done")
     # movers per step (particles whose cell changed) over a short sample
-    mv = []
+    mv, mp = [], {}
     for _ in range(20):
         ctx.step(1)
         mv.append(ctx.movers())
+        pth = ctx.resort_path()
+        mp[pth] = mp.get(pth, 0) + 1
     mover_stats = {"min": int(min(mv)), "mean": float(np.mean(mv)), "max": int(max(mv)),
-                   "path": mover_path(int(np.mean(mv)))}
+                   "path_steps": mp}
     log(f"movers per step {mover_stats}")
     window_first = args.window_start + 24 * len(combos) + args.warmup + 20
 

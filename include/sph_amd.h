@@ -241,6 +241,11 @@ int sph_set_step_kernel(void* ctx, int32_t variant);
 int sph_set_resort(void* ctx, int32_t mode);
 /* Particles whose cell key changed in the last incremental re-sort. Synchronises. */
 int sph_movers(void* ctx, int64_t* count);
+/* Which way the last incremental re-sort ordered its movers: 1 through the
+ * per-tile buckets (every tile's movers within its bucket capacity), 0 through
+ * the sorted-movers fallback. Synchronises. No counterpart in the reference
+ * (its sort is a full std::sort / thrust sort every step). */
+int sph_resort_path(void* ctx, int32_t* path);
 /* Re-evaluate the block kernels on the current state with counters on:
  * out[0..2] density (or fused) kernel: occupied blocks, blocks over the LDS
  * capacity, staged particles; out[3..5] the same for the force kernel. */

@@ -164,6 +164,7 @@ def lib():
         "sph_set_step_kernel": (i32, [vp, i32]),
         "sph_set_resort": (i32, [vp, i32]),
         "sph_movers": (i32, [vp, P(i64)]),
+        "sph_resort_path": (i32, [vp, P(i32)]),
         "sph_block_stats": (i32, [vp, P(i64)]),
         "sph_get_sorted_keys": (i32, [vp, vp]),
         "sph_get_permutation": (i32, [vp, vp]),
@@ -221,7 +222,7 @@ EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_
             "sph_density_pressure",
             "sph_forces", "sph_integrate", "sph_update_velocities", "sph_update_positions",
             "sph_set_dt", "sph_measure", "sph_prepare", "sph_step",
-            "sph_set_graph", "sph_ell_overflow", "sph_set_step_kernel", "sph_set_resort", "sph_movers", "sph_block_stats", "sph_get_sorted_keys", "sph_get_permutation",
+            "sph_set_graph", "sph_ell_overflow", "sph_set_step_kernel", "sph_set_resort", "sph_movers", "sph_resort_path", "sph_block_stats", "sph_get_sorted_keys", "sph_get_permutation",
             "sph_get_cell_ranges", "sph_get_neighbor_lists", "sph_get_ell",
             "sph_get_device_view", "sph_set_timing", "sph_stage_times", "sph_synchronize",
             "sph_batch_create", "sph_batch_scenes", "sph_batch_scene_info",
@@ -426,6 +427,12 @@ class Context:
         c = C.c_int64()
         self._chk(self._L.sph_movers(self._h, C.byref(c)), "sph_movers")
         return int(c.value)
+
+    def resort_path(self) -> str:
+        """'bucket' or 'sorted': how the last incremental re-sort ordered its movers."""
+        c = C.c_int32()
+        self._chk(self._L.sph_resort_path(self._h, C.byref(c)), "sph_resort_path")
+        return "bucket" if c.value else "sorted"
 
     def block_stats(self):
         out = np.zeros(6, np.int64)

@@ -445,7 +445,8 @@ int fused_iteration(Ctx* c) {
 // Re-sort the integrated twin T (keys in sb.keys_in) into S with new ranges.
 int resort_integrated(Ctx* c, bool flags_ready) {
   DevState& s = c->s;
-  if (c->resort == SPH_RESORT_INCREMENTAL) return incremental_resort(c, flags_ready);
+  if (c->resort == SPH_RESORT_INCREMENTAL && s.n <= kResortMaxParticles)
+    return incremental_resort(c, flags_ready);
   int rc = sort_keys(c);
   if (rc) return rc;
   {
@@ -503,11 +504,6 @@ int sync_and_check(Ctx* c) {
   uint32_t st = 0;
   HIPCHK(c, hipMemcpyAsync(&st, c->sb.status, sizeof(st), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (st & 4u)
-    return fail(c, SPH_ERR_HIP,
-                "incremental re-sort: a grid barrier of the mover kernel gave up waiting "
-                "(device status 4; its workgroups were not all resident) and the merge was "
-                "skipped: the state is invalid");
   if (st != 0)
     return fail(c, SPH_ERR_HIP, "radix sort look-back gave up waiting (device status " +
                                     std::to_string(st) + ")");
@@ -598,13 +594,12 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
         (rc = dev_alloc(c, &r.ktmp, N)) || (rc = dev_alloc(c, &r.keys_alt, N)) ||
         (rc = dev_alloc(c, &r.flags, resort_flag_words((int)N))) ||
         (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
-        (rc = dev_alloc(c, &r.tile_lo, resort_tiles((int)N) + 1)) ||
         (rc = dev_alloc(c, &s.tile_first, resort_tiles((int)N))) ||
-        (rc = dev_alloc(c, &s.tile_count, resort_tiles((int)N))) ||
-        (rc = dev_alloc(c, &r.vtmp, N)) || (rc = dev_alloc(c, &r.counts, kMoverGrid * 256)) ||
-        (rc = dev_alloc(c, &r.bar, 1)) || (rc = dev_alloc(c, &r.n_movers, 1)) ||
-        (rc = dev_alloc(c, &r.bucket, resort_tiles((int)N) * kBucketCap)) ||
-        (rc = dev_alloc(c, &r.bovf, 1)) || (rc = dev_alloc(c, &r.mode, 1)))
+        (rc = dev_alloc(c, &s.tile_count, resort_count_words((int)N))) ||
+        (rc = dev_alloc(c, &r.vtmp, N)) || (rc = dev_alloc(c, &r.tally, 1)) ||
+        (rc = dev_alloc(c, &r.n_movers, 1)) ||
+        (rc = dev_alloc(c, &r.bucket, resort_tiles((int)N) * kBucketCap * 2)) ||
+        (rc = dev_alloc(c, &r.mode, 1)))
       return bail(rc);
   }
   b.blocks = s.blocks;
@@ -624,10 +619,8 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
     return bail(rc);
   e = hipMemsetAsync(b.hist, 0, kMaxPasses * 256 * sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(b.status, 0, sizeof(uint32_t), c->stream);
-  c->rb.status = b.status;
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.n_movers, 0, sizeof(int), c->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(c->rb.bar, 0, sizeof(uint32_t), c->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(c->rb.bovf, 0, sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->rb.tally, 0, sizeof(unsigned long long), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.mode, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, n_keys);
   if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);
@@ -1278,6 +1271,16 @@ int sph_movers(void* ctx, int64_t* count) {
   HIPCHK(c, hipMemcpyAsync(&m, c->rb.n_movers, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *count = m;
+  return SPH_OK;
+}
+
+int sph_resort_path(void* ctx, int32_t* path) {
+  Ctx* c = as_ctx(ctx);
+  if (!c || !path) return SPH_ERR_INVALID;
+  uint32_t m = 0;
+  HIPCHK(c, hipMemcpyAsync(&m, c->rb.mode, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *path = m != 0u ? 1 : 0;
   return SPH_OK;
 }
 

@@ -11,11 +11,13 @@
 namespace sphamd {
 
 constexpr int kMaxPasses = 8;  // 64-bit keys, 8-bit digits
-constexpr int kMoverLds = 4096;      // LDS key-buffer entries of the mover radix path
-constexpr int kMoverGrid = 64;       // workgroups of the mover sort (all resident)
 constexpr int kResortTile = 256;     // particles per tile of the incremental re-sort
-constexpr int kBucketCap = 32;       // movers a merge tile's bucket holds
-constexpr uint32_t kBucketOverflow = 0x80000000u;  // in the bucket count word
+constexpr int kBucketCap = 128;      // movers a merge tile's bucket holds
+constexpr int kBucketFast = 32;      // bucket entries the merge loads with its first round
+constexpr int kChunkTiles = 64;      // merge tiles per counter chunk (one k_bucket workgroup)
+// k_bucket's tally word counts its workgroups (one per chunk) in 16 bits:
+// larger sets re-sort in full
+constexpr int64_t kResortMaxParticles = 65535ll * kResortTile * kChunkTiles;
 
 struct SortBuffers {
   // n passed to the launchers sizes the grids; when n_dev is set the kernels
@@ -77,7 +79,10 @@ struct DevState {
   int32_t* csr_idx;
   uint32_t* hist;  // the sort's histogram, cleared by the permutation kernel
   uint64_t* tile_first;  // key of every kResortTile-th sorted slot (with the ranges)
-  uint32_t* tile_count;  // per kResortTile tile, zeroed with the ranges (re-sort counters)
+  // re-sort counters per kResortTile tile, then per chunk of kChunkTiles tiles
+  // (at tiles + c), zeroed with the ranges: movers by previous index (low 32
+  // bits) and by owner tile (high 32 bits)
+  unsigned long long* tile_count;
   // Morton blocks (square groups of 2^block_bits cells per side, a contiguous key
   // range): particle range per block, -1 when empty
   int block_bits;   // 0 when the order has no block structure (row-major)
@@ -89,23 +94,25 @@ enum { NB_STENCIL = 0, NB_CSR = 1, NB_ELL = 2 };
 
 // Buffers of the incremental re-sort (sph_resort.hip).
 struct ResortBuffers {
-  uint64_t* mkey;                  // movers' new keys, previous-index order
-  uint32_t* midx;                  // movers' previous indices
-  uint64_t* skey;                  // movers sorted by (key, previous index)
+  unsigned long long* flags;       // mover bit per particle, one word per wave
+  // [tiles][kBucketCap] pairs: (key lo, key hi, previous index, pid), (x, y, vx, vy)
+  uint4* bucket;
+  // k_bucket's tally, one atomic add per workgroup: movers (bits 0-31),
+  // workgroups done (32-47; the last one finishes the step), workgroups with a
+  // full bucket (48-63)
+  unsigned long long* tally;
+  uint32_t* mode;                  // 1: the merge windows come from the buckets, 0: sorted
+  int* n_movers;
+  uint64_t* keys_alt;              // second sorted-key buffer (ping-pong)
+  // sorted path (a bucket overflowed): movers in previous-index order, then
+  // sorted by (key, previous index), and the movers before each 256-particle tile
+  uint64_t* mkey;
+  uint32_t* midx;
+  uint64_t* skey;
   uint32_t* sidx;
   uint64_t* ktmp;                  // sort scratch
   uint32_t* vtmp;
-  uint32_t* counts;                // [kMoverGrid][256] digit counts
-  uint32_t* bar;                   // grid barrier counter
-  unsigned long long* flags;       // mover bit per particle, one word per wave
-  uint32_t* tile_base;             // movers before each 256-particle tile
-  uint32_t* tile_lo;               // [tiles + 1] sorted movers before each tile's first particle
-  int* n_movers;
-  uint32_t* status;
-  uint64_t* keys_alt;              // second sorted-key buffer (ping-pong)
-  uint4* bucket;                   // [tiles][kBucketCap] (key lo, key hi, previous index, 0)
-  uint32_t* bovf;                  // movers bucketed this step; kBucketOverflow: the radix path
-  uint32_t* mode;                  // 1: the merge windows come from the buckets, 0: sorted
+  uint32_t* tile_base;
 };
 
 // Where the producer of the new keys reports the movers: one ballot word per
@@ -151,7 +158,7 @@ hipError_t launch_pack(const Launch& L, const DevState& s);
 hipError_t launch_unpack(const Launch& L, const DevState& s);
 // fused: also integrate into T and write the next keys; movers (fused only,
 // flags may be null): the particles whose next key differs from s.keys_sorted
-// (the input of the incremental re-sort), as flag words and the mover list
+// (the input of the incremental re-sort), as flag words
 hipError_t launch_forces(const Launch& L, const DevState& s, int nb_mode, bool fused,
                          uint64_t* keys_out, MoverSink movers = MoverSink{},
                          bool packed = false);
@@ -162,11 +169,15 @@ hipError_t launch_integrate(const Launch& L, const DevState& s);
 hipError_t launch_block_step(const Launch& L, const DevState& s, uint64_t* keys_out, int grid,
                              int variant, unsigned long long* stats, int part = 0);
 int block_step_grid(int64_t n_blocks, int cus);
-// incremental re-sort: movers -> (sort movers) -> merge -> ranges
+// incremental re-sort: flags -> buckets (or sorted movers) -> merge -> ranges
 int64_t resort_flag_words(int n);
 int64_t resort_tiles(int n);
-// tile prefixes, the movers sorted into skey / sidx and the merge windows (one
-// launch; a second one first when the producer of knew reported no movers)
+// words of DevState::tile_count for n particles: the tiles, then the chunks
+int64_t resort_count_words(int n);
+// movers into their owner tiles' buckets with the tile and chunk counters
+// (k_bucket; when a bucket overflowed its last workgroup sorts the movers into
+// skey / sidx instead); k_mover_flags first when the producer of knew wrote no
+// flag words
 hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers& r,
                          const uint64_t* knew, int key_bits, bool flags_ready);
 // soa: also write the separate arrays rx, ry, vx, vy (else only s.xyuv);

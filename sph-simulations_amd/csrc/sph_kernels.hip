@@ -44,7 +44,7 @@ __device__ __forceinline__ bool write_ranges(int i, int n, const uint64_t* __res
                                              int2* __restrict__ cells, int64_t n_keys,
                                              int2* __restrict__ blocks, int block_bits,
                                              uint64_t* __restrict__ tile_first,
-                                             uint32_t* __restrict__ tile_count) {
+                                             unsigned long long* __restrict__ tile_count) {
   const bool live = i < n;
   const int ic = min(i, n - 1);
   const uint64_t kc = keys[ic], kpc = keys[max(ic - 1, 0)], knc = keys[min(ic + 1, n - 1)];
@@ -52,6 +52,8 @@ __device__ __forceinline__ bool write_ranges(int i, int n, const uint64_t* __res
   if (live && i % kResortTile == 0) {
     tile_first[i / kResortTile] = k;
     tile_count[i / kResortTile] = 0;
+    if (i % (kResortTile * kChunkTiles) == 0)
+      tile_count[(n + kResortTile - 1) / kResortTile + i / (kResortTile * kChunkTiles)] = 0;
   }
   const int lane = threadIdx.x & (kWave - 1);
   uint64_t prev = __shfl_up(k, 1);
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(BLK) void k_gather(
     float* __restrict__ dvx, float* __restrict__ dvy, uint32_t* __restrict__ dpid,
     float* __restrict__ dax, float* __restrict__ day, int2* __restrict__ cells,
     int64_t n_keys, uint32_t* __restrict__ hist, int hist_words, int2* __restrict__ blocks,
-    int block_bits, uint64_t* __restrict__ tile_first, uint32_t* __restrict__ tile_count,
+    int block_bits, uint64_t* __restrict__ tile_first, unsigned long long* __restrict__ tile_count,
     float4* __restrict__ xyuv) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(BLK) void k_ranges(int n, const uint64_t* __restric
                                                 int2* __restrict__ blocks, int block_bits,
                                                 uint32_t* __restrict__ hist, int hist_words,
                                                 uint64_t* __restrict__ tile_first,
-                                                uint32_t* __restrict__ tile_count) {
+                                                unsigned long long* __restrict__ tile_count) {
   const int stride = gridDim.x * BLK;
   const int i0 = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
@@ -165,6 +167,8 @@ __global__ __launch_bounds__(BLK) void k_ranges(int n, const uint64_t* __restric
     if (i % kResortTile == 0) {
       tile_first[i / kResortTile] = k;
       tile_count[i / kResortTile] = 0;
+      if (i % (kResortTile * kChunkTiles) == 0)
+        tile_count[(n + kResortTile - 1) / kResortTile + i / (kResortTile * kChunkTiles)] = 0;
     }
     if (k >= (uint64_t)n_keys) continue;
     if (i == 0 || k != kp[q]) cells[k].x = i;

@@ -426,12 +426,12 @@ def test_capacity_and_state_errors(sph):
             ctx.step(1)
 
 
-def test_incremental_resort_listed_movers_many_tiles(sph, orc):
-    """At most 4096 movers among more than 4096 re-sort tiles (1.44 M particles,
-    5625 tiles of 256): the movers come from the force kernel's lists and the
-    per-tile counters live in global memory instead of LDS; the permutation is
-    still the oracle's stable sort and the mover count the number of changed
-    keys."""
+def test_incremental_resort_buckets_many_chunks(sph, orc):
+    """About 2,700 movers among 5625 re-sort tiles (1.44 M particles, 88
+    counter chunks of 64 tiles, so the merge's prefixes sum chunk counters
+    over more than one wave): every mover goes into its owner tile's bucket;
+    the permutation is still the oracle's stable sort and the mover count the
+    number of changed keys."""
     s = sph.scenes.dam_break(1200, 1200, box=(4.0, 4.0), seed=5)
     rng = np.random.default_rng(11)
     pick = rng.choice(s.n, 1000, replace=False)  # plus about 1,700 slow movers
@@ -445,11 +445,12 @@ def test_incremental_resort_listed_movers_many_tiles(sph, orc):
         after = ctx.download()
         perm = ctx.permutation()
         m = ctx.movers()
+        path = ctx.resort_path()
         sk = ctx.sorted_keys()
         st, en = ctx.cell_ranges()
         n_keys = int(ctx.grid().n_keys)
-    note(f"[gpu] listed movers, {s.n} particles: {m}")
-    assert 0 < m <= 4096
+    note(f"[gpu] bucketed movers, {s.n} particles: {m} ({path})")
+    assert 0 < m <= 4096 and path == "bucket"
     ost, oen = orc.cell_ranges(sk, n_keys)
     assert np.array_equal(st, ost) and np.array_equal(en, oen)
     x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
@@ -458,6 +459,41 @@ def test_incremental_resort_listed_movers_many_tiles(sph, orc):
     assert m == int(np.sum(k_new != k_before))
     operm, _ = orc.stable_sort(k_new)
     assert np.array_equal(perm, operm)
+
+
+def test_incremental_resort_bucket_rounds(sph, orc):
+    """One bucketing workgroup's 16,384 particles with about a quarter of them
+    moving one cell: more movers than its LDS list holds (listed in rounds of
+    2048) and windows of 32 to 128 movers per tile (the merge loads the bucket
+    entries past the first 32 in a second round); still the bucket path, and
+    the oracle's permutation and cell ranges."""
+    s = sph.scenes.dam_break(128, 128, seed=21)
+    with sph.Context(s.params, s.n) as ctx:
+        cs = float(ctx.grid().cs_x)
+        rng = np.random.default_rng(23)
+        # displacements of about 0.15 cell per axis and step
+        s.vx = (0.15 * cs / s.params.dt * rng.normal(0, 1, s.n)).astype(np.float32)
+        s.vy = (0.15 * cs / s.params.dt * rng.normal(0, 1, s.n)).astype(np.float32)
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.prepare()
+        for step in range(2):
+            k_before = ctx.sorted_keys()
+            ctx.step(1)
+            after = ctx.download()
+            perm = ctx.permutation()
+            m = ctx.movers()
+            path = ctx.resort_path()
+            note(f"[gpu] bucket rounds, step {step}: {m} of {s.n} ({path})")
+            assert m > 2048 and path == "bucket"
+            x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
+            x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
+            k_new = orc.keys(s.params, x_prev, y_prev)
+            assert m == int(np.sum(k_new != k_before))
+            operm, sk = orc.stable_sort(k_new)
+            assert np.array_equal(perm, operm)
+            st, en = ctx.cell_ranges()
+            ost, oen = orc.cell_ranges(sk, int(ctx.grid().n_keys))
+            assert np.array_equal(st, ost) and np.array_equal(en, oen)
 
 
 @pytest.mark.parametrize("name", ["dam_break_16k", "periodic_9k", "random_clusters"])
@@ -487,31 +523,30 @@ def test_step_call_splits_equivalent(sph, all_scenes, name):
 
 
 MOVER_PATTERNS = {
-    # name: (particles sent, kind, listed-mover bounds of the first step)
-    "spread": (18000, "spread", (16384, 32768)),        # grid grouping
-    "spread_cells": (9000, "spread", (4096, 16384)),    # workgroup 0, cell-table owners
+    # name: (particles sent, kind, mover bounds of the first step, its path)
     # (a sent particle's neighbors feel its approach through the artificial
     # viscosity and move too: about three movers per particle sent)
-    "converge": (2500, "converge", (4096, 16384)),      # one window of thousands of movers
-    "converge_grid": (6000, "converge", (16384, 32768)),  # grid path -> radix
-    "empty_cells": (4500, "empty", (4096, 16384)),      # owner searches past the queue
+    "spread": (18000, "spread", (16384, 32768), "bucket"),
+    "spread_cells": (9000, "spread", (4096, 16384), "bucket"),
+    "converge": (2500, "converge", (4096, 16384), "sorted"),
+    "converge_grid": (6000, "converge", (16384, 32768), "sorted"),
+    "empty_cells": (4500, "empty", (4096, 16384), "sorted"),
 }
 
 
 @pytest.mark.parametrize("pattern", list(MOVER_PATTERNS))
 def test_incremental_resort_mid_movers(sph, orc, pattern):
-    """Between 4096 and 32768 movers, all listed. Up to 16384 workgroup 0 takes
-    each mover's owner tile from the previous order's range of its new cell
-    (movers into a previously empty cell search the tile-first table: queued,
-    and past the queue's 4096 entries each by itself), groups and ranks them;
-    above, the whole grid groups them against the sampled tile-first table
-    (5625 tiles of 256). "spread": fast movers over many merge windows;
-    "converge": particles sent into one spot, one window of thousands of
-    movers (ranked by counting on the workgroup path; on the grid path the
-    radix sort takes over); "empty_cells": particles sent into empty space.
-    Two steps in a row; the permutation is the oracle's stable sort, the
-    count exact and the cell ranges the oracle's."""
-    count, kind, (lo, hi) = MOVER_PATTERNS[pattern]
+    """Thousands of movers over 5625 tiles of 256. "spread": fast movers over
+    many tiles, each tile's owned movers within its bucket; "converge":
+    particles sent into one spot, one tile owning thousands of movers, so its
+    bucket overflows and the step takes the sorted path (ordered compaction
+    and a stable radix sort in one workgroup); "empty_cells": particles sent
+    into empty space, many of them owned by one tile (sorted path; the
+    owner search in the sampled tile-first table runs for the "spread" movers
+    that land in empty cells). Two steps in
+    a row; the permutation is the oracle's stable sort, the count exact and
+    the cell ranges the oracle's."""
+    count, kind, (lo, hi), want = MOVER_PATTERNS[pattern]
     s = sph.scenes.dam_break(1200, 1200, box=(4.0, 4.0), seed=5)
     rng = np.random.default_rng(17)
     pick = rng.choice(s.n, count, replace=False)
@@ -533,11 +568,13 @@ def test_incremental_resort_mid_movers(sph, orc, pattern):
             after = ctx.download()
             perm = ctx.permutation()
             m = ctx.movers()
+            path = ctx.resort_path()
             sk = ctx.sorted_keys()
             st, en = ctx.cell_ranges()
-            note(f"[gpu] mid movers ({pattern}), step {step}: {m} of {s.n}")
+            note(f"[gpu] mid movers ({pattern}), step {step}: {m} of {s.n} ({path})")
             if step == 0:
                 assert lo < m <= hi
+                assert path == want
             x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
             x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
             k_new = orc.keys(s.params, x_prev, y_prev)
@@ -551,11 +588,11 @@ def test_incremental_resort_mid_movers(sph, orc, pattern):
 
 @pytest.mark.parametrize("side,sigma", [(100, 40.0), (512, 400.0)])
 def test_incremental_resort_many_movers(sph, orc, side, sigma):
-    """More movers than the small workgroup path takes, two steps in a row: 10 K
-    particles (workgroup 0's cell-table path) and 262 K particles (more
-    movers than are listed: the grid-wide radix sort of the mover kernel, one
-    and several 1024-element sub-tiles per workgroup); the permutation is
-    still the oracle's stable sort and the mover count exact."""
+    """Most particles moving 0.5 to 5 cells per step, two steps in a row: 10 K
+    particles (one bucketing workgroup) and 262 K particles (16 of them, more
+    than 65,536 movers); tiles own more movers than a bucket holds, so both
+    take the sorted path; the permutation is still the oracle's stable sort
+    and the mover count exact."""
     s = sph.scenes.dam_break(side, side, seed=12)
     rng = np.random.default_rng(3)
     s.vx = rng.normal(0, sigma, s.n).astype(np.float32)  # ~0.5 to 5 cells per step
@@ -563,16 +600,15 @@ def test_incremental_resort_many_movers(sph, orc, side, sigma):
     with sph.Context(s.params, s.n) as ctx:
         ctx.upload(s.rx, s.ry, s.vx, s.vy)
         ctx.prepare()
-        # at 262 K particles the second step follows one with more than 32768
-        # movers: the force kernel then lists none and marks the lists full
         for step in range(2):
             k_before = ctx.sorted_keys()
             ctx.step(1)
             after = ctx.download()
             perm = ctx.permutation()
             m = ctx.movers()
-            note(f"[gpu] many movers, step {step}: {m} of {s.n}")
-            assert m > 4096
+            path = ctx.resort_path()
+            note(f"[gpu] many movers, step {step}: {m} of {s.n} ({path})")
+            assert m > 4096 and path == "sorted"
             x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
             x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
             k_new = orc.keys(s.params, x_prev, y_prev)
