@@ -230,7 +230,9 @@ int sph_snapshot_load(void* ctx, const char* path, int64_t* step);
  * cell-range kernel. */
 int sph_prepare(void* ctx);
 int sph_step(void* ctx, int32_t n_steps);
-/* Capture one fused iteration in a hipGraph and replay it in sph_step. */
+/* Capture fused iterations in a hipGraph and replay it in sph_step: enable = 0
+ * off, k >= 1 (at most 64) a graph of 2k iterations (steps left over run
+ * without it). */
 int sph_set_graph(void* ctx, int32_t enable);
 /* Number of particles whose neighbor count exceeded the fixed list width in the
  * last density pass (they are handled by a stencil fallback). Synchronises. */

@@ -414,8 +414,9 @@ class Context:
     def step(self, n_steps=1):
         self._chk(self._L.sph_step(self._h, int(n_steps)), "sph_step")
 
-    def set_graph(self, enable=True):
-        self._chk(self._L.sph_set_graph(self._h, int(bool(enable))), "sph_set_graph")
+    def set_graph(self, enable=True, pairs: int = 4):
+        """Replay sph_step's iterations from a hipGraph of 2 * pairs iterations."""
+        self._chk(self._L.sph_set_graph(self._h, int(pairs) if enable else 0), "sph_set_graph")
 
     def set_step_kernel(self, variant: int):
         self._chk(self._L.sph_set_step_kernel(self._h, int(variant)), "sph_set_step_kernel")

@@ -66,6 +66,7 @@ struct Ctx {
   bool fields_prev_order = false;
   // graph
   bool use_graph = false;
+  int graph_pairs = 4;  // iteration pairs a captured graph holds
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int graph_n = -1;
@@ -467,19 +468,23 @@ void destroy_graph(Ctx* c) {
   c->graph_n = -1;
 }
 
-// The graph holds one fused iteration with fixed buffer pointers. The pid
-// buffers swap once per iteration, so two iterations are captured (an even
-// number of swaps returns every pointer to where it started).
+// The graph holds fused iterations with fixed buffer pointers. The pid
+// buffers swap once per iteration, so an even number of iterations is captured
+// (2 * graph_pairs: an even number of swaps returns every pointer to where it
+// started).
 int capture_graph(Ctx* c) {
   destroy_graph(c);
   bool was_timing = c->timing;
   c->timing = false;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  // the first of the two iterations leaves the observable fields to the second
-  c->fields_out = false;
-  int rc = fused_iteration(c);
+  // only the last iteration writes the observable fields
+  int rc = SPH_OK;
+  const int iters = 2 * c->graph_pairs;
+  for (int it = 0; it < iters && rc == SPH_OK; ++it) {
+    c->fields_out = it + 1 == iters;
+    rc = fused_iteration(c);
+  }
   c->fields_out = true;
-  if (rc == SPH_OK) rc = fused_iteration(c);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   c->timing = was_timing;
@@ -1199,8 +1204,10 @@ int sph_step(void* ctx, int32_t n_steps) {
   auto run = [&]() -> int {
     int done = 0, rc2 = SPH_OK;
     if (c->use_graph && !c->timing) {
-      if (n_steps >= 2 && !graph_matches(c) && (rc2 = capture_graph(c))) return rc2;
-      for (; done + 2 <= n_steps; done += 2) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+      const int per = 2 * c->graph_pairs;
+      if (n_steps >= per && !graph_matches(c) && (rc2 = capture_graph(c))) return rc2;
+      for (; done + per <= n_steps; done += per)
+        HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     }
     for (; done < n_steps; ++done) {
       c->fields_out = done + 1 == n_steps;
@@ -1226,7 +1233,12 @@ int sph_step(void* ctx, int32_t n_steps) {
 int sph_set_graph(void* ctx, int32_t enable) {
   Ctx* c = as_ctx(ctx);
   if (!c) return SPH_ERR_INVALID;
+  if (enable < 0 || enable > 64) return fail(c, SPH_ERR_INVALID, "graph: 0 to 64 iteration pairs");
   c->use_graph = enable != 0;
+  if (enable != 0 && enable != c->graph_pairs) {
+    c->graph_pairs = enable;
+    destroy_graph(c);
+  }
   if (!c->use_graph) destroy_graph(c);
   return SPH_OK;
 }

@@ -165,10 +165,10 @@ def test_batch_256_scenes_full_size(sph, orc):
     every scene in its segment with finite state; the smallest scene matches its
     own oracle run bit for bit."""
     b = sph.scenes.batched_dam_breaks()
-    steps = 6
+    steps = 7  # run(): prepare, then six steps, one replay of a six-step graph
     note(f"[gpu] batch 256 scenes n={b.n}")
     with sph.Batch(b.params, b.counts) as ctx:
-        ctx.set_graph(True)
+        ctx.set_graph(True, 3)
         ctx.upload(b.rx, b.ry, b.vx, b.vy)
         ctx.run(steps)
         d = ctx.download()

@@ -349,23 +349,33 @@ def test_fused_step_equals_stage_path(sph, all_scenes):
             assert np.array_equal(fa[k], fb[k]), k
 
 
-def test_graph_replay_equals_direct(sph, all_scenes):
+@pytest.mark.parametrize("pairs", [1, 4])
+def test_graph_replay_equals_direct(sph, all_scenes, pairs):
+    """Graphs of 2 * pairs iterations: several replays with eager work in
+    between, then two replays and three direct iterations in one call, equal
+    direct steps, state and fields."""
     s = all_scenes["periodic_9k"]
+    per = 2 * pairs
     with sph.Context(s.params, s.n) as a, sph.Context(s.params, s.n) as b:
         for c in (a, b):
             c.upload(s.rx, s.ry, s.vx, s.vy)
             c.prepare()
-        b.set_graph(True)
+        b.set_graph(True, pairs)
         for _ in range(3):   # several replays with eager work in between
-            a.step(4)
-            b.step(4)
+            a.step(per)
+            b.step(per)
             a.measure(); b.measure()
             da, db = a.download(), b.download()
             for k in ("rx", "ry", "vx", "vy", "pid"):
                 assert np.array_equal(da[k], db[k]), k
-        b.step(3)   # odd count: one direct iteration after the graph
-        a.step(3)
-        assert np.array_equal(a.download()["rx"], b.download()["rx"])
+        b.step(2 * per + 3)
+        a.step(2 * per + 3)
+        da, db = a.download(), b.download()
+        for k in ("rx", "ry", "vx", "vy", "pid"):
+            assert np.array_equal(da[k], db[k]), k
+        fa, fb = a.fields(), b.fields()
+        for k in ("ax", "ay", "rho"):
+            assert np.array_equal(fa[k], fb[k]), k
 
 
 @pytest.mark.parametrize("name", ["dam_break_16k", "random_clusters"])

@@ -50,11 +50,11 @@ def test_full_size_structures(sph, orc, big, name):
 
 def test_full_size_steps_match_oracle(sph, orc, big):
     s = big["dam_break_1m"]
-    steps = 4
+    steps = 5  # run(): prepare, then four steps, one replay of a four-step graph
     o = orc.run(s.params, s.rx, s.ry, s.vx, s.vy, steps)
     with sph.Context(s.params, s.n) as ctx:
         ctx.upload(s.rx, s.ry, s.vx, s.vy)
-        ctx.set_graph(True)
+        ctx.set_graph(True, 2)
         ctx.run(steps)
         d = ctx.download()
         f = ctx.fields()
