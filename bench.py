@@ -36,8 +36,8 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
     """Algorithmic HBM bytes per particle of one launch of each stage (FP32
     state, u64 keys, u32 indices; neighbor and halo re-reads counted as cache
     hits). flags_in_force: the force kernel reports the re-sort's movers
-    (reading the previous key; flag bits and the mover list). None for a stage
-    without a model."""
+    (reading the previous key, writing one flag bit). None for a stage without
+    a model."""
     if variant == "split_block":
         # density: R r 8, W rho,pr 8; force: R r,v,rho,pr 24, W a 8, T r,v 16, key 8
         phys = {"approximate_density": 16.0, "force_computation": 56.0}
@@ -47,13 +47,12 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
     else:
         # density: R r 8, W the packed (rho, pr) copy 8, count 4, list 4k;
         # force: R r,v 16, rho,pr 8, count 4, list 4k, T 16, key 8 (+ R
-        # previous key 8, W one flag bit and the movers' list entries 12 f when
-        # it reports the movers); the separate rho, pr and a arrays (8 + 8) are
-        # written only in the last step of a sph_step call; neighbors come from
-        # the packed copies (cache hits)
+        # previous key 8 and W one flag bit when it reports the movers); the
+        # separate rho, pr and a arrays (8 + 8) are written only in the last
+        # step of a sph_step call; neighbors come from the packed copies (cache
+        # hits)
         phys = {"approximate_density": 20.0 + 4.0 * k,
-                "force_computation": 52.0 + 4.0 * k
-                + ((8.125 + 12.0 * movers_frac) if flags_in_force else 0.0)}
+                "force_computation": 52.0 + 4.0 * k + (8.125 if flags_in_force else 0.0)}
     if resort == "full":
         sort = {
             # R new keys (8) + previous sorted keys (8); clear previous ranges (8/occ. cell)
@@ -67,24 +66,27 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
     else:
         f = movers_frac
         sort = {
-            # up to 4096 movers: R tile-first keys 8 and W tile prefixes and
-            # window starts 8 per 256-particle tile; R list key+idx 12, W sorted
-            # key+idx 12 per mover (+ R new and old keys 16, W flag bit and list
-            # 12 f when no kernel before reported the movers)
-            "movers": 16.0 / 256.0 + 24.0 * f
-            + (0.0 if flags_in_force else 16.125 + 12.0 * f),
-            # R old key 8, T r,v 16, pid 4; W r,v 16 (only the packed copy
-            # inside sph_step), pid 4, key 8 (the permutation 4 only in the
-            # last step of a call); clear the old ranges 8 occ
-            "merge": 56.0 + 8.0 * occ,
-            # ranges: R key 8, W 8 occ
-            "neighbor_search": 8.0 + 8.0 * occ,
+            # k_bucket: R the flag bits 1/8, a packed 8-byte counter per
+            # 256-particle tile; per mover R new key 8, T r,v 16, pid 4, its new
+            # cell's range 8, W its bucket entry 32, its owner's counter 8 (+ R
+            # new and old keys 16, W the flag bit when no kernel before
+            # reported the movers)
+            "movers": 0.125 + 8.0 / 256.0 + 76.0 * f
+            + (0.0 if flags_in_force else 16.125),
+            # k_merge: R old key 8, T r,v 16, pid 4, flag bits 1/8, the movers'
+            # bucket entries 32 f; W r,v 16 (only the packed copy inside
+            # sph_step), pid 4, key 8 (the permutation 4 only in the last step
+            # of a call); clear the old ranges 8 occ
+            "merge": 56.125 + 32.0 * f + 8.0 * occ,
+            # k_ranges: R key 8, W 8 occ, the tile-first key and the zeroed
+            # counter 16 per 256-particle tile
+            "neighbor_search": 8.0 + 16.0 / 256.0 + 8.0 * occ,
         }
     return {**phys, **sort}.get(stage)
 
 
 STAGE_KERNEL = {"approximate_density": "k_density", "force_computation": "k_forces",
-                "movers": "k_movers", "merge": "k_merge", "neighbor_search": "k_ranges"}
+                "movers": "k_bucket", "merge": "k_merge", "neighbor_search": "k_ranges"}
 
 
 def measured_traffic(kernel):
