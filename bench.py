@@ -389,7 +389,7 @@ def bench_small(sph, steps):
     return {"workload": sc.name, "particles": n, "steps": steps,
             "value": n * steps / wall, "unit": "particle-steps/s",
             "us_per_step": wall * 1e6 / steps, "us_per_launch_events": kern,
-            "bound": "launch latency (six dependent kernels per step, graph-replayed)",
+            "bound": "launch latency (five dependent kernels per step, graph-replayed)",
             "cell_block_mappings": mappings}
 
 
