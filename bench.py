@@ -435,7 +435,9 @@ def bench_batched(sph, steps, trial_steps=10):
         if launches == 0:
             continue
         per = ms / launches
-        bpp = alg_bytes_per_particle(name, k, occ, passes, "particle", resort, mfrac)
+        # the batched force kernel reports the movers itself (incremental re-sort)
+        bpp = alg_bytes_per_particle(name, k, occ, passes, "particle", resort, mfrac,
+                                     flags_in_force=resort == "incremental")
         stages[name] = {"launches": launches, "us_per_launch": round(per * 1e3, 3)}
         kern_ms += ms
         if bpp is None:

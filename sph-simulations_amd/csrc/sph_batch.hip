@@ -65,10 +65,13 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
     const int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ ax_out,
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
     float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out,
-    const float4* __restrict__ xyuv, const float2* __restrict__ rp) {
+    const float4* __restrict__ xyuv, const float2* __restrict__ rp,
+    const uint64_t* __restrict__ kold, unsigned long long* __restrict__ flags) {
   const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int i = t.begin + (int)threadIdx.x;
   if (i >= t.end) return;
+  // the previous key, for the re-sort's mover report (flags null: none)
+  const uint64_t ko = flags ? kold[i] : 0ull;
   const SceneDev& S = scenes[t.scene];
   const DevGrid g = S.g;
   const DevPhys c = S.c;
@@ -91,8 +94,20 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
   ty[i] = y;
   tvx[i] = u;
   tvy[i] = v;
-  keys_out[i] = base + cell_key(g, cell_coord(x, g.lo_x, g.cs_x, g.nx),
-                                cell_coord(y, g.lo_y, g.cs_y, g.ny));
+  const uint64_t kn = base + cell_key(g, cell_coord(x, g.lo_x, g.cs_x, g.nx),
+                                      cell_coord(y, g.lo_y, g.cs_y, g.ny));
+  keys_out[i] = kn;
+  if (flags) {
+    // The mover flags of the incremental re-sort: a workgroup's particles start
+    // anywhere in the order, so a wave's 64 bits straddle at most two flag
+    // words; OR them in (the merge zeroes the words after reading them).
+    const unsigned long long bal = __ballot(kn != ko);
+    if ((threadIdx.x & (kWave - 1)) == 0 && bal != 0ull) {
+      const int w0 = i / kWave, sh = i % kWave;
+      atomicOr(&flags[w0], bal << sh);
+      if (sh != 0 && (bal >> (kWave - sh)) != 0ull) atomicOr(&flags[w0 + 1], bal >> (kWave - sh));
+    }
+  }
 }
 
 bool tile_ok(const BatchLaunch& B) {
@@ -133,21 +148,23 @@ hipError_t launch_batch_density(const Launch& L, const BatchLaunch& B, const Dev
 
 template <int PHYS, int BC>
 static void bforces_go(const Launch& L, const BatchLaunch& B, const DevState& s,
-                       uint64_t* keys_out) {
+                       uint64_t* keys_out, unsigned long long* flags) {
   k_bforces<PHYS, BC><<<B.n_tiles, B.tile, 0, L.stream>>>(
       s.n, B.tiles, B.scenes, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.ell_count,
-      s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out, s.xyuv, s.rp);
+      s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out, s.xyuv, s.rp,
+      s.keys_sorted, flags);
 }
 
 hipError_t launch_batch_forces(const Launch& L, const BatchLaunch& B, const DevState& s,
-                               uint64_t* keys_out) {
+                               uint64_t* keys_out, MoverSink movers) {
   if (s.n <= 0) return hipSuccess;
   if (!tile_ok(B)) return hipErrorInvalidValue;
   const int ph = L.phys.physics, bc = L.phys.boundary;
-  if (ph == 0 && bc == 0) bforces_go<0, 0>(L, B, s, keys_out);
-  else if (ph == 0 && bc == 1) bforces_go<0, 1>(L, B, s, keys_out);
-  else if (ph == 1 && bc == 0) bforces_go<1, 0>(L, B, s, keys_out);
-  else bforces_go<1, 1>(L, B, s, keys_out);
+  unsigned long long* f = movers.flags;
+  if (ph == 0 && bc == 0) bforces_go<0, 0>(L, B, s, keys_out, f);
+  else if (ph == 0 && bc == 1) bforces_go<0, 1>(L, B, s, keys_out, f);
+  else if (ph == 1 && bc == 0) bforces_go<1, 0>(L, B, s, keys_out, f);
+  else bforces_go<1, 1>(L, B, s, keys_out, f);
   return hipGetLastError();
 }
 

@@ -405,8 +405,14 @@ int fused_iteration(Ctx* c) {
       StageTimer t(c, SPH_T_DENSITY);
       HIPCHK(c, launch_batch_density(c->L, c->B, fs));
     }
-    StageTimer t(c, SPH_T_FORCE);
-    HIPCHK(c, launch_batch_forces(c->L, c->B, fs, c->sb.keys_in));
+    // the force kernel also flags the movers for the incremental re-sort
+    const bool inc = c->resort == SPH_RESORT_INCREMENTAL;
+    {
+      StageTimer t(c, SPH_T_FORCE);
+      HIPCHK(c, launch_batch_forces(c->L, c->B, fs, c->sb.keys_in,
+                                    inc ? mover_sink(c->rb) : MoverSink{}));
+    }
+    return resort_integrated(c, inc);
   } else if (s.block_bits && c->step_kernel == SPH_STEP_FUSED_BLOCK) {
     // Morton order, one kernel: density, forces, integration and keys
     StageTimer t(c, SPH_T_BLOCK_STEP);
@@ -626,6 +632,10 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
   if (e == hipSuccess) e = hipMemsetAsync(b.status, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.n_movers, 0, sizeof(int), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.tally, 0, sizeof(unsigned long long), c->stream);
+  // producers that OR their mover bits in expect zero flag words
+  if (e == hipSuccess)
+    e = hipMemsetAsync(c->rb.flags, 0, resort_flag_words((int)N) * sizeof(unsigned long long),
+                       c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->rb.mode, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, n_keys);
   if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);

@@ -232,8 +232,10 @@ hipError_t launch_batch_keys(const Launch& L, const BatchLaunch& B, const DevSta
                              uint64_t* keys_out);
 // density + EOS + fixed-width list of every scene
 hipError_t launch_batch_density(const Launch& L, const BatchLaunch& B, const DevState& s);
-// forces + integration into T + next keys of every scene
+// forces + integration into T + next keys of every scene; movers (flags may be
+// null): the particles whose next key differs from s.keys_sorted, ORed into
+// flag words that are zero on entry (the merge zeroes them after reading)
 hipError_t launch_batch_forces(const Launch& L, const BatchLaunch& B, const DevState& s,
-                               uint64_t* keys_out);
+                               uint64_t* keys_out, MoverSink movers = MoverSink{});
 
 }  // namespace sphamd

@@ -479,7 +479,7 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
 // Sorted path: the movers before the tile from tile_base, the window's ends by
 // binary search of the tile-first keys in the sorted movers.
 __global__ __launch_bounds__(TW) void k_merge(
-    int n, const uint64_t* __restrict__ kold, const unsigned long long* __restrict__ flags,
+    int n, const uint64_t* __restrict__ kold, unsigned long long* __restrict__ flags,
     const uint64_t* __restrict__ tfirst, const unsigned long long* __restrict__ tcount,
     const float* __restrict__ tx, const float* __restrict__ ty, const float* __restrict__ tvx,
     const float* __restrict__ tvy, const uint32_t* __restrict__ pid_src, float* __restrict__ rx,
@@ -558,6 +558,8 @@ __global__ __launch_bounds__(TW) void k_merge(
     s_wpre[0] = 0;
     for (int w = 1; w <= TW / kWave; ++w) s_wpre[w] += s_wpre[w - 1];
   }
+  // the tile's flag words are read (s_fw): zero them for producers that OR in
+  if (lane == 0) flags[j0 / kWave + wave] = 0ull;
   uint32_t tb, wlo, whi;
   if (from_bucket) {
     uint32_t slo = 0, shi = 0;
