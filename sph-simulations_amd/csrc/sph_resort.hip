@@ -466,10 +466,10 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
 }
 
 // ------------------------------------------------------------------- merge
-// One workgroup per tile of the previous order [j0, j0 + TW). Its window of
-// sorted movers [lo, hi) holds the movers ordered after the tile's first
-// particle and before the next tile's first particle (tile 0: from the first
-// mover), so every mover belongs to exactly one window.
+// Per tile of the previous order [j0, j0 + TW): its window of sorted movers
+// [lo, hi) holds the movers ordered after the tile's first particle and
+// before the next tile's first particle (tile 0: from the first mover), so
+// every mover belongs to exactly one window.
 //   stayer j:  slot = (j - movers before j) + lo + (window movers before it)
 //   mover r:   slot = r + (stayers before it), where the previous-order
 //              particles before it are j0 + (tile particles before it)
@@ -478,6 +478,14 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
 // earlier tiles and of the earlier chunks; the window is the tile's bucket.
 // Sorted path: the movers before the tile from tile_base, the window's ends by
 // binary search of the tile-first keys in the sorted movers.
+// A workgroup takes MT consecutive tiles, each thread one particle of each:
+// every load of a particle's state is issued in one round for all of them, so
+// at 1M particles the whole grid is resident at once.
+constexpr int MT = 2;          // tiles per merge workgroup
+constexpr int BH = TW / MT;    // threads that hold one tile's bucket entries
+static_assert(kBucketCap <= BH && MT * kBucketCap <= WIN,
+              "a tile's bucket maps to its share of the threads and of the window buffer");
+
 __global__ __launch_bounds__(TW) void k_merge(
     int n, const uint64_t* __restrict__ kold, unsigned long long* __restrict__ flags,
     const uint64_t* __restrict__ tfirst, const unsigned long long* __restrict__ tcount,
@@ -487,207 +495,259 @@ __global__ __launch_bounds__(TW) void k_merge(
     uint32_t* __restrict__ pid_dst, uint64_t* __restrict__ keys_dst, uint32_t* __restrict__ perm,
     int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
     float4* __restrict__ xyuv, ResortBuffers r) {
-  __shared__ uint64_t s_tk[TW];  // the tile's previous keys (all ones past n)
+  __shared__ uint64_t s_tk[MT][TW];  // the tiles' previous keys (all ones past n)
+  // window keys and previous indices: bucket path tile q at q * kBucketCap,
+  // sorted path one tile's chunk at a time
   __shared__ uint64_t s_mk[WIN];
   __shared__ uint32_t s_mi[WIN];
-  __shared__ uint32_t s_wpre[TW / kWave + 1];  // movers of the tile before each wave
-  __shared__ unsigned long long s_fw[TW / kWave];  // the tile's flag words
+  __shared__ uint64_t s_ak[MT][kBucketCap];  // bucket path: keys, arrival order
+  __shared__ uint32_t s_ai[MT][kBucketCap];
+  __shared__ float4 s_ms[MT][kBucketCap];    // bucket path: state, sorted order
+  __shared__ uint32_t s_mp[MT][kBucketCap];
+  __shared__ uint32_t s_wpre[MT][TW / kWave + 1];  // movers of a tile before each wave
+  __shared__ unsigned long long s_fw[MT][TW / kWave];  // the tiles' flag words
   __shared__ uint32_t s_plo[TW / kWave], s_phi[TW / kWave];  // counter sums per wave
-  __shared__ uint32_t s_win[2];                    // sorted path: the window's ends
-  __shared__ uint64_t s_bk_k[kBucketCap];          // the tile's bucket, arrival order
-  __shared__ uint32_t s_bk_i[kBucketCap];
-  __shared__ uint4 s_bk_s[kBucketCap];             // (x, y, vx, vy) bits, arrival order
-  __shared__ uint32_t s_bk_p[kBucketCap];          // pid, arrival order
-  __shared__ float4 s_ms[kBucketCap];              // the window's state, sorted order
-  __shared__ uint32_t s_mp[kBucketCap];
+  __shared__ uint32_t s_win[MT + 1];  // sorted path: the windows' ends
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const int tile = blockIdx.x;
   const int tiles = (n + TW - 1) / TW;
-  const int j0 = tile * TW, j = j0 + tid;
-  const bool live = j < n;
-  // every load that depends only on j, issued together
-  const unsigned long long fw = flag_word(flags, j0 / kWave + wave, n);
-  // the packed counters: the chunk's tiles before this one (lanes of wave 0)
-  // and the chunks before this one
-  const int ch = tile / kChunkTiles, tin = tile % kChunkTiles;
+  const int t0 = (int)blockIdx.x * MT;
+  // every load that depends only on the particles, issued together
+  unsigned long long fw[MT];
+  uint64_t k[MT], kp[MT];
+  float x[MT], y[MT], u[MT], v[MT];
+  uint32_t id[MT];
+  bool live[MT];
+#pragma unroll
+  for (int q = 0; q < MT; ++q) {
+    const int j0 = (t0 + q) * TW, j = j0 + tid;
+    live[q] = j < n;
+    fw[q] = flag_word(flags, j0 / kWave + wave, n);
+    const int jc = min(j, n - 1);
+    const uint64_t kc = kold[jc], kpc = kold[max(jc - 1, 0)];
+    k[q] = live[q] ? kc : ~0ull;
+    kp[q] = (live[q] && j > 0) ? kpc : ~k[q];
+    x[q] = tx[jc];
+    y[q] = ty[jc];
+    u[q] = tvx[jc];
+    v[q] = tvy[jc];
+    id[q] = pid_src[jc];
+  }
+  // the packed counters: the chunk's tiles before t0 (lanes of wave 0), the
+  // chunks before it, and the workgroup's own tiles
+  const int ch = t0 / kChunkTiles, tin = t0 % kChunkTiles;
   const unsigned long long tcv =
       tcount[min(ch * kChunkTiles + (tid & (kChunkTiles - 1)), tiles - 1)];
-  const unsigned long long own = tcount[tile];
+  unsigned long long own[MT];
+#pragma unroll
+  for (int q = 0; q < MT; ++q) own[q] = tcount[min(t0 + q, tiles - 1)];
   unsigned long long pc = tid < tin ? tcv : 0ull;
   for (int q = tid; q < ch; q += TW) pc += tcount[tiles + q];
-  const int jc = min(j, n - 1);
-  const uint64_t kc = kold[jc], kpc = kold[max(jc - 1, 0)];
-  const uint64_t k = live ? kc : ~0ull;
-  const uint64_t kp = (live && j > 0) ? kpc : ~k;
-  const float x = tx[jc], y = ty[jc], u = tvx[jc], v = tvy[jc];
-  const uint32_t id = pid_src[jc];
-  // the first entries of the tile's bucket, loaded whichever path the step
-  // took (entries past the window's length are stale and not used)
-  const uint4* be = r.bucket + 2 * ((size_t)tile * kBucketCap + (tid & (kBucketFast - 1)));
+  // the first entries of each tile's bucket (thread qb * BH + rb: entry rb of
+  // tile t0 + qb), loaded whichever path the step took (entries past the
+  // window's length are stale and not used)
+  const int qb = tid / BH, rb = tid % BH;
+  const int tq = min(t0 + qb, tiles - 1);
+  const uint4* be = r.bucket + 2 * ((size_t)tq * kBucketCap + (rb & (kBucketFast - 1)));
   uint4 b0 = be[0], b1 = be[1];
   const bool from_bucket = *r.mode != 0u;
-  if (tile == 0 && tid == 0) *r.tally = 0ull;  // for the next k_bucket (this one has ended)
+  if (blockIdx.x == 0 && tid == 0) *r.tally = 0ull;  // for the next k_bucket (this one has ended)
   // the previous order's ranges (k_ranges writes the new ones)
-  if (live && k != kp && k < n_keys) {
-    cells[k] = make_int2(-1, -1);
-    if (block_bits > 0 && (k >> (2 * block_bits)) != (kp >> (2 * block_bits)))
-      blocks[k >> (2 * block_bits)] = make_int2(-1, -1);
-  }
-  s_tk[tid] = k;
+#pragma unroll
+  for (int q = 0; q < MT; ++q)
+    if (live[q] && k[q] != kp[q] && k[q] < n_keys) {
+      cells[k[q]] = make_int2(-1, -1);
+      if (block_bits > 0 && (k[q] >> (2 * block_bits)) != (kp[q] >> (2 * block_bits)))
+        blocks[k[q] >> (2 * block_bits)] = make_int2(-1, -1);
+    }
+#pragma unroll
+  for (int q = 0; q < MT; ++q) s_tk[q][tid] = k[q];
   {
     const uint32_t plo = wave_sum((uint32_t)pc), phi = wave_sum((uint32_t)(pc >> 32));
     if (lane == 0) {
       s_plo[wave] = plo;
       s_phi[wave] = phi;
-      s_wpre[wave + 1] = (uint32_t)__popcll(fw);
-      s_fw[wave] = fw;
+#pragma unroll
+      for (int q = 0; q < MT; ++q) {
+        s_wpre[q][wave + 1] = (uint32_t)__popcll(fw[q]);
+        s_fw[q][wave] = fw[q];
+      }
     }
   }
-  if (!from_bucket && tid < 2) {
-    // sorted path: movers ordered before this tile's (tid 0) and the next
-    // tile's (tid 1) first particle
+  if (!from_bucket && tid <= MT) {
+    // sorted path: movers ordered before the first particle of tile t0 + tid
+    // (all of them past the last tile)
     const int m = *r.n_movers;
-    const int t = tile + tid;
+    const int t = t0 + tid;
     s_win[tid] = t == 0 ? 0u
                  : t >= tiles
                      ? (uint32_t)m
                      : (uint32_t)rank_in_sorted(r.skey, r.sidx, 0, m, tfirst[t], (uint32_t)t * TW);
   }
   __syncthreads();
-  if (tid == 0) {
-    s_wpre[0] = 0;
-    for (int w = 1; w <= TW / kWave; ++w) s_wpre[w] += s_wpre[w - 1];
+  if (tid < MT) {
+    s_wpre[tid][0] = 0;
+    for (int w = 1; w <= TW / kWave; ++w) s_wpre[tid][w] += s_wpre[tid][w - 1];
   }
-  // the tile's flag words are read (s_fw): zero them for producers that OR in
-  if (lane == 0) flags[j0 / kWave + wave] = 0ull;
-  uint32_t tb, wlo, whi;
-  if (from_bucket) {
-    uint32_t slo = 0, shi = 0;
+  // the tiles' flag words are read (s_fw): zero them for producers that OR in
+  if (lane == 0)
 #pragma unroll
-    for (int w = 0; w < TW / kWave; ++w) {
-      slo += s_plo[w];
-      shi += s_phi[w];
-    }
-    tb = slo;
-    wlo = shi;
-    whi = shi + (uint32_t)(own >> 32);
-  } else {
-    tb = r.tile_base[tile];
-    wlo = s_win[0];
-    whi = s_win[1];
-  }
-  // windows lie in [0, n) (clamped, so that a bad table cannot read past skey)
-  const int lo = min((int)wlo, n), hi = max(lo, min((int)whi, n));
-  __syncthreads();
-  const bool mv = live && ((fw >> lane) & 1ull);
-  int before_movers = 0;  // window movers ordered before stayer j
-  for (int c0 = lo; c0 < hi; c0 += WIN) {
-    const int len = min(WIN, hi - c0);
+    for (int q = 0; q < MT; ++q)
+      if (t0 + q < tiles) flags[(t0 + q) * (TW / kWave) + wave] = 0ull;
+  uint32_t tb[MT];
+  int lo[MT], hi[MT];
+  {
+    uint32_t wlo[MT], whi[MT];
     if (from_bucket) {
-      // the window is the tile's bucket (at most kBucketCap movers, one
-      // chunk): sorted here by (key, previous index), rank by counting
-      const int blen = min(len, kBucketCap);
-      if (blen > kBucketFast && tid >= kBucketFast && tid < blen) {
-        const uint4* bx = r.bucket + 2 * ((size_t)tile * kBucketCap + tid);
+      uint32_t slo = 0, shi = 0;
+#pragma unroll
+      for (int w = 0; w < TW / kWave; ++w) {
+        slo += s_plo[w];
+        shi += s_phi[w];
+      }
+#pragma unroll
+      for (int q = 0; q < MT; ++q) {
+        const unsigned long long o = t0 + q < tiles ? own[q] : 0ull;
+        tb[q] = slo;
+        wlo[q] = shi;
+        whi[q] = shi + (uint32_t)(o >> 32);
+        slo += (uint32_t)o;
+        shi = whi[q];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < MT; ++q) {
+        tb[q] = r.tile_base[min(t0 + q, tiles - 1)];
+        wlo[q] = s_win[q];
+        whi[q] = s_win[q + 1];
+      }
+    }
+    // windows lie in [0, n) (clamped, so that a bad table cannot read past skey)
+#pragma unroll
+    for (int q = 0; q < MT; ++q) {
+      lo[q] = min((int)wlo[q], n);
+      hi[q] = max(lo[q], min((int)whi[q], n));
+    }
+  }
+  __syncthreads();
+  bool mv[MT];
+  int before[MT];  // window movers ordered before a stayer
+#pragma unroll
+  for (int q = 0; q < MT; ++q) {
+    mv[q] = live[q] && ((fw[q] >> lane) & 1ull);
+    before[q] = 0;
+  }
+  // a mover with key km, previous index src and its state goes to window slot
+  // c of tile q: rank among the tile's keys, minus the tile's movers before it
+  auto place = [&](int q, int c, uint64_t km, uint32_t src, float mx, float my, float mu,
+                   float mw, uint32_t mid_) {
+    const int j0 = (t0 + q) * TW;
+    int a = 0, z = TW;
+    while (a < z) {
+      const int mid = (a + z) >> 1;
+      if (pair_less(s_tk[q][mid], (uint32_t)(j0 + mid), km, src)) a = mid + 1;
+      else z = mid;
+    }
+    const int w = a / kWave, l = a % kWave;
+    uint32_t mb = tb[q] + s_wpre[q][w];
+    if (w < TW / kWave) mb += (uint32_t)__popcll(s_fw[q][w] & ((1ull << l) - 1ull));
+    const int pos = c + (j0 + a - (int)mb);
+    if ((unsigned)pos >= (unsigned)n) return;  // only a corrupt table gets here
+    if (rx) {
+      rx[pos] = mx;
+      ry[pos] = my;
+      vx[pos] = mu;
+      vy[pos] = mw;
+    }
+    xyuv[pos] = make_float4(mx, my, mu, mw);
+    pid_dst[pos] = mid_;
+    keys_dst[pos] = km;
+    if (perm) perm[pos] = src;
+  };
+  auto stayer_rank = [&](const uint64_t* mk, const uint32_t* mi, int len, uint64_t kk,
+                         uint32_t j) {
+    int a = 0, z = len;
+    while (a < z) {
+      const int mid = (a + z) >> 1;
+      if (pair_less(mk[mid], mi[mid], kk, j)) a = mid + 1;
+      else z = mid;
+    }
+    return a;
+  };
+  if (from_bucket) {
+    // each tile's window is its bucket (at most kBucketCap movers): keys in
+    // arrival order into LDS, ranked by counting, then placed
+    const int blen = min(hi[qb] - lo[qb], kBucketCap);
+    if (rb < blen) {
+      if (rb >= kBucketFast) {
+        const uint4* bx = r.bucket + 2 * ((size_t)tq * kBucketCap + rb);
         b0 = bx[0];
         b1 = bx[1];
       }
-      if (tid < blen) {
-        s_bk_k[tid] = (uint64_t)b0.x | ((uint64_t)b0.y << 32);
-        s_bk_i[tid] = b0.z;
-        s_bk_p[tid] = b0.w;
-        s_bk_s[tid] = b1;
-      }
-      __syncthreads();
-      for (int q = tid; q < blen; q += TW) {
-        const uint64_t km = s_bk_k[q];
-        const uint32_t im = s_bk_i[q];
-        int rank = 0;
-        for (int e = 0; e < blen; ++e) rank += pair_less(s_bk_k[e], s_bk_i[e], km, im) ? 1 : 0;
-        s_mk[rank] = km;
-        s_mi[rank] = im;
-        const uint4 st = s_bk_s[q];
-        s_ms[rank] = make_float4(__uint_as_float(st.x), __uint_as_float(st.y),
-                                 __uint_as_float(st.z), __uint_as_float(st.w));
-        s_mp[rank] = s_bk_p[q];
-      }
-    } else {
-      for (int q = tid; q < len; q += TW) {
-        s_mk[q] = r.skey[c0 + q];
-        s_mi[q] = r.sidx[c0 + q];
-      }
+      s_ak[qb][rb] = (uint64_t)b0.x | ((uint64_t)b0.y << 32);
+      s_ai[qb][rb] = b0.z;
     }
     __syncthreads();
-    if (live && !mv) {
-      int a = 0, z = len;
-      while (a < z) {
-        const int mid = (a + z) >> 1;
-        if (pair_less(s_mk[mid], s_mi[mid], k, (uint32_t)j)) a = mid + 1;
-        else z = mid;
-      }
-      before_movers += a;
-    }
-    for (int q = tid; q < len; q += TW) {
-      const uint64_t km = s_mk[q];
-      const uint32_t src = s_mi[q];
-      // the mover's state: from its bucket entry, or gathered from T (in
-      // flight during the search)
-      float mx, my, mu, mw;
-      uint32_t mid_;
-      if (from_bucket) {
-        const float4 st = s_ms[q];
-        mx = st.x;
-        my = st.y;
-        mu = st.z;
-        mw = st.w;
-        mid_ = s_mp[q];
-      } else {
-        const uint32_t sc = min(src, (uint32_t)(n - 1));
-        mx = tx[sc];
-        my = ty[sc];
-        mu = tvx[sc];
-        mw = tvy[sc];
-        mid_ = pid_src[sc];
-      }
-      // tile particles ordered before the mover (keys past n compare greater)
-      int a = 0, z = TW;
-      while (a < z) {
-        const int mid = (a + z) >> 1;
-        if (pair_less(s_tk[mid], (uint32_t)(j0 + mid), km, src)) a = mid + 1;
-        else z = mid;
-      }
-      // movers among the tile particles [j0, j0 + a)
-      const int w = a / kWave, l = a % kWave;
-      uint32_t mb = tb + s_wpre[w];
-      if (w < TW / kWave) mb += (uint32_t)__popcll(s_fw[w] & ((1ull << l) - 1ull));
-      const int pos = (c0 + q) + (j0 + a - (int)mb);
-      if ((unsigned)pos >= (unsigned)n) continue;  // only a corrupt table gets here
-      if (rx) {
-        rx[pos] = mx;
-        ry[pos] = my;
-        vx[pos] = mu;
-        vy[pos] = mw;
-      }
-      xyuv[pos] = make_float4(mx, my, mu, mw);
-      pid_dst[pos] = mid_;
-      keys_dst[pos] = km;
-      if (perm) perm[pos] = src;
+    if (rb < blen) {
+      const uint64_t km = (uint64_t)b0.x | ((uint64_t)b0.y << 32);
+      const uint32_t im = b0.z;
+      int rank = 0;
+      for (int e = 0; e < blen; ++e) rank += pair_less(s_ak[qb][e], s_ai[qb][e], km, im) ? 1 : 0;
+      s_mk[qb * kBucketCap + rank] = km;
+      s_mi[qb * kBucketCap + rank] = im;
+      s_ms[qb][rank] = make_float4(__uint_as_float(b1.x), __uint_as_float(b1.y),
+                                   __uint_as_float(b1.z), __uint_as_float(b1.w));
+      s_mp[qb][rank] = b0.w;
     }
     __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MT; ++q)
+      if (live[q] && !mv[q])
+        before[q] = stayer_rank(s_mk + q * kBucketCap, s_mi + q * kBucketCap,
+                                min(hi[q] - lo[q], kBucketCap), k[q],
+                                (uint32_t)((t0 + q) * TW + tid));
+    if (rb < blen) {
+      const float4 st = s_ms[qb][rb];
+      place(qb, lo[qb] + rb, s_mk[qb * kBucketCap + rb], s_mi[qb * kBucketCap + rb], st.x, st.y,
+            st.z, st.w, s_mp[qb][rb]);
+    }
+  } else {
+    // sorted path: each tile's window in chunks of WIN sorted movers, the
+    // movers' state gathered from T
+    for (int q = 0; q < MT; ++q) {
+      for (int c0 = lo[q]; c0 < hi[q]; c0 += WIN) {
+        const int len = min(WIN, hi[q] - c0);
+        for (int e = tid; e < len; e += TW) {
+          s_mk[e] = r.skey[c0 + e];
+          s_mi[e] = r.sidx[c0 + e];
+        }
+        __syncthreads();
+        if (live[q] && !mv[q])
+          before[q] += stayer_rank(s_mk, s_mi, len, k[q], (uint32_t)((t0 + q) * TW + tid));
+        for (int e = tid; e < len; e += TW) {
+          const uint32_t src = s_mi[e];
+          const uint32_t sc = min(src, (uint32_t)(n - 1));
+          place(q, c0 + e, s_mk[e], src, tx[sc], ty[sc], tvx[sc], tvy[sc], pid_src[sc]);
+        }
+        __syncthreads();
+      }
+    }
   }
-  if (live && !mv) {
-    const uint32_t mb = tb + s_wpre[wave] + (uint32_t)__popcll(fw & ((1ull << lane) - 1ull));
-    const int pos = (j - (int)mb) + lo + before_movers;
-    if ((unsigned)pos >= (unsigned)n) return;  // only a corrupt table gets here
+#pragma unroll
+  for (int q = 0; q < MT; ++q) {
+    if (!live[q] || mv[q]) continue;
+    const int j = (t0 + q) * TW + tid;
+    const uint32_t mb = tb[q] + s_wpre[q][wave] + (uint32_t)__popcll(fw[q] & ((1ull << lane) - 1ull));
+    const int pos = (j - (int)mb) + lo[q] + before[q];
+    if ((unsigned)pos >= (unsigned)n) continue;  // only a corrupt table gets here
     if (rx) {
-      rx[pos] = x;
-      ry[pos] = y;
-      vx[pos] = u;
-      vy[pos] = v;
+      rx[pos] = x[q];
+      ry[pos] = y[q];
+      vx[pos] = u[q];
+      vy[pos] = v[q];
     }
-    xyuv[pos] = make_float4(x, y, u, v);
-    pid_dst[pos] = id;
-    keys_dst[pos] = k;
+    xyuv[pos] = make_float4(x[q], y[q], u[q], v[q]);
+    pid_dst[pos] = id[q];
+    keys_dst[pos] = k[q];
     if (perm) perm[pos] = (uint32_t)j;
   }
 }
@@ -723,11 +783,10 @@ hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers&
                         uint64_t* keys_dst, bool soa, bool perm) {
   if (s.n <= 0) return hipSuccess;
   const int tiles = (int)resort_tiles(s.n);
-  k_merge<<<tiles, TW, 0, L.stream>>>(s.n, s.keys_sorted, r.flags, s.tile_first, s.tile_count,
-                                      s.tx, s.ty, s.tvx, s.tvy, s.pid, soa ? s.rx : nullptr,
-                                      s.ry, s.vx, s.vy, s.pid_alt, keys_dst,
-                                      perm ? s.perm : nullptr, s.cells, s.blocks, s.block_bits,
-                                      (uint64_t)s.n_keys, s.xyuv, r);
+  k_merge<<<(tiles + MT - 1) / MT, TW, 0, L.stream>>>(
+      s.n, s.keys_sorted, r.flags, s.tile_first, s.tile_count, s.tx, s.ty, s.tvx, s.tvy, s.pid,
+      soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt, keys_dst, perm ? s.perm : nullptr,
+      s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv, r);
   return hipGetLastError();
 }
 
