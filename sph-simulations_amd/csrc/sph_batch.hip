@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kMaxTile) void k_bdensity(
   rp_out[i] = make_float2(rho, pr);
 }
 
-template <int PHYS, int BC>
+template <int PHYS, int BC, typename K>
 __global__ __launch_bounds__(kMaxTile) void k_bforces(
     int n, const TileDev* __restrict__ tiles, const SceneDev* __restrict__ scenes,
     const float* __restrict__ rx, const float* __restrict__ ry, const float* __restrict__ vx,
@@ -64,14 +64,14 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
     const int2* __restrict__ cells, const int32_t* __restrict__ ell_count,
     const int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ ax_out,
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
-    float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out,
+    float* __restrict__ tvx, float* __restrict__ tvy, K* __restrict__ keys_out,
     const float4* __restrict__ xyuv, const float2* __restrict__ rp,
-    const uint64_t* __restrict__ kold, unsigned long long* __restrict__ flags) {
+    const K* __restrict__ kold, unsigned long long* __restrict__ flags) {
   const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int i = t.begin + (int)threadIdx.x;
   if (i >= t.end) return;
   // the previous key, for the re-sort's mover report (flags null: none)
-  const uint64_t ko = flags ? kold[i] : 0ull;
+  const uint64_t ko = flags ? (uint64_t)kold[i] : 0ull;
   const SceneDev& S = scenes[t.scene];
   const DevGrid g = S.g;
   const DevPhys c = S.c;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
   tvy[i] = v;
   const uint64_t kn = base + cell_key(g, cell_coord(x, g.lo_x, g.cs_x, g.nx),
                                       cell_coord(y, g.lo_y, g.cs_y, g.ny));
-  keys_out[i] = kn;
+  keys_out[i] = (K)kn;
   if (flags) {
     // The mover flags of the incremental re-sort: a workgroup's particles start
     // anywhere in the order, so a wave's 64 bits straddle at most two flag
@@ -149,10 +149,16 @@ hipError_t launch_batch_density(const Launch& L, const BatchLaunch& B, const Dev
 template <int PHYS, int BC>
 static void bforces_go(const Launch& L, const BatchLaunch& B, const DevState& s,
                        uint64_t* keys_out, unsigned long long* flags) {
-  k_bforces<PHYS, BC><<<B.n_tiles, B.tile, 0, L.stream>>>(
-      s.n, B.tiles, B.scenes, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.ell_count,
-      s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out, s.xyuv, s.rp,
-      s.keys_sorted, flags);
+  if (s.narrow)  // the step's narrow keys: read nkeys, write nkeys_in
+    k_bforces<PHYS, BC, uint32_t><<<B.n_tiles, B.tile, 0, L.stream>>>(
+        s.n, B.tiles, B.scenes, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.ell_count,
+        s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, s.nkeys_in, s.xyuv, s.rp,
+        s.nkeys, flags);
+  else
+    k_bforces<PHYS, BC, uint64_t><<<B.n_tiles, B.tile, 0, L.stream>>>(
+        s.n, B.tiles, B.scenes, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.ell_count,
+        s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out, s.xyuv, s.rp,
+        s.keys_sorted, flags);
 }
 
 hipError_t launch_batch_forces(const Launch& L, const BatchLaunch& B, const DevState& s,

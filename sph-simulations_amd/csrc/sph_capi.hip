@@ -98,7 +98,8 @@ struct Ctx {
 void graph_signature(const Ctx* c, const void** sig) {
   const DevState& s = c->s;
   const void* v[10] = {s.rx, s.tx, s.vx, s.tvx, s.ax, s.tax, s.pid, s.pid_alt,
-                       s.keys_sorted, reinterpret_cast<const void*>((intptr_t)s.n)};
+                       s.narrow ? (const void*)s.nkeys : (const void*)s.keys_sorted,
+                       reinterpret_cast<const void*>((intptr_t)s.n * 2 + (s.narrow ? 1 : 0))};
   for (int i = 0; i < 10; ++i) sig[i] = v[i];
 }
 
@@ -380,8 +381,12 @@ int incremental_resort(Ctx* c, bool flags_ready) {
     StageTimer t(c, SPH_T_MERGE);
     HIPCHK(c, launch_merge(c->L, s, r, r.keys_alt, !c->soa_lazy, c->fields_out));
   }
-  std::swap(s.keys_sorted, r.keys_alt);
-  c->sb.keys_out = s.keys_sorted;
+  if (s.narrow) {
+    std::swap(s.nkeys, s.nkeys_alt);
+  } else {
+    std::swap(s.keys_sorted, r.keys_alt);
+    c->sb.keys_out = s.keys_sorted;
+  }
   std::swap(s.pid, s.pid_alt);
   {
     StageTimer t(c, SPH_T_NEIGHBOR_SEARCH);
@@ -605,6 +610,8 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
         (rc = dev_alloc(c, &r.ktmp, N)) || (rc = dev_alloc(c, &r.keys_alt, N)) ||
         (rc = dev_alloc(c, &r.flags, resort_flag_words((int)N))) ||
         (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
+        (rc = dev_alloc(c, &s.nkeys, N)) || (rc = dev_alloc(c, &s.nkeys_in, N)) ||
+        (rc = dev_alloc(c, &s.nkeys_alt, N)) ||
         (rc = dev_alloc(c, &s.tile_first, resort_tiles((int)N))) ||
         (rc = dev_alloc(c, &s.tile_count, resort_count_words((int)N))) ||
         (rc = dev_alloc(c, &r.vtmp, N)) || (rc = dev_alloc(c, &r.tally, 1)) ||
@@ -1195,13 +1202,6 @@ int sph_step(void* ctx, int32_t n_steps) {
   int rc = check_n(c);
   if (rc) return rc;
   if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_prepare first");
-  // the packed copy of the sorted state the step kernels gather from (other
-  // calls may have changed the state since the last re-sort wrote it)
-  HIPCHK(c, launch_pack(c->L, c->s));
-  if (n_steps <= 0) {
-    c->csr_total = -1;
-    return SPH_OK;
-  }
   // per-particle kernels (one scene or batched), two steps or more: the steps
   // keep only the packed copy current and the separate arrays are written back
   // once at the end (a single step has the merge write them, which costs less
@@ -1210,6 +1210,23 @@ int sph_step(void* ctx, int32_t n_steps) {
       n_steps >= 2 &&
       (c->n_scenes || !(c->s.block_bits && (c->step_kernel == SPH_STEP_FUSED_BLOCK ||
                                             c->step_kernel == SPH_STEP_SPLIT_BLOCK)));
+  // and with cell keys below 2^32 and the incremental re-sort, the steps keep
+  // the sorted keys as u32 (k_pack narrows them, k_unpack widens them back)
+  c->s.narrow = lazy && c->resort == SPH_RESORT_INCREMENTAL && c->grid.key_bits <= 32 &&
+                c->s.n <= kResortMaxParticles;
+  // the packed copy of the sorted state the step kernels gather from (other
+  // calls may have changed the state since the last re-sort wrote it)
+  {
+    const hipError_t e = launch_pack(c->L, c->s);
+    if (e != hipSuccess) {
+      c->s.narrow = false;
+      return hip_fail(c, e, "pack");
+    }
+  }
+  if (n_steps <= 0) {
+    c->csr_total = -1;
+    return SPH_OK;
+  }
   c->soa_lazy = lazy;
   auto run = [&]() -> int {
     int done = 0, rc2 = SPH_OK;
@@ -1233,6 +1250,7 @@ int sph_step(void* ctx, int32_t n_steps) {
     const hipError_t e = launch_unpack(c->L, c->s);
     if (rc == SPH_OK && e != hipSuccess) rc = hip_fail(c, e, "unpack");
   }
+  c->s.narrow = false;
   if (rc) return rc;
   c->csr_total = -1;
   c->density_ell = false;

@@ -128,7 +128,8 @@ __global__ __launch_bounds__(BLK) void k_gather(
 // after) loaded in one round: few enough waves that they all run at once.
 constexpr int kRangesPer = 4;
 
-__global__ __launch_bounds__(BLK) void k_ranges(int n, const uint64_t* __restrict__ keys,
+template <typename K>
+__global__ __launch_bounds__(BLK) void k_ranges(int n, const K* __restrict__ keys,
                                                 int2* __restrict__ cells, int64_t n_keys,
                                                 int2* __restrict__ blocks, int block_bits,
                                                 uint32_t* __restrict__ hist, int hist_words,
@@ -235,7 +236,9 @@ __global__ __launch_bounds__(BLK) void k_density(
 // The separate arrays of the sorted state from its packed copy.
 __global__ __launch_bounds__(BLK) void k_unpack(int n, const float4* __restrict__ xyuv,
                                                 float* __restrict__ rx, float* __restrict__ ry,
-                                                float* __restrict__ vx, float* __restrict__ vy) {
+                                                float* __restrict__ vx, float* __restrict__ vy,
+                                                const uint32_t* __restrict__ k32,
+                                                uint64_t* __restrict__ k64) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   const float4 q = xyuv[i];
@@ -243,6 +246,7 @@ __global__ __launch_bounds__(BLK) void k_unpack(int n, const float4* __restrict_
   ry[i] = q.y;
   vx[i] = q.z;
   vy[i] = q.w;
+  if (k32) k64[i] = k32[i];  // the narrow keys back (null: none)
 }
 
 // The sorted state packed per particle (x, y, vx, vy) for the neighbor gathers.
@@ -250,14 +254,19 @@ __global__ __launch_bounds__(BLK) void k_pack(int n, const float* __restrict__ r
                                               const float* __restrict__ ry,
                                               const float* __restrict__ vx,
                                               const float* __restrict__ vy,
-                                              float4* __restrict__ xyuv) {
+                                              float4* __restrict__ xyuv,
+                                              const uint64_t* __restrict__ k64,
+                                              uint32_t* __restrict__ k32) {
   const int i = blockIdx.x * BLK + threadIdx.x;
-  if (i < n) xyuv[i] = make_float4(rx[i], ry[i], vx[i], vy[i]);
+  if (i >= n) return;
+  xyuv[i] = make_float4(rx[i], ry[i], vx[i], vy[i]);
+  if (k32) k32[i] = (uint32_t)k64[i];  // narrow copy of the sorted keys (null: none)
 }
 
 // Accelerations. FUSED additionally integrates (symplectic Euler) into the T
-// arrays and writes the cell key of the new position for the next sort.
-template <int PHYS, int BC, int NB, bool FUSED>
+// arrays and writes the cell key of the new position for the next sort (keys
+// of type K: u32 for narrow keys).
+template <int PHYS, int BC, int NB, bool FUSED, typename K = uint64_t>
 __global__ __launch_bounds__(BLK) void k_forces(
     int n, DevGrid g, DevPhys c, const float* __restrict__ rx, const float* __restrict__ ry,
     const float* __restrict__ vx, const float* __restrict__ vy,
@@ -266,8 +275,8 @@ __global__ __launch_bounds__(BLK) void k_forces(
     const int32_t* __restrict__ csr_idx, const int32_t* __restrict__ ell_count,
     const int32_t* __restrict__ ell_idx, int ell_w, float* __restrict__ ax_out,
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
-    float* __restrict__ tvx, float* __restrict__ tvy, uint64_t* __restrict__ keys_out,
-    const uint64_t* __restrict__ kold, MoverSink movers, const float4* __restrict__ xyuv,
+    float* __restrict__ tvx, float* __restrict__ tvy, K* __restrict__ keys_out,
+    const K* __restrict__ kold, MoverSink movers, const float4* __restrict__ xyuv,
     const float2* __restrict__ rp) {
   const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
@@ -280,7 +289,7 @@ __global__ __launch_bounds__(BLK) void k_forces(
   } else {
     xi = rx[i]; yi = ry[i]; vxi = vx[i]; vyi = vy[i]; rhoi = rho[i]; pri = pr[i];
   }
-  const uint64_t ko = (FUSED && movers.flags) ? kold[i] : 0ull;
+  const uint64_t ko = (FUSED && movers.flags) ? (uint64_t)kold[i] : 0ull;
   Acc a{0.f, 0.f, 0.f, 0.f};
   auto pair = [&](int j, float dx, float dy, float r2) {
     force_pair<PHYS>(c, dx, dy, r2, rhoi, rho[j], pri, pr[j], vxi, vyi, vx[j], vy[j], a);
@@ -317,7 +326,7 @@ __global__ __launch_bounds__(BLK) void k_forces(
     tvy[i] = v;
     const uint64_t kn = cell_key(g, cell_coord(x, g.lo_x, g.cs_x, g.nx),
                                  cell_coord(y, g.lo_y, g.cs_y, g.ny));
-    keys_out[i] = kn;
+    keys_out[i] = (K)kn;
     // particles whose key changes (tiles are BLK-aligned, so wave w of the
     // tile covers flag word i / 64; waves past n write none)
     if (movers.flags) record_movers(movers, i, kn != ko);
@@ -639,10 +648,17 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
 }
 
 hipError_t launch_ranges(const Launch& L, const DevState& s, int hist_words) {
-  if (s.n > 0)
-    k_ranges<<<(blocks_for(s.n) + kRangesPer - 1) / kRangesPer, BLK, 0, L.stream>>>(
-        s.n, s.keys_sorted, s.cells, s.n_keys, s.blocks, s.block_bits, s.hist, hist_words,
-        s.tile_first, s.tile_count);
+  if (s.n > 0) {
+    const int grid = (blocks_for(s.n) + kRangesPer - 1) / kRangesPer;
+    if (s.narrow)
+      k_ranges<uint32_t><<<grid, BLK, 0, L.stream>>>(s.n, s.nkeys, s.cells, s.n_keys, s.blocks,
+                                                     s.block_bits, s.hist, hist_words,
+                                                     s.tile_first, s.tile_count);
+    else
+      k_ranges<uint64_t><<<grid, BLK, 0, L.stream>>>(s.n, s.keys_sorted, s.cells, s.n_keys,
+                                                     s.blocks, s.block_bits, s.hist, hist_words,
+                                                     s.tile_first, s.tile_count);
+  }
   return hipGetLastError();
 }
 
@@ -715,19 +731,30 @@ hipError_t launch_settle_fields(const Launch& L, const DevState& s, bool gather_
 
 hipError_t launch_unpack(const Launch& L, const DevState& s) {
   if (s.n > 0)
-    k_unpack<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, s.xyuv, s.rx, s.ry, s.vx, s.vy);
+    k_unpack<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, s.xyuv, s.rx, s.ry, s.vx, s.vy,
+                                                    s.narrow ? s.nkeys : nullptr, s.keys_sorted);
   return hipGetLastError();
 }
 
 hipError_t launch_pack(const Launch& L, const DevState& s) {
   if (s.n > 0)
-    k_pack<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, s.rx, s.ry, s.vx, s.vy, s.xyuv);
+    k_pack<<<blocks_for(s.n), BLK, 0, L.stream>>>(s.n, s.rx, s.ry, s.vx, s.vy, s.xyuv,
+                                                  s.keys_sorted, s.narrow ? s.nkeys : nullptr);
   return hipGetLastError();
 }
 
 template <int PHYS, int BC, int NB, bool FUSED>
 static hipError_t forces_go(const Launch& L, const DevState& s, uint64_t* keys_out,
                             MoverSink movers, bool packed) {
+  if constexpr (NB == NB_ELL && FUSED) {
+    if (s.narrow) {  // the step's narrow keys: read nkeys, write nkeys_in
+      k_forces<PHYS, BC, NB, FUSED, uint32_t><<<blocks_for(s.n), BLK, 0, L.stream>>>(
+          s.n, L.grid, L.phys, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.csr_off,
+          s.csr_idx, s.ell_count, s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy,
+          s.nkeys_in, s.nkeys, movers, packed ? s.xyuv : nullptr, packed ? s.rp : nullptr);
+      return hipGetLastError();
+    }
+  }
   k_forces<PHYS, BC, NB, FUSED><<<blocks_for(s.n), BLK, 0, L.stream>>>(
       s.n, L.grid, L.phys, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.csr_off, s.csr_idx,
       s.ell_count, s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out,

@@ -212,7 +212,8 @@ __device__ uint32_t compact_movers(int n, int tiles, const unsigned long long* _
 // previous indices midx holds in previous-index order, by their new keys (read
 // through midx in the first pass); result in skey / sidx. One workgroup; s:
 // KB_WAVES * 256 + 512 words of LDS.
-__device__ void sort_movers(int m, int passes, const uint64_t* __restrict__ knew,
+template <typename K>
+__device__ void sort_movers(int m, int passes, const K* __restrict__ knew,
                             const ResortBuffers& r, uint32_t* s) {
   uint32_t* s_cnt = s;                    // [KB_WAVES][256] per-wave digit counts
   uint32_t* s_off = s + KB_WAVES * 256;  // [256] running digit offsets
@@ -225,7 +226,7 @@ __device__ void sort_movers(int m, int passes, const uint64_t* __restrict__ knew
     const uint32_t* vin = p == 0 ? r.midx : ((p & 1) ? r.vtmp : r.midx);
     uint64_t* kout = p == passes - 1 ? r.skey : ((p & 1) ? r.mkey : r.ktmp);
     uint32_t* vout = p == passes - 1 ? r.sidx : ((p & 1) ? r.midx : r.vtmp);
-    auto key_of = [&](int e) { return p == 0 ? knew[vin[e]] : kin[e]; };
+    auto key_of = [&](int e) { return p == 0 ? (uint64_t)knew[vin[e]] : kin[e]; };
     const int shift = 8 * p;
     s_off[tid] = 0u;
     __syncthreads();
@@ -313,9 +314,9 @@ constexpr int KB_LDS = kBucketList + 2 * kBucketTab;
 static_assert(FB_WPT * KB_TW <= KB_LDS && KB_WAVES * 256 + 512 <= KB_LDS,
               "the sorted path's LDS exceeds the list and table");
 
-template <bool WIDE>
+template <bool WIDE, typename K>
 __global__ __launch_bounds__(KB_TW) void k_bucket(
-    int n, int tiles, int passes, const uint64_t* __restrict__ knew,
+    int n, int tiles, int passes, const K* __restrict__ knew,
     const unsigned long long* __restrict__ flags, const uint64_t* __restrict__ tfirst,
     unsigned long long* __restrict__ tcount, const int2* __restrict__ cells, uint64_t n_keys,
     const float* __restrict__ tx, const float* __restrict__ ty, const float* __restrict__ tvx,
@@ -386,7 +387,7 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
         const uint32_t rr = r0 + (uint32_t)tid;
         if (rr >= cnt) break;
         const uint32_t mi = s_list[rr];
-        const uint64_t mk = knew[mi];
+        const uint64_t mk = (uint64_t)knew[mi];
         const float mx = tx[mi], my = ty[mi], mu = tvx[mi], mv = tvy[mi];
         const uint32_t mp = pid_src[mi];
         const int2 cr = cells[min(mk, n_keys - 1)];
@@ -486,13 +487,14 @@ constexpr int BH = TW / MT;    // threads that hold one tile's bucket entries
 static_assert(kBucketCap <= BH && MT * kBucketCap <= WIN,
               "a tile's bucket maps to its share of the threads and of the window buffer");
 
+template <typename K>
 __global__ __launch_bounds__(TW) void k_merge(
-    int n, const uint64_t* __restrict__ kold, unsigned long long* __restrict__ flags,
+    int n, const K* __restrict__ kold, unsigned long long* __restrict__ flags,
     const uint64_t* __restrict__ tfirst, const unsigned long long* __restrict__ tcount,
     const float* __restrict__ tx, const float* __restrict__ ty, const float* __restrict__ tvx,
     const float* __restrict__ tvy, const uint32_t* __restrict__ pid_src, float* __restrict__ rx,
     float* __restrict__ ry, float* __restrict__ vx, float* __restrict__ vy,
-    uint32_t* __restrict__ pid_dst, uint64_t* __restrict__ keys_dst, uint32_t* __restrict__ perm,
+    uint32_t* __restrict__ pid_dst, K* __restrict__ keys_dst, uint32_t* __restrict__ perm,
     int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
     float4* __restrict__ xyuv, ResortBuffers r) {
   __shared__ uint64_t s_tk[MT][TW];  // the tiles' previous keys (all ones past n)
@@ -523,7 +525,7 @@ __global__ __launch_bounds__(TW) void k_merge(
     live[q] = j < n;
     fw[q] = flag_word(flags, j0 / kWave + wave, n);
     const int jc = min(j, n - 1);
-    const uint64_t kc = kold[jc], kpc = kold[max(jc - 1, 0)];
+    const uint64_t kc = (uint64_t)kold[jc], kpc = (uint64_t)kold[max(jc - 1, 0)];
     k[q] = live[q] ? kc : ~0ull;
     kp[q] = (live[q] && j > 0) ? kpc : ~k[q];
     x[q] = tx[jc];
@@ -660,7 +662,7 @@ __global__ __launch_bounds__(TW) void k_merge(
     }
     xyuv[pos] = make_float4(mx, my, mu, mw);
     pid_dst[pos] = mid_;
-    keys_dst[pos] = km;
+    keys_dst[pos] = (K)km;
     if (perm) perm[pos] = src;
   };
   auto stayer_rank = [&](const uint64_t* mk, const uint32_t* mi, int len, uint64_t kk,
@@ -747,7 +749,7 @@ __global__ __launch_bounds__(TW) void k_merge(
     }
     xyuv[pos] = make_float4(x[q], y[q], u[q], v[q]);
     pid_dst[pos] = id[q];
-    keys_dst[pos] = k[q];
+    keys_dst[pos] = (K)k[q];
     if (perm) perm[pos] = (uint32_t)j;
   }
 }
@@ -767,15 +769,27 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
   const int tiles = (int)resort_tiles(s.n);
   const int passes = (key_bits + 7) / 8;
   if (passes < 1 || passes > kMaxPasses) return hipErrorInvalidValue;
-  if (!flags_ready)
+  if (!flags_ready) {
+    // producers without their own report write u64 keys
+    if (s.narrow) return hipErrorInvalidValue;
     k_mover_flags<<<(s.n + MF_TW - 1) / MF_TW, MF_TW, 0, L.stream>>>(
         s.n, (int)resort_flag_words(s.n), knew, s.keys_sorted, mover_sink(r));
-  // keys of at most 32 bits: (key, index) in one word
-  auto bk = passes <= 4 ? k_bucket<false> : k_bucket<true>;
+  }
   const int words = (int)((s.n + kWave - 1) / kWave);
-  bk<<<(words + KB_TW - 1) / KB_TW, KB_TW, 0, L.stream>>>(
-      s.n, tiles, passes, knew, r.flags, s.tile_first, s.tile_count, s.cells,
-      (uint64_t)s.n_keys, s.tx, s.ty, s.tvx, s.tvy, s.pid, r);
+  const int grid = (words + KB_TW - 1) / KB_TW;
+  // keys of at most 32 bits: (key, index) in one word
+  if (s.narrow)
+    k_bucket<false, uint32_t><<<grid, KB_TW, 0, L.stream>>>(
+        s.n, tiles, passes, s.nkeys_in, r.flags, s.tile_first, s.tile_count, s.cells,
+        (uint64_t)s.n_keys, s.tx, s.ty, s.tvx, s.tvy, s.pid, r);
+  else if (passes <= 4)
+    k_bucket<false, uint64_t><<<grid, KB_TW, 0, L.stream>>>(
+        s.n, tiles, passes, knew, r.flags, s.tile_first, s.tile_count, s.cells,
+        (uint64_t)s.n_keys, s.tx, s.ty, s.tvx, s.tvy, s.pid, r);
+  else
+    k_bucket<true, uint64_t><<<grid, KB_TW, 0, L.stream>>>(
+        s.n, tiles, passes, knew, r.flags, s.tile_first, s.tile_count, s.cells,
+        (uint64_t)s.n_keys, s.tx, s.ty, s.tvx, s.tvy, s.pid, r);
   return hipGetLastError();
 }
 
@@ -783,10 +797,17 @@ hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers&
                         uint64_t* keys_dst, bool soa, bool perm) {
   if (s.n <= 0) return hipSuccess;
   const int tiles = (int)resort_tiles(s.n);
-  k_merge<<<(tiles + MT - 1) / MT, TW, 0, L.stream>>>(
-      s.n, s.keys_sorted, r.flags, s.tile_first, s.tile_count, s.tx, s.ty, s.tvx, s.tvy, s.pid,
-      soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt, keys_dst, perm ? s.perm : nullptr,
-      s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv, r);
+  const int grid = (tiles + MT - 1) / MT;
+  if (s.narrow)  // the step's narrow keys: nkeys into nkeys_alt
+    k_merge<uint32_t><<<grid, TW, 0, L.stream>>>(
+        s.n, s.nkeys, r.flags, s.tile_first, s.tile_count, s.tx, s.ty, s.tvx, s.tvy, s.pid,
+        soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt, s.nkeys_alt, perm ? s.perm : nullptr,
+        s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv, r);
+  else
+    k_merge<uint64_t><<<grid, TW, 0, L.stream>>>(
+        s.n, s.keys_sorted, r.flags, s.tile_first, s.tile_count, s.tx, s.ty, s.tvx, s.tvy, s.pid,
+        soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt, keys_dst, perm ? s.perm : nullptr,
+        s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv, r);
   return hipGetLastError();
 }
 
