@@ -66,7 +66,8 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
     float* __restrict__ tvx, float* __restrict__ tvy, K* __restrict__ keys_out,
     const float4* __restrict__ xyuv, const float2* __restrict__ rp,
-    const K* __restrict__ kold, unsigned long long* __restrict__ flags) {
+    const K* __restrict__ kold, unsigned long long* __restrict__ flags,
+    float4* __restrict__ t4) {
   const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int i = t.begin + (int)threadIdx.x;
   if (i >= t.end) return;
@@ -90,10 +91,14 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
   }
   float x = xi, y = yi, u = vxi, v = vyi;
   integrate_one<BC>(c, x, y, u, v, axi, ayi);
-  tx[i] = x;
-  ty[i] = y;
-  tvx[i] = u;
-  tvy[i] = v;
+  if constexpr (sizeof(K) == 4) {
+    t4[i] = make_float4(x, y, u, v);  // a narrow step keeps T packed
+  } else {
+    tx[i] = x;
+    ty[i] = y;
+    tvx[i] = u;
+    tvy[i] = v;
+  }
   const uint64_t kn = base + cell_key(g, cell_coord(x, g.lo_x, g.cs_x, g.nx),
                                       cell_coord(y, g.lo_y, g.cs_y, g.ny));
   keys_out[i] = (K)kn;
@@ -153,12 +158,12 @@ static void bforces_go(const Launch& L, const BatchLaunch& B, const DevState& s,
     k_bforces<PHYS, BC, uint32_t><<<B.n_tiles, B.tile, 0, L.stream>>>(
         s.n, B.tiles, B.scenes, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.ell_count,
         s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, s.nkeys_in, s.xyuv, s.rp,
-        s.nkeys, flags);
+        s.nkeys, flags, s.t4);
   else
     k_bforces<PHYS, BC, uint64_t><<<B.n_tiles, B.tile, 0, L.stream>>>(
         s.n, B.tiles, B.scenes, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.ell_count,
         s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out, s.xyuv, s.rp,
-        s.keys_sorted, flags);
+        s.keys_sorted, flags, nullptr);
 }
 
 hipError_t launch_batch_forces(const Launch& L, const BatchLaunch& B, const DevState& s,

@@ -611,7 +611,7 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
         (rc = dev_alloc(c, &r.flags, resort_flag_words((int)N))) ||
         (rc = dev_alloc(c, &r.tile_base, resort_tiles((int)N))) ||
         (rc = dev_alloc(c, &s.nkeys, N)) || (rc = dev_alloc(c, &s.nkeys_in, N)) ||
-        (rc = dev_alloc(c, &s.nkeys_alt, N)) ||
+        (rc = dev_alloc(c, &s.nkeys_alt, N)) || (rc = dev_alloc(c, &s.t4, N)) ||
         (rc = dev_alloc(c, &s.tile_first, resort_tiles((int)N))) ||
         (rc = dev_alloc(c, &s.tile_count, resort_count_words((int)N))) ||
         (rc = dev_alloc(c, &r.vtmp, N)) || (rc = dev_alloc(c, &r.tally, 1)) ||

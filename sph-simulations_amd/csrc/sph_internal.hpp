@@ -88,13 +88,16 @@ struct DevState {
   int block_bits;   // 0 when the order has no block structure (row-major)
   int2* blocks;
   int64_t n_blocks;
-  // Narrow keys (cell keys below 2^32, inside a sph_step call): the sorted keys,
-  // the next keys and the merge's second buffer as u32. When narrow is set the
-  // step's kernels (forces, re-sort, ranges) read and write these in place of
-  // keys_sorted / the new keys / the merge's u64 buffer; k_pack fills nkeys
-  // from keys_sorted on entry and k_unpack writes it back on exit.
+  // Narrow steps (cell keys below 2^32, inside a sph_step call): the sorted
+  // keys, the next keys and the merge's second buffer as u32, and the
+  // integrated state T packed per particle (x, y, vx, vy). When narrow is set
+  // the step's kernels (forces, re-sort, ranges) use these in place of
+  // keys_sorted / the new keys / the merge's u64 buffer and of tx, ty, tvx,
+  // tvy; k_pack fills nkeys from keys_sorted on entry and k_unpack writes it
+  // back on exit.
   bool narrow;
   uint32_t *nkeys, *nkeys_in, *nkeys_alt;
+  float4* t4;
 };
 
 enum { NB_STENCIL = 0, NB_CSR = 1, NB_ELL = 2 };

@@ -277,7 +277,7 @@ __global__ __launch_bounds__(BLK) void k_forces(
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
     float* __restrict__ tvx, float* __restrict__ tvy, K* __restrict__ keys_out,
     const K* __restrict__ kold, MoverSink movers, const float4* __restrict__ xyuv,
-    const float2* __restrict__ rp) {
+    const float2* __restrict__ rp, float4* __restrict__ t4) {
   const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
   float xi, yi, vxi, vyi, rhoi, pri;
@@ -320,10 +320,14 @@ __global__ __launch_bounds__(BLK) void k_forces(
   if (FUSED) {
     float x = xi, y = yi, u = vxi, v = vyi;
     integrate_one<BC>(c, x, y, u, v, axi, ayi);
-    tx[i] = x;
-    ty[i] = y;
-    tvx[i] = u;
-    tvy[i] = v;
+    if constexpr (sizeof(K) == 4) {
+      t4[i] = make_float4(x, y, u, v);  // a narrow step keeps T packed
+    } else {
+      tx[i] = x;
+      ty[i] = y;
+      tvx[i] = u;
+      tvy[i] = v;
+    }
     const uint64_t kn = cell_key(g, cell_coord(x, g.lo_x, g.cs_x, g.nx),
                                  cell_coord(y, g.lo_y, g.cs_y, g.ny));
     keys_out[i] = (K)kn;
@@ -751,7 +755,8 @@ static hipError_t forces_go(const Launch& L, const DevState& s, uint64_t* keys_o
       k_forces<PHYS, BC, NB, FUSED, uint32_t><<<blocks_for(s.n), BLK, 0, L.stream>>>(
           s.n, L.grid, L.phys, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.csr_off,
           s.csr_idx, s.ell_count, s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy,
-          s.nkeys_in, s.nkeys, movers, packed ? s.xyuv : nullptr, packed ? s.rp : nullptr);
+          s.nkeys_in, s.nkeys, movers, packed ? s.xyuv : nullptr, packed ? s.rp : nullptr,
+          s.t4);
       return hipGetLastError();
     }
   }
@@ -759,7 +764,7 @@ static hipError_t forces_go(const Launch& L, const DevState& s, uint64_t* keys_o
       s.n, L.grid, L.phys, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.csr_off, s.csr_idx,
       s.ell_count, s.ell_idx, s.ell_width, s.ax, s.ay, s.tx, s.ty, s.tvx, s.tvy, keys_out,
       s.keys_sorted, FUSED ? movers : MoverSink{}, packed && NB == NB_ELL ? s.xyuv : nullptr,
-      packed && NB == NB_ELL ? s.rp : nullptr);
+      packed && NB == NB_ELL ? s.rp : nullptr, nullptr);
   return hipGetLastError();
 }
 

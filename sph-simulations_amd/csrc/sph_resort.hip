@@ -320,8 +320,10 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
     const unsigned long long* __restrict__ flags, const uint64_t* __restrict__ tfirst,
     unsigned long long* __restrict__ tcount, const int2* __restrict__ cells, uint64_t n_keys,
     const float* __restrict__ tx, const float* __restrict__ ty, const float* __restrict__ tvx,
-    const float* __restrict__ tvy, const uint32_t* __restrict__ pid_src, ResortBuffers r) {
+    const float* __restrict__ tvy, const uint32_t* __restrict__ pid_src,
+    const float4* __restrict__ t4, ResortBuffers r) {
   using O = Ord<WIDE>;
+  constexpr bool NARROW = sizeof(K) == 4;  // a narrow step: T packed in t4
   __shared__ uint64_t s_mem64[KB_LDS / 2];
   __shared__ uint32_t s_w[KB_WAVES];
   __shared__ uint32_t s_cc[3];  // owners in the chunks before, of and after this one
@@ -388,7 +390,19 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
         if (rr >= cnt) break;
         const uint32_t mi = s_list[rr];
         const uint64_t mk = (uint64_t)knew[mi];
-        const float mx = tx[mi], my = ty[mi], mu = tvx[mi], mv = tvy[mi];
+        float mx, my, mu, mv;
+        if constexpr (NARROW) {
+          const float4 tq = t4[mi];
+          mx = tq.x;
+          my = tq.y;
+          mu = tq.z;
+          mv = tq.w;
+        } else {
+          mx = tx[mi];
+          my = ty[mi];
+          mu = tvx[mi];
+          mv = tvy[mi];
+        }
         const uint32_t mp = pid_src[mi];
         const int2 cr = cells[min(mk, n_keys - 1)];
         const O mo = O::make(mk, mi);
@@ -496,7 +510,8 @@ __global__ __launch_bounds__(TW) void k_merge(
     float* __restrict__ ry, float* __restrict__ vx, float* __restrict__ vy,
     uint32_t* __restrict__ pid_dst, K* __restrict__ keys_dst, uint32_t* __restrict__ perm,
     int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
-    float4* __restrict__ xyuv, ResortBuffers r) {
+    float4* __restrict__ xyuv, const float4* __restrict__ t4, ResortBuffers r) {
+  constexpr bool NARROW = sizeof(K) == 4;  // a narrow step: T packed in t4
   __shared__ uint64_t s_tk[MT][TW];  // the tiles' previous keys (all ones past n)
   // window keys and previous indices: bucket path tile q at q * kBucketCap,
   // sorted path one tile's chunk at a time
@@ -528,10 +543,18 @@ __global__ __launch_bounds__(TW) void k_merge(
     const uint64_t kc = (uint64_t)kold[jc], kpc = (uint64_t)kold[max(jc - 1, 0)];
     k[q] = live[q] ? kc : ~0ull;
     kp[q] = (live[q] && j > 0) ? kpc : ~k[q];
-    x[q] = tx[jc];
-    y[q] = ty[jc];
-    u[q] = tvx[jc];
-    v[q] = tvy[jc];
+    if constexpr (NARROW) {
+      const float4 tq = t4[jc];
+      x[q] = tq.x;
+      y[q] = tq.y;
+      u[q] = tq.z;
+      v[q] = tq.w;
+    } else {
+      x[q] = tx[jc];
+      y[q] = ty[jc];
+      u[q] = tvx[jc];
+      v[q] = tvy[jc];
+    }
     id[q] = pid_src[jc];
   }
   // the packed counters: the chunk's tiles before t0 (lanes of wave 0), the
@@ -728,7 +751,12 @@ __global__ __launch_bounds__(TW) void k_merge(
         for (int e = tid; e < len; e += TW) {
           const uint32_t src = s_mi[e];
           const uint32_t sc = min(src, (uint32_t)(n - 1));
-          place(q, c0 + e, s_mk[e], src, tx[sc], ty[sc], tvx[sc], tvy[sc], pid_src[sc]);
+          if constexpr (NARROW) {
+            const float4 tq = t4[sc];
+            place(q, c0 + e, s_mk[e], src, tq.x, tq.y, tq.z, tq.w, pid_src[sc]);
+          } else {
+            place(q, c0 + e, s_mk[e], src, tx[sc], ty[sc], tvx[sc], tvy[sc], pid_src[sc]);
+          }
         }
         __syncthreads();
       }
@@ -781,15 +809,15 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
   if (s.narrow)
     k_bucket<false, uint32_t><<<grid, KB_TW, 0, L.stream>>>(
         s.n, tiles, passes, s.nkeys_in, r.flags, s.tile_first, s.tile_count, s.cells,
-        (uint64_t)s.n_keys, s.tx, s.ty, s.tvx, s.tvy, s.pid, r);
+        (uint64_t)s.n_keys, s.tx, s.ty, s.tvx, s.tvy, s.pid, s.t4, r);
   else if (passes <= 4)
     k_bucket<false, uint64_t><<<grid, KB_TW, 0, L.stream>>>(
         s.n, tiles, passes, knew, r.flags, s.tile_first, s.tile_count, s.cells,
-        (uint64_t)s.n_keys, s.tx, s.ty, s.tvx, s.tvy, s.pid, r);
+        (uint64_t)s.n_keys, s.tx, s.ty, s.tvx, s.tvy, s.pid, nullptr, r);
   else
     k_bucket<true, uint64_t><<<grid, KB_TW, 0, L.stream>>>(
         s.n, tiles, passes, knew, r.flags, s.tile_first, s.tile_count, s.cells,
-        (uint64_t)s.n_keys, s.tx, s.ty, s.tvx, s.tvy, s.pid, r);
+        (uint64_t)s.n_keys, s.tx, s.ty, s.tvx, s.tvy, s.pid, nullptr, r);
   return hipGetLastError();
 }
 
@@ -802,12 +830,12 @@ hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers&
     k_merge<uint32_t><<<grid, TW, 0, L.stream>>>(
         s.n, s.nkeys, r.flags, s.tile_first, s.tile_count, s.tx, s.ty, s.tvx, s.tvy, s.pid,
         soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt, s.nkeys_alt, perm ? s.perm : nullptr,
-        s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv, r);
+        s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv, s.t4, r);
   else
     k_merge<uint64_t><<<grid, TW, 0, L.stream>>>(
         s.n, s.keys_sorted, r.flags, s.tile_first, s.tile_count, s.tx, s.ty, s.tvx, s.tvy, s.pid,
         soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt, keys_dst, perm ? s.perm : nullptr,
-        s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv, r);
+        s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv, nullptr, r);
   return hipGetLastError();
 }
 
