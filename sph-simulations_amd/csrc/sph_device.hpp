@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace sphamd {
 
 constexpr int kWave = 64;
@@ -283,6 +285,13 @@ constexpr int kEllLds = kEllStage + 9;  // LDS words per thread: staged slots + 
 #endif
 constexpr int kCandBatch = SPH_CAND_BATCH;  // stencil candidates loaded together (density)
 constexpr int kListBatch = SPH_LIST_BATCH;  // list slots loaded together (forces)
+#ifndef SPH_LIST_BATCH2
+#define SPH_LIST_BATCH2 3
+#endif
+// later batches: 2D lists hold about five neighbors, rarely more than seven,
+// so the batch after the first loads three slots, not four
+constexpr int kListBatch2 = SPH_LIST_BATCH2;
+static_assert(kListBatch2 <= kListBatch, "later batches reuse the first batch's registers");
 
 // Cell ranges of the 3x3 stencil of position (x, y) in visiting order (x offset
 // outer, y offset inner); cells outside a walled box give (0, 0) and empty
@@ -517,11 +526,11 @@ __device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
   int jl[kListBatch];
 #pragma unroll
   for (int u = 0; u < kListBatch; ++u) jl[u] = ell_idx[(int64_t)min(u, ell_w - 1) * n + i];
-  auto batch = [&](int k0) {
-    float xb[kListBatch], yb[kListBatch], ub[kListBatch], wb[kListBatch], db[kListBatch],
-        pb[kListBatch];
+  auto batch = [&](int k0, auto size) {
+    constexpr int B = decltype(size)::value;
+    float xb[B], yb[B], ub[B], wb[B], db[B], pb[B];
 #pragma unroll
-    for (int u = 0; u < kListBatch; ++u) {
+    for (int u = 0; u < B; ++u) {
       const int j = k0 + u < cnt ? jl[u] : i;
       if constexpr (PACKED) {
         const float4 q = ld16(xyuv, j);
@@ -542,7 +551,7 @@ __device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
       }
     }
 #pragma unroll
-    for (int u = 0; u < kListBatch; ++u) {
+    for (int u = 0; u < B; ++u) {
       pin(xb[u]);
       pin(yb[u]);
       pin(ub[u]);
@@ -551,7 +560,7 @@ __device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
       pin(pb[u]);
     }
 #pragma unroll
-    for (int u = 0; u < kListBatch; ++u) {
+    for (int u = 0; u < B; ++u) {
       if (k0 + u >= cnt) continue;
       float dx, dy;
       separation<BC>(c, xi, yi, xb[u], yb[u], dx, dy);
@@ -560,12 +569,12 @@ __device__ __forceinline__ Acc forces_ell(const DevGrid& g, const DevPhys& c,
     }
   };
   if (cnt <= ell_w) {
-    batch(0);
-    for (int k0 = kListBatch; k0 < cnt; k0 += kListBatch) {
+    batch(0, std::integral_constant<int, kListBatch>{});
+    for (int k0 = kListBatch; k0 < cnt; k0 += kListBatch2) {
 #pragma unroll
-      for (int u = 0; u < kListBatch; ++u)
+      for (int u = 0; u < kListBatch2; ++u)
         jl[u] = ell_idx[(int64_t)min(k0 + u, ell_w - 1) * n + i];
-      batch(k0);
+      batch(k0, std::integral_constant<int, kListBatch2>{});
     }
   } else if constexpr (PACKED) {
     // the list overflowed: the stencil, state from the packed copies
