@@ -281,15 +281,18 @@ constexpr int kEllLds = kEllStage + 9;  // LDS words per thread: staged slots + 
 #define SPH_CAND_BATCH 4
 #endif
 #ifndef SPH_LIST_BATCH
-#define SPH_LIST_BATCH 4
+#define SPH_LIST_BATCH 7
 #endif
 constexpr int kCandBatch = SPH_CAND_BATCH;  // stencil candidates loaded together (density)
 constexpr int kListBatch = SPH_LIST_BATCH;  // list slots loaded together (forces)
 #ifndef SPH_LIST_BATCH2
-#define SPH_LIST_BATCH2 3
+#define SPH_LIST_BATCH2 4
 #endif
-// later batches: 2D lists hold about five neighbors, rarely more than seven,
-// so the batch after the first loads three slots, not four
+// 2D lists at h = 1.3 dx hold about five neighbors and rarely more than
+// seven: the first batch loads seven slots with the count, so one round of
+// loads serves almost every wave; longer lists continue four at a time
+// (first / later batch of 4/4, 4/3, 6/1, 7/1, 7/4, 8/1 slots measured on the
+// 1M dam break: 7/1 and 7/4 fastest, 0.8-1.4 us per step ahead of 4/4)
 constexpr int kListBatch2 = SPH_LIST_BATCH2;
 static_assert(kListBatch2 <= kListBatch, "later batches reuse the first batch's registers");
 
