@@ -11,7 +11,10 @@ namespace sphamd {
 
 namespace {
 
-constexpr int BLK = 256;
+#ifndef SPH_BLK
+#define SPH_BLK 256  // 128, 256 and 512 measured at 1M: 256 fastest
+#endif
+constexpr int BLK = SPH_BLK;
 
 inline int blocks_for(int64_t n, int blk = BLK) { return (int)((n + blk - 1) / blk); }
 
@@ -126,7 +129,10 @@ __global__ __launch_bounds__(BLK) void k_gather(
 // the digit histogram its mover sort used. Each thread takes kRangesPer slots a
 // grid stride apart, every key it needs (its own, the one before and the one
 // after) loaded in one round: few enough waves that they all run at once.
-constexpr int kRangesPer = 4;
+#ifndef SPH_RANGES_PER
+#define SPH_RANGES_PER 2  // 1, 2, 4 and 8 measured at 1M: 2 fastest
+#endif
+constexpr int kRangesPer = SPH_RANGES_PER;
 
 template <typename K>
 __global__ __launch_bounds__(BLK) void k_ranges(int n, const K* __restrict__ keys,
