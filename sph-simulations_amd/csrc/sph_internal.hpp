@@ -12,7 +12,10 @@ namespace sphamd {
 
 constexpr int kMaxPasses = 8;  // 64-bit keys, 8-bit digits
 constexpr int kResortTile = 256;     // particles per tile of the incremental re-sort
-constexpr int kBucketCap = 128;      // movers a merge tile's bucket holds
+#ifndef SPH_BUCKET_CAP
+#define SPH_BUCKET_CAP 64  // at most the merge's threads per tile (256 / tiles per workgroup)
+#endif
+constexpr int kBucketCap = SPH_BUCKET_CAP;  // movers a merge tile's bucket holds
 constexpr int kBucketFast = 8;       // bucket entries the merge loads with its first round
 constexpr int kChunkTiles = 64;      // merge tiles per counter chunk (one k_bucket workgroup)
 // k_bucket's tally word counts its workgroups (one per chunk) in 16 bits:

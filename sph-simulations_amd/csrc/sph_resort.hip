@@ -496,7 +496,10 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
 // A workgroup takes MT consecutive tiles, each thread one particle of each:
 // every load of a particle's state is issued in one round for all of them, so
 // at 1M particles the whole grid is resident at once.
-constexpr int MT = 2;          // tiles per merge workgroup
+#ifndef SPH_MERGE_TILES
+#define SPH_MERGE_TILES 4  // 1, 2, 4 (buckets of 64) and 8 (of 32) measured at 1M: 4 fastest
+#endif
+constexpr int MT = SPH_MERGE_TILES;  // tiles per merge workgroup
 constexpr int BH = TW / MT;    // threads that hold one tile's bucket entries
 static_assert(kBucketCap <= BH && MT * kBucketCap <= WIN,
               "a tile's bucket maps to its share of the threads and of the window buffer");

@@ -472,18 +472,18 @@ def test_incremental_resort_buckets_many_chunks(sph, orc):
 
 
 def test_incremental_resort_bucket_rounds(sph, orc):
-    """One bucketing workgroup's 16,384 particles with about a quarter of them
-    moving one cell: more movers than its LDS list holds (listed in rounds of
-    2048) and windows of 32 to 128 movers per tile (the merge loads the bucket
-    entries past the first 32 in a second round); still the bucket path, and
-    the oracle's permutation and cell ranges."""
+    """One bucketing workgroup's 16,384 particles with about one in seven of
+    them moving one cell: more movers than its LDS list holds (listed in rounds
+    of 2048) and windows of tens of movers per tile, within the 64 a bucket
+    holds (the merge loads the entries past its first 8 in a second round);
+    still the bucket path, and the oracle's permutation and cell ranges."""
     s = sph.scenes.dam_break(128, 128, seed=21)
     with sph.Context(s.params, s.n) as ctx:
         cs = float(ctx.grid().cs_x)
         rng = np.random.default_rng(23)
-        # displacements of about 0.15 cell per axis and step
-        s.vx = (0.15 * cs / s.params.dt * rng.normal(0, 1, s.n)).astype(np.float32)
-        s.vy = (0.15 * cs / s.params.dt * rng.normal(0, 1, s.n)).astype(np.float32)
+        # displacements of about 0.095 cell per axis and step
+        s.vx = (0.095 * cs / s.params.dt * rng.normal(0, 1, s.n)).astype(np.float32)
+        s.vy = (0.095 * cs / s.params.dt * rng.normal(0, 1, s.n)).astype(np.float32)
         ctx.upload(s.rx, s.ry, s.vx, s.vy)
         ctx.prepare()
         for step in range(2):
