@@ -32,12 +32,14 @@ def log(msg):
 
 
 def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers_frac=0.0,
-                           flags_in_force=False):
+                           flags_in_force=False, narrow=False):
     """Algorithmic HBM bytes per particle of one launch of each stage (FP32
     state, u64 keys, u32 indices; neighbor and halo re-reads counted as cache
     hits). flags_in_force: the force kernel reports the re-sort's movers
-    (reading the previous key, writing one flag bit). None for a stage without
-    a model."""
+    (reading the previous key, writing one flag bit). narrow: the step keeps
+    its keys as u32 (kb = 4 bytes, else 8). None for a stage without a
+    model."""
+    kb = 4.0 if narrow else 8.0
     if variant == "split_block":
         # density: R r 8, W rho,pr 8; force: R r,v,rho,pr 24, W a 8, T r,v 16, key 8
         phys = {"approximate_density": 16.0, "force_computation": 56.0}
@@ -46,13 +48,14 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
         phys = {"block_step": 56.0}
     else:
         # density: R r 8, W the packed (rho, pr) copy 8, count 4, list 4k;
-        # force: R r,v 16, rho,pr 8, count 4, list 4k, T 16, key 8 (+ R
-        # previous key 8 and W one flag bit when it reports the movers); the
+        # force: R r,v 16, rho,pr 8, count 4, list 4k, T 16, key kb (+ R
+        # previous key kb and W one flag bit when it reports the movers); the
         # separate rho, pr and a arrays (8 + 8) are written only in the last
         # step of a sph_step call; neighbors come from the packed copies (cache
         # hits)
         phys = {"approximate_density": 20.0 + 4.0 * k,
-                "force_computation": 52.0 + 4.0 * k + (8.125 if flags_in_force else 0.0)}
+                "force_computation": 44.0 + kb + 4.0 * k
+                + ((kb + 0.125) if flags_in_force else 0.0)}
     if resort == "full":
         sort = {
             # R new keys (8) + previous sorted keys (8); clear previous ranges (8/occ. cell)
@@ -67,20 +70,20 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
         f = movers_frac
         sort = {
             # k_bucket: R the flag bits 1/8, a packed 8-byte counter per
-            # 256-particle tile; per mover R new key 8, T r,v 16, pid 4, its new
-            # cell's range 8, W its bucket entry 32, its owner's counter 8 (+ R
-            # new and old keys 16, W the flag bit when no kernel before
+            # 256-particle tile; per mover R new key kb, T r,v 16, pid 4, its
+            # new cell's range 8, W its bucket entry 32, its owner's counter 8
+            # (+ R new and old keys 2 kb, W the flag bit when no kernel before
             # reported the movers)
-            "movers": 0.125 + 8.0 / 256.0 + 76.0 * f
-            + (0.0 if flags_in_force else 16.125),
-            # k_merge: R old key 8, T r,v 16, pid 4, flag bits 1/8, the movers'
-            # bucket entries 32 f; W r,v 16 (only the packed copy inside
-            # sph_step), pid 4, key 8 (the permutation 4 only in the last step
-            # of a call); clear the old ranges 8 occ
-            "merge": 56.125 + 32.0 * f + 8.0 * occ,
-            # k_ranges: R key 8, W 8 occ, the tile-first key and the zeroed
+            "movers": 0.125 + 8.0 / 256.0 + (68.0 + kb) * f
+            + (0.0 if flags_in_force else 2.0 * kb + 0.125),
+            # k_merge: R old key kb, T r,v 16, pid 4, flag bits 1/8, the
+            # movers' bucket entries 32 f; W r,v 16 (only the packed copy
+            # inside sph_step), pid 4, key kb (the permutation 4 only in the
+            # last step of a call); clear the old ranges 8 occ
+            "merge": 40.125 + 2.0 * kb + 32.0 * f + 8.0 * occ,
+            # k_ranges: R key kb, W 8 occ, the tile-first key and the zeroed
             # counter 16 per 256-particle tile
-            "neighbor_search": 8.0 + 16.0 / 256.0 + 8.0 * occ,
+            "neighbor_search": kb + 16.0 / 256.0 + 8.0 * occ,
         }
     return {**phys, **sort}.get(stage)
 
@@ -146,7 +149,7 @@ def cpu_info():
     return model, len(cores) or (os.cpu_count() or 1)
 
 
-def cpu_baseline(scene, budget_s):
+def cpu_baseline(scene, budget_s, max_steps=200):
     """The reference-style OpenMP CPU step (oracle/cpu_baseline.cpp: compute_type
     mp-lists, and sorted) on a bounded sample of the same workload, in a fresh
     child process started with the reference's OpenMP recipe
@@ -168,7 +171,8 @@ def cpu_baseline(scene, budget_s):
         json.dump(dataclasses.asdict(scene.params), open(prm, "w"))
         log(f"cpu baseline: {threads} threads ({phys} physical cores visible, {model})")
         r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline_run", npz, prm,
-                            str(budget_s)], cwd=ROOT, env=env, capture_output=True, text=True,
+                            str(budget_s), "mp-lists,sorted", str(max_steps)], cwd=ROOT,
+                           env=env, capture_output=True, text=True,
                            timeout=max(120.0, 6 * budget_s))
     sys.stderr.write(r.stderr)
     if r.returncode != 0:
@@ -197,6 +201,8 @@ def stage_table(stage, steps, n, k, occ, passes, variant, resort, mfrac):
     """Per-stage launches, time and achieved algorithmic GB/s of an events pass
     of `steps` steps; returns (stages, algorithmic bytes per particle-step, kernel ms)."""
     flags_in_force = variant == "particle" and resort == "incremental"
+    # steps of a multi-step call with keys below 2^32 keep them as u32
+    narrow = flags_in_force and passes <= 4 and steps >= 2
     stages, step_bytes, kern_ms = {}, 0.0, 0.0
     for name, (ms, launches) in stage.items():
         if launches == 0:
@@ -205,7 +211,7 @@ def stage_table(stage, steps, n, k, occ, passes, variant, resort, mfrac):
         entry = {"launches": launches, "ms_total": round(ms, 4),
                  "us_per_launch": round(per * 1e3, 3)}
         bpp = alg_bytes_per_particle(name, k, occ, passes, variant, resort, mfrac,
-                                     flags_in_force)
+                                     flags_in_force, narrow)
         if bpp is not None:
             entry["alg_bytes_per_launch"] = int(bpp * n)
             entry["achieved_GBs"] = round(bpp * n / (per * 1e-3) / 1e9, 1)
@@ -437,7 +443,8 @@ def bench_batched(sph, steps, trial_steps=10):
         per = ms / launches
         # the batched force kernel reports the movers itself (incremental re-sort)
         bpp = alg_bytes_per_particle(name, k, occ, passes, "particle", resort, mfrac,
-                                     flags_in_force=resort == "incremental")
+                                     flags_in_force=resort == "incremental",
+                                     narrow=resort == "incremental" and passes <= 4)
         stages[name] = {"launches": launches, "us_per_launch": round(per * 1e3, 3)}
         kern_ms += ms
         if bpp is None:
@@ -787,6 +794,18 @@ def main():
                                            args.train_scenes, args.train_steps)
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget)
+        # the other plain-SPH configurations on the same host: the 16,384-particle
+        # dam break (up to its 1,000 steps) and the 4M periodic square (bounded)
+        cfgs = {}
+        for sc, budget, cap in ((sph.scenes.dam_break(), args.cpu_budget / 2, 1000),
+                                (sph.scenes.periodic_shear(), args.cpu_budget, 200)):
+            r = cpu_baseline(sc, budget, cap)
+            cfgs[sc.name] = {"particles": sc.n, "configured_steps": sc.steps,
+                             "value": r["value"], "unit": "particle-steps/s",
+                             "variants": {k: {kk: v[kk] for kk in ("steps", "value", "s_per_step",
+                                                                   "stage_seconds")}
+                                          for k, v in r["variants"].items()}}
+        out["cpu_baseline"]["configs"] = cfgs
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

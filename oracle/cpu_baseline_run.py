@@ -7,7 +7,7 @@ them when it first starts. Reads the workload (npz: rx, ry, vx, vy; json: the
 parameters), times the reference-style CPU step for each compute type on a
 bounded sample and prints one JSON line.
 
-usage: python -m oracle.cpu_baseline_run <state.npz> <params.json> <budget_s> [types]
+usage: python -m oracle.cpu_baseline_run <state.npz> <params.json> <budget_s> [types] [max_steps]
 """
 import json
 import sys
@@ -21,6 +21,7 @@ from oracle import oracle as orc
 def main():
     state, prm_path, budget = sys.argv[1], sys.argv[2], float(sys.argv[3])
     types = sys.argv[4].split(",") if len(sys.argv) > 4 else ["mp-lists", "sorted"]
+    max_steps = int(sys.argv[5]) if len(sys.argv) > 5 else 200
     z = np.load(state)
     d = json.load(open(prm_path))
     p = SimpleNamespace(**d)
@@ -30,7 +31,7 @@ def main():
     for ct in types:
         threads, t, _ = orc.baseline_run(p, z["rx"], z["ry"], z["vx"], z["vy"], 2, ct)
         per_step = t["main_loop"] / 2.0
-        steps = int(max(2, min(200, budget / max(per_step, 1e-6))))
+        steps = int(max(2, min(max_steps, budget / max(per_step, 1e-6))))
         threads, t, _ = orc.baseline_run(p, z["rx"], z["ry"], z["vx"], z["vy"], steps, ct)
         out[ct] = {"steps": steps, "threads": threads,
                    "value": n * steps / t["main_loop"],
