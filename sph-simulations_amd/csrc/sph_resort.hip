@@ -23,8 +23,9 @@
 //             word (one atomic per workgroup) chooses the path: when a bucket
 //             overflowed it orders the movers itself (ordered compaction, then
 //             a stable radix sort) for the sorted path.
-//   k_merge   one workgroup per two tiles of the previous order (a thread per
-//             particle of each): clears the tiles' old cell / block ranges; bucket path: its prefixes from the
+//   k_merge   one workgroup per MT tiles of the previous order (four by
+//             default; a thread per particle of each): clears the tiles' old
+//             cell / block ranges; bucket path: its prefixes from the
 //             packed counters, its window (the movers it owns) from its bucket,
 //             sorted in LDS by counting; sorted path: the window from the sorted
 //             movers. It places its stayers (rank among the window's movers,
