@@ -390,7 +390,7 @@ int incremental_resort(Ctx* c, bool flags_ready) {
   std::swap(s.pid, s.pid_alt);
   {
     StageTimer t(c, SPH_T_NEIGHBOR_SEARCH);
-    HIPCHK(c, launch_ranges(c->L, s, hist_words(c)));
+    HIPCHK(c, launch_ranges(c->L, s, r, hist_words(c)));
   }
   s.n_prev = s.n;
   return SPH_OK;
@@ -614,11 +614,13 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
         (rc = dev_alloc(c, &s.nkeys_alt, N)) || (rc = dev_alloc(c, &s.t4, N)) ||
         (rc = dev_alloc(c, &s.tile_first, resort_tiles((int)N))) ||
         (rc = dev_alloc(c, &s.tile_count, resort_count_words((int)N))) ||
-        (rc = dev_alloc(c, &r.vtmp, N)) || (rc = dev_alloc(c, &r.tally, 1)) ||
-        (rc = dev_alloc(c, &r.n_movers, 1)) ||
-        (rc = dev_alloc(c, &r.bucket, resort_tiles((int)N) * kBucketCap * 2)) ||
-        (rc = dev_alloc(c, &r.mode, 1)))
+        (rc = dev_alloc(c, &r.vtmp, N)) || (rc = dev_alloc(c, &r.mode, kResortStateWords)) ||
+        (rc = dev_alloc(c, &r.bucket, resort_tiles((int)N) * kBucketCap * 2)))
       return bail(rc);
+    r.n_movers = reinterpret_cast<int*>(r.mode + 1);
+    r.claim = r.mode + 2;
+    r.n_last = reinterpret_cast<int*>(r.mode + 3);
+    r.mode_last = r.mode + 4;
   }
   b.blocks = s.blocks;
   b.block_bits = s.block_bits;
@@ -637,13 +639,14 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
     return bail(rc);
   e = hipMemsetAsync(b.hist, 0, kMaxPasses * 256 * sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(b.status, 0, sizeof(uint32_t), c->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(c->rb.n_movers, 0, sizeof(int), c->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(c->rb.tally, 0, sizeof(unsigned long long), c->stream);
+  // the re-sort's step words: count 0, path buckets, claim free
+  if (e == hipSuccess)
+    e = hipMemsetAsync(c->rb.mode, 0, kResortStateWords * sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = hipMemsetD32Async(c->rb.mode, 1, 1, c->stream);
   // producers that OR their mover bits in expect zero flag words
   if (e == hipSuccess)
     e = hipMemsetAsync(c->rb.flags, 0, resort_flag_words((int)N) * sizeof(unsigned long long),
                        c->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(c->rb.mode, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = launch_fill_cells(c->L, s.cells, n_keys);
   if (e == hipSuccess && s.block_bits) e = launch_fill_cells(c->L, s.blocks, s.n_blocks);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1308,7 +1311,7 @@ int sph_movers(void* ctx, int64_t* count) {
   Ctx* c = as_ctx(ctx);
   if (!c || !count) return SPH_ERR_INVALID;
   int m = 0;
-  HIPCHK(c, hipMemcpyAsync(&m, c->rb.n_movers, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&m, c->rb.n_last, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *count = m;
   return SPH_OK;
@@ -1318,7 +1321,8 @@ int sph_resort_path(void* ctx, int32_t* path) {
   Ctx* c = as_ctx(ctx);
   if (!c || !path) return SPH_ERR_INVALID;
   uint32_t m = 0;
-  HIPCHK(c, hipMemcpyAsync(&m, c->rb.mode, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&m, c->rb.mode_last, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *path = m != 0u ? 1 : 0;
   return SPH_OK;

@@ -105,17 +105,24 @@ struct DevState {
 
 enum { NB_STENCIL = 0, NB_CSR = 1, NB_ELL = 2 };
 
+constexpr int kResortStateWords = 8;
+
 // Buffers of the incremental re-sort (sph_resort.hip).
 struct ResortBuffers {
   unsigned long long* flags;       // mover bit per particle, one word per wave
   // [tiles][kBucketCap] pairs: (key lo, key hi, previous index, pid), (x, y, vx, vy)
   uint4* bucket;
-  // k_bucket's tally, one atomic add per workgroup: movers (bits 0-31),
-  // workgroups done (32-47; the last one finishes the step), workgroups with a
-  // full bucket (48-63)
-  unsigned long long* tally;
-  uint32_t* mode;                  // 1: the merge windows come from the buckets, 0: sorted
+  // Per step (k_ranges copies the first two to the last two and resets them
+  // for the next step): the movers, one add per k_bucket workgroup; the path
+  // (1: the merge windows come from the buckets, 0: sorted -- set by the
+  // workgroup that wins the claim word when a bucket overflows); the claim
+  // word; the last step's mover count and path, for the host. One allocation
+  // of kResortStateWords words.
   int* n_movers;
+  uint32_t* mode;
+  uint32_t* claim;
+  int* n_last;
+  uint32_t* mode_last;
   uint64_t* keys_alt;              // second sorted-key buffer (ping-pong)
   // sorted path (a bucket overflowed): movers in previous-index order, then
   // sorted by (key, previous index), and the movers before each 256-particle tile
@@ -198,7 +205,9 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
 hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
                         uint64_t* keys_dst, bool soa = true, bool perm = true);
 // cell and block ranges of the sorted keys s.keys_sorted; clears the histogram
-hipError_t launch_ranges(const Launch& L, const DevState& s, int hist_words);
+// and resets the re-sort's step words in r for the next step
+hipError_t launch_ranges(const Launch& L, const DevState& s, const ResortBuffers& r,
+                         int hist_words);
 hipError_t launch_update_velocities(const Launch& L, const DevState& s);
 hipError_t launch_update_positions(const Launch& L, const DevState& s);
 hipError_t launch_csr_count(const Launch& L, const DevState& s, bool self);

@@ -140,11 +140,21 @@ __global__ __launch_bounds__(BLK) void k_ranges(int n, const K* __restrict__ key
                                                 int2* __restrict__ blocks, int block_bits,
                                                 uint32_t* __restrict__ hist, int hist_words,
                                                 uint64_t* __restrict__ tile_first,
-                                                unsigned long long* __restrict__ tile_count) {
+                                                unsigned long long* __restrict__ tile_count,
+                                                ResortBuffers r) {
   const int stride = gridDim.x * BLK;
   const int i0 = blockIdx.x * BLK + threadIdx.x;
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < hist_words; w += BLK) hist[w] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // the re-sort that just ended: its count and path for the host; then the
+    // next step's start values (buckets unless a bucket overflows)
+    *r.n_last = *r.n_movers;
+    *r.mode_last = *r.mode;
+    *r.n_movers = 0;
+    *r.mode = 1u;
+    *r.claim = 0u;
+  }
   // each slot's key, and one neighbor key per lane (the one before for lane
   // 0, after for the others) loaded together; the neighbors inside the wave
   // come from the adjacent lanes (slot q of the wave's lanes is 64
@@ -657,17 +667,18 @@ hipError_t launch_gather(const Launch& L, const DevState& s, const float* src_rx
   return hipGetLastError();
 }
 
-hipError_t launch_ranges(const Launch& L, const DevState& s, int hist_words) {
+hipError_t launch_ranges(const Launch& L, const DevState& s, const ResortBuffers& r,
+                         int hist_words) {
   if (s.n > 0) {
     const int grid = (blocks_for(s.n) + kRangesPer - 1) / kRangesPer;
     if (s.narrow)
       k_ranges<uint32_t><<<grid, BLK, 0, L.stream>>>(s.n, s.nkeys, s.cells, s.n_keys, s.blocks,
                                                      s.block_bits, s.hist, hist_words,
-                                                     s.tile_first, s.tile_count);
+                                                     s.tile_first, s.tile_count, r);
     else
       k_ranges<uint64_t><<<grid, BLK, 0, L.stream>>>(s.n, s.keys_sorted, s.cells, s.n_keys,
                                                      s.blocks, s.block_bits, s.hist, hist_words,
-                                                     s.tile_first, s.tile_count);
+                                                     s.tile_first, s.tile_count, r);
   }
   return hipGetLastError();
 }

@@ -19,10 +19,10 @@
 //             cell (a search of the sampled tile-first table when that cell was
 //             empty). Packed 64-bit counters per tile and per chunk of
 //             kChunkTiles tiles count the movers by previous index (low half)
-//             and by owner (high half). The workgroup that completes the tally
-//             word (one atomic per workgroup) chooses the path: when a bucket
-//             overflowed it orders the movers itself (ordered compaction, then
-//             a stable radix sort) for the sorted path.
+//             and by owner (high half). When a bucket overflows, the workgroup
+//             whose mover first finds it full (and first claims the step)
+//             orders every mover itself (ordered compaction, then a stable
+//             radix sort) for the sorted path.
 //   k_merge   one workgroup per MT tiles of the previous order (four by
 //             default; a thread per particle of each): clears the tiles' old
 //             cell / block ranges; bucket path: its prefixes from the
@@ -33,7 +33,9 @@
 //             keys, searched in LDS); scatters r, v, pid, key and the
 //             permutation
 //   k_ranges  cell start/end and Morton block ranges of the new order, the
-//             tile-first keys, and zeroes the counters for the next step
+//             tile-first keys; zeroes the counters and resets the step's
+//             mover count, path and claim word for the next step (keeping the
+//             count and path for the host)
 #include <hip/hip_runtime.h>
 
 #include "sph_internal.hpp"
@@ -300,12 +302,15 @@ __device__ void sort_movers(int m, int passes, const K* __restrict__ knew,
 // over their tiles, so the merge finds its prefixes from at most kChunkTiles +
 // chunks words. A bucket entry carries the mover's key, previous index and
 // state (pid, x, y, vx, vy from T), so the merge reads it in one round. Each
-// workgroup adds its movers, itself and whether one of its movers found a full
-// bucket to the tally word in one atomic; the one that completes the tally
-// tells the merge which path (*r.mode: 1 buckets, 0 sorted) and, for the
-// sorted path, orders the movers itself. It reads nothing other workgroups
-// wrote except the tally (flags and keys come from the previous kernel), so
-// no workgroup needs a device-scope fence.
+// workgroup adds its movers to the step's count without waiting for the
+// result. The path stays the buckets (*r.mode = 1, set by k_ranges) unless a
+// bucket overflows: the mover that takes the first slot past a bucket's end
+// claims the sort (one compare-and-swap on the claim word, so one workgroup
+// of the step wins), and its workgroup then orders every mover itself --
+// compaction and sort read only the flags and keys of the previous kernel --
+// and sets the sorted path. No workgroup reads what another one wrote, so
+// none needs a device-scope fence, and none waits for an atomic's result
+// unless it overflowed a bucket.
 constexpr int kBucketList = 2048;
 constexpr int kBucketTab = 1024;  // sampled tile-first keys in LDS (every S-th tile)
 static_assert(kBucketTab % KB_TW == 0, "tile-first table loads");
@@ -328,8 +333,7 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
   __shared__ uint64_t s_mem64[KB_LDS / 2];
   __shared__ uint32_t s_w[KB_WAVES];
   __shared__ uint32_t s_cc[3];  // owners in the chunks before, of and after this one
-  __shared__ uint32_t s_full;   // a mover of this workgroup found its bucket full
-  __shared__ unsigned long long s_tally;  // the tally with this workgroup added, last one only
+  __shared__ uint32_t s_sort;  // this workgroup won the claim: it orders the movers
   uint32_t* s_mem = reinterpret_cast<uint32_t*>(s_mem64);
   uint32_t* s_list = s_mem;
   uint64_t* s_tab = s_mem64 + kBucketList / 2;
@@ -356,7 +360,7 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
   const uint32_t x = wave_incl_scan(c);
   if (lane == kWave - 1) s_w[wave] = x;
   if (tid < 3) s_cc[tid] = 0u;
-  if (tid == 0) s_full = 0u;
+  if (tid == 0) s_sort = 0u;
   __syncthreads();
   uint32_t off = x - c, tot = 0;
 #pragma unroll
@@ -439,8 +443,8 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
           e[0] = make_uint4((uint32_t)mk, (uint32_t)(mk >> 32), mi, mp);
           e[1] = make_uint4(__float_as_uint(mx), __float_as_uint(my), __float_as_uint(mu),
                             __float_as_uint(mv));
-        } else {
-          s_full = 1u;
+        } else if (slot == (uint32_t)kBucketCap && atomicCAS(r.claim, 0u, 1u) == 0u) {
+          s_sort = 1u;  // the first overflow of a bucket, and the first claim of the step
         }
       }
       __syncthreads();
@@ -452,33 +456,16 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
       if (v != 0ull && oc >= 0 && oc * kChunkTiles < tiles) atomicAdd(&tcount[tiles + oc], v);
     }
   }
-  // the tally: this workgroup's movers, itself, and whether it found a full
-  // bucket (s_full is complete after the last round's barrier)
-  if (tid == 0) {
-    const unsigned long long add =
-        (unsigned long long)tot + (1ull << 32) + (s_full ? (1ull << 48) : 0ull);
-    const unsigned long long t = atomicAdd(r.tally, add) + add;
-    s_tally = ((t >> 32) & 0xffffull) == gridDim.x ? t : 0ull;
-  }
-  __syncthreads();
-  const unsigned long long tally = s_tally;
-  if (tally == 0ull) return;  // not the last workgroup
-  if ((tally >> 48) == 0ull) {
-    if (tid == 0) {
-      *r.n_movers = (int)(uint32_t)tally;
-      *r.mode = 1u;
-    }
-    return;
-  }
+  // the step's mover count (read by the merge of the sorted path and, through
+  // k_ranges, by the host); s_sort is complete after the last round's barrier
+  if (tid == 0 && tot != 0u) atomicAdd(r.n_movers, (int)tot);
+  if (s_sort == 0u) return;
   // sorted path: every mover in previous-index order, then sorted by (key,
   // previous index); the merge searches the tile-first keys in them
   const uint32_t m = compact_movers(n, tiles, flags, r.tile_base, r.midx, s_mem, s_w);
   __syncthreads();
   sort_movers((int)m, passes, knew, r, s_mem);
-  if (tid == 0) {
-    *r.n_movers = (int)m;
-    *r.mode = 0u;
-  }
+  if (tid == 0) *r.mode = 0u;
 }
 
 // ------------------------------------------------------------------- merge
@@ -579,7 +566,6 @@ __global__ __launch_bounds__(TW) void k_merge(
   const uint4* be = r.bucket + 2 * ((size_t)tq * kBucketCap + (rb & (kBucketFast - 1)));
   uint4 b0 = be[0], b1 = be[1];
   const bool from_bucket = *r.mode != 0u;
-  if (blockIdx.x == 0 && tid == 0) *r.tally = 0ull;  // for the next k_bucket (this one has ended)
   // the previous order's ranges (k_ranges writes the new ones)
 #pragma unroll
   for (int q = 0; q < MT; ++q)
