@@ -24,8 +24,8 @@
 //             orders every mover itself (ordered compaction, then a stable
 //             radix sort) for the sorted path.
 //   k_merge   one workgroup per MT tiles of the previous order (four by
-//             default; a thread per particle of each): clears the tiles' old
-//             cell / block ranges; bucket path: its prefixes from the
+//             default; a thread per particle of each): clears the old cell /
+//             block ranges its movers left; bucket path: its prefixes from the
 //             packed counters, its window (the movers it owns) from its bucket,
 //             sorted in LDS by counting; sorted path: the window from the sorted
 //             movers. It places its stayers (rank among the window's movers,
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(TW) void k_merge(
   const int t0 = (int)blockIdx.x * MT;
   // every load that depends only on the particles, issued together
   unsigned long long fw[MT];
-  uint64_t k[MT], kp[MT];
+  uint64_t k[MT];
   float x[MT], y[MT], u[MT], v[MT];
   uint32_t id[MT];
   bool live[MT];
@@ -531,9 +531,8 @@ __global__ __launch_bounds__(TW) void k_merge(
     live[q] = j < n;
     fw[q] = flag_word(flags, j0 / kWave + wave, n);
     const int jc = min(j, n - 1);
-    const uint64_t kc = (uint64_t)kold[jc], kpc = (uint64_t)kold[max(jc - 1, 0)];
+    const uint64_t kc = (uint64_t)kold[jc];
     k[q] = live[q] ? kc : ~0ull;
-    kp[q] = (live[q] && j > 0) ? kpc : ~k[q];
     if constexpr (NARROW) {
       const float4 tq = t4[jc];
       x[q] = tq.x;
@@ -566,13 +565,14 @@ __global__ __launch_bounds__(TW) void k_merge(
   const uint4* be = r.bucket + 2 * ((size_t)tq * kBucketCap + (rb & (kBucketFast - 1)));
   uint4 b0 = be[0], b1 = be[1];
   const bool from_bucket = *r.mode != 0u;
-  // the previous order's ranges (k_ranges writes the new ones)
+  // k_ranges writes the range of every cell and block that holds particles in
+  // the new order, so of the previous ranges only those of cells and blocks
+  // that may have become empty are cleared: the ones a mover left
 #pragma unroll
   for (int q = 0; q < MT; ++q)
-    if (live[q] && k[q] != kp[q] && k[q] < n_keys) {
+    if (live[q] && ((fw[q] >> lane) & 1ull) && k[q] < n_keys) {
       cells[k[q]] = make_int2(-1, -1);
-      if (block_bits > 0 && (k[q] >> (2 * block_bits)) != (kp[q] >> (2 * block_bits)))
-        blocks[k[q] >> (2 * block_bits)] = make_int2(-1, -1);
+      if (block_bits > 0) blocks[k[q] >> (2 * block_bits)] = make_int2(-1, -1);
     }
 #pragma unroll
   for (int q = 0; q < MT; ++q) s_tk[q][tid] = k[q];
