@@ -280,6 +280,7 @@ def bench_periodic(sph, steps, ref_steps=20):
     ctx.prepare()
     ctx.set_graph(True)
     ctx.step(10)
+    ctx.step_graphs(steps)
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.step(steps)
@@ -357,6 +358,7 @@ def bench_small(sph, steps):
     ctx.prepare()
     ctx.set_graph(True)
     ctx.step(20)
+    ctx.step_graphs(steps)
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.step(steps)
@@ -384,6 +386,7 @@ def bench_small(sph, steps):
                           ("fused_block", sph.STEP_FUSED_BLOCK)):
             c2.set_step_kernel(kv)
             c2.step(10)
+            c2.step_graphs(200)
             c2.synchronize()
             t0 = time.perf_counter()
             c2.step(200)
@@ -415,6 +418,7 @@ def bench_batched(sph, steps, trial_steps=10):
     ctx.prepare()
     ctx.set_graph(True)
     ctx.step(10)
+    ctx.step_graphs(steps)
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.step(steps)
@@ -705,7 +709,9 @@ def main():
     log(f"movers per step {mover_stats}")
     window_first = args.window_start + 24 * len(combos) + args.warmup + 20
 
-    # timed region: exactly `steps` graph-replayed steps, synchronised on both sides
+    # timed region: exactly `steps` graph-replayed steps, synchronised on both
+    # sides (the graphs that call replays are captured beforehand)
+    ctx.step_graphs(args.steps)
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.step(args.steps)

@@ -231,9 +231,15 @@ int sph_snapshot_load(void* ctx, const char* path, int64_t* step);
 int sph_prepare(void* ctx);
 int sph_step(void* ctx, int32_t n_steps);
 /* Capture fused iterations in a hipGraph and replay it in sph_step: enable = 0
- * off, k >= 1 (at most 64) a graph of 2k iterations (steps left over run
- * without it). */
+ * off, k >= 1 (at most 64) a graph of 2k iterations; the even part of the steps
+ * left over replays one shorter graph, an odd last step runs without. Graphs
+ * are kept per buffer assignment (the pid and key buffers alternate every
+ * step) and iteration count, up to four at once. */
 int sph_set_graph(void* ctx, int32_t enable);
+/* Capture, without running anything, the graphs that sph_step(ctx, n_steps)
+ * would replay from the current state, so that call spends no time capturing
+ * (no reference counterpart: for callers that time or schedule sph_step). */
+int sph_step_graphs(void* ctx, int32_t n_steps);
 /* Number of particles whose neighbor count exceeded the fixed list width in the
  * last density pass (they are handled by a stencil fallback). Synchronises. */
 int sph_ell_overflow(void* ctx, int64_t* count);

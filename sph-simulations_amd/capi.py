@@ -160,6 +160,7 @@ def lib():
         "sph_prepare": (i32, [vp]),
         "sph_step": (i32, [vp, i32]),
         "sph_set_graph": (i32, [vp, i32]),
+        "sph_step_graphs": (i32, [vp, i32]),
         "sph_ell_overflow": (i32, [vp, P(i64)]),
         "sph_set_step_kernel": (i32, [vp, i32]),
         "sph_set_resort": (i32, [vp, i32]),
@@ -222,7 +223,9 @@ EXPORTED = ("sph_create", "sph_destroy", "sph_last_error", "sph_get_grid", "sph_
             "sph_density_pressure",
             "sph_forces", "sph_integrate", "sph_update_velocities", "sph_update_positions",
             "sph_set_dt", "sph_measure", "sph_prepare", "sph_step",
-            "sph_set_graph", "sph_ell_overflow", "sph_set_step_kernel", "sph_set_resort", "sph_movers", "sph_resort_path", "sph_block_stats", "sph_get_sorted_keys", "sph_get_permutation",
+            "sph_set_graph", "sph_step_graphs", "sph_ell_overflow", "sph_set_step_kernel",
+            "sph_set_resort", "sph_movers", "sph_resort_path", "sph_block_stats",
+            "sph_get_sorted_keys", "sph_get_permutation",
             "sph_get_cell_ranges", "sph_get_neighbor_lists", "sph_get_ell",
             "sph_get_device_view", "sph_set_timing", "sph_stage_times", "sph_synchronize",
             "sph_batch_create", "sph_batch_scenes", "sph_batch_scene_info",
@@ -417,6 +420,11 @@ class Context:
     def set_graph(self, enable=True, pairs: int = 4):
         """Replay sph_step's iterations from a hipGraph of 2 * pairs iterations."""
         self._chk(self._L.sph_set_graph(self._h, int(pairs) if enable else 0), "sph_set_graph")
+
+    def step_graphs(self, n_steps: int):
+        """Capture (without running) the graphs that step(n_steps) replays from
+        the current state."""
+        self._chk(self._L.sph_step_graphs(self._h, int(n_steps)), "sph_step_graphs")
 
     def set_step_kernel(self, variant: int):
         self._chk(self._L.sph_set_step_kernel(self._h, int(variant)), "sph_set_step_kernel")

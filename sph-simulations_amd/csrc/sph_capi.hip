@@ -67,10 +67,17 @@ struct Ctx {
   // graph
   bool use_graph = false;
   int graph_pairs = 4;  // iteration pairs a captured graph holds
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
+  // captured graphs, each for one buffer assignment and iteration count (the
+  // pid and key buffers swap every iteration, so a call that starts after an
+  // odd number of steps needs the other assignment's graph)
+  struct GraphEntry {
+    const void* sig[10];  // buffer assignment the graph was captured with
+    int iters;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
   int graph_n = -1;
-  const void* graph_sig[10] = {};  // buffer assignment the graph was captured with
   int block_grid = 0;             // persistent workgroups of the block kernels
   int step_kernel = SPH_STEP_PARTICLE;
   int resort = SPH_RESORT_INCREMENTAL;  // how sph_step re-sorts
@@ -103,13 +110,18 @@ void graph_signature(const Ctx* c, const void** sig) {
   for (int i = 0; i < 10; ++i) sig[i] = v[i];
 }
 
-bool graph_matches(const Ctx* c) {
-  if (!c->gexec) return false;
+// The cached graph of `iters` iterations for the current buffer assignment, or
+// null.
+hipGraphExec_t find_graph(const Ctx* c, int iters) {
   const void* sig[10];
   graph_signature(c, sig);
-  for (int i = 0; i < 10; ++i)
-    if (sig[i] != c->graph_sig[i]) return false;
-  return true;
+  for (const Ctx::GraphEntry& g : c->graphs) {
+    if (g.iters != iters) continue;
+    bool same = true;
+    for (int i = 0; i < 10; ++i) same = same && sig[i] == g.sig[i];
+    if (same) return g.exec;
+  }
+  return nullptr;
 }
 
 int fail(Ctx* c, int code, const std::string& msg) {
@@ -472,25 +484,28 @@ int resort_integrated(Ctx* c, bool flags_ready) {
 }
 
 void destroy_graph(Ctx* c) {
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->gexec = nullptr;
-  c->graph = nullptr;
+  for (Ctx::GraphEntry& g : c->graphs) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
+  c->graphs.clear();
   c->graph_n = -1;
 }
 
-// The graph holds fused iterations with fixed buffer pointers. The pid
-// buffers swap once per iteration, so an even number of iterations is captured
-// (2 * graph_pairs: an even number of swaps returns every pointer to where it
-// started).
-int capture_graph(Ctx* c) {
-  destroy_graph(c);
+// Graphs kept at once: both buffer assignments of the full-length graph and of
+// a shorter one for a call's remaining steps.
+constexpr size_t kGraphCache = 4;
+
+// A graph of `iters` fused iterations with fixed buffer pointers. The pid and
+// key buffers swap once per iteration, so `iters` is even: an even number of
+// swaps returns every pointer to where it started, and capturing runs nothing.
+int capture_graph(Ctx* c, int iters, hipGraphExec_t* out) {
+  if (c->graph_n != c->s.n || c->graphs.size() >= kGraphCache) destroy_graph(c);
   bool was_timing = c->timing;
   c->timing = false;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   // only the last iteration writes the observable fields
   int rc = SPH_OK;
-  const int iters = 2 * c->graph_pairs;
   for (int it = 0; it < iters && rc == SPH_OK; ++it) {
     c->fields_out = it + 1 == iters;
     rc = fused_iteration(c);
@@ -504,15 +519,28 @@ int capture_graph(Ctx* c) {
     return rc;
   }
   if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
-  c->graph = g;
-  e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   if (e != hipSuccess) {
-    destroy_graph(c);
+    (void)hipGraphDestroy(g);
     return hip_fail(c, e, "hipGraphInstantiate");
   }
+  Ctx::GraphEntry ent{};
+  graph_signature(c, ent.sig);
+  ent.iters = iters;
+  ent.graph = g;
+  ent.exec = x;
+  c->graphs.push_back(ent);
   c->graph_n = c->s.n;
-  graph_signature(c, c->graph_sig);
+  *out = x;
   return SPH_OK;
+}
+
+// The graph of `iters` iterations for the current buffer assignment, captured
+// if it is not cached.
+int get_graph(Ctx* c, int iters, hipGraphExec_t* out) {
+  if ((*out = find_graph(c, iters))) return SPH_OK;
+  return capture_graph(c, iters, out);
 }
 
 // Synchronise and report a radix-sort look-back that gave up (see sph_sort.hip).
@@ -1199,9 +1227,9 @@ int sph_prepare(void* ctx) {
   return SPH_OK;
 }
 
-int sph_step(void* ctx, int32_t n_steps) {
-  Ctx* c = as_ctx(ctx);
-  if (!c) return SPH_ERR_INVALID;
+// sph_step, or with graphs_only the capture of the graphs such a call replays
+// (nothing launched, no state changed).
+int step_impl(Ctx* c, int32_t n_steps, bool graphs_only) {
   int rc = check_n(c);
   if (rc) return rc;
   if (!c->sorted) return fail(c, SPH_ERR_STATE, "call sph_prepare first");
@@ -1217,6 +1245,19 @@ int sph_step(void* ctx, int32_t n_steps) {
   // the sorted keys as u32 (k_pack narrows them, k_unpack widens them back)
   c->s.narrow = lazy && c->resort == SPH_RESORT_INCREMENTAL && c->grid.key_bits <= 32 &&
                 c->s.n <= kResortMaxParticles;
+  if (graphs_only) {
+    c->soa_lazy = lazy;
+    rc = SPH_OK;
+    if (c->use_graph && n_steps > 0) {
+      const int per = 2 * c->graph_pairs, rest = (n_steps % per) & ~1;
+      hipGraphExec_t x = nullptr;
+      if (n_steps >= per) rc = get_graph(c, per, &x);
+      if (rc == SPH_OK && rest >= 2) rc = get_graph(c, rest, &x);
+    }
+    c->soa_lazy = false;
+    c->s.narrow = false;
+    return rc;
+  }
   // the packed copy of the sorted state the step kernels gather from (other
   // calls may have changed the state since the last re-sort wrote it)
   {
@@ -1227,6 +1268,7 @@ int sph_step(void* ctx, int32_t n_steps) {
     }
   }
   if (n_steps <= 0) {
+    c->s.narrow = false;
     c->csr_total = -1;
     return SPH_OK;
   }
@@ -1234,10 +1276,20 @@ int sph_step(void* ctx, int32_t n_steps) {
   auto run = [&]() -> int {
     int done = 0, rc2 = SPH_OK;
     if (c->use_graph && !c->timing) {
+      // whole graphs, then the even part of what is left in one shorter graph
+      // (an odd last step runs without)
       const int per = 2 * c->graph_pairs;
-      if (n_steps >= per && !graph_matches(c) && (rc2 = capture_graph(c))) return rc2;
-      for (; done + per <= n_steps; done += per)
-        HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+      hipGraphExec_t x = nullptr;
+      for (; done + per <= n_steps; done += per) {
+        if (!x && (rc2 = get_graph(c, per, &x))) return rc2;
+        HIPCHK(c, hipGraphLaunch(x, c->stream));
+      }
+      const int rest = (n_steps - done) & ~1;
+      if (rest >= 2) {
+        if ((rc2 = get_graph(c, rest, &x))) return rc2;
+        HIPCHK(c, hipGraphLaunch(x, c->stream));
+        done += rest;
+      }
     }
     for (; done < n_steps; ++done) {
       c->fields_out = done + 1 == n_steps;
@@ -1259,6 +1311,19 @@ int sph_step(void* ctx, int32_t n_steps) {
   c->density_ell = false;
   c->fields_prev_order = true;  // computed before the final re-sort
   return SPH_OK;
+}
+
+int sph_step(void* ctx, int32_t n_steps) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  return step_impl(c, n_steps, false);
+}
+
+int sph_step_graphs(void* ctx, int32_t n_steps) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return SPH_ERR_INVALID;
+  if (n_steps < 0) return fail(c, SPH_ERR_INVALID, "negative step count");
+  return step_impl(c, n_steps, true);
 }
 
 int sph_set_graph(void* ctx, int32_t enable) {

@@ -378,6 +378,37 @@ def test_graph_replay_equals_direct(sph, all_scenes, pairs):
             assert np.array_equal(fa[k], fb[k]), k
 
 
+def test_graph_cache_parities_and_remainders(sph, all_scenes):
+    """Graphs are kept per buffer assignment (which alternates every step) and
+    per length: calls that start after odd and even step counts, whole graphs
+    followed by a shorter graph for the even rest and an odd last step, and
+    graphs captured ahead by step_graphs (which runs nothing) all equal direct
+    steps, state and fields."""
+    s = all_scenes["dam_break_16k"]
+    with sph.Context(s.params, s.n) as a, sph.Context(s.params, s.n) as b:
+        for c in (a, b):
+            c.upload(s.rx, s.ry, s.vx, s.vy)
+            c.prepare()
+        b.set_graph(True, 4)
+        for k, ahead in ((3, False), (12, True), (1, False), (20, True), (5, False),
+                         (20, False), (14, True), (9, True)):
+            note(f"[gpu] graph cache: call of {k} steps, captured ahead={ahead}")
+            if ahead:
+                before = b.download()
+                b.step_graphs(k)
+                after = b.download()
+                for q in ("rx", "vx", "pid"):
+                    assert np.array_equal(before[q], after[q]), ("step_graphs ran steps", q)
+            a.step(k)
+            b.step(k)
+            da, db = a.download(), b.download()
+            for q in ("rx", "ry", "vx", "vy", "pid"):
+                assert np.array_equal(da[q], db[q]), (k, q)
+            fa, fb = a.fields(), b.fields()
+            for q in ("ax", "ay", "rho"):
+                assert np.array_equal(fa[q], fb[q]), (k, q)
+
+
 @pytest.mark.parametrize("name", ["dam_break_16k", "random_clusters"])
 def test_measure_fp64(sph, orc, all_scenes, name):
     s = all_scenes[name]
