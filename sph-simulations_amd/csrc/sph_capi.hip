@@ -382,12 +382,13 @@ int find_neighbors(Ctx* c) {
 // sorted order: mover flags -> sorted movers (one launch) -> merge/scatter
 // T -> S -> ranges. Same permutation as sort_keys + gather.
 // flags_ready: the producer of the new keys wrote the mover flags.
-int incremental_resort(Ctx* c, bool flags_ready) {
+int incremental_resort(Ctx* c, bool flags_ready, bool bucketed) {
   DevState& s = c->s;
   ResortBuffers& r = c->rb;
   {
     StageTimer t(c, SPH_T_MOVERS);
-    HIPCHK(c, launch_movers(c->L, s, r, c->sb.keys_in, c->grid.key_bits, flags_ready));
+    HIPCHK(c,
+           launch_movers(c->L, s, r, c->sb.keys_in, c->grid.key_bits, flags_ready, bucketed));
   }
   {
     StageTimer t(c, SPH_T_MERGE);
@@ -408,7 +409,7 @@ int incremental_resort(Ctx* c, bool flags_ready) {
   return SPH_OK;
 }
 
-int resort_integrated(Ctx* c, bool flags_ready = false);
+int resort_integrated(Ctx* c, bool flags_ready = false, bool bucketed = false);
 
 // One fused iteration: forces(ELL) + integrate + keys -> sort -> gather T->S
 // + ranges -> density + EOS + ELL.
@@ -454,23 +455,28 @@ int fused_iteration(Ctx* c) {
       StageTimer t(c, SPH_T_DENSITY);
       HIPCHK(c, launch_density(c->L, fs, NB_ELL, false, true));
     }
-    // the force kernel also flags the movers for the incremental re-sort
+    // the force kernel also flags the movers for the incremental re-sort and
+    // puts them into their owner tiles' buckets
     const bool inc = c->resort == SPH_RESORT_INCREMENTAL;
+    const bool bucketed = inc && kFusedBucket && s.n <= kResortMaxParticles;
     {
       StageTimer t(c, SPH_T_FORCE);
       HIPCHK(c, launch_forces(c->L, fs, NB_ELL, true, c->sb.keys_in,
-                              inc ? mover_sink(c->rb) : MoverSink{}, true));
+                              !inc       ? MoverSink{}
+                              : bucketed ? bucket_sink(c->rb, s, (int)resort_tiles(s.n))
+                                         : mover_sink(c->rb),
+                              true));
     }
-    return resort_integrated(c, inc);
+    return resort_integrated(c, inc, bucketed);
   }
   return resort_integrated(c);
 }
 
 // Re-sort the integrated twin T (keys in sb.keys_in) into S with new ranges.
-int resort_integrated(Ctx* c, bool flags_ready) {
+int resort_integrated(Ctx* c, bool flags_ready, bool bucketed) {
   DevState& s = c->s;
   if (c->resort == SPH_RESORT_INCREMENTAL && s.n <= kResortMaxParticles)
-    return incremental_resort(c, flags_ready);
+    return incremental_resort(c, flags_ready, bucketed);
   int rc = sort_keys(c);
   if (rc) return rc;
   {

@@ -21,6 +21,12 @@ constexpr int kChunkTiles = 64;      // merge tiles per counter chunk (one k_buc
 // k_bucket's tally word counts its workgroups (one per chunk) in 16 bits:
 // larger sets re-sort in full
 constexpr int64_t kResortMaxParticles = 65535ll * kResortTile * kChunkTiles;
+// the single-scene force kernel buckets the movers itself (bucket_movers)
+// instead of a k_bucket launch
+#ifndef SPH_FUSED_BUCKET
+#define SPH_FUSED_BUCKET 1
+#endif
+constexpr bool kFusedBucket = SPH_FUSED_BUCKET != 0;
 
 struct SortBuffers {
   // n passed to the launchers sizes the grids; when n_dev is set the kernels
@@ -136,18 +142,155 @@ struct ResortBuffers {
 };
 
 // Where the producer of the new keys reports the movers: one ballot word per
-// 64 particles of the previous order (flags == nullptr: report nothing).
+// 64 particles of the previous order (flags == nullptr: report nothing). With
+// bucket set the producer also puts every mover into its owner tile's bucket
+// (bucket_movers), the work of k_bucket, so the re-sort needs no bucketing
+// launch; the other members are then the re-sort's tables of the previous
+// order.
 struct MoverSink {
   unsigned long long* flags = nullptr;
+  uint4* bucket = nullptr;
+  unsigned long long* tcount = nullptr;  // packed tile counters, then chunk counters
+  const uint64_t* tfirst = nullptr;      // first key of each tile
+  const int2* cells = nullptr;           // cell ranges of the previous order
+  const uint32_t* pid = nullptr;         // ids in the previous order
+  uint32_t* claim = nullptr;             // set when a bucket overflows
+  uint64_t n_keys = 0;
+  int tiles = 0;
 };
 
-inline MoverSink mover_sink(const ResortBuffers& r) { return MoverSink{r.flags}; }
+inline MoverSink mover_sink(const ResortBuffers& r) {
+  MoverSink m;
+  m.flags = r.flags;
+  return m;
+}
+
+// The report and the bucketing in the producer of the new keys.
+inline MoverSink bucket_sink(const ResortBuffers& r, const DevState& s, int tiles) {
+  MoverSink m = mover_sink(r);
+  m.bucket = r.bucket;
+  m.tcount = s.tile_count;
+  m.tfirst = s.tile_first;
+  m.cells = s.cells;
+  m.pid = s.pid;
+  m.claim = r.claim;
+  m.n_keys = (uint64_t)s.n_keys;
+  m.tiles = tiles;
+  return m;
+}
 
 // One wave's part of the report, called by all its active lanes (the wave's
 // particles are the 64 indices from a multiple of 64, its first lane active).
 __device__ __forceinline__ void record_movers(const MoverSink& m, int i, bool moved) {
   const unsigned long long bal = __ballot(moved);
   if ((threadIdx.x & (kWave - 1)) == 0) m.flags[i / kWave] = bal;
+}
+
+// (key, index) < (tile t's first key, its first index)
+__device__ __forceinline__ bool tile_before(const MoverSink& m, int t, uint64_t k, uint32_t i) {
+  const uint64_t tk = m.tfirst[t];
+  return tk < k || (tk == k && (uint32_t)t * (uint32_t)kResortTile < i);
+}
+
+// Owner tile of a mover (key k, previous index i) whose new cell held no
+// particle: the last tile whose first (key, index) is ordered before it, or
+// tile 0. With all 64 lanes active the wave searches together: a sample of
+// every S-th tile's first key (S = tiles / 256 rounded up; four per lane, one
+// round of loads, counted by ballots), then the tiles between two samples
+// (64 per round). Otherwise the calling lane alone bisects the whole table.
+__device__ __forceinline__ uint32_t owner_of_empty(const MoverSink& m, bool full, uint64_t k,
+                                                   uint32_t i) {
+  const int lane = threadIdx.x & (kWave - 1);
+  if (!full) {
+    int lo = 0, hi = m.tiles;  // tiles [0, lo) are before the mover
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (tile_before(m, mid, k, i)) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo > 0 ? (uint32_t)(lo - 1) : 0u;
+  }
+  const int S = (m.tiles + 4 * kWave - 1) / (4 * kWave);
+  bool lt[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int t = (q * kWave + lane) * S;
+    lt[q] = t < m.tiles && tile_before(m, min(t, m.tiles - 1), k, i);
+  }
+  int c1 = 0;  // samples before the mover (a prefix of the sampled tiles)
+#pragma unroll
+  for (int q = 0; q < 4; ++q) c1 += __popcll(__ballot(lt[q]));
+  if (c1 == 0) return 0u;
+  const int base = (c1 - 1) * S, end = min(base + S, m.tiles);
+  int c2 = 0;  // tiles after the sample that are before the mover
+  for (int t0 = base + 1; t0 < end; t0 += kWave) {
+    const int t = t0 + lane;
+    const bool b = t < end && tile_before(m, min(t, m.tiles - 1), k, i);
+    c2 += __popcll(__ballot(b));
+  }
+  return (uint32_t)(base + c2);
+}
+
+// The bucketing of k_bucket done by the wave that found its movers: the count
+// of the wave's movers by previous index (low half of its tile's counter),
+// then each mover's owner tile -- the
+// tile of the last previous-order particle ordered before it by (key,
+// previous index): from the previous range [s, e) of its new cell k, s - 1 if
+// its index i < s (its old key was smaller) and e - 1 otherwise, or by
+// owner_of_empty when that cell was empty -- one 64-bit atomic on the owner's
+// counter (high half) for the mover's slot in the owner's bucket, and the
+// entry (key, previous index, pid, integrated state). A bucket that is full
+// sets the claim word: the step then orders the movers in k_chunks. Only the
+// tile counters are added to here: the chunk counters and the step's mover
+// count are sums that k_chunks and the merge form afterwards. (Adds to the
+// same word from every wave serialise at the memory: the 64 chunk counters
+// of a 1M set took 20-25 us per step that way.)
+// Called by all active lanes of a wave; st is the lane's integrated state.
+__device__ __forceinline__ void bucket_movers(const MoverSink& m, int i, uint64_t k,
+                                              bool moved, float4 st) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned long long bal = __ballot(moved);
+  if (bal == 0ull) return;
+  const unsigned long long active = __ballot(true);
+  const int tile = i / kResortTile;
+  if (lane == 0) atomicAdd(&m.tcount[tile], (unsigned long long)__popcll(bal));
+  int2 cr = make_int2(0, 0);
+  uint32_t pid = 0u;
+  if (moved) {
+    cr = m.cells[min(k, m.n_keys - 1)];
+    pid = m.pid[i];
+  }
+  uint32_t own = 0u;
+  if (cr.x >= 0) {
+    const int p = i < cr.x ? cr.x - 1 : cr.y - 1;
+    own = p < 0 ? 0u : (uint32_t)p / (uint32_t)kResortTile;
+  }
+  unsigned long long empty = __ballot(moved && cr.x < 0);
+  while (empty != 0ull) {  // wave-uniform
+    const int l = __ffsll((long long)empty) - 1;
+    empty &= empty - 1ull;
+    const uint64_t kl = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(k >> 32), l)
+                         << 32) |
+                        (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, l);
+    const uint32_t il = (uint32_t)__builtin_amdgcn_readlane(i, l);
+    if (active == ~0ull) {
+      const uint32_t o = owner_of_empty(m, true, kl, il);
+      if (lane == l) own = o;
+    } else if (lane == l) {
+      own = owner_of_empty(m, false, kl, il);
+    }
+  }
+  if (!moved) return;
+  constexpr unsigned long long kOwnOne = 1ull << 32;
+  const uint32_t slot = (uint32_t)(atomicAdd(&m.tcount[own], kOwnOne) >> 32);
+  if (slot < (uint32_t)kBucketCap) {
+    uint4* e = m.bucket + 2 * ((size_t)own * kBucketCap + slot);
+    e[0] = make_uint4((uint32_t)k, (uint32_t)(k >> 32), (uint32_t)i, pid);
+    e[1] = make_uint4(__float_as_uint(st.x), __float_as_uint(st.y), __float_as_uint(st.z),
+                      __float_as_uint(st.w));
+  } else {
+    *m.claim = 1u;
+  }
 }
 
 struct Launch {
@@ -195,11 +338,14 @@ int64_t resort_tiles(int n);
 // words of DevState::tile_count for n particles: the tiles, then the chunks
 int64_t resort_count_words(int n);
 // movers into their owner tiles' buckets with the tile and chunk counters
-// (k_bucket; when a bucket overflowed its last workgroup sorts the movers into
-// skey / sidx instead); k_mover_flags first when the producer of knew wrote no
-// flag words
+// (k_bucket; when a bucket overflowed the workgroup that claimed the step sorts
+// the movers into skey / sidx instead); k_mover_flags first when the producer
+// of knew wrote no flag words. bucketed: the producer bucketed the movers
+// itself (bucket_sink) and added only to the tile counters; k_chunks sums them
+// into the chunk counters and runs the sorted path when a bucket overflowed
 hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers& r,
-                         const uint64_t* knew, int key_bits, bool flags_ready);
+                         const uint64_t* knew, int key_bits, bool flags_ready,
+                         bool bucketed = false);
 // soa: also write the separate arrays rx, ry, vx, vy (else only s.xyuv);
 // perm: also write the permutation s.perm (read only through the API)
 hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,

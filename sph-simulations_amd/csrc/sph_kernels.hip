@@ -349,7 +349,10 @@ __global__ __launch_bounds__(BLK) void k_forces(
     keys_out[i] = (K)kn;
     // particles whose key changes (tiles are BLK-aligned, so wave w of the
     // tile covers flag word i / 64; waves past n write none)
-    if (movers.flags) record_movers(movers, i, kn != ko);
+    if (movers.flags) {
+      record_movers(movers, i, kn != ko);
+      if (movers.bucket) bucket_movers(movers, i, kn, kn != ko, make_float4(x, y, u, v));
+    }
   }
 }
 

@@ -23,6 +23,10 @@
 //             whose mover first finds it full (and first claims the step)
 //             orders every mover itself (ordered compaction, then a stable
 //             radix sort) for the sorted path.
+//   k_chunks  in place of k_bucket when the force kernel bucketed the movers
+//             itself (bucket_movers, adding to the tile counters only): the
+//             chunk counters as sums of the tile counters, and the sorted path
+//             when a bucket overflowed
 //   k_merge   one workgroup per MT tiles of the previous order (four by
 //             default; a thread per particle of each): clears the old cell /
 //             block ranges its movers left; bucket path: its prefixes from the
@@ -468,6 +472,40 @@ __global__ __launch_bounds__(KB_TW) void k_bucket(
   if (tid == 0) *r.mode = 0u;
 }
 
+// After a producer that bucketed the movers itself (bucket_movers), which
+// adds only to the tile counters: each wave sums one chunk's kChunkTiles tile
+// counters (both halves) into the chunk counter the merge reads; and when a
+// bucket overflowed (the claim word), workgroup 0 then orders every mover for
+// the sorted path, with the same ordered compaction and stable radix sort as
+// k_bucket's.
+template <typename K>
+__global__ __launch_bounds__(KB_TW) void k_chunks(int n, int tiles, int passes,
+                                                  const K* __restrict__ knew,
+                                                  const unsigned long long* __restrict__ flags,
+                                                  unsigned long long* __restrict__ tcount,
+                                                  ResortBuffers r) {
+  __shared__ uint64_t s_mem64[KB_LDS / 2];
+  __shared__ uint32_t s_w[KB_WAVES];
+  static_assert(kChunkTiles == kWave, "one wave per chunk, one lane per tile");
+  const int lane = threadIdx.x & (kWave - 1);
+  const int chunks = (tiles + kChunkTiles - 1) / kChunkTiles;
+  const int ch = (int)blockIdx.x * KB_WAVES + (int)threadIdx.x / kWave;
+  const int t = ch * kChunkTiles + lane;
+  const unsigned long long v = tcount[min(t, tiles - 1)];
+  const bool in = ch < chunks && t < tiles;
+  const uint32_t lo = wave_sum(in ? (uint32_t)v : 0u), hi = wave_sum(in ? (uint32_t)(v >> 32) : 0u);
+  if (ch < chunks && lane == 0) tcount[tiles + ch] = ((unsigned long long)hi << 32) | lo;
+  if (blockIdx.x != 0 || *r.claim == 0u) return;
+  uint32_t* s_mem = reinterpret_cast<uint32_t*>(s_mem64);
+  const uint32_t m = compact_movers(n, tiles, flags, r.tile_base, r.midx, s_mem, s_w);
+  __syncthreads();
+  sort_movers((int)m, passes, knew, r, s_mem);
+  if (threadIdx.x == 0) {
+    *r.n_movers = (int)m;
+    *r.mode = 0u;
+  }
+}
+
 // ------------------------------------------------------------------- merge
 // Per tile of the previous order [j0, j0 + TW): its window of sorted movers
 // [lo, hi) holds the movers ordered after the tile's first particle and
@@ -628,6 +666,9 @@ __global__ __launch_bounds__(TW) void k_merge(
         slo += (uint32_t)o;
         shi = whi[q];
       }
+      // the last workgroup's prefix is the step's mover count (producers that
+      // bucket the movers themselves do not add it up)
+      if (t0 + MT >= tiles && tid == 0) *r.n_movers = (int)slo;
     } else {
 #pragma unroll
       for (int q = 0; q < MT; ++q) {
@@ -782,11 +823,25 @@ int64_t resort_count_words(int n) {
 }
 
 hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers& r,
-                         const uint64_t* knew, int key_bits, bool flags_ready) {
+                         const uint64_t* knew, int key_bits, bool flags_ready, bool bucketed) {
   if (s.n <= 0) return hipSuccess;
   const int tiles = (int)resort_tiles(s.n);
   const int passes = (key_bits + 7) / 8;
   if (passes < 1 || passes > kMaxPasses) return hipErrorInvalidValue;
+  if (bucketed) {
+    // the producer bucketed the movers: the chunk counters, and the sorted
+    // path when claimed
+    if (!flags_ready) return hipErrorInvalidValue;
+    const int chunks = (tiles + kChunkTiles - 1) / kChunkTiles;
+    const int grid = (chunks + KB_WAVES - 1) / KB_WAVES;
+    if (s.narrow)
+      k_chunks<uint32_t><<<grid, KB_TW, 0, L.stream>>>(s.n, tiles, passes, s.nkeys_in, r.flags,
+                                                       s.tile_count, r);
+    else
+      k_chunks<uint64_t><<<grid, KB_TW, 0, L.stream>>>(s.n, tiles, passes, knew, r.flags,
+                                                       s.tile_count, r);
+    return hipGetLastError();
+  }
   if (!flags_ready) {
     // producers without their own report write u64 keys
     if (s.narrow) return hipErrorInvalidValue;

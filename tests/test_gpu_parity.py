@@ -627,6 +627,63 @@ def test_incremental_resort_mid_movers(sph, orc, pattern):
             assert np.array_equal(st, ost) and np.array_equal(en, oen)
 
 
+@pytest.mark.parametrize("steps", [1, 2])
+def test_incremental_resort_empty_cell_owners(sph, orc, steps):
+    """Movers into cells that held no particle, few enough that every bucket
+    holds its owner's movers (bucket path): their owner tiles come from the
+    search of the tile-first keys, by the whole wave, or by one lane in the
+    last, partial wave (10,000 particles: its last wave has 16 lanes, and the
+    last 16 particles in sorted order are among the movers). One- and two-step
+    calls (64- and 32-bit keys); the permutation is the oracle's stable sort."""
+    s = sph.scenes.dam_break(100, 100, box=(1.0, 1.0), seed=21)
+    assert s.n % 64 == 16
+    rng = np.random.default_rng(23)
+    dt, h = s.params.dt, s.params.h
+    k0 = orc.keys(s.params, s.rx, s.ry)
+    order, _ = orc.stable_sort(k0)
+    last = order[-16:]
+    pick = np.unique(np.concatenate([last, rng.choice(s.n, 60, replace=False)]))
+    # targets in empty cells of the two Morton quadrants of 64 x 64 cells that
+    # the 77 x 77-cell block only partly fills (cells 80-125 by 2-61 and 2-61
+    # by 80-125), so their keys interleave with the block's and their owners
+    # spread over many tiles (at most 40 per bucket on this scene)
+    side = rng.random(pick.size) < 0.5
+    a = (80 + 46 * rng.random(pick.size)) * h
+    b = (2 + 60 * rng.random(pick.size)) * h
+    tx = np.where(side, a, b)
+    ty = np.where(side, b, a)
+    s.vx[pick] = ((tx - s.rx[pick]) / dt).astype(np.float32)
+    s.vy[pick] = ((ty - s.ry[pick]) / dt).astype(np.float32)
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.prepare()
+        k_before = ctx.sorted_keys()
+        pid_before = ctx.download()["pid"]
+        ctx.step(steps)
+        after = ctx.download()
+        perm = ctx.permutation()
+        m = ctx.movers()
+        path = ctx.resort_path()
+        note(f"[gpu] empty-cell owners, {steps}-step call: {m} movers ({path})")
+        assert path == "bucket"
+        # the last step's movers, from the state it started from
+        x_prev = np.empty_like(after["rx"]); y_prev = np.empty_like(after["ry"])
+        x_prev[perm] = after["rx"]; y_prev[perm] = after["ry"]
+        k_new = orc.keys(s.params, x_prev, y_prev)
+        if steps == 1:
+            assert m == int(np.sum(k_new != k_before))
+            assert m >= pick.size
+            # the last wave's particles were all sent away
+            moved_ids = set(pid_before[np.nonzero(k_new != k_before)[0]].tolist())
+            assert set(last.tolist()) <= moved_ids
+        operm, osk = orc.stable_sort(k_new)
+        assert np.array_equal(perm, operm)
+        assert np.array_equal(ctx.sorted_keys(), osk)
+        st, en = ctx.cell_ranges()
+        ost, oen = orc.cell_ranges(osk, int(ctx.grid().n_keys))
+        assert np.array_equal(st, ost) and np.array_equal(en, oen)
+
+
 @pytest.mark.parametrize("side,sigma", [(100, 40.0), (512, 400.0)])
 def test_incremental_resort_many_movers(sph, orc, side, sigma):
     """Most particles moving 0.5 to 5 cells per step, two steps in a row: 10 K
