@@ -385,14 +385,18 @@ int find_neighbors(Ctx* c) {
 int incremental_resort(Ctx* c, bool flags_ready, bool bucketed) {
   DevState& s = c->s;
   ResortBuffers& r = c->rb;
-  {
+  if (!bucketed || !kMergePrefix) {
     StageTimer t(c, SPH_T_MOVERS);
     HIPCHK(c,
            launch_movers(c->L, s, r, c->sb.keys_in, c->grid.key_bits, flags_ready, bucketed));
   }
   {
+    // after a bucketing producer the merge also forms the chunk counters and
+    // runs the sorted path (no launch between the force kernel and the merge)
     StageTimer t(c, SPH_T_MERGE);
-    HIPCHK(c, launch_merge(c->L, s, r, r.keys_alt, !c->soa_lazy, c->fields_out));
+    const bool pub = bucketed && kMergePrefix;
+    HIPCHK(c, launch_merge(c->L, s, r, r.keys_alt, !c->soa_lazy, c->fields_out,
+                           pub ? c->sb.keys_in : nullptr, c->grid.key_bits));
   }
   if (s.narrow) {
     std::swap(s.nkeys, s.nkeys_alt);
@@ -555,8 +559,9 @@ int sync_and_check(Ctx* c) {
   HIPCHK(c, hipMemcpyAsync(&st, c->sb.status, sizeof(st), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (st != 0)
-    return fail(c, SPH_ERR_HIP, "radix sort look-back gave up waiting (device status " +
-                                    std::to_string(st) + ")");
+    return fail(c, SPH_ERR_HIP,
+                std::string(st & 8u ? "incremental re-sort merge" : "radix sort look-back") +
+                    " gave up waiting (device status " + std::to_string(st) + ")");
   return SPH_OK;
 }
 
@@ -655,6 +660,8 @@ int alloc_state(Ctx* c, int64_t n_max, int64_t n_keys, int key_bits, int block_b
     r.claim = r.mode + 2;
     r.n_last = reinterpret_cast<int*>(r.mode + 3);
     r.mode_last = r.mode + 4;
+    r.ready = r.mode + 5;
+    r.status = b.status;
   }
   b.blocks = s.blocks;
   b.block_bits = s.block_bits;

@@ -27,6 +27,12 @@ constexpr int64_t kResortMaxParticles = 65535ll * kResortTile * kChunkTiles;
 #define SPH_FUSED_BUCKET 1
 #endif
 constexpr bool kFusedBucket = SPH_FUSED_BUCKET != 0;
+// after it, the merge forms the chunk counters and runs the sorted path
+// itself (no k_chunks launch)
+#ifndef SPH_MERGE_PREFIX
+#define SPH_MERGE_PREFIX 1
+#endif
+constexpr bool kMergePrefix = SPH_MERGE_PREFIX != 0;
 
 struct SortBuffers {
   // n passed to the launchers sizes the grids; when n_dev is set the kernels
@@ -129,6 +135,8 @@ struct ResortBuffers {
   uint32_t* claim;
   int* n_last;
   uint32_t* mode_last;
+  uint32_t* ready;   // the merge's sorted movers are complete (reset by k_ranges)
+  uint32_t* status;  // device status word: bits for waits that gave up
   uint64_t* keys_alt;              // second sorted-key buffer (ping-pong)
   // sorted path (a bucket overflowed): movers in previous-index order, then
   // sorted by (key, previous index), and the movers before each 256-particle tile
@@ -347,9 +355,16 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
                          const uint64_t* knew, int key_bits, bool flags_ready,
                          bool bucketed = false);
 // soa: also write the separate arrays rx, ry, vx, vy (else only s.xyuv);
-// perm: also write the permutation s.perm (read only through the API)
+// perm: also write the permutation s.perm (read only through the API).
+// knew (the new keys, key_bits significant) given: the producer bucketed the
+// movers and added only to the tile counters (bucket_movers), and no
+// k_bucket / k_chunks ran: the merge forms the chunk counters itself (each
+// published by one workgroup, waited for by the later ones) and, when a
+// bucket overflowed, its workgroup 0 orders the movers for the sorted path
+// while the others wait
 hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
-                        uint64_t* keys_dst, bool soa = true, bool perm = true);
+                        uint64_t* keys_dst, bool soa = true, bool perm = true,
+                        const uint64_t* knew = nullptr, int key_bits = 0);
 // cell and block ranges of the sorted keys s.keys_sorted; clears the histogram
 // and resets the re-sort's step words in r for the next step
 hipError_t launch_ranges(const Launch& L, const DevState& s, const ResortBuffers& r,

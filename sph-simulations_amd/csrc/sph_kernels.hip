@@ -154,6 +154,7 @@ __global__ __launch_bounds__(BLK) void k_ranges(int n, const K* __restrict__ key
     *r.n_movers = 0;
     *r.mode = 1u;
     *r.claim = 0u;
+    *r.ready = 0u;
   }
   // each slot's key, and one neighbor key per lane (the one before for lane
   // 0, after for the others) loaded together; the neighbors inside the wave

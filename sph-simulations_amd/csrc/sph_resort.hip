@@ -164,13 +164,18 @@ constexpr int FB_WPT = 12;  // flag words per thread and batch of the ordered co
 static_assert(KB_TW == 256, "the digit scans take one digit per thread");
 
 // Movers in previous-index order into midx, and the movers before each tile
-// (tile_base); returns their number. Per batch each wave takes FB_WPT * 64
+// (tile_base); returns their number. Per batch each wave takes WPT * 64
 // consecutive flag words (lane l: words q * 64 + l, coalesced); the prefixes
-// come from wave scans plus the waves' offsets. s_pre: FB_WPT * KB_TW words,
-// s_w: KB_WAVES words (LDS).
-__device__ uint32_t compact_movers(int n, int tiles, const unsigned long long* __restrict__ flags,
-                                   uint32_t* __restrict__ tile_base,
-                                   uint32_t* __restrict__ midx, uint32_t* s_pre, uint32_t* s_w) {
+// come from wave scans plus the waves' offsets. s_pre: WPT * KB_TW words,
+// s_w: KB_WAVES words (LDS). (The merge's copy takes fewer words per batch to
+// keep its register count.)
+template <int WPT = FB_WPT>
+__device__ __forceinline__ uint32_t compact_movers(int n, int tiles,
+                                                   const unsigned long long* __restrict__ flags,
+                                                   uint32_t* __restrict__ tile_base,
+                                                   uint32_t* __restrict__ midx, uint32_t* s_pre,
+                                                   uint32_t* s_w) {
+  constexpr int FB_WPT = WPT;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int words = 4 * tiles;
   uint32_t done = 0;
@@ -219,15 +224,14 @@ __device__ uint32_t compact_movers(int n, int tiles, const unsigned long long* _
 // previous indices midx holds in previous-index order, by their new keys (read
 // through midx in the first pass); result in skey / sidx. One workgroup; s:
 // KB_WAVES * 256 + 512 words of LDS.
-template <typename K>
-__device__ void sort_movers(int m, int passes, const K* __restrict__ knew,
-                            const ResortBuffers& r, uint32_t* s) {
+template <typename K, int HB = 8>  // HB: keys per thread in flight while counting
+__device__ __forceinline__ void sort_movers(int m, int passes, const K* __restrict__ knew,
+                                            const ResortBuffers& r, uint32_t* s) {
   uint32_t* s_cnt = s;                    // [KB_WAVES][256] per-wave digit counts
   uint32_t* s_off = s + KB_WAVES * 256;  // [256] running digit offsets
   uint32_t* s_tot = s_off + 256;          // [256] digit totals of a chunk
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
-  constexpr int HB = 8;  // keys per thread in flight while counting
   for (int p = 0; p < passes; ++p) {
     const uint64_t* kin = (p & 1) ? r.ktmp : r.mkey;
     const uint32_t* vin = p == 0 ? r.midx : ((p & 1) ? r.vtmp : r.midx);
@@ -530,10 +534,62 @@ constexpr int BH = TW / MT;    // threads that hold one tile's bucket entries
 static_assert(kBucketCap <= BH && MT * kBucketCap <= WIN,
               "a tile's bucket maps to its share of the threads and of the window buffer");
 
-template <typename K>
+// The merge's LDS, also the scratch of the sorted path that workgroup 0 of a
+// claimed step runs first when the merge forms the chunk counters itself.
+struct MergeShared {
+  uint64_t tk[MT][TW];
+  uint64_t mk[WIN];
+  uint32_t mi[WIN];
+  uint64_t ak[MT][kBucketCap];
+  uint32_t ai[MT][kBucketCap];
+  alignas(16) uint32_t ms[MT][kBucketCap][4];
+  uint32_t mp[MT][kBucketCap];
+  uint32_t wpre[MT][TW / kWave + 1];
+  unsigned long long fw[MT][TW / kWave];
+  uint32_t plo[TW / kWave], phi[TW / kWave];
+  uint32_t win[MT + 1];
+};
+union MergeLds {
+  MergeShared m;
+  struct {
+    uint32_t mem[KB_LDS];
+    uint32_t w[KB_WAVES];
+  } sort;
+};
+
+// Published chunk counters (bit 63 set by the publishing workgroup).
+constexpr unsigned long long kPublished = 1ull << 63;
+constexpr uint32_t kMaxChunkPolls = 1u << 20;   // ~60 ms of polling at ~128 clk each
+constexpr uint32_t kMaxReadyPolls = 1u << 24;   // ~1 s: the sorted path's one-workgroup sort
+constexpr uint32_t kStatusMergeWait = 8u;       // a merge wait gave up (status word)
+
+__device__ __forceinline__ void st_agent64(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The counter a workgroup of lower id publishes, without its flag bit; gives
+// up (status bit, value 0) if it never appears.
+__device__ __forceinline__ unsigned long long wait_published(unsigned long long* p,
+                                                             uint32_t* status) {
+  unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t polls = 0;
+  while (!(v & kPublished)) {
+    if (++polls >= kMaxChunkPolls) {
+      atomicOr(status, kStatusMergeWait);
+      return 0ull;
+    }
+    __builtin_amdgcn_s_sleep(2);
+    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return v & ~kPublished;
+}
+
+
+template <typename K, bool PUB>
 __global__ __launch_bounds__(TW) void k_merge(
-    int n, const K* __restrict__ kold, unsigned long long* __restrict__ flags,
-    const uint64_t* __restrict__ tfirst, const unsigned long long* __restrict__ tcount,
+    int n, int passes, const K* __restrict__ knew, const K* __restrict__ kold,
+    unsigned long long* __restrict__ flags,
+    const uint64_t* __restrict__ tfirst, unsigned long long* __restrict__ tcount,
     const float* __restrict__ tx, const float* __restrict__ ty, const float* __restrict__ tvx,
     const float* __restrict__ tvy, const uint32_t* __restrict__ pid_src, float* __restrict__ rx,
     float* __restrict__ ry, float* __restrict__ vx, float* __restrict__ vy,
@@ -541,22 +597,58 @@ __global__ __launch_bounds__(TW) void k_merge(
     int2* __restrict__ cells, int2* __restrict__ blocks, int block_bits, uint64_t n_keys,
     float4* __restrict__ xyuv, const float4* __restrict__ t4, ResortBuffers r) {
   constexpr bool NARROW = sizeof(K) == 4;  // a narrow step: T packed in t4
-  __shared__ uint64_t s_tk[MT][TW];  // the tiles' previous keys (all ones past n)
+  // (workgroup 0 of a claimed PUB step first orders the movers in the same LDS)
+  __shared__ MergeLds lds;
+  auto& s_tk = lds.m.tk;  // the tiles' previous keys (all ones past n)
   // window keys and previous indices: bucket path tile q at q * kBucketCap,
   // sorted path one tile's chunk at a time
-  __shared__ uint64_t s_mk[WIN];
-  __shared__ uint32_t s_mi[WIN];
-  __shared__ uint64_t s_ak[MT][kBucketCap];  // bucket path: keys, arrival order
-  __shared__ uint32_t s_ai[MT][kBucketCap];
-  __shared__ float4 s_ms[MT][kBucketCap];    // bucket path: state, sorted order
-  __shared__ uint32_t s_mp[MT][kBucketCap];
-  __shared__ uint32_t s_wpre[MT][TW / kWave + 1];  // movers of a tile before each wave
-  __shared__ unsigned long long s_fw[MT][TW / kWave];  // the tiles' flag words
-  __shared__ uint32_t s_plo[TW / kWave], s_phi[TW / kWave];  // counter sums per wave
-  __shared__ uint32_t s_win[MT + 1];  // sorted path: the windows' ends
+  auto& s_mk = lds.m.mk;
+  auto& s_mi = lds.m.mi;
+  auto& s_ak = lds.m.ak;  // bucket path: keys, arrival order
+  auto& s_ai = lds.m.ai;
+  auto& s_ms = *reinterpret_cast<float4 (*)[MT][kBucketCap]>(&lds.m.ms);  // bucket path:
+  auto& s_mp = lds.m.mp;                                                  // sorted order
+  auto& s_wpre = lds.m.wpre;  // movers of a tile before each wave
+  auto& s_fw = lds.m.fw;      // the tiles' flag words
+  auto& s_plo = lds.m.plo;    // counter sums per wave
+  auto& s_phi = lds.m.phi;
+  auto& s_win = lds.m.win;  // sorted path: the windows' ends
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int tiles = (n + TW - 1) / TW;
   const int t0 = (int)blockIdx.x * MT;
+  uint32_t claimed = 0u;
+  if constexpr (PUB) {
+    // chunk blockIdx.x's counter from its tile counters, published with a
+    // flag bit before anything else (a workgroup waits only for chunks of
+    // lower-id workgroups, so every chunk it waits for is published by a
+    // workgroup that is resident or done)
+    const int cb = (int)blockIdx.x;
+    if (wave == 0 && cb * kChunkTiles < tiles) {
+      const int t = cb * kChunkTiles + lane;
+      const unsigned long long cv = tcount[min(t, tiles - 1)];
+      const bool in = t < tiles;
+      const uint32_t lo = wave_sum(in ? (uint32_t)cv : 0u);
+      const uint32_t hi = wave_sum(in ? (uint32_t)(cv >> 32) : 0u);
+      if (lane == 0) st_agent64(&tcount[tiles + cb], kPublished | ((uint64_t)hi << 32) | lo);
+    }
+    claimed = *r.claim;
+    if (claimed != 0u && blockIdx.x == 0) {
+      // a bucket overflowed: order every mover here first (ordered compaction
+      // and a stable radix sort, as k_bucket's sorted path), then release the
+      // other workgroups, which wait for it before they place their windows
+      const uint32_t m =
+          compact_movers<4>(n, tiles, flags, r.tile_base, r.midx, lds.sort.mem, lds.sort.w);
+      __syncthreads();
+      sort_movers<K, 2>((int)m, passes, knew, r, lds.sort.mem);
+      __syncthreads();
+      if (tid == 0) {
+        *r.n_movers = (int)m;
+        *r.mode = 0u;
+        __hip_atomic_store(r.ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __syncthreads();
+    }
+  }
   // every load that depends only on the particles, issued together
   unsigned long long fw[MT];
   uint64_t k[MT];
@@ -594,7 +686,6 @@ __global__ __launch_bounds__(TW) void k_merge(
 #pragma unroll
   for (int q = 0; q < MT; ++q) own[q] = tcount[min(t0 + q, tiles - 1)];
   unsigned long long pc = tid < tin ? tcv : 0ull;
-  for (int q = tid; q < ch; q += TW) pc += tcount[tiles + q];
   // the first entries of each tile's bucket (thread qb * BH + rb: entry rb of
   // tile t0 + qb), loaded whichever path the step took (entries past the
   // window's length are stale and not used)
@@ -602,7 +693,15 @@ __global__ __launch_bounds__(TW) void k_merge(
   const int tq = min(t0 + qb, tiles - 1);
   const uint4* be = r.bucket + 2 * ((size_t)tq * kBucketCap + (rb & (kBucketFast - 1)));
   uint4 b0 = be[0], b1 = be[1];
-  const bool from_bucket = *r.mode != 0u;
+  bool from_bucket;
+  if constexpr (PUB) {
+    // the earlier chunks' counters, waited for (see the start of the kernel)
+    for (int q = tid; q < ch; q += TW) pc += wait_published(&tcount[tiles + q], r.status);
+    from_bucket = claimed == 0u;
+  } else {
+    for (int q = tid; q < ch; q += TW) pc += tcount[tiles + q];
+    from_bucket = *r.mode != 0u;
+  }
   // k_ranges writes the range of every cell and block that holds particles in
   // the new order, so of the previous ranges only those of cells and blocks
   // that may have become empty are cleared: the ones a mover left
@@ -624,6 +723,22 @@ __global__ __launch_bounds__(TW) void k_merge(
         s_wpre[q][wave + 1] = (uint32_t)__popcll(fw[q]);
         s_fw[q][wave] = fw[q];
       }
+    }
+  }
+  if constexpr (PUB) {
+    if (!from_bucket) {
+      // the sorted movers of workgroup 0
+      if (tid == 0) {
+        uint32_t polls = 0;
+        while (__hip_atomic_load(r.ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+          if (++polls >= kMaxReadyPolls) {
+            atomicOr(r.status, kStatusMergeWait);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      __syncthreads();
     }
   }
   if (!from_bucket && tid <= MT) {
@@ -866,22 +981,37 @@ hipError_t launch_movers(const Launch& L, const DevState& s, const ResortBuffers
   return hipGetLastError();
 }
 
-hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
-                        uint64_t* keys_dst, bool soa, bool perm) {
-  if (s.n <= 0) return hipSuccess;
+template <bool PUB>
+static hipError_t merge_go(const Launch& L, const DevState& s, const ResortBuffers& r,
+                           const uint64_t* knew, int passes, uint64_t* keys_dst, bool soa,
+                           bool perm) {
   const int tiles = (int)resort_tiles(s.n);
   const int grid = (tiles + MT - 1) / MT;
   if (s.narrow)  // the step's narrow keys: nkeys into nkeys_alt
-    k_merge<uint32_t><<<grid, TW, 0, L.stream>>>(
-        s.n, s.nkeys, r.flags, s.tile_first, s.tile_count, s.tx, s.ty, s.tvx, s.tvy, s.pid,
-        soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt, s.nkeys_alt, perm ? s.perm : nullptr,
-        s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv, s.t4, r);
+    k_merge<uint32_t, PUB><<<grid, TW, 0, L.stream>>>(
+        s.n, passes, s.nkeys_in, s.nkeys, r.flags, s.tile_first, s.tile_count, s.tx, s.ty,
+        s.tvx, s.tvy, s.pid, soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt, s.nkeys_alt,
+        perm ? s.perm : nullptr, s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv,
+        s.t4, r);
   else
-    k_merge<uint64_t><<<grid, TW, 0, L.stream>>>(
-        s.n, s.keys_sorted, r.flags, s.tile_first, s.tile_count, s.tx, s.ty, s.tvx, s.tvy, s.pid,
-        soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt, keys_dst, perm ? s.perm : nullptr,
-        s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv, nullptr, r);
+    k_merge<uint64_t, PUB><<<grid, TW, 0, L.stream>>>(
+        s.n, passes, knew, s.keys_sorted, r.flags, s.tile_first, s.tile_count, s.tx, s.ty,
+        s.tvx, s.tvy, s.pid, soa ? s.rx : nullptr, s.ry, s.vx, s.vy, s.pid_alt, keys_dst,
+        perm ? s.perm : nullptr, s.cells, s.blocks, s.block_bits, (uint64_t)s.n_keys, s.xyuv,
+        nullptr, r);
   return hipGetLastError();
+}
+
+hipError_t launch_merge(const Launch& L, const DevState& s, const ResortBuffers& r,
+                        uint64_t* keys_dst, bool soa, bool perm, const uint64_t* knew,
+                        int key_bits) {
+  if (s.n <= 0) return hipSuccess;
+  if (knew) {
+    const int passes = (key_bits + 7) / 8;
+    if (passes < 1 || passes > kMaxPasses) return hipErrorInvalidValue;
+    return merge_go<true>(L, s, r, knew, passes, keys_dst, soa, perm);
+  }
+  return merge_go<false>(L, s, r, nullptr, 0, keys_dst, soa, perm);
 }
 
 }  // namespace sphamd
