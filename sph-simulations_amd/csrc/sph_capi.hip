@@ -56,6 +56,14 @@ struct Ctx {
   // inside sph_step on the per-particle kernels: the re-sort writes only the
   // packed copy of the state, the separate arrays are restored when it ends
   bool soa_lazy = false;
+  // the packed copy (and, with packed_ok_narrow, the narrow keys) still holds
+  // the sorted state a lazy sph_step call left: the next call needs no k_pack.
+  // Every entry point but the step calls and a few that only read clears it
+  // (as_ctx); never set once a device view was handed out (the caller may then
+  // write the state between calls without an API call)
+  bool packed_ok = false;
+  bool packed_ok_narrow = false;
+  bool view_out = false;
   // the fused step writes the accelerations, the separate density and
   // pressure arrays and the incremental re-sort's permutation (off for the
   // steps of a sph_step call but its last: only the last step's are
@@ -276,7 +284,15 @@ int dev_alloc(Ctx* c, T** p, int64_t count) {
   return SPH_OK;
 }
 
-Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
+// The context of an entry point that may change the state: the packed copy
+// then no longer counts as current.
+Ctx* as_ctx(void* p) {
+  Ctx* c = static_cast<Ctx*>(p);
+  if (c) c->packed_ok = false;
+  return c;
+}
+// The context of the step calls and of calls that change no state.
+Ctx* as_ctx_keep(void* p) { return static_cast<Ctx*>(p); }
 
 // ---- timing helpers
 hipEvent_t take_event(Ctx* c) {
@@ -958,7 +974,7 @@ int sph_upload(void* ctx, int64_t n, const float* rx, const float* ry, const flo
 }
 
 int sph_download(void* ctx, float* rx, float* ry, float* vx, float* vy, uint32_t* pid) {
-  Ctx* c = as_ctx(ctx);
+  Ctx* c = as_ctx_keep(ctx);
   if (!c) return SPH_ERR_INVALID;
   DevState& s = c->s;
   size_t fb = (size_t)s.n * sizeof(float);
@@ -1272,8 +1288,11 @@ int step_impl(Ctx* c, int32_t n_steps, bool graphs_only) {
     return rc;
   }
   // the packed copy of the sorted state the step kernels gather from (other
-  // calls may have changed the state since the last re-sort wrote it)
-  {
+  // calls may have changed the state since the last re-sort wrote it; not
+  // after a lazy step call with nothing in between)
+  const bool pack = !(c->packed_ok && (!c->s.narrow || c->packed_ok_narrow));
+  c->packed_ok = false;
+  if (pack) {
     const hipError_t e = launch_pack(c->L, c->s);
     if (e != hipSuccess) {
       c->s.narrow = false;
@@ -1318,8 +1337,11 @@ int step_impl(Ctx* c, int32_t n_steps, bool graphs_only) {
     const hipError_t e = launch_unpack(c->L, c->s);
     if (rc == SPH_OK && e != hipSuccess) rc = hip_fail(c, e, "unpack");
   }
+  const bool narrow = c->s.narrow;
   c->s.narrow = false;
   if (rc) return rc;
+  c->packed_ok = lazy && !c->view_out;
+  c->packed_ok_narrow = narrow;
   c->csr_total = -1;
   c->density_ell = false;
   c->fields_prev_order = true;  // computed before the final re-sort
@@ -1327,13 +1349,13 @@ int step_impl(Ctx* c, int32_t n_steps, bool graphs_only) {
 }
 
 int sph_step(void* ctx, int32_t n_steps) {
-  Ctx* c = as_ctx(ctx);
+  Ctx* c = as_ctx_keep(ctx);
   if (!c) return SPH_ERR_INVALID;
   return step_impl(c, n_steps, false);
 }
 
 int sph_step_graphs(void* ctx, int32_t n_steps) {
-  Ctx* c = as_ctx(ctx);
+  Ctx* c = as_ctx_keep(ctx);
   if (!c) return SPH_ERR_INVALID;
   if (n_steps < 0) return fail(c, SPH_ERR_INVALID, "negative step count");
   return step_impl(c, n_steps, true);
@@ -1386,7 +1408,7 @@ int sph_set_resort(void* ctx, int32_t mode) {
 }
 
 int sph_movers(void* ctx, int64_t* count) {
-  Ctx* c = as_ctx(ctx);
+  Ctx* c = as_ctx_keep(ctx);
   if (!c || !count) return SPH_ERR_INVALID;
   int m = 0;
   HIPCHK(c, hipMemcpyAsync(&m, c->rb.n_last, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1396,7 +1418,7 @@ int sph_movers(void* ctx, int64_t* count) {
 }
 
 int sph_resort_path(void* ctx, int32_t* path) {
-  Ctx* c = as_ctx(ctx);
+  Ctx* c = as_ctx_keep(ctx);
   if (!c || !path) return SPH_ERR_INVALID;
   uint32_t m = 0;
   HIPCHK(c, hipMemcpyAsync(&m, c->rb.mode_last, sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -1496,6 +1518,7 @@ int sph_get_ell(void* ctx, int32_t* counts, int32_t* slots) {
 int sph_get_device_view(void* ctx, sph_device_view* out) {
   Ctx* c = as_ctx(ctx);
   if (!c || !out) return SPH_ERR_INVALID;
+  c->view_out = true;  // from now on every sph_step call packs the state afresh
   int rc = settle_fields(c);  // the view's fields follow the current order
   if (rc) return rc;
   DevState& s = c->s;
@@ -1535,7 +1558,7 @@ int sph_stage_times(void* ctx, double* ms, int64_t* launches, int32_t count) {
 }
 
 int sph_synchronize(void* ctx) {
-  Ctx* c = as_ctx(ctx);
+  Ctx* c = as_ctx_keep(ctx);
   if (!c) return SPH_ERR_INVALID;
   return sync_and_check(c);
 }
@@ -1726,6 +1749,7 @@ int sph_batch_autotune(void* ctx, const sph_batch_tuning* candidates, int32_t n_
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->use_graph = was_graph;
+  c->packed_ok = false;  // the trials' state is replaced below
   c->timing = was_timing;
   if (rc == SPH_OK && best_i < 0) rc = fail(c, SPH_ERR_INVALID, "no candidate could be applied");
   const std::string err = c->err;

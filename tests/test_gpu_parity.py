@@ -409,6 +409,62 @@ def test_graph_cache_parities_and_remainders(sph, all_scenes):
                 assert np.array_equal(fa[q], fb[q]), (k, q)
 
 
+def test_step_calls_repack_after_changes(sph, all_scenes):
+    """A step call right after another one reuses the packed copy of the state
+    the first one left; any other call that may change the state, and any
+    write through a device view, makes the next step call pack afresh. Each
+    case equals the same sequence with a state-preserving call (measure) that
+    forces the pack, state and fields."""
+    import ctypes as C
+    s = all_scenes["dam_break_16k"]
+
+    def run(seq):
+        with sph.Context(s.params, s.n) as ctx:
+            ctx.upload(s.rx, s.ry, s.vx, s.vy)
+            ctx.prepare()
+            ctx.set_graph(True, 1)
+            view = None
+            for op in seq:
+                if op == "view":
+                    view = ctx.device_view()
+                elif op == "measure":
+                    ctx.measure()
+                elif op == "kick":
+                    ctx.update_velocities()
+                elif op == "write_vx":
+                    # halve vx through the view's pointer, no call on the context
+                    ctx.synchronize()
+                    buf = np.empty(s.n, np.float32)
+                    assert sph.lib().sph_device_copy(buf.ctypes.data_as(C.c_void_p),
+                                                     C.c_void_p(view.vx), s.n * 4, 1) == 0
+                    buf *= np.float32(0.5)
+                    assert sph.lib().sph_device_copy(C.c_void_p(view.vx),
+                                                     buf.ctypes.data_as(C.c_void_p),
+                                                     s.n * 4, 0) == 0
+                else:
+                    ctx.step(op)
+            if view is not None:
+                assert ctx.device_view().vx == view.vx   # the pointer written through
+            return ctx.download(), ctx.fields()
+
+    cases = [([4, 4], [4, "measure", 4], [8]),
+             ([4, "kick", 4], [4, "kick", "measure", 4], None),
+             (["view", 4, "write_vx", 4], ["view", 4, "write_vx", "measure", 4], None),
+             ([3, 5, 2], [3, "measure", 5, "measure", 2], [10])]
+    for seq, ref, direct in cases:
+        note(f"[gpu] step calls {seq} vs {ref}")
+        (d, f), (d0, f0) = run(seq), run(ref)
+        outs = [(d0, f0)] + ([run(direct)] if direct else [])
+        for dr, fr in outs:
+            for k in ("pid", "rx", "ry", "vx", "vy"):
+                assert np.array_equal(d[k], dr[k]), (seq, k)
+            for k in ("ax", "ay", "rho"):
+                assert np.array_equal(f[k], fr[k]), (seq, k)
+    # and the write through the view did change the outcome
+    (d1, _), (d2, _) = run(["view", 4, "write_vx", 4]), run(["view", 4, 4])
+    assert not np.array_equal(d1["vx"], d2["vx"])
+
+
 @pytest.mark.parametrize("name", ["dam_break_16k", "random_clusters"])
 def test_measure_fp64(sph, orc, all_scenes, name):
     s = all_scenes[name]
