@@ -32,14 +32,16 @@ def log(msg):
 
 
 def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers_frac=0.0,
-                           flags_in_force=False, narrow=False):
+                           flags_in_force=False, narrow=False, bucket_in_force=False):
     """Algorithmic HBM bytes per particle of one launch of each stage (FP32
     state, u64 keys, u32 indices; neighbor and halo re-reads counted as cache
     hits). flags_in_force: the force kernel reports the re-sort's movers
-    (reading the previous key, writing one flag bit). narrow: the step keeps
-    its keys as u32 (kb = 4 bytes, else 8). None for a stage without a
-    model."""
+    (reading the previous key, writing one flag bit). bucket_in_force: it also
+    puts each mover into its owner tile's bucket, and the merge forms the
+    chunk counters (no bucketing launch). narrow: the step keeps its keys as
+    u32 (kb = 4 bytes, else 8). None for a stage without a model."""
     kb = 4.0 if narrow else 8.0
+    f = movers_frac
     if variant == "split_block":
         # density: R r 8, W rho,pr 8; force: R r,v,rho,pr 24, W a 8, T r,v 16, key 8
         phys = {"approximate_density": 16.0, "force_computation": 56.0}
@@ -55,7 +57,10 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
         # hits)
         phys = {"approximate_density": 20.0 + 4.0 * k,
                 "force_computation": 44.0 + kb + 4.0 * k
-                + ((kb + 0.125) if flags_in_force else 0.0)}
+                + ((kb + 0.125) if flags_in_force else 0.0)
+                # per mover: R its new cell's range 8, pid 4; W its bucket
+                # entry 32, its owner's counter 8
+                + (52.0 * f if bucket_in_force else 0.0)}
     if resort == "full":
         sort = {
             # R new keys (8) + previous sorted keys (8); clear previous ranges (8/occ. cell)
@@ -67,7 +72,6 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
             "neighbor_search": 68.0 + 8.0 * occ,
         }
     else:
-        f = movers_frac
         sort = {
             # k_bucket: R the flag bits 1/8, a packed 8-byte counter per
             # 256-particle tile; per mover R new key kb, T r,v 16, pid 4, its
@@ -80,7 +84,9 @@ def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers
             # movers' bucket entries 32 f; W r,v 16 (only the packed copy
             # inside sph_step), pid 4, key kb (the permutation 4 only in the
             # last step of a call); clear the old ranges 8 occ
-            "merge": 40.125 + 2.0 * kb + 32.0 * f + 8.0 * occ,
+            "merge": 40.125 + 2.0 * kb + 32.0 * f + 8.0 * occ
+            # (bucket_in_force: R the tile counters, W the chunk counters)
+            + (16.0 / 256.0 if bucket_in_force else 0.0),
             # k_ranges: R key kb, W 8 occ, the tile-first key and the zeroed
             # counter 16 per 256-particle tile
             "neighbor_search": kb + 16.0 / 256.0 + 8.0 * occ,
@@ -211,7 +217,7 @@ def stage_table(stage, steps, n, k, occ, passes, variant, resort, mfrac):
         entry = {"launches": launches, "ms_total": round(ms, 4),
                  "us_per_launch": round(per * 1e3, 3)}
         bpp = alg_bytes_per_particle(name, k, occ, passes, variant, resort, mfrac,
-                                     flags_in_force, narrow)
+                                     flags_in_force, narrow, bucket_in_force=flags_in_force)
         if bpp is not None:
             entry["alg_bytes_per_launch"] = int(bpp * n)
             entry["achieved_GBs"] = round(bpp * n / (per * 1e-3) / 1e9, 1)

@@ -417,7 +417,7 @@ class Context:
     def step(self, n_steps=1):
         self._chk(self._L.sph_step(self._h, int(n_steps)), "sph_step")
 
-    def set_graph(self, enable=True, pairs: int = 4):
+    def set_graph(self, enable=True, pairs: int = 16):
         """Replay sph_step's iterations from a hipGraph of 2 * pairs iterations."""
         self._chk(self._L.sph_set_graph(self._h, int(pairs) if enable else 0), "sph_set_graph")
 

@@ -74,7 +74,7 @@ struct Ctx {
   bool fields_prev_order = false;
   // graph
   bool use_graph = false;
-  int graph_pairs = 4;  // iteration pairs a captured graph holds
+  int graph_pairs = 16;  // iteration pairs a captured graph holds
   // captured graphs, each for one buffer assignment and iteration count (the
   // pid and key buffers swap every iteration, so a call that starts after an
   // odd number of steps needs the other assignment's graph)
