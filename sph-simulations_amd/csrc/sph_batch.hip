@@ -66,8 +66,8 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
     float* __restrict__ ay_out, float* __restrict__ tx, float* __restrict__ ty,
     float* __restrict__ tvx, float* __restrict__ tvy, K* __restrict__ keys_out,
     const float4* __restrict__ xyuv, const float2* __restrict__ rp,
-    const K* __restrict__ kold, unsigned long long* __restrict__ flags,
-    float4* __restrict__ t4) {
+    const K* __restrict__ kold, MoverSink movers, float4* __restrict__ t4) {
+  unsigned long long* __restrict__ flags = movers.flags;
   const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int i = t.begin + (int)threadIdx.x;
   if (i >= t.end) return;
@@ -112,6 +112,8 @@ __global__ __launch_bounds__(kMaxTile) void k_bforces(
       atomicOr(&flags[w0], bal << sh);
       if (sh != 0 && (bal >> (kWave - sh)) != 0ull) atomicOr(&flags[w0 + 1], bal >> (kWave - sh));
     }
+    // and each mover into its owner tile's bucket (bucket_sink)
+    if (movers.bucket) bucket_movers<false>(movers, i, kn, kn != ko, make_float4(x, y, u, v));
   }
 }
 
@@ -153,7 +155,7 @@ hipError_t launch_batch_density(const Launch& L, const BatchLaunch& B, const Dev
 
 template <int PHYS, int BC>
 static void bforces_go(const Launch& L, const BatchLaunch& B, const DevState& s,
-                       uint64_t* keys_out, unsigned long long* flags) {
+                       uint64_t* keys_out, const MoverSink& flags) {
   if (s.narrow)  // the step's narrow keys: read nkeys, write nkeys_in
     k_bforces<PHYS, BC, uint32_t><<<B.n_tiles, B.tile, 0, L.stream>>>(
         s.n, B.tiles, B.scenes, s.rx, s.ry, s.vx, s.vy, s.rho, s.pr, s.cells, s.ell_count,
@@ -171,7 +173,7 @@ hipError_t launch_batch_forces(const Launch& L, const BatchLaunch& B, const DevS
   if (s.n <= 0) return hipSuccess;
   if (!tile_ok(B)) return hipErrorInvalidValue;
   const int ph = L.phys.physics, bc = L.phys.boundary;
-  unsigned long long* f = movers.flags;
+  const MoverSink& f = movers;
   if (ph == 0 && bc == 0) bforces_go<0, 0>(L, B, s, keys_out, f);
   else if (ph == 0 && bc == 1) bforces_go<0, 1>(L, B, s, keys_out, f);
   else if (ph == 1 && bc == 0) bforces_go<1, 0>(L, B, s, keys_out, f);

@@ -443,14 +443,18 @@ int fused_iteration(Ctx* c) {
       StageTimer t(c, SPH_T_DENSITY);
       HIPCHK(c, launch_batch_density(c->L, c->B, fs));
     }
-    // the force kernel also flags the movers for the incremental re-sort
+    // the force kernel also flags the movers for the incremental re-sort and
+    // puts them into their owner tiles' buckets
     const bool inc = c->resort == SPH_RESORT_INCREMENTAL;
+    const bool bucketed = inc && kFusedBucket && s.n <= kResortMaxParticles;
     {
       StageTimer t(c, SPH_T_FORCE);
       HIPCHK(c, launch_batch_forces(c->L, c->B, fs, c->sb.keys_in,
-                                    inc ? mover_sink(c->rb) : MoverSink{}));
+                                    !inc       ? MoverSink{}
+                                    : bucketed ? bucket_sink(c->rb, s, (int)resort_tiles(s.n))
+                                               : mover_sink(c->rb)));
     }
-    return resort_integrated(c, inc);
+    return resort_integrated(c, inc, bucketed);
   } else if (s.block_bits && c->step_kernel == SPH_STEP_FUSED_BLOCK) {
     // Morton order, one kernel: density, forces, integration and keys
     StageTimer t(c, SPH_T_BLOCK_STEP);

@@ -254,6 +254,10 @@ __device__ __forceinline__ uint32_t owner_of_empty(const MoverSink& m, bool full
 // same word from every wave serialise at the memory: the 64 chunk counters
 // of a 1M set took 20-25 us per step that way.)
 // Called by all active lanes of a wave; st is the lane's integrated state.
+// ALIGNED: the wave's particles are the 64 indices from a multiple of 64 (one
+// tile; lane 0 adds the wave's count), else consecutive indices from anywhere
+// (the batched scenes' segments: each mover adds itself to its tile).
+template <bool ALIGNED = true>
 __device__ __forceinline__ void bucket_movers(const MoverSink& m, int i, uint64_t k,
                                               bool moved, float4 st) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -261,7 +265,11 @@ __device__ __forceinline__ void bucket_movers(const MoverSink& m, int i, uint64_
   if (bal == 0ull) return;
   const unsigned long long active = __ballot(true);
   const int tile = i / kResortTile;
-  if (lane == 0) atomicAdd(&m.tcount[tile], (unsigned long long)__popcll(bal));
+  if (ALIGNED) {
+    if (lane == 0) atomicAdd(&m.tcount[tile], (unsigned long long)__popcll(bal));
+  } else if (moved) {
+    atomicAdd(&m.tcount[tile], 1ull);
+  }
   int2 cr = make_int2(0, 0);
   uint32_t pid = 0u;
   if (moved) {
