@@ -64,6 +64,12 @@ struct Ctx {
   bool packed_ok = false;
   bool packed_ok_narrow = false;
   bool view_out = false;
+  // after a lazy sph_step call (no device view out) the separate state
+  // arrays, and with soa_stale_narrow the u64 sorted keys, are written back
+  // from the packed copy only when a later call needs them (settle_soa, from
+  // as_ctx / as_ctx_read, or the next step call of another kind)
+  bool soa_stale = false;
+  bool soa_stale_narrow = false;
   // the fused step writes the accelerations, the separate density and
   // pressure arrays and the incremental re-sort's permutation (off for the
   // steps of a sph_step call but its last: only the last step's are
@@ -284,14 +290,25 @@ int dev_alloc(Ctx* c, T** p, int64_t count) {
   return SPH_OK;
 }
 
-// The context of an entry point that may change the state: the packed copy
-// then no longer counts as current.
+int settle_soa(Ctx* c);
+
+// The context of an entry point that may read or change the state: the
+// separate arrays are brought up to date first, and the packed copy no longer
+// counts as current. Null when bringing them up to date failed (the reason is
+// in sph_last_error).
 Ctx* as_ctx(void* p) {
   Ctx* c = static_cast<Ctx*>(p);
-  if (c) c->packed_ok = false;
-  return c;
+  if (!c) return nullptr;
+  c->packed_ok = false;
+  return settle_soa(c) == SPH_OK ? c : nullptr;
 }
-// The context of the step calls and of calls that change no state.
+// The context of a call that reads the state and changes nothing.
+Ctx* as_ctx_read(void* p) {
+  Ctx* c = static_cast<Ctx*>(p);
+  if (!c) return nullptr;
+  return settle_soa(c) == SPH_OK ? c : nullptr;
+}
+// The context of the step calls and of calls that touch no state array.
 Ctx* as_ctx_keep(void* p) { return static_cast<Ctx*>(p); }
 
 // ---- timing helpers
@@ -589,6 +606,17 @@ int sync_and_check(Ctx* c) {
 // arrays, which are free between calls) puts them into the current order,
 // so that index i of every array is the same particle. Costs one gather of
 // five arrays, once; the steps themselves never pay for it.
+// The separate state arrays (and the u64 sorted keys after narrow steps)
+// from the packed copy a lazy sph_step call left: one pass, once.
+int settle_soa(Ctx* c) {
+  if (!c->soa_stale) return SPH_OK;
+  c->soa_stale = false;
+  DevState s = c->s;
+  s.narrow = c->soa_stale_narrow;
+  const hipError_t e = launch_unpack(c->L, s);
+  return e == hipSuccess ? SPH_OK : hip_fail(c, e, "unpack");
+}
+
 int settle_fields(Ctx* c) {
   if (!c->fields_prev_order) return SPH_OK;
   c->fields_prev_order = false;
@@ -898,7 +926,7 @@ int sph_create(const sph_params* params, int64_t n_max, int device, void** out_c
 }
 
 int sph_destroy(void* ctx) {
-  Ctx* c = as_ctx(ctx);
+  Ctx* c = as_ctx_keep(ctx);
   if (!c) return SPH_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   destroy_graph(c);
@@ -978,7 +1006,7 @@ int sph_upload(void* ctx, int64_t n, const float* rx, const float* ry, const flo
 }
 
 int sph_download(void* ctx, float* rx, float* ry, float* vx, float* vy, uint32_t* pid) {
-  Ctx* c = as_ctx_keep(ctx);
+  Ctx* c = as_ctx_read(ctx);
   if (!c) return SPH_ERR_INVALID;
   DevState& s = c->s;
   size_t fb = (size_t)s.n * sizeof(float);
@@ -1291,6 +1319,13 @@ int step_impl(Ctx* c, int32_t n_steps, bool graphs_only) {
     c->s.narrow = false;
     return rc;
   }
+  // separate arrays a lazy call left stale are written back unless this call
+  // is lazy in the same way (it reads only the packed copy and the keys it
+  // left)
+  if (c->soa_stale && !(lazy && c->s.narrow == c->soa_stale_narrow) && (rc = settle_soa(c))) {
+    c->s.narrow = false;
+    return rc;
+  }
   // the packed copy of the sorted state the step kernels gather from (other
   // calls may have changed the state since the last re-sort wrote it; not
   // after a lazy step call with nothing in between)
@@ -1337,11 +1372,17 @@ int step_impl(Ctx* c, int32_t n_steps, bool graphs_only) {
   rc = run();
   c->fields_out = true;
   c->soa_lazy = false;
-  if (lazy) {
-    const hipError_t e = launch_unpack(c->L, c->s);
-    if (rc == SPH_OK && e != hipSuccess) rc = hip_fail(c, e, "unpack");
-  }
   const bool narrow = c->s.narrow;
+  if (lazy) {
+    if (rc == SPH_OK && !c->view_out) {
+      // the separate arrays are written back when something needs them
+      c->soa_stale = true;
+      c->soa_stale_narrow = narrow;
+    } else {
+      const hipError_t e = launch_unpack(c->L, c->s);
+      if (rc == SPH_OK && e != hipSuccess) rc = hip_fail(c, e, "unpack");
+    }
+  }
   c->s.narrow = false;
   if (rc) return rc;
   c->packed_ok = lazy && !c->view_out;
@@ -1734,6 +1775,7 @@ int sph_batch_autotune(void* ctx, const sph_batch_tuning* candidates, int32_t n_
   for (int i = 0; i < n_candidates && rc == SPH_OK; ++i) {
     if (ms_per_step) ms_per_step[i] = -1.0;
     if (batch_configure(c, candidates[i]) != SPH_OK) continue;  // not applicable
+    c->soa_stale = false;  // the trial's state is replaced
     hipError_t e = copy_state(false);
     if (e != hipSuccess) { rc = hip_fail(c, e, "restore state"); break; }
     if ((rc = sph_prepare(c)) || (rc = sph_step(c, 2))) break;
@@ -1759,6 +1801,7 @@ int sph_batch_autotune(void* ctx, const sph_batch_tuning* candidates, int32_t n_
   const std::string err = c->err;
   // leave the chosen (or, after a failure, the starting) tuning and the saved state
   int rc2 = batch_configure(c, rc == SPH_OK ? candidates[best_i] : start);
+  c->soa_stale = false;  // the trials' state is replaced
   if (rc2 == SPH_OK) {
     hipError_t e = copy_state(false);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
