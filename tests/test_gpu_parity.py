@@ -740,6 +740,32 @@ def test_incremental_resort_empty_cell_owners(sph, orc, steps):
         assert np.array_equal(st, ost) and np.array_equal(en, oen)
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_sorted_path_inside_multistep_call(sph, orc, graph):
+    """A bucket overflows inside a multi-step call (32-bit keys, the merge
+    orders the movers in its first workgroup while the others wait): 2,000
+    particles of a 90,000-particle dam break sent into one spot, three
+    reference iterations (prepare + a two-step call + the final density and
+    forces), against the FP32 oracle: the same order and bit-identical state."""
+    s = sph.scenes.dam_break(300, 300, seed=31)
+    rng = np.random.default_rng(37)
+    pick = rng.choice(s.n, 2000, replace=False)
+    dt = s.params.dt
+    s.vx[pick] = ((0.3 - s.rx[pick]) / dt).astype(np.float32)
+    s.vy[pick] = ((0.3 - s.ry[pick]) / dt).astype(np.float32)
+    o = orc.run(s.params, s.rx, s.ry, s.vx, s.vy, 3)
+    with sph.Context(s.params, s.n) as ctx:
+        ctx.upload(s.rx, s.ry, s.vx, s.vy)
+        ctx.set_graph(graph, 1)
+        ctx.run(3)
+        path = ctx.resort_path()
+        d = ctx.download()
+    note(f"[gpu] sorted path in a two-step call (graph={graph}): last step {path}")
+    assert np.array_equal(d["pid"], o["pid"])
+    for k in ("rx", "ry", "vx", "vy"):
+        assert np.array_equal(d[k], o[k]), k
+
+
 @pytest.mark.parametrize("side,sigma", [(100, 40.0), (512, 400.0)])
 def test_incremental_resort_many_movers(sph, orc, side, sigma):
     """Most particles moving 0.5 to 5 cells per step, two steps in a row: 10 K
