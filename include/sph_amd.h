@@ -89,7 +89,9 @@ typedef struct sph_grid_info {
 } sph_grid_info;
 
 /* Device view of the current (sorted) state, for consumers that read it in
- * place on the same stream (all arrays hold n elements, float unless noted). */
+ * place on the same stream (all arrays hold n elements, float unless noted).
+ * The pid buffer alternates from step to step: take the view again after a
+ * step call. */
 typedef struct sph_device_view {
   int64_t n;
   float *rx, *ry, *vx, *vy, *ax, *ay, *rho, *pr; /* pr = p / rho^2 */
@@ -223,11 +225,14 @@ int sph_snapshot_load(void* ctx, const char* path, int64_t* step);
  *   sph_density_pressure + sph_forces refresh).
  * Reference-equivalent run of S iterations (simulation_configuration.h:297-387):
  *   sph_prepare; sph_step(S-1); sph_density_pressure(SORTED); sph_forces(SORTED).
- * By default (SPH_STEP_PARTICLE, SPH_RESORT_INCREMENTAL) one iteration is: the
- * density kernel (density + EOS + fixed-width list), the force kernel (forces +
- * integration + next keys + the movers' flags and lists), the incremental
- * re-sort (the movers grouped and ranked by merge tile, then the merge) and the
- * cell-range kernel. */
+ * By default (SPH_STEP_PARTICLE, SPH_RESORT_INCREMENTAL) one iteration is four
+ * launches: the density kernel (density + EOS + fixed-width list), the force
+ * kernel (forces + integration + next keys + the movers' flags, each mover put
+ * into the bucket of its merge tile), the merge of the incremental re-sort
+ * (each tile's bucket ranked in LDS and merged with the tile's other
+ * particles) and the cell-range kernel. A multi-step call writes the separate
+ * state arrays back when a later call first needs them, unless a device view
+ * was handed out. */
 int sph_prepare(void* ctx);
 int sph_step(void* ctx, int32_t n_steps);
 /* Capture fused iterations in a hipGraph and replay it in sph_step: enable = 0
