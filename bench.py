@@ -487,14 +487,16 @@ def bench_batched(sph, steps, trial_steps=10):
 
 
 def bench_surrogate(sph, scene, steps, train_iters, train_scenes=16, train_steps=1000,
-                    check_every=10, threshold=0.02):
+                    check_every=10, threshold=0.02, fast_steps=500):
     """Surrogate in the loop on the 1M-particle dam break: a network trained on
     solver snapshots (`train_scenes` 65,536-particle dam breaks of `train_steps`
     steps, a snapshot every 10, the last scene held out; Adam with a gradient-norm
     limit) supplies the accelerations; plain SPH is evaluated every `check_every` steps
     and a window whose RMS relative acceleration error exceeds `threshold` is
     rolled back and recomputed with plain SPH. Compared with a plain run of the
-    same steps from the same state (throughput and drift)."""
+    same steps from the same state (throughput and drift). The network-only run
+    (every check accepted) takes `fast_steps` steps against a plain run of as
+    many."""
     sur = importlib.import_module("sph-simulations_amd.surrogate")
     n = scene.n
     log(f"surrogate: training ({train_iters} iterations)")
@@ -536,15 +538,26 @@ def bench_surrogate(sph, scene, steps, train_iters, train_scenes=16, train_steps
     plain2.step(steps)
     plain2.synchronize()
     t_p = time.perf_counter() - t0
-    # network-only steps (every check accepted): the surrogate path's best case
+    log(f"surrogate in the loop: {steps} steps, {t_s * 1e3 / steps:.3f} ms/step")
+    # network-only steps (every check accepted): the surrogate path's best case,
+    # against a plain run of as many steps
+    fs = min(fast_steps, steps)
     fast = sph.Context(scene.params, n)
     fast.upload(scene.rx, scene.ry, scene.vx, scene.vy)
     fast.prepare()
     fast.synchronize()
     t0 = time.perf_counter()
-    st_fast = sur.surrogate_run(fast, net, steps, check_every, 1e30, 0)
+    st_fast = sur.surrogate_run(fast, net, fs, check_every, 1e30, 0)
     fast.synchronize()
     t_f = time.perf_counter() - t0
+    plain3 = sph.Context(scene.params, n)
+    plain3.upload(scene.rx, scene.ry, scene.vx, scene.vy)
+    plain3.prepare()
+    plain3.set_graph(True)
+    plain3.step(fs)
+    plain3.synchronize()
+    log(f"network only: {fs} steps, {t_f * 1e3 / fs:.3f} ms/step")
+
     # drift of the surrogate trajectory against the plain one (matched ids)
     def state(c):
         d = c.download()
@@ -553,11 +566,11 @@ def bench_surrogate(sph, scene, steps, train_iters, train_scenes=16, train_steps
         o = np.argsort(d["pid"])
         return {k: d[k][o].astype(np.float64) for k in ("rx", "ry")} | {
             "rho": f["rho"][o].astype(np.float64)}
-    a, b, f_ = state(surr), state(plain2), state(fast)
+    a, b, f_, b3 = state(surr), state(plain2), state(fast), state(plain3)
     h = scene.params.h
     rho0 = scene.params.rho0
 
-    def drift(x):
+    def drift(x, b=b):
         dpos = np.hypot(x["rx"] - b["rx"], x["ry"] - b["ry"])
         drho = np.abs(x["rho"] - b["rho"]) / rho0
         return {"position_rms_over_h": float(np.sqrt(np.mean(dpos ** 2)) / h),
@@ -571,9 +584,10 @@ def bench_surrogate(sph, scene, steps, train_iters, train_scenes=16, train_steps
         "fallback_rate": st["failed_checks"] / max(st["checks"], 1),
         "stats": st,
         "drift_vs_plain": drift(a),
-        "network_only": {"value": n * steps / t_f, "ms_per_step": t_f * 1e3 / steps,
-                         "vs_plain": t_p / t_f, "mean_error": st_fast["mean_error"],
-                         "max_error": st_fast["max_error"], "drift_vs_plain": drift(f_)},
+        "network_only": {"steps": fs, "value": n * fs / t_f, "ms_per_step": t_f * 1e3 / fs,
+                         "vs_plain": (t_p / steps) / (t_f / fs),
+                         "mean_error": st_fast["mean_error"],
+                         "max_error": st_fast["max_error"], "drift_vs_plain": drift(f_, b3)},
     })
     # the network alone against the BF16 MFMA peak: FLOPs of its node GEMMs over
     # the time of whole forward passes (edge passes included)
@@ -596,7 +610,7 @@ def bench_surrogate(sph, scene, steps, train_iters, train_scenes=16, train_steps
                        "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                        "frac": tflops / 2500.0, "forward_ms": fwd_ms,
                        "gemm_flops_per_forward": flops}
-    for c in (plain, surr, plain2, fast):
+    for c in (plain, surr, plain2, fast, plain3):
         c.close()
     net.close()
     return out
@@ -619,7 +633,7 @@ def main():
                     help="steps of the 16,384-particle dam break (0 disables)")
     ap.add_argument("--batched-steps", type=int, default=500,
                     help="steps of the 256-scene batched workload (0 disables)")
-    ap.add_argument("--surrogate-steps", type=int, default=200,
+    ap.add_argument("--surrogate-steps", type=int, default=5000,
                     help="steps of the surrogate-in-the-loop comparison (0 disables)")
     ap.add_argument("--train-iters", type=int, default=4000,
                     help="training iterations of the surrogate network")
