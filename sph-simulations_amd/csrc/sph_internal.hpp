@@ -239,6 +239,21 @@ __device__ __forceinline__ uint32_t owner_of_empty(const MoverSink& m, bool full
   return (uint32_t)(base + c2);
 }
 
+// First keys of a tile of the previous order and of the next tile (all ones
+// past the last tile). A mover of that tile whose (key, index) lies between
+// the two is owned by its own tile. Wave-uniform: the producer reads them with
+// scalar loads at its start, so they are at hand when its movers are known.
+struct TileKeys {
+  uint64_t first = 0, next = 0;
+};
+
+__device__ __forceinline__ TileKeys tile_keys(const MoverSink& m, int t) {
+  TileKeys b;
+  b.first = m.tfirst[t];
+  b.next = t + 1 < m.tiles ? m.tfirst[t + 1] : ~0ull;
+  return b;
+}
+
 // The bucketing of k_bucket done by the wave that found its movers: the count
 // of the wave's movers by previous index (low half of its tile's counter),
 // then each mover's owner tile -- the
@@ -255,33 +270,54 @@ __device__ __forceinline__ uint32_t owner_of_empty(const MoverSink& m, bool full
 // of a 1M set took 20-25 us per step that way.)
 // Called by all active lanes of a wave; st is the lane's integrated state.
 // ALIGNED: the wave's particles are the 64 indices from a multiple of 64 (one
-// tile; lane 0 adds the wave's count), else consecutive indices from anywhere
-// (the batched scenes' segments: each mover adds itself to its tile).
+// tile, whose first keys are tk), else consecutive indices from anywhere (the
+// batched scenes' segments: each mover adds itself to its tile). In the
+// aligned form a mover between tk.first and tk.next (about nine in ten: most
+// movers step to a cell the tile already spans) is owned by its own tile,
+// known without its cell range: lane 0 adds the wave's count and takes the
+// slots of all such movers in one returning atomic, so a wave whose movers
+// are all of this kind waits for one memory round (that atomic, with the
+// pids) instead of two (the cell ranges, then the atomics).
 template <bool ALIGNED = true>
 __device__ __forceinline__ void bucket_movers(const MoverSink& m, int i, uint64_t k,
-                                              bool moved, float4 st) {
+                                              bool moved, float4 st, TileKeys tk = TileKeys{}) {
   const int lane = threadIdx.x & (kWave - 1);
   const unsigned long long bal = __ballot(moved);
   if (bal == 0ull) return;
   const unsigned long long active = __ballot(true);
   const int tile = i / kResortTile;
-  if (ALIGNED) {
-    if (lane == 0) atomicAdd(&m.tcount[tile], (unsigned long long)__popcll(bal));
-  } else if (moved) {
-    atomicAdd(&m.tcount[tile], 1ull);
-  }
+  // (tk.first, first index of the tile) < (k, i) and (k, i) < (tk.next, first
+  // index of the next tile); i lies in the tile, so the second is k <= tk.next
+  const bool home = ALIGNED && moved &&
+                    (tk.first < k || (tk.first == k && (uint32_t)tile * kResortTile < (uint32_t)i)) &&
+                    k <= tk.next;
+  const unsigned long long hbal = __ballot(home);
   int2 cr = make_int2(0, 0);
   uint32_t pid = 0u;
   if (moved) {
-    cr = m.cells[min(k, m.n_keys - 1)];
     pid = m.pid[i];
+    if (!home) cr = m.cells[min(k, m.n_keys - 1)];
   }
-  uint32_t own = 0u;
-  if (cr.x >= 0) {
+  uint32_t hbase = 0u;
+  if (ALIGNED) {
+    const unsigned long long add =
+        (unsigned long long)__popcll(bal) | ((unsigned long long)__popcll(hbal) << 32);
+    if (hbal != 0ull) {  // wave-uniform
+      unsigned long long old = 0ull;
+      if (lane == 0) old = atomicAdd(&m.tcount[tile], add);
+      hbase = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(old >> 32), 0);
+    } else if (lane == 0) {
+      atomicAdd(&m.tcount[tile], add);
+    }
+  } else if (moved) {
+    atomicAdd(&m.tcount[tile], 1ull);
+  }
+  uint32_t own = (uint32_t)tile;
+  if (moved && !home && cr.x >= 0) {
     const int p = i < cr.x ? cr.x - 1 : cr.y - 1;
     own = p < 0 ? 0u : (uint32_t)p / (uint32_t)kResortTile;
   }
-  unsigned long long empty = __ballot(moved && cr.x < 0);
+  unsigned long long empty = __ballot(moved && !home && cr.x < 0);
   while (empty != 0ull) {  // wave-uniform
     const int l = __ffsll((long long)empty) - 1;
     empty &= empty - 1ull;
@@ -298,7 +334,8 @@ __device__ __forceinline__ void bucket_movers(const MoverSink& m, int i, uint64_
   }
   if (!moved) return;
   constexpr unsigned long long kOwnOne = 1ull << 32;
-  const uint32_t slot = (uint32_t)(atomicAdd(&m.tcount[own], kOwnOne) >> 32);
+  const uint32_t slot = home ? hbase + (uint32_t)__popcll(hbal & ((1ull << lane) - 1ull))
+                             : (uint32_t)(atomicAdd(&m.tcount[own], kOwnOne) >> 32);
   if (slot < (uint32_t)kBucketCap) {
     uint4* e = m.bucket + 2 * ((size_t)own * kBucketCap + slot);
     e[0] = make_uint4((uint32_t)k, (uint32_t)(k >> 32), (uint32_t)i, pid);

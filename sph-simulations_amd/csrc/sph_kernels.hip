@@ -295,8 +295,15 @@ __global__ __launch_bounds__(BLK) void k_forces(
     float* __restrict__ tvx, float* __restrict__ tvy, K* __restrict__ keys_out,
     const K* __restrict__ kold, MoverSink movers, const float4* __restrict__ xyuv,
     const float2* __restrict__ rp, float4* __restrict__ t4) {
-  const int i = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int i = tile * BLK + threadIdx.x;
   if (i >= n) return;
+  // the bucketing's bounds of the wave's re-sort tile, read first (scalar
+  // loads when a workgroup is one tile)
+  int rtile = tile;
+  if constexpr (BLK != kResortTile)
+    rtile = (tile * BLK + (int)(threadIdx.x & ~(unsigned)(kWave - 1))) / kResortTile;
+  const TileKeys tk = (FUSED && movers.bucket) ? tile_keys(movers, rtile) : TileKeys{};
   float xi, yi, vxi, vyi, rhoi, pri;
   if (NB == NB_ELL && xyuv) {
     // the particle's own state from the packed copies: two loads instead of six
@@ -352,7 +359,7 @@ __global__ __launch_bounds__(BLK) void k_forces(
     // tile covers flag word i / 64; waves past n write none)
     if (movers.flags) {
       record_movers(movers, i, kn != ko);
-      if (movers.bucket) bucket_movers(movers, i, kn, kn != ko, make_float4(x, y, u, v));
+      if (movers.bucket) bucket_movers(movers, i, kn, kn != ko, make_float4(x, y, u, v), tk);
     }
   }
 }
