@@ -174,6 +174,115 @@ def test_ell_lists_bit_exact(sph, orc, all_scenes, name):
         assert ctx.ell_overflow() == int(np.sum(cnt > w))
 
 
+def corner_pairs(sph, orc, periodic):
+    """Pairs straddling the support radius across stencil corners: one particle
+    of each pair on a cell corner (the nearest float inside the diagonal cell),
+    the other inside the cell at distance h (1 + e) from it, |e| <= 2e-5, for
+    the four corners of cells spread over the box (and across the seam when
+    periodic): the diagonal stencil cells at the edge of the support, where a
+    walk that prunes cells or rounds the cell assignment differently would
+    lose or gain neighbors."""
+    base = (sph.scenes.periodic_shear(n_side=96, seed=3) if periodic
+            else sph.scenes.dam_break(40, 40, seed=3, name="corner_pairs"))
+    p = base.params
+    g = orc.grid(p)
+    lo = np.array([g.lo_x, g.lo_y], np.float64)
+    cs = np.array([g.cs_x, g.cs_y], np.float64)
+    nc = np.array([g.nx, g.ny])
+    length = np.array([g.len_x, g.len_y], np.float64)
+    rng = np.random.default_rng(11)
+    first = 0 if periodic else 2
+    cand = [(cx, cy) for cx in range(first, g.nx - first, 3) for cy in range(first, g.ny - first, 3)]
+    picks = [cand[k] for k in rng.choice(len(cand), 40, replace=False)]
+    if periodic:
+        picks += [(0, 0), (g.nx - 1, g.ny - 1), (0, g.ny // 2)]
+    pts = []
+    for cx, cy in picks:
+        for sx in (-1, 1):
+            for sy in (-1, 1):
+                c = np.array([cx, cy])
+                s = np.array([sx, sy])
+                corner = lo + (c + (s > 0)) * cs
+                want = c + s
+                if periodic:
+                    want = np.mod(want, nc)
+                elif np.any(want < 0) or np.any(want >= nc):
+                    continue
+                # the corner particle: the float32 nearest the corner that is
+                # classified into the diagonal cell (wrapped across the seam)
+                j = corner.astype(np.float32)
+                ok = True
+                for ax in (0, 1):
+                    lo_a, hi_a = np.float32(lo[ax]), np.float32(lo[ax] + length[ax])
+                    step = np.float32(-np.inf if s[ax] < 0 else np.inf)
+                    for _ in range(16):
+                        if periodic and j[ax] < lo_a:
+                            j[ax] = np.nextafter(hi_a, np.float32(-np.inf))
+                        elif periodic and j[ax] >= hi_a:
+                            j[ax] = lo_a
+                        if orc.cell_coord(float(j[ax]), float(lo_a), float(np.float32(cs[ax])),
+                                          int(nc[ax])) == want[ax]:
+                            break
+                        j[ax] = np.nextafter(j[ax], step)
+                    else:
+                        ok = False
+                if not ok:
+                    continue
+                for th in np.linspace(0.15, 1.42, 6):
+                    for e in np.linspace(-2e-5, 2e-5, 9):
+                        r = p.h * (1.0 + e)
+                        i = corner - s * r * np.array([np.cos(th), np.sin(th)])
+                        if periodic:
+                            i = lo + np.mod(i - lo, length)
+                        i32 = i.astype(np.float32)
+                        if (orc.cell_coord(float(i32[0]), float(g.lo_x), float(g.cs_x), g.nx) != cx or
+                                orc.cell_coord(float(i32[1]), float(g.lo_y), float(g.cs_y), g.ny) != cy):
+                            continue
+                        pts.append(i32)
+                        pts.append(j.copy())
+    pts = np.array(pts, np.float32)
+    n = len(pts)
+    z = np.zeros(n, np.float32)
+    return p, pts[:, 0].copy(), pts[:, 1].copy(), z, z.copy()
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_corner_pairs_lists_and_density(sph, orc, periodic):
+    """Neighbor lists and densities bit-exact for pairs across stencil corners
+    at h (1 + e): accepted and rejected exactly as the oracle's 3x3 walk
+    does."""
+    p, rx, ry, vx, vy = corner_pairs(sph, orc, periodic)
+    note(f"[gpu] corner pairs periodic={periodic}: n={rx.size}")
+    o = oracle_sorted(orc, p, rx, ry, vx, vy)
+    cnt, off, idx = orc.neighbor_csr(p, o["rx"], o["ry"], o["start"], o["end"])
+    rho, _, _ = orc.density(p, o["rx"], o["ry"], off, idx)
+    # the set exercises both sides of the threshold: some pairs whose FP32
+    # squared distance is within 1e-4 of h^2 are accepted, some rejected
+    x, y = o["rx"], o["ry"]
+    h2 = np.float32(p.h) * np.float32(p.h)
+    k0 = np.arange(0, rx.size, 2)
+    a = np.argsort(o["perm"])  # sorted position of each input particle
+    dx = (x[a[k0]] - x[a[k0 + 1]]).astype(np.float64)
+    dy = (y[a[k0]] - y[a[k0 + 1]]).astype(np.float64)
+    if periodic:
+        dx -= np.round(dx / (p.hi[0] - p.lo[0])) * (p.hi[0] - p.lo[0])
+        dy -= np.round(dy / (p.hi[1] - p.lo[1])) * (p.hi[1] - p.lo[1])
+    rr = (dx * dx + dy * dy) / float(h2)
+    near = np.abs(rr - 1.0) < 1e-4
+    assert np.sum(near & (rr <= 1.0)) > 20 and np.sum(near & (rr > 1.0)) > 20
+    with sph.Context(p, rx.size) as ctx:
+        ctx.upload(rx, ry, vx, vy)
+        ctx.prepare()
+        gcnt, slots = ctx.ell()
+        f = ctx.fields()
+    assert np.array_equal(gcnt, cnt)
+    w = slots.shape[1]
+    for i in range(rx.size):
+        k = min(cnt[i], w)
+        assert np.array_equal(slots[i, :k], idx[off[i]:off[i] + k]), i
+    assert np.array_equal(f["rho"], rho)
+
+
 def test_ell_overflow_fallback(sph, orc):
     # a narrow list forces the stencil fallback; forces must not change
     s = sph.scenes.reference_example()
