@@ -31,6 +31,44 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+_HIP = None
+
+
+def device_sync():
+    """hipDeviceSynchronize (what torch.cuda.synchronize calls), through the
+    HIP runtime the library links: waits for all work on the device, every
+    stream, without the library's read of its device status word."""
+    global _HIP
+    if _HIP is None:
+        import ctypes
+        for name in ("libamdhip64.so.7", "libamdhip64.so"):
+            try:
+                _HIP = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _HIP is None:
+            raise RuntimeError("HIP runtime library not found")
+    rc = _HIP.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize -> {rc}")
+
+
+def timed_steps(ctx, steps):
+    """Wall time of one `steps`-step call: the call alone between two
+    device-wide synchronisations; the context's status word (set on the
+    device when a bounded wait gave up) is checked after the clock stops,
+    so a failed run still raises."""
+    ctx.synchronize()
+    device_sync()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    device_sync()
+    wall = time.perf_counter() - t0
+    ctx.synchronize()
+    return wall
+
+
 def alg_bytes_per_particle(stage, k, occ, passes, variant, resort="full", movers_frac=0.0,
                            flags_in_force=False, narrow=False, bucket_in_force=False):
     """Algorithmic HBM bytes per particle of one launch of each stage (FP32
@@ -240,11 +278,7 @@ def full_run_profile(sph, scene, total, chunk=100):
     rows, t_sum, done = [], 0.0, 0
     while done < total:
         kk = min(chunk, total - done)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        ctx.step(kk)
-        ctx.synchronize()
-        dt = time.perf_counter() - t0
+        dt = timed_steps(ctx, kk)
         t_sum += dt
         done += kk
         rows.append([done, round(dt * 1e3 / kk, 4), ctx.movers(), ctx.resort_path()])
@@ -287,11 +321,7 @@ def bench_periodic(sph, steps, ref_steps=20):
     ctx.set_graph(True)
     ctx.step(10)
     ctx.step_graphs(steps)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.step(steps)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
+    wall = timed_steps(ctx, steps)
     value = n * steps / wall
     ev = min(steps, 100)
     ctx.set_timing(True)
@@ -365,11 +395,7 @@ def bench_small(sph, steps):
     ctx.set_graph(True)
     ctx.step(20)
     ctx.step_graphs(steps)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.step(steps)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
+    wall = timed_steps(ctx, steps)
     ctx.set_timing(True)
     ctx.step(100)
     ctx.synchronize()
@@ -393,11 +419,7 @@ def bench_small(sph, steps):
             c2.set_step_kernel(kv)
             c2.step(10)
             c2.step_graphs(200)
-            c2.synchronize()
-            t0 = time.perf_counter()
-            c2.step(200)
-            c2.synchronize()
-            row[kname] = round((time.perf_counter() - t0) * 1e6 / 200, 2)
+            row[kname] = round(timed_steps(c2, 200) * 1e6 / 200, 2)
         c2.close()
         mappings[f"{sc2.n}_particles_us_per_step"] = row
     log(f"mappings {mappings}")
@@ -425,11 +447,7 @@ def bench_batched(sph, steps, trial_steps=10):
     ctx.set_graph(True)
     ctx.step(10)
     ctx.step_graphs(steps)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.step(steps)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
+    wall = timed_steps(ctx, steps)
     value = n * steps / wall
     ctx.set_timing(True)
     ctx.step(min(steps, 50))
@@ -625,6 +643,11 @@ def main():
     ap.add_argument("--window-start", type=int, default=2500,
                     help="steps of the run from rest before the timed window (the collapse "
                          "is under way; movers per step grow with it)")
+    ap.add_argument("--settle", type=int, default=200,
+                    help="steps run in one call after the variant timings and the mover "
+                         "sample, before the warm-up: the timed steps then start from the "
+                         "state a long run is in (the first ~200 steps after short, "
+                         "synchronised calls run ~1.5 % slower)")
     ap.add_argument("--full-steps", type=int, default=5000,
                     help="length of the whole-run profile from rest (0 disables)")
     ap.add_argument("--periodic-steps", type=int, default=200,
@@ -724,22 +747,26 @@ def main():
     mover_stats = {"min": int(min(mv)), "mean": float(np.mean(mv)), "max": int(max(mv)),
                    "path_steps": mp}
     log(f"movers per step {mover_stats}")
+    # the run continues without a break before the warm-up (the variant
+    # timings and the mover sample are short calls with synchronisations
+    # between them; the steps after such calls run slower until the device is
+    # back in the state of a long run)
+    if args.settle > 0:
+        ctx.step(args.settle)
+        ctx.synchronize()
+        log(f"settle {args.settle} steps done")
     # warm-up steps right before the timed call, as one call (a step call that
     # follows another one reuses the packed state it left)
     ctx.step(args.warmup)
     ctx.synchronize()
     log(f"warmup {args.warmup} steps done")
-    window_first = args.window_start + 24 * len(combos) + 20 + args.warmup
+    window_first = args.window_start + 24 * len(combos) + 20 + max(args.settle, 0) + args.warmup
 
-    # timed region: exactly `steps` graph-replayed steps, synchronised on both
-    # sides (the graphs that call replays are captured beforehand)
+    # timed region: exactly `steps` graph-replayed steps, a device-wide
+    # synchronisation on both sides (the graphs that call replays are captured
+    # beforehand; the status check follows the clock)
     ctx.step_graphs(args.steps)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.step(args.steps)
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    wall = timed_steps(ctx, args.steps)
     value = n * args.steps / wall
     log(f"timed {args.steps} steps: {wall * 1e3 / args.steps:.4f} ms/step")
 
@@ -787,6 +814,7 @@ def main():
         "config": {"workload": scene.name, "particles": n, "box": list(scene.params.hi),
                    "h": scene.params.h, "dt": scene.params.dt, "physics": "wcsph",
                    "window": {"first_step": window_first, "steps": args.steps,
+                              "settle_steps": max(args.settle, 0),
                               "note": "timed late in the collapse, after the run from rest"},
                    "mean_neighbors": round(k, 3), "occupied_cells_per_particle": round(occ, 4),
                    "radix_passes": passes, "step_kernel": best, "resort": best_resort,
