@@ -677,7 +677,8 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE "
-              f"{world}); multi-GPU sharding of this path is described in DESIGN.md, not run.",
+              f"{world}): the step shards by Morton-key ranges with a two-cell halo "
+              "exchange, a design that is not built here.",
               file=sys.stderr)
         sys.exit(2)
 
