@@ -29,7 +29,8 @@ def scenes(sph):
     out["reference_rowmajor"] = sph.scenes.reference_example(order=sph.ORDER_ROWMAJOR)
     out["dam_break_16k"] = sph.scenes.dam_break()
     s = sph.scenes.periodic_shear(n_side=96, seed=9)
-    out["periodic_9k"] = s
+    out["periodic_9k"] = s        # 73 x 73 cells (odd: stencil rows of three loads)
+    out["periodic_even"] = sph.scenes.periodic_shear(n_side=100, seed=10)  # 76 x 76 cells
     # random gas with clusters / duplicates
     d = sph.scenes.dam_break(80, 80, seed=5, name="random")
     d.rx = rng.random(d.n).astype(np.float32) * 0.3
@@ -157,7 +158,8 @@ def test_density_forces_stages(sph, orc, all_scenes, name):
         note(f"[gpu] bit-identical to FP32 oracle (rho, ax): {exact}")
 
 
-@pytest.mark.parametrize("name", ["dam_break_16k", "periodic_9k", "reference_example"])
+@pytest.mark.parametrize("name", ["dam_break_16k", "periodic_9k", "periodic_even",
+                                  "reference_example"])
 def test_ell_lists_bit_exact(sph, orc, all_scenes, name):
     s = all_scenes[name]
     o = oracle_sorted(orc, s.params, s.rx, s.ry, s.vx, s.vy)
@@ -306,7 +308,8 @@ def test_ell_overflow_fallback(sph, orc):
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("name,steps", [("reference_example", 50), ("reference_rowmajor", 20),
                                         ("dam_break_16k", 40), ("periodic_9k", 40),
-                                        ("random_clusters", 20), ("dense_cluster", 3)])
+                                        ("periodic_even", 30), ("random_clusters", 20),
+                                        ("dense_cluster", 3)])
 def test_multistep_run_bit_exact_fp32(sph, orc, all_scenes, name, steps, variant, resort):
     """The whole pipeline, `steps` iterations, with each step-kernel variant
     (split blocks, fused blocks, per particle) and each re-sort (full radix,
