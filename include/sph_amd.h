@@ -405,6 +405,11 @@ int sph_get_device_view(void* ctx, sph_device_view* out);
 int sph_set_timing(void* ctx, int32_t enable);
 /* Accumulated milliseconds per enum sph_stage and launch counts; resets them. */
 int sph_stage_times(void* ctx, double* ms, int64_t* launches, int32_t count);
+/* Waits for the context stream and reads the device status word (through a
+ * pinned host word): SPH_ERR_HIP when a bounded wait on the device gave up
+ * (radix-sort look-back, re-sort merge) since the context was created. About
+ * 12 us on an idle stream; a caller timing only the steps can wait with
+ * hipDeviceSynchronize and call this afterwards. */
 int sph_synchronize(void* ctx);
 
 #ifdef __cplusplus
