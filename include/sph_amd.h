@@ -269,7 +269,8 @@ int sph_block_stats(void* ctx, int64_t* out6);
  * A batch context holds n_scenes independent particle systems that share
  * physics, boundary, order and ell_width but each have their own box, h, dt,
  * mass and derived constants. The scenes' keys live in one key space (scene s
- * owns the key range [base_s, base_s + its cell-table size)), so one radix sort
+ * owns the key range [base_s, base_s + its cell-table size), base_s the sum of
+ * the earlier scenes' table sizes each rounded up to even), so one radix sort
  * per step serves all scenes and each scene keeps a fixed, contiguous segment
  * [first_s, first_s + count_s) of the sorted order; every workgroup of the
  * physics kernels covers one tile of one segment. Upload and download take the

@@ -790,7 +790,9 @@ int batch_layout(const Ctx* c, const sph_batch_tuning& t, std::vector<sph_grid_i
     }
     derive_phys(p, grids[i], dev[i].c, dev[i].g);
     dev[i].base = (uint64_t)total;
-    total += grids[i].n_keys;
+    // every scene's table starts at an even entry (16-byte aligned): the
+    // density walk loads pairs of ranges
+    total += (grids[i].n_keys + 1) & ~(int64_t)1;
     if (total > ((int64_t)1 << 31)) {
       why = "batch cell table larger than 2^31 entries";
       return -1;

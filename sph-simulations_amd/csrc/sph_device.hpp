@@ -342,7 +342,8 @@ __device__ __forceinline__ void stencil_ranges(const DevGrid& g, const int2* __r
     // 8-byte load of its third cell (six loads, not nine). Needs the pair's
     // odd member to be the row's neighbor: always in a walled box (a pair
     // past the last column reads a padded cell, masked below) and in a
-    // periodic one with an even number of columns.
+    // periodic one with an even number of columns. Tables start 16-byte
+    // aligned (a batched scene's table at an even entry).
     if (kPairLoads && g.nx >= 2 && (BC == 0 || (g.nx & 1) == 0)) {
       const bool odd = (cx & 1) != 0;
       const uint32_t pxd = odd ? dxm : dx;        // the pair's even member
