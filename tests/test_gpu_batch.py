@@ -137,6 +137,34 @@ def test_batch_measure_per_scene(sph, orc, small_batch):
         np.testing.assert_allclose(m[s], o, rtol=1e-12, atol=0)
 
 
+def test_batch_periodic_scenes_bit_exact(sph, orc):
+    """A batch of periodic scenes with odd and even column counts (46, 49 and
+    76 cells a side): every scene bit-identical to its own oracle run, and
+    the scenes' cell tables at their documented key bases."""
+    parts = [sph.scenes.periodic_shear(n_side=ns, seed=0x5EED0200 + ns) for ns in (60, 64, 100)]
+    cat = lambda k: np.concatenate([getattr(sc, k) for sc in parts])
+    b = sph.scenes.BatchScene("periodic_batch", [sc.params for sc in parts],
+                              np.array([sc.n for sc in parts], np.int64), cat("rx"), cat("ry"),
+                              cat("vx"), cat("vy"), 15)
+    assert sorted(orc.grid(sc.params).nx % 2 for sc in parts) == [0, 0, 1]
+    with sph.Batch(b.params, b.counts) as ctx:
+        ctx.set_tuning(1.0, 256, 1)
+        ctx.set_graph(True)
+        ctx.upload(b.rx, b.ry, b.vx, b.vy)
+        ctx.run(b.steps)
+        d = ctx.download()
+        f = ctx.fields()
+        keys = ctx.sorted_keys()
+        infos = [ctx.scene_info(s) for s in range(len(b.counts))]
+    check_scenes_vs_oracle(orc, b, d, f, 1.0, b.steps)
+    base = 0
+    for s, (first, count, g) in enumerate(infos):
+        seg = slice(first, first + count)
+        local = orc.keys(scene_params(b.params[s], 1.0), d["rx"][seg], d["ry"][seg])
+        assert np.array_equal(keys[seg] - np.uint64(base), local)
+        base += (int(g.n_keys) + 1) & ~1
+
+
 def test_batch_errors(sph, small_batch):
     b = small_batch
     with sph.Batch(b.params, b.counts) as ctx:
